@@ -1,0 +1,454 @@
+"""MI355X (gfx950) backend for the leveled RNS-CKKS hot path of ccmm-rs.
+
+Python binding of ``libckks_hip.so`` (the C ABI declared in
+``include/ckks_hip.h``).  Class and method names mirror the reference's Rust
+API (``PreparedRnsNttPlan::forward``, ``rescale_rns_ckks_to_next``,
+``rns_key_switch_with_prepared_dynamic_ntt`` ...) so callers and tests read
+like the reference's own.  Device memory is held in torch tensors of dtype
+int64 whose bits are the uint64 residues; torch is only the allocator and
+stream provider.  There is no CPU fallback: every operation runs the HIP
+kernels, and a missing library raises.
+
+The directory name contains a hyphen, so load it with :func:`load` from a
+path (``importlib``) rather than ``import``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libckks_hip.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "ckks_hip.h")
+
+_lib = None
+
+_sz = C.c_size_t
+_u64 = C.c_uint64
+_p = C.c_void_p
+
+
+class CkksError(RuntimeError):
+    """A library call returned a non-zero status; message from ckks_last_error()."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(msg)
+        self.code = code
+
+
+def build(quiet: bool = True) -> str:
+    """Compile libckks_hip.so in-tree for gfx950 (hipcc cross-compiles)."""
+    import subprocess
+
+    cmd = ["make", "-j8", "-C", _HERE]
+    subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL if quiet else None)
+    return LIB_PATH
+
+
+_SIGNATURES = {
+    "ckks_last_error": (C.c_char_p, []),
+    "ckks_version": (C.c_char_p, []),
+    "ckks_context_create": (C.c_int, [C.c_int, _p, C.POINTER(_p)]),
+    "ckks_context_destroy": (C.c_int, [_p]),
+    "ckks_context_set_stream": (C.c_int, [_p, _p]),
+    "ckks_sync": (C.c_int, [_p]),
+    "ckks_malloc": (C.c_int, [_p, _sz, C.POINTER(_p)]),
+    "ckks_free": (C.c_int, [_p, _p]),
+    "ckks_upload": (C.c_int, [_p, _p, _p, _sz]),
+    "ckks_download": (C.c_int, [_p, _p, _p, _sz]),
+    "ckks_find_ntt_primes": (C.c_int, [C.c_uint, _sz, _sz, _p, _sz, _p, C.POINTER(_sz)]),
+    "ckks_rns_plan_create": (C.c_int, [_p, _p, _sz, _sz, C.POINTER(_p)]),
+    "ckks_rns_plan_destroy": (C.c_int, [_p]),
+    "ckks_rns_plan_psi": (C.c_int, [_p, _p]),
+    "ckks_ntt_forward": (C.c_int, [_p, _p, _p, _sz, _sz, _sz]),
+    "ckks_ntt_inverse": (C.c_int, [_p, _p, _p, _sz, _sz, _sz]),
+    "ckks_ntt_forward_batch": (C.c_int, [_p, _p, _sz]),
+    "ckks_ntt_inverse_batch": (C.c_int, [_p, _p, _sz]),
+    "ckks_poly_mul": (C.c_int, [_p, _p, _p, _p, _sz, _sz, _sz]),
+    "ckks_poly_mac": (C.c_int, [_p, _p, _p, _p, _sz, _sz, _sz]),
+    "ckks_poly_add": (C.c_int, [_p, _p, _p, _p, _sz, _sz, _sz]),
+    "ckks_poly_sub": (C.c_int, [_p, _p, _p, _p, _sz, _sz, _sz]),
+    "ckks_poly_neg": (C.c_int, [_p, _p, _p, _sz, _sz, _sz]),
+    "ckks_automorphism": (C.c_int, [_p, _p, _p, _sz, _sz, _sz, _u64]),
+    "ckks_ntt_galois_permute": (C.c_int, [_p, _p, _p, _sz, _sz, _sz, _u64]),
+    "ckks_rescale": (C.c_int, [_p, _p, _p, _sz, _sz, _sz, _sz]),
+    "ckks_mod_switch": (C.c_int, [_p, _p, _p, _sz, _sz, _sz, _sz]),
+    "ckks_tensor": (C.c_int, [_p, _p, _p, _p, _sz, _sz, _sz, _sz]),
+    "ckks_rns_ksk_create": (C.c_int, [_p, _p, _sz, _p, _p, C.POINTER(_p)]),
+    "ckks_hybrid_ksk_create": (C.c_int, [_p, _sz, _p, _sz, _p, _p, C.POINTER(_p)]),
+    "ckks_ksk_destroy": (C.c_int, [_p]),
+    "ckks_key_switch": (C.c_int, [_p, _p, _p, _sz, _sz, _sz]),
+    "ckks_apply_galois": (C.c_int, [_p, _u64, _p, _p, _sz, _sz, _sz]),
+    "ckks_relinearize": (C.c_int, [_p, _p, _p, _sz, _sz, _sz]),
+    "ckks_multiply_relinearize_rescale": (C.c_int, [_p, _p, _p, _p, _sz, _sz, _sz]),
+}
+
+
+def header_symbols() -> list[str]:
+    """Every function name declared in include/ckks_hip.h."""
+    text = open(HEADER_PATH).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(ckks_\w+)\s*\(", text, re.M)))
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise CkksError(-1, f"{LIB_PATH} is missing: build it with make -C {_HERE}")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(L, name, None)
+            if fn is None:
+                continue
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise CkksError(rc, lib().ckks_last_error().decode())
+
+
+# --------------------------------------------------------------- tensors
+def _torch():
+    import torch
+
+    return torch
+
+
+def to_device(x: np.ndarray, device="cuda"):
+    """uint64 numpy array -> int64 torch tensor on the GPU (same bits)."""
+    torch = _torch()
+    a = np.ascontiguousarray(x, dtype=np.uint64)
+    return torch.from_numpy(a.view(np.int64).copy()).to(device)
+
+
+def to_host(t) -> np.ndarray:
+    return t.detach().cpu().numpy().view(np.uint64)
+
+
+def empty(shape, device="cuda"):
+    torch = _torch()
+    return torch.empty(shape, dtype=torch.int64, device=device)
+
+
+def _ptr(t) -> int:
+    if t is None:
+        return None
+    assert t.is_contiguous() or t.dim() == 0
+    return t.data_ptr()
+
+
+# --------------------------------------------------------------- parameters
+def find_ntt_primes(bits: int, degree: int, count: int, skip=()) -> list[int]:
+    """find_ntt_prime_below_bits (ring/ntt_prime.rs:67) continued downward."""
+    out = np.zeros(max(count, 1), dtype=np.uint64)
+    sk = np.ascontiguousarray(list(skip) or [0], dtype=np.uint64)
+    found = _sz(0)
+    _check(lib().ckks_find_ntt_primes(bits, degree, count, sk.ctypes.data, len(skip),
+                                      out.ctypes.data, C.byref(found)))
+    return [int(v) for v in out[: found.value]]
+
+
+def chain_moduli(levels: int, special: int, degree: int = 65536):
+    """Moduli of the benchmark chains: one 60-bit base prime, `levels`
+    40-bit primes and `special` 60-bit special primes, all q = 1 mod 2N
+    (found for N = 2^16, so they also serve every smaller degree).
+    Returns (Q moduli, P moduli)."""
+    q0 = find_ntt_primes(60, degree, 1)
+    qs = find_ntt_primes(40, degree, levels)
+    ps = find_ntt_primes(60, degree, special, skip=q0) if special else []
+    return q0 + qs, ps
+
+
+# --------------------------------------------------------------- context
+class Context:
+    """ckks_context on one GPU; runs on torch's current stream by default so
+    library calls and torch ops are ordered."""
+
+    def __init__(self, device: int = 0, stream=None):
+        torch = _torch()
+        self.device = device
+        if stream is None:
+            stream = torch.cuda.current_stream(device)
+        self._stream = stream
+        h = _p()
+        # handle 0 is the null stream, which the library then uses too
+        _check(lib().ckks_context_create(device, _p(stream.cuda_stream or None), C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def sync(self):
+        _check(lib().ckks_sync(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ckks_context_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class PreparedRnsNttPlan:
+    """PreparedRnsNttPlan (ring/prepared_rns_ntt.rs) on the GPU: one prepared
+    negacyclic plan per modulus of the basis."""
+
+    def __init__(self, ctx: Context, moduli, degree: int):
+        self.ctx = ctx
+        self.moduli = [int(q) for q in moduli]
+        self.degree = int(degree)
+        arr = np.ascontiguousarray(self.moduli, dtype=np.uint64)
+        h = _p()
+        _check(lib().ckks_rns_plan_create(ctx.handle, arr.ctypes.data, len(arr), self.degree,
+                                          C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __len__(self):
+        return len(self.moduli)
+
+    def psi(self) -> list[int]:
+        out = np.zeros(len(self.moduli), dtype=np.uint64)
+        _check(lib().ckks_rns_plan_psi(self._h, out.ctypes.data))
+        return [int(v) for v in out]
+
+    def _shape(self, t, n_limbs):
+        n_limbs = n_limbs or len(self.moduli)
+        per = n_limbs * self.degree
+        assert t.numel() % per == 0, "tensor size must be a multiple of limbs x N"
+        return t.numel() // per, n_limbs, per
+
+    def forward(self, x, out=None, n_limbs=None):
+        """Forward negacyclic NTT of every limb of every polynomial in x
+        (limb-major [poly][limb][N]); out may be x."""
+        n_polys, n_limbs, per = self._shape(x, n_limbs)
+        out = x if out is None else out
+        _check(lib().ckks_ntt_forward(self._h, _ptr(x), _ptr(out), n_polys, n_limbs, per))
+        return out
+
+    def inverse(self, x, out=None, n_limbs=None):
+        n_polys, n_limbs, per = self._shape(x, n_limbs)
+        out = x if out is None else out
+        _check(lib().ckks_ntt_inverse(self._h, _ptr(x), _ptr(out), n_polys, n_limbs, per))
+        return out
+
+    # --- pointwise (NttPolynomial / Polynomial arithmetic) ---
+    def _ew(self, fn, x, y, out, n_limbs):
+        n_polys, n_limbs, per = self._shape(x, n_limbs)
+        _check(fn(self._h, _ptr(x), _ptr(y), _ptr(out), n_polys, n_limbs, per))
+        return out
+
+    def pointwise_mul(self, x, y, out=None, n_limbs=None):
+        out = empty(x.shape) if out is None else out
+        return self._ew(lib().ckks_poly_mul, x, y, out, n_limbs)
+
+    def pointwise_mul_add_assign(self, acc, x, y, n_limbs=None):
+        n_polys, n_limbs, per = self._shape(acc, n_limbs)
+        _check(lib().ckks_poly_mac(self._h, _ptr(acc), _ptr(x), _ptr(y), n_polys, n_limbs, per))
+        return acc
+
+    def add(self, x, y, out=None, n_limbs=None):
+        out = empty(x.shape) if out is None else out
+        return self._ew(lib().ckks_poly_add, x, y, out, n_limbs)
+
+    def sub(self, x, y, out=None, n_limbs=None):
+        out = empty(x.shape) if out is None else out
+        return self._ew(lib().ckks_poly_sub, x, y, out, n_limbs)
+
+    def neg(self, x, out=None, n_limbs=None):
+        out = empty(x.shape) if out is None else out
+        n_polys, n_limbs, per = self._shape(x, n_limbs)
+        _check(lib().ckks_poly_neg(self._h, _ptr(x), _ptr(out), n_polys, n_limbs, per))
+        return out
+
+    def apply_automorphism(self, x, exponent: int, out=None, n_limbs=None):
+        """apply_automorphism (ckks/automorphism.rs:8) per limb, coefficient domain."""
+        out = empty(x.shape) if out is None else out
+        n_polys, n_limbs, per = self._shape(x, n_limbs)
+        _check(lib().ckks_automorphism(self._h, _ptr(x), _ptr(out), n_polys, n_limbs, per, exponent))
+        return out
+
+    def permute_negacyclic_ntt_galois(self, x, exponent: int, out=None, n_limbs=None):
+        """permute_negacyclic_ntt_galois (ring/ntt_galois_permutation.rs:5)."""
+        out = empty(x.shape) if out is None else out
+        n_polys, n_limbs, per = self._shape(x, n_limbs)
+        _check(lib().ckks_ntt_galois_permute(self._h, _ptr(x), _ptr(out), n_polys, n_limbs, per,
+                                             exponent))
+        return out
+
+    # --- ciphertext level ---
+    def rescale_to_next(self, ct, n_limbs: int, out=None):
+        """rescale_rns_ckks_to_next (ckks/rns_rescale.rs:19) on a batch of
+        ciphertexts [ct][2][n_limbs][N] -> [ct][2][n_limbs-1][N]."""
+        n = self.degree
+        n_cts = ct.numel() // (2 * n_limbs * n)
+        out = empty((n_cts, 2, n_limbs - 1, n)) if out is None else out
+        _check(lib().ckks_rescale(self._h, _ptr(ct), _ptr(out), n_cts, n_limbs, 2 * n_limbs * n,
+                                  2 * (n_limbs - 1) * n))
+        return out
+
+    def mod_switch_to_next(self, ct, n_limbs: int, out=None):
+        n = self.degree
+        n_cts = ct.numel() // (2 * n_limbs * n)
+        out = empty((n_cts, 2, n_limbs - 1, n)) if out is None else out
+        _check(lib().ckks_mod_switch(self._h, _ptr(ct), _ptr(out), n_cts, n_limbs,
+                                     2 * n_limbs * n, 2 * (n_limbs - 1) * n))
+        return out
+
+    def tensor(self, ct0, ct1, n_limbs: int, out=None):
+        """rns_tensor_with_prepared_ntt (grafting/rns_key_switch.rs:1724)."""
+        n = self.degree
+        n_cts = ct0.numel() // (2 * n_limbs * n)
+        out = empty((n_cts, 3, n_limbs, n)) if out is None else out
+        _check(lib().ckks_tensor(self._h, _ptr(ct0), _ptr(ct1), _ptr(out), n_cts, n_limbs,
+                                 2 * n_limbs * n, 3 * n_limbs * n))
+        return out
+
+    def close(self):
+        # the context releases its plans itself when it is closed first
+        if getattr(self, "_h", None) and self.ctx.handle:
+            lib().ckks_rns_plan_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class _KeySwitchKey:
+    """Device-resident key-switching key; `switch`, `apply_galois`,
+    `relinearize` and `multiply_relinearize_rescale` take ciphertext batches
+    [ct][2][L][N] (relinearize: [ct][3][L][N]) over the key's Q basis."""
+
+    q_limbs: int
+
+    def _finish_init(self, plan, h):
+        self.plan = plan
+        self.ctx = plan.ctx
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def _n_cts(self, t, polys):
+        per = polys * self.q_limbs * self.plan.degree
+        assert t.numel() % per == 0, "ciphertext batch size mismatch"
+        return t.numel() // per
+
+    def switch(self, ct, out=None):
+        """rns_key_switch_with_prepared_dynamic_ntt (grafting/rns_key_switch.rs:2526)
+        or the dnum hybrid switch."""
+        n = self._n_cts(ct, 2)
+        out = empty((n, 2, self.q_limbs, self.plan.degree)) if out is None else out
+        _check(lib().ckks_key_switch(self._h, _ptr(ct), _ptr(out), n, 0, 0))
+        return out
+
+    def apply_galois(self, ct, exponent: int, out=None):
+        """apply_rns_galois_automorphism_with_prepared_ntt (ckks/rns_galois_key.rs:439)."""
+        n = self._n_cts(ct, 2)
+        out = empty((n, 2, self.q_limbs, self.plan.degree)) if out is None else out
+        _check(lib().ckks_apply_galois(self._h, exponent, _ptr(ct), _ptr(out), n, 0, 0))
+        return out
+
+    def relinearize(self, ct3, out=None):
+        """rns_relinearize_with_prepared_dynamic_ntt (grafting/rns_key_switch.rs:2113)."""
+        n = self._n_cts(ct3, 3)
+        out = empty((n, 2, self.q_limbs, self.plan.degree)) if out is None else out
+        _check(lib().ckks_relinearize(self._h, _ptr(ct3), _ptr(out), n, 0, 0))
+        return out
+
+    def multiply_relinearize_rescale(self, ct0, ct1, out=None):
+        """multiply_relinearize_rescale_rns_ckks_with_prepared_ntt (ckks/rns_multiply.rs:161)."""
+        n = self._n_cts(ct0, 2)
+        out = empty((n, 2, self.q_limbs - 1, self.plan.degree)) if out is None else out
+        _check(lib().ckks_multiply_relinearize_rescale(self._h, _ptr(ct0), _ptr(ct1), _ptr(out), n,
+                                                       0, 0))
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None) and self.ctx.handle:
+            lib().ckks_ksk_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _sizes(seq):
+    return np.ascontiguousarray(list(seq), dtype=np.uintp)
+
+
+class RnsKeySwitchKey(_KeySwitchKey):
+    """PreparedRnsKeySwitchKey / PreparedRnsMultiplicationKey
+    (grafting/rns_key_switch.rs:1930-2430): NTT-domain entries
+    key_b/key_a of shape [blocks][L][N] over the plan's full basis."""
+
+    def __init__(self, plan: PreparedRnsNttPlan, block_sizes, key_b, key_a):
+        self.q_limbs = len(plan.moduli)
+        bs = _sizes(block_sizes)
+        kb = np.ascontiguousarray(key_b, dtype=np.uint64)
+        ka = np.ascontiguousarray(key_a, dtype=np.uint64)
+        h = _p()
+        _check(lib().ckks_rns_ksk_create(plan.handle, bs.ctypes.data, len(bs), kb.ctypes.data,
+                                         ka.ctypes.data, C.byref(h)))
+        self._finish_init(plan, h)
+
+
+class HybridKeySwitchKey(_KeySwitchKey):
+    """dnum-block hybrid key over Q u P: entry j encrypts P*E_j*s' (NTT
+    domain, [blocks][Q+P limbs][N]); `plan` holds the Q limbs then P."""
+
+    def __init__(self, plan: PreparedRnsNttPlan, q_limbs: int, block_sizes, key_b, key_a):
+        self.q_limbs = int(q_limbs)
+        bs = _sizes(block_sizes)
+        kb = np.ascontiguousarray(key_b, dtype=np.uint64)
+        ka = np.ascontiguousarray(key_a, dtype=np.uint64)
+        h = _p()
+        _check(lib().ckks_hybrid_ksk_create(plan.handle, self.q_limbs, bs.ctypes.data, len(bs),
+                                            kb.ctypes.data, ka.ctypes.data, C.byref(h)))
+        self._finish_init(plan, h)
+
+
+class NttBatchEntry(C.Structure):
+    _fields_ = [("plan", _p), ("input", _p), ("output", _p), ("n_polys", _sz),
+                ("n_limbs", _sz), ("poly_stride", _sz)]
+
+
+def ntt_batch(ctx: Context, entries, forward: bool = True):
+    """One launch per pass over entries [(plan, x, out, n_limbs)] of mixed
+    degree and level."""
+    arr = (NttBatchEntry * len(entries))()
+    for i, (plan, x, out, n_limbs) in enumerate(entries):
+        n_polys, n_limbs, per = plan._shape(x, n_limbs)
+        arr[i] = NttBatchEntry(plan.handle, _ptr(x), _ptr(out if out is not None else x), n_polys,
+                               n_limbs, per)
+    fn = lib().ckks_ntt_forward_batch if forward else lib().ckks_ntt_inverse_batch
+    _check(fn(ctx.handle, C.cast(arr, _p), len(entries)))
+
+
+def load():
+    """Return this module (for callers that locate the package by path)."""
+    import sys
+
+    return sys.modules[__name__]

@@ -1,0 +1,103 @@
+// Internal state shared by the library's translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <set>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/ckks_hip.h"
+#include "modarith.hpp"
+#include "ntt_tile.hpp"
+
+namespace ck {
+
+// Per-limb constants as the kernels see them.
+struct DevPrime {
+    u64 q;
+    const Tw *fwd;  // psi^bitrev(k) with Shoup quotients, k in [0, N)
+    const Tw *inv;  // psi^-bitrev(k)
+    u64 ninv, ninv_q;
+    u64 br_hi, br_lo;  // floor(2^128 / q)
+    u32 logn, pad;
+};
+
+// Constants of the exact rescale by a dropped limb l into limb i < l
+// (ckks/rns_rescale.rs:107-167).
+struct RescaleConst {
+    u64 inv, inv_q;  // q_l^-1 mod q_i and its Shoup quotient
+    u64 ql_mod;      // q_l mod q_i
+};
+
+struct Error {
+    int code;
+    std::string msg;
+};
+
+void set_error(int code, const std::string &msg);
+int fail(int code, const std::string &msg);
+
+#define CK_HIP(call)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return ::ck::fail(CKKS_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct TwiddleTables {
+    u64 q;
+    u32 logn;
+    Tw *fwd = nullptr, *inv = nullptr;  // device
+};
+
+}  // namespace ck
+
+struct ckks_rns_plan;
+struct ckks_ksk;
+
+struct ckks_context {
+    int device = 0;
+    hipStream_t stream = nullptr;  // nullptr = the null stream
+    // plans and keys created from this context, released with it
+    std::set<ckks_rns_plan *> plans;
+    std::set<ckks_ksk *> keys;
+    // (q, logn) -> device twiddle tables, shared by plans of this context
+    std::map<std::pair<uint64_t, uint32_t>, ck::TwiddleTables> tables;
+    // grow-only scratch slots: slot 0 is the NTT's inter-pass buffer, the
+    // others belong to the composite operations (tensor, key switch).
+    static constexpr int kSlots = 8;
+    void *ws[kSlots] = {};
+    size_t ws_bytes[kSlots] = {};
+    std::vector<void *> owned;  // misc device allocations freed at destroy
+
+    int workspace(int slot, size_t bytes, void **out);
+};
+
+struct ckks_rns_plan {
+    ckks_context *ctx = nullptr;
+    std::vector<uint64_t> moduli;
+    std::vector<uint64_t> psi;
+    uint32_t logn = 0;
+    size_t degree = 0;
+    ck::DevPrime *d_primes = nullptr;  // device array, one per limb
+    std::vector<ck::DevPrime> h_primes;
+    ck::RescaleConst *d_rescale = nullptr;  // [dropped l][target i], L x L
+};
+
+namespace ck {
+
+// Launch helpers (ntt.hip).
+struct NttCall {
+    const ckks_rns_plan *plan;
+    const u64 *in;
+    u64 *out;
+    size_t n_polys, n_limbs;
+    size_t in_stride, out_stride;  // elements between consecutive polynomials
+};
+int launch_ntt(ckks_context *ctx, const NttCall *calls, size_t count, bool forward);
+
+int check_plan_limbs(const ckks_rns_plan *plan, size_t n_limbs, const char *what);
+
+}  // namespace ck

@@ -1,0 +1,650 @@
+// Key switching (gfx950): RNS-gadget (reference family, no special primes)
+// and the dnum-block hybrid with special primes P.
+//
+// Both run the same exact pipeline over the key basis B (B = Q for the RNS
+// gadget, B = Q u P for the hybrid):
+//   1. ModUp: digit_j = canonical x mod Q_j (grafting/rns_gadget.rs:143-228)
+//      lifted exactly to every limb of B by mixed-radix (Garner) digits
+//      (ring/basis_transition.rs:72-138).  For limbs inside block j the digit
+//      is the input residue itself.
+//   2. forward NTT of every digit limb;
+//   3. MAC with the key: acc_c[i] = sum_j D_j[i] * key_c[j][i]  (c = b, a),
+//      one 128-bit accumulation and one reduction per residue;
+//   4. inverse NTT of both accumulators;
+//   5. hybrid: centered ModDown by P (ring/auxiliary_modulus.rs:89-171) with
+//      an exact P-side Garner and a lexicographic mixed-radix comparison
+//      against floor(P/2); then the pass-through component is added
+//      (b for key switch / rotation, c0 and c1 for relinearization).
+// Adding b after ModDown gives the same residues as ModDown(P b + acc_b)
+// (ckks/hybrid_rns_galois_key.rs:633-663) because P b = 0 mod P.
+#include <algorithm>
+#include <cstring>
+
+#include "host_math.hpp"
+#include "internal.hpp"
+
+namespace ck {
+
+constexpr int kMaxAlpha = 12;   // limbs per gadget block handled by ModUp
+constexpr int kMaxSpecial = 8;  // special primes handled by ModDown
+constexpr int kKsThreads = 256;
+
+// Constants of the exact lift of one gadget block.
+struct ModUpBlock {
+    u32 start, size;
+    // Garner: c_k = (x_k - sum_{m<k} c_m Pm[k][m]) * inv[k]  mod q_{start+k}
+    u64 pm[kMaxAlpha][kMaxAlpha];
+    u64 inv[kMaxAlpha], inv_q[kMaxAlpha];
+    // evaluation of the mixed-radix value at every basis limb i:
+    // w[i * kMaxAlpha + m] = (q_start ... q_{start+m-1}) mod q_i
+    const u64 *w;
+};
+
+struct ModDownTable {
+    u32 q_limbs, p_limbs;
+    u64 pm[kMaxSpecial][kMaxSpecial];  // Garner over P
+    u64 inv[kMaxSpecial], inv_q[kMaxSpecial];
+    u64 half[kMaxSpecial];  // mixed-radix digits of floor(P/2)
+    // per Q limb i: w[i*kMaxSpecial + m] = (p_0..p_{m-1}) mod q_i,
+    // pmod[i] = P mod q_i, pinv[i] = P^-1 mod q_i (Shoup pair)
+    const u64 *w;
+    const u64 *pmod, *pinv, *pinv_q;
+};
+
+}  // namespace ck
+
+struct ckks_ksk {
+    ckks_context *ctx = nullptr;
+    const ckks_rns_plan *plan = nullptr;  // key basis B
+    int hybrid = 0;
+    size_t q_limbs = 0, p_limbs = 0, limbs = 0, dnum = 0, degree = 0;
+    std::vector<size_t> block_start, block_size;
+    ck::u64 *d_kb = nullptr, *d_ka = nullptr;  // [dnum][B][N], NTT domain
+    ck::ModUpBlock *d_modup = nullptr;
+    std::vector<ck::ModUpBlock> h_modup;
+    ck::u64 *d_consts = nullptr;
+    ck::ModDownTable *d_moddown = nullptr;
+};
+
+namespace ck {
+
+// ---- device helpers --------------------------------------------------------
+
+CK_HD u64 reduce128(const Acc128 &a, const DevPrime &P) {
+    return barrett_reduce128(a.hi, a.lo, Barrett{P.q, P.br_hi, P.br_lo});
+}
+
+CK_HD u64 reduce64(u64 x, const DevPrime &P) {
+    return x >= P.q ? barrett_reduce128(0, x, Barrett{P.q, P.br_hi, P.br_lo}) : x;
+}
+
+// ---- 1. ModUp -------------------------------------------------------------------
+
+struct ModUpArgs {
+    const u64 *in;  // component being decomposed: limb i of ct c at in + c*in_stride + i*N
+    u64 in_stride;
+    u64 *digits;  // [ct][dnum][B][N]
+    const ModUpBlock *blocks;
+    const DevPrime *primes;  // key basis B
+    u32 dnum, limbs, logn, chunks;
+};
+
+template <int A>
+__device__ void modup_thread(const ModUpArgs &a, const ModUpBlock &blk, u32 ct, u32 j, u64 n) {
+    const u64 N = (u64)1 << a.logn;
+    const u64 *src = a.in + ct * a.in_stride + (u64)blk.start * N + n;
+    u64 x[A], c[A];
+#pragma unroll
+    for (int m = 0; m < A; ++m) x[m] = src[(u64)m * N];
+    // mixed-radix digits
+#pragma unroll
+    for (int k = 0; k < A; ++k) {
+        const DevPrime &Pk = a.primes[blk.start + k];
+        if (k == 0) {
+            c[0] = x[0];
+        } else {
+            Acc128 t;
+            t.zero();
+#pragma unroll
+            for (int m = 0; m < k; ++m) t.mac(c[m], blk.pm[k][m]);
+            u64 tr = reduce128(t, Pk);
+            c[k] = shoup(sub_mod(x[k], tr, Pk.q), blk.inv[k], blk.inv_q[k], Pk.q);
+        }
+    }
+    u64 *dst = a.digits + (((u64)ct * a.dnum + j) * a.limbs) * N + n;
+    for (u32 i = 0; i < a.limbs; ++i) {
+        u64 v;
+        if (i >= blk.start && i < blk.start + A) {
+            // inside the block the canonical digit reduces to the residue
+            v = src[(u64)(i - blk.start) * N];
+        } else {
+            const DevPrime &Pi = a.primes[i];
+            const u64 *w = blk.w + (u64)i * kMaxAlpha;
+            if (A == 1) {
+                v = reduce64(c[0], Pi);
+            } else {
+                Acc128 t;
+                t.zero();
+#pragma unroll
+                for (int m = 0; m < A; ++m) t.mac(c[m], w[m]);
+                v = reduce128(t, Pi);
+            }
+        }
+        dst[(u64)i * N] = v;
+    }
+}
+
+__global__ __launch_bounds__(kKsThreads) void k_modup(ModUpArgs a) {
+    const u32 b = blockIdx.x;
+    const u32 chunk = b % a.chunks;
+    const u32 cj = b / a.chunks;
+    const u32 j = cj % a.dnum, ct = cj / a.dnum;
+    const ModUpBlock &blk = a.blocks[j];
+    const u64 n = (u64)chunk * kKsThreads + threadIdx.x;
+    switch (blk.size) {
+#define CK_MU(A) \
+    case A: modup_thread<A>(a, blk, ct, j, n); break;
+        CK_MU(1) CK_MU(2) CK_MU(3) CK_MU(4) CK_MU(5) CK_MU(6)
+        CK_MU(7) CK_MU(8) CK_MU(9) CK_MU(10) CK_MU(11) CK_MU(12)
+#undef CK_MU
+        default: break;
+    }
+}
+
+// ---- 3. key MAC -------------------------------------------------------------------
+
+struct MacArgs {
+    const u64 *digits;  // [ct][dnum][B][N] NTT domain
+    const u64 *kb, *ka;  // [dnum][B][N]
+    u64 *acc;            // [ct][2][B][N]
+    const DevPrime *primes;
+    u32 n_cts, dnum, limbs, logn, chunks;
+};
+
+// Each thread owns one residue position (i, n) and keeps the key entries of
+// every digit in registers while it walks the batch, so a key is read from
+// HBM once per launch, not once per ciphertext.
+template <int D>
+__global__ __launch_bounds__(kKsThreads) void k_ks_mac(MacArgs a) {
+    const u32 b = blockIdx.x;
+    const u32 chunk = b % a.chunks;
+    const u32 i = b / a.chunks;
+    const u64 N = (u64)1 << a.logn;
+    const u64 n = (u64)chunk * kKsThreads + threadIdx.x;
+    const DevPrime P = a.primes[i];
+    u64 kb[D], ka[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+        kb[j] = a.kb[((u64)j * a.limbs + i) * N + n];
+        ka[j] = a.ka[((u64)j * a.limbs + i) * N + n];
+    }
+    for (u32 ct = 0; ct < a.n_cts; ++ct) {
+        const u64 *d = a.digits + (((u64)ct * D) * a.limbs + i) * N + n;
+        Acc128 sb, sa;
+        sb.zero();
+        sa.zero();
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            u64 dv = d[(u64)j * a.limbs * N];
+            sb.mac(dv, kb[j]);
+            sa.mac(dv, ka[j]);
+        }
+        u64 *o = a.acc + ((u64)ct * 2 * a.limbs + i) * N + n;
+        o[0] = reduce128(sb, P);
+        o[(u64)a.limbs * N] = reduce128(sa, P);
+    }
+}
+
+// Generic digit count (the RNS gadget with singleton blocks has dnum = L):
+// key entries are streamed per ciphertext; partial sums are reduced every 8
+// products so the 128-bit accumulator cannot overflow.
+__global__ __launch_bounds__(kKsThreads) void k_ks_mac_any(MacArgs a) {
+    const u32 b = blockIdx.x;
+    const u32 chunk = b % a.chunks;
+    const u32 rest = b / a.chunks;
+    const u32 i = rest % a.limbs, ct = rest / a.limbs;
+    const u64 N = (u64)1 << a.logn;
+    const u64 n = (u64)chunk * kKsThreads + threadIdx.x;
+    const DevPrime P = a.primes[i];
+    const u64 *d = a.digits + (((u64)ct * a.dnum) * a.limbs + i) * N + n;
+    u64 rb = 0, ra = 0;
+    for (u32 j0 = 0; j0 < a.dnum; j0 += 8) {
+        Acc128 sb, sa;
+        sb.zero();
+        sa.zero();
+        u32 j1 = j0 + 8 < a.dnum ? j0 + 8 : a.dnum;
+        for (u32 j = j0; j < j1; ++j) {
+            u64 dv = d[(u64)j * a.limbs * N];
+            u64 off = ((u64)j * a.limbs + i) * N + n;
+            sb.mac(dv, a.kb[off]);
+            sa.mac(dv, a.ka[off]);
+        }
+        rb = add_mod(rb, reduce128(sb, P), P.q);
+        ra = add_mod(ra, reduce128(sa, P), P.q);
+    }
+    u64 *o = a.acc + ((u64)ct * 2 * a.limbs + i) * N + n;
+    o[0] = rb;
+    o[(u64)a.limbs * N] = ra;
+}
+
+// ---- 5. ModDown / finalize ---------------------------------------------------------
+
+struct DownArgs {
+    const u64 *acc;  // [ct][2][B][N] coefficient domain
+    const u64 *add0, *add1;  // pass-through for component 0 / 1 (may be null)
+    u64 add_stride;
+    u64 *out;  // [ct][2][q_limbs][N] with out_stride
+    u64 out_stride;
+    const ModDownTable *md;  // null: no special primes
+    const DevPrime *primes;  // key basis
+    u32 limbs, q_limbs, logn, chunks;
+};
+
+template <int K>
+__global__ __launch_bounds__(kKsThreads) void k_moddown(DownArgs a) {
+    const u32 b = blockIdx.x;
+    const u32 chunk = b % a.chunks;
+    const u32 rest = b / a.chunks;
+    const u32 comp = rest % 2, ct = rest / 2;
+    const u64 N = (u64)1 << a.logn;
+    const u64 n = (u64)chunk * kKsThreads + threadIdx.x;
+    const u64 *x = a.acc + ((u64)ct * 2 + comp) * a.limbs * N + n;
+    const u64 *add = comp == 0 ? a.add0 : a.add1;
+    if (add) add += ct * a.add_stride + n;
+    u64 *o = a.out + ct * a.out_stride + (u64)comp * a.q_limbs * N + n;
+    if (K == 0) {
+        for (u32 i = 0; i < a.q_limbs; ++i) {
+            u64 v = x[(u64)i * N];
+            if (add) v = add_mod(v, add[(u64)i * N], a.primes[i].q);
+            o[(u64)i * N] = v;
+        }
+        return;
+    }
+    const ModDownTable &md = *a.md;
+    u64 c[K > 0 ? K : 1];
+    bool neg = false;
+    if (K > 0) {
+        u64 xp[K > 0 ? K : 1];
+#pragma unroll
+        for (int m = 0; m < K; ++m) xp[m] = x[(u64)(a.q_limbs + m) * N];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const DevPrime &Pk = a.primes[a.q_limbs + k];
+            if (k == 0) {
+                c[0] = xp[0];
+            } else {
+                Acc128 t;
+                t.zero();
+#pragma unroll
+                for (int m = 0; m < k; ++m) t.mac(c[m], md.pm[k][m]);
+                c[k] = shoup(sub_mod(xp[k], reduce128(t, Pk), Pk.q), md.inv[k], md.inv_q[k], Pk.q);
+            }
+        }
+        // x_P > floor(P/2)  <=>  mixed-radix digits compare greater from the top
+        int cmp = 0;
+#pragma unroll
+        for (int k = K - 1; k >= 0; --k)
+            if (cmp == 0 && c[k] != md.half[k]) cmp = c[k] > md.half[k] ? 1 : -1;
+        neg = cmp > 0;
+    }
+    for (u32 i = 0; i < a.q_limbs; ++i) {
+        const DevPrime &Pi = a.primes[i];
+        const u64 *w = md.w + (u64)i * kMaxSpecial;
+        Acc128 t;
+        t.zero();
+#pragma unroll
+        for (int m = 0; m < K; ++m) t.mac(c[m], w[m]);
+        u64 r = reduce128(t, Pi);  // x_P mod q_i
+        if (neg) r = sub_mod(r, md.pmod[i], Pi.q);  // (x_P - P) mod q_i
+        u64 v = shoup(sub_mod(x[(u64)i * N], r, Pi.q), md.pinv[i], md.pinv_q[i], Pi.q);
+        if (add) v = add_mod(v, add[(u64)i * N], Pi.q);
+        o[(u64)i * N] = v;
+    }
+}
+
+// ---- host-side setup ----------------------------------------------------------------
+
+static void fill_garner(const std::vector<u64> &mods, u64 (*pm)[kMaxAlpha], u64 *inv, u64 *inv_q,
+                        int stride_dummy) {
+    (void)stride_dummy;
+    size_t A = mods.size();
+    for (size_t k = 0; k < A; ++k) {
+        u64 qk = mods[k];
+        u64 prod = 1 % qk;
+        for (size_t m = 0; m < k; ++m) {
+            pm[k][m] = prod;  // (q_0 .. q_{m-1}) mod q_k
+            prod = host::mulmod(prod, mods[m] % qk, qk);
+        }
+        if (k > 0) {
+            inv[k] = host::invmod(prod, qk);
+            inv_q[k] = host::shoup_quotient(inv[k], qk);
+        }
+    }
+}
+
+static int create_key(const ckks_rns_plan *plan, size_t q_limbs, const size_t *block_sizes,
+                      size_t n_blocks, const u64 *key_b, const u64 *key_a, bool hybrid,
+                      ckks_ksk **out) {
+    if (!plan || !block_sizes || !key_b || !key_a || !out) return fail(CKKS_ERR_INVALID, "null argument");
+    if (n_blocks == 0) return fail(CKKS_ERR_INVALID, "RNS gadget layout must contain at least one block");
+    const size_t B = plan->moduli.size();
+    if (q_limbs == 0 || q_limbs > B) return fail(CKKS_ERR_INVALID, "Q limb count must be within the key basis");
+    const size_t K = B - q_limbs;
+    if (!hybrid && K != 0) return fail(CKKS_ERR_INVALID, "RNS gadget keys have no special primes");
+    if (hybrid && (K == 0 || K > (size_t)kMaxSpecial))
+        return fail(CKKS_ERR_UNSUPPORTED, "hybrid keys need 1..8 special primes");
+    size_t total = 0;
+    for (size_t j = 0; j < n_blocks; ++j) {
+        if (block_sizes[j] == 0) return fail(CKKS_ERR_INVALID, "RNS gadget blocks must be nonempty");
+        if (block_sizes[j] > (size_t)kMaxAlpha)
+            return fail(CKKS_ERR_UNSUPPORTED, "gadget blocks of at most 12 limbs are supported");
+        total += block_sizes[j];
+    }
+    if (total != q_limbs) return fail(CKKS_ERR_INVALID, "RNS gadget blocks must cover the full basis exactly");
+    ckks_context *ctx = plan->ctx;
+    CK_HIP(hipSetDevice(ctx->device));
+    std::unique_ptr<ckks_ksk> k(new ckks_ksk());
+    k->ctx = ctx;
+    k->plan = plan;
+    k->hybrid = hybrid ? 1 : 0;
+    k->q_limbs = q_limbs;
+    k->p_limbs = K;
+    k->limbs = B;
+    k->dnum = n_blocks;
+    k->degree = plan->degree;
+    const size_t N = plan->degree;
+    const auto &mods = plan->moduli;
+    // ModUp constants per block
+    std::vector<u64> consts;  // w tables of every block, then ModDown tables
+    size_t start = 0;
+    for (size_t j = 0; j < n_blocks; ++j) {
+        ModUpBlock mb{};
+        mb.start = (u32)start;
+        mb.size = (u32)block_sizes[j];
+        std::vector<u64> bm(mods.begin() + start, mods.begin() + start + block_sizes[j]);
+        fill_garner(bm, mb.pm, mb.inv, mb.inv_q, 0);
+        mb.w = nullptr;  // patched after upload
+        k->h_modup.push_back(mb);
+        k->block_start.push_back(start);
+        k->block_size.push_back(block_sizes[j]);
+        for (size_t i = 0; i < B; ++i) {
+            u64 prod = 1 % mods[i];
+            for (int m = 0; m < kMaxAlpha; ++m) {
+                consts.push_back(m < (int)bm.size() ? prod : 0);
+                if (m < (int)bm.size()) prod = host::mulmod(prod, bm[m] % mods[i], mods[i]);
+            }
+        }
+        start += block_sizes[j];
+    }
+    size_t md_w_off = consts.size();
+    ModDownTable md{};
+    if (hybrid) {
+        md.q_limbs = (u32)q_limbs;
+        md.p_limbs = (u32)K;
+        std::vector<u64> pmods(mods.begin() + q_limbs, mods.end());
+        u64 tmp_pm[kMaxAlpha][kMaxAlpha] = {};
+        u64 tmp_inv[kMaxAlpha] = {}, tmp_inv_q[kMaxAlpha] = {};
+        fill_garner(pmods, tmp_pm, tmp_inv, tmp_inv_q, 0);
+        for (size_t a = 0; a < K; ++a) {
+            for (size_t b = 0; b < K; ++b) md.pm[a][b] = tmp_pm[a][b];
+            md.inv[a] = tmp_inv[a];
+            md.inv_q[a] = tmp_inv_q[a];
+        }
+        // mixed-radix digits of floor(P/2): P odd, so floor(P/2) = (P-1)/2.
+        // Digits of P-1 are (p_0 - 1, p_1 - 1, ...) ... computed by halving
+        // the positional representation from the top digit down.
+        {
+            std::vector<u64> d(K);
+            for (size_t m = 0; m < K; ++m) d[m] = pmods[m] - 1;  // P - 1 = sum (p_m - 1) M_m
+            // divide by 2 in mixed radix: process from most significant digit
+            unsigned __int128 rem = 0;
+            for (size_t m = K; m-- > 0;) {
+                unsigned __int128 cur = rem * pmods[m] + d[m];
+                md.half[m] = (u64)(cur / 2);
+                rem = cur % 2;
+            }
+        }
+        for (size_t i = 0; i < q_limbs; ++i) {
+            u64 prod = 1 % mods[i];
+            for (int m = 0; m < kMaxSpecial; ++m) {
+                consts.push_back(m < (int)K ? prod : 0);
+                if (m < (int)K) prod = host::mulmod(prod, pmods[m] % mods[i], mods[i]);
+            }
+        }
+        // P mod q_i, P^-1 mod q_i
+        for (size_t i = 0; i < q_limbs; ++i) {
+            u64 p = 1 % mods[i];
+            for (size_t m = 0; m < K; ++m) p = host::mulmod(p, pmods[m] % mods[i], mods[i]);
+            consts.push_back(p);
+        }
+        for (size_t i = 0; i < q_limbs; ++i) {
+            u64 p = 1 % mods[i];
+            for (size_t m = 0; m < K; ++m) p = host::mulmod(p, pmods[m] % mods[i], mods[i]);
+            u64 inv = host::invmod(p, mods[i]);
+            consts.push_back(inv);
+        }
+        for (size_t i = 0; i < q_limbs; ++i)
+            consts.push_back(host::shoup_quotient(consts[md_w_off + q_limbs * kMaxSpecial + q_limbs + i],
+                                                  mods[i]));
+    }
+    CK_HIP(hipMalloc(&k->d_consts, std::max<size_t>(consts.size(), 1) * sizeof(u64)));
+    CK_HIP(hipMemcpy(k->d_consts, consts.data(), consts.size() * sizeof(u64), hipMemcpyHostToDevice));
+    for (size_t j = 0; j < n_blocks; ++j) k->h_modup[j].w = k->d_consts + j * B * kMaxAlpha;
+    CK_HIP(hipMalloc(&k->d_modup, n_blocks * sizeof(ModUpBlock)));
+    CK_HIP(hipMemcpy(k->d_modup, k->h_modup.data(), n_blocks * sizeof(ModUpBlock), hipMemcpyHostToDevice));
+    if (hybrid) {
+        md.w = k->d_consts + md_w_off;
+        md.pmod = md.w + q_limbs * kMaxSpecial;
+        md.pinv = md.pmod + q_limbs;
+        md.pinv_q = md.pinv + q_limbs;
+        CK_HIP(hipMalloc(&k->d_moddown, sizeof(ModDownTable)));
+        CK_HIP(hipMemcpy(k->d_moddown, &md, sizeof(ModDownTable), hipMemcpyHostToDevice));
+    }
+    // key material
+    const size_t kelems = n_blocks * B * N;
+    for (size_t e = 0; e < kelems; ++e) {
+        u64 q = mods[(e / N) % B];
+        if (key_b[e] >= q || key_a[e] >= q)
+            return fail(CKKS_ERR_INVALID, "key residues must be canonical");
+    }
+    CK_HIP(hipMalloc(&k->d_kb, kelems * sizeof(u64)));
+    CK_HIP(hipMalloc(&k->d_ka, kelems * sizeof(u64)));
+    CK_HIP(hipMemcpy(k->d_kb, key_b, kelems * sizeof(u64), hipMemcpyHostToDevice));
+    CK_HIP(hipMemcpy(k->d_ka, key_a, kelems * sizeof(u64), hipMemcpyHostToDevice));
+    ctx->keys.insert(k.get());
+    *out = k.release();
+    return CKKS_OK;
+}
+
+// ---- the pipeline --------------------------------------------------------------------
+
+// Switches the key of component `x` (coefficient domain, q_limbs limbs per
+// ct at x + c*x_stride) and writes
+//   out[c][0] = add0 + KS_b(x),  out[c][1] = add1 + KS_a(x)
+// (add0/add1 may be null) over the key's Q limbs.
+static int key_switch_core(const ckks_ksk *k, const u64 *x, u64 x_stride, const u64 *add0,
+                           const u64 *add1, u64 add_stride, u64 *out, u64 out_stride, size_t n_cts) {
+    ckks_context *ctx = k->ctx;
+    const ckks_rns_plan *plan = k->plan;
+    const u64 N = k->degree, B = k->limbs, D = k->dnum;
+    const u32 logn = plan->logn;
+    hipStream_t s = ctx->stream;
+    int rc;
+    void *wd, *wa;
+    if ((rc = ctx->workspace(4, n_cts * D * B * N * sizeof(u64), &wd))) return rc;
+    if ((rc = ctx->workspace(5, n_cts * 2 * B * N * sizeof(u64), &wa))) return rc;
+    u64 *digits = (u64 *)wd, *acc = (u64 *)wa;
+    const u32 chunks = (u32)(N / kKsThreads);
+    // 1. ModUp
+    ModUpArgs mu{};
+    mu.in = x;
+    mu.in_stride = x_stride;
+    mu.digits = digits;
+    mu.blocks = k->d_modup;
+    mu.primes = plan->d_primes;
+    mu.dnum = (u32)D;
+    mu.limbs = (u32)B;
+    mu.logn = logn;
+    mu.chunks = chunks;
+    hipLaunchKernelGGL(k_modup, dim3((u32)(n_cts * D * chunks)), dim3(kKsThreads), 0, s, mu);
+    CK_HIP(hipGetLastError());
+    // 2. forward NTT of every digit limb
+    NttCall f{plan, digits, digits, n_cts * D, B, B * N, B * N};
+    if ((rc = launch_ntt(ctx, &f, 1, true))) return rc;
+    // 3. MAC
+    MacArgs ma{};
+    ma.digits = digits;
+    ma.kb = k->d_kb;
+    ma.ka = k->d_ka;
+    ma.acc = acc;
+    ma.primes = plan->d_primes;
+    ma.n_cts = (u32)n_cts;
+    ma.dnum = (u32)D;
+    ma.limbs = (u32)B;
+    ma.logn = logn;
+    ma.chunks = chunks;
+    switch (D) {
+#define CK_MAC(DD) \
+    case DD: hipLaunchKernelGGL(k_ks_mac<DD>, dim3((u32)(B * chunks)), dim3(kKsThreads), 0, s, ma); break;
+        CK_MAC(1) CK_MAC(2) CK_MAC(3) CK_MAC(4) CK_MAC(5) CK_MAC(6) CK_MAC(7) CK_MAC(8)
+#undef CK_MAC
+        default:
+            hipLaunchKernelGGL(k_ks_mac_any, dim3((u32)(n_cts * B * chunks)), dim3(kKsThreads), 0, s, ma);
+    }
+    CK_HIP(hipGetLastError());
+    // 4. inverse NTT of both accumulators
+    NttCall iv{plan, acc, acc, n_cts * 2, B, B * N, B * N};
+    if ((rc = launch_ntt(ctx, &iv, 1, false))) return rc;
+    // 5. ModDown (or plain finalize) with pass-through adds
+    DownArgs da{};
+    da.acc = acc;
+    da.add0 = add0;
+    da.add1 = add1;
+    da.add_stride = add_stride;
+    da.out = out;
+    da.out_stride = out_stride;
+    da.md = k->d_moddown;
+    da.primes = plan->d_primes;
+    da.limbs = (u32)B;
+    da.q_limbs = (u32)k->q_limbs;
+    da.logn = logn;
+    da.chunks = chunks;
+    dim3 g((u32)(n_cts * 2 * chunks));
+    switch (k->p_limbs) {
+#define CK_MD(KK) \
+    case KK: hipLaunchKernelGGL(k_moddown<KK>, g, dim3(kKsThreads), 0, s, da); break;
+        CK_MD(0) CK_MD(1) CK_MD(2) CK_MD(3) CK_MD(4) CK_MD(5) CK_MD(6) CK_MD(7) CK_MD(8)
+#undef CK_MD
+        default: return fail(CKKS_ERR_UNSUPPORTED, "too many special primes");
+    }
+    CK_HIP(hipGetLastError());
+    return CKKS_OK;
+}
+
+static int check_ct_args(const ckks_ksk *k, const void *in, const void *out, size_t n_cts, size_t &in_stride,
+                         size_t &out_stride, size_t in_polys) {
+    if (!k) return fail(CKKS_ERR_INVALID, "null key");
+    if (n_cts && (!in || !out)) return fail(CKKS_ERR_INVALID, "null buffer");
+    const size_t poly = k->q_limbs * k->degree;
+    if (in_stride == 0) in_stride = in_polys * poly;
+    if (out_stride == 0) out_stride = 2 * poly;
+    if (in_stride < in_polys * poly || out_stride < 2 * poly)
+        return fail(CKKS_ERR_INVALID, "ciphertext stride smaller than its limbs");
+    return CKKS_OK;
+}
+
+}  // namespace ck
+
+using namespace ck;
+
+extern "C" {
+
+int ckks_rns_ksk_create(const ckks_rns_plan *plan, const size_t *block_sizes, size_t n_blocks,
+                        const uint64_t *key_b, const uint64_t *key_a, ckks_ksk **out) {
+    if (!plan) return fail(CKKS_ERR_INVALID, "null plan");
+    return create_key(plan, plan->moduli.size(), block_sizes, n_blocks, key_b, key_a, false, out);
+}
+
+int ckks_hybrid_ksk_create(const ckks_rns_plan *qp_plan, size_t q_limbs, const size_t *block_sizes,
+                           size_t n_blocks, const uint64_t *key_b, const uint64_t *key_a, ckks_ksk **out) {
+    return create_key(qp_plan, q_limbs, block_sizes, n_blocks, key_b, key_a, true, out);
+}
+
+int ckks_ksk_destroy(ckks_ksk *k) {
+    if (!k) return CKKS_OK;
+    hipSetDevice(k->ctx->device);
+    hipStreamSynchronize(k->ctx->stream);
+    k->ctx->keys.erase(k);
+    hipFree(k->d_kb);
+    hipFree(k->d_ka);
+    hipFree(k->d_modup);
+    hipFree(k->d_consts);
+    hipFree(k->d_moddown);
+    delete k;
+    return CKKS_OK;
+}
+
+int ckks_key_switch(const ckks_ksk *key, const uint64_t *in, uint64_t *out, size_t n_cts,
+                    size_t in_stride, size_t out_stride) {
+    int rc = check_ct_args(key, in, out, n_cts, in_stride, out_stride, 2);
+    if (rc || n_cts == 0) return rc;
+    const size_t poly = key->q_limbs * key->degree;
+    if (in == out) {
+        // the a component is consumed by ModUp before any output is written
+        // only if ModUp finishes first, which stream order guarantees; b is
+        // read by the last kernel while out is written element-by-element at
+        // the same positions, which is safe.
+    }
+    return key_switch_core(key, in + poly, in_stride, in, nullptr, in_stride, out, out_stride, n_cts);
+}
+
+int ckks_apply_galois(const ckks_ksk *key, uint64_t exponent, const uint64_t *in, uint64_t *out,
+                      size_t n_cts, size_t in_stride, size_t out_stride) {
+    int rc = check_ct_args(key, in, out, n_cts, in_stride, out_stride, 2);
+    if (rc || n_cts == 0) return rc;
+    const size_t poly = key->q_limbs * key->degree;
+    ckks_context *ctx = key->ctx;
+    void *w;
+    if ((rc = ctx->workspace(3, n_cts * 2 * poly * sizeof(u64), &w))) return rc;
+    u64 *rot = (u64 *)w;
+    // sigma_k on both components (apply_rns_automorphism, ckks/rns_automorphism.rs:12)
+    if (in_stride == 2 * poly) {
+        if ((rc = ckks_automorphism(key->plan, in, rot, 2 * n_cts, key->q_limbs, poly, exponent))) return rc;
+    } else {
+        for (size_t c = 0; c < n_cts; ++c)
+            if ((rc = ckks_automorphism(key->plan, in + c * in_stride, rot + c * 2 * poly, 2, key->q_limbs, poly,
+                                        exponent)))
+                return rc;
+    }
+    return key_switch_core(key, rot + poly, 2 * poly, rot, nullptr, 2 * poly, out, out_stride, n_cts);
+}
+
+int ckks_relinearize(const ckks_ksk *key, const uint64_t *in3, uint64_t *out, size_t n_cts,
+                     size_t in_stride, size_t out_stride) {
+    int rc = check_ct_args(key, in3, out, n_cts, in_stride, out_stride, 3);
+    if (rc || n_cts == 0) return rc;
+    const size_t poly = key->q_limbs * key->degree;
+    return key_switch_core(key, in3 + 2 * poly, in_stride, in3, in3 + poly, in_stride, out, out_stride, n_cts);
+}
+
+int ckks_multiply_relinearize_rescale(const ckks_ksk *key, const uint64_t *ct0, const uint64_t *ct1,
+                                      uint64_t *out, size_t n_cts, size_t in_stride, size_t out_stride) {
+    if (!key) return fail(CKKS_ERR_INVALID, "null key");
+    if (n_cts == 0) return CKKS_OK;
+    const size_t L = key->q_limbs, N = key->degree, poly = L * N;
+    if (L < 2) return fail(CKKS_ERR_INVALID, "RNS CKKS rescale requires at least two modulus limbs");
+    if (in_stride == 0) in_stride = 2 * poly;
+    if (out_stride == 0) out_stride = 2 * (L - 1) * N;
+    ckks_context *ctx = key->ctx;
+    int rc;
+    void *w6, *w7;
+    if ((rc = ctx->workspace(6, n_cts * 3 * poly * sizeof(u64), &w6))) return rc;
+    if ((rc = ctx->workspace(7, n_cts * 2 * poly * sizeof(u64), &w7))) return rc;
+    u64 *t3 = (u64 *)w6, *t2 = (u64 *)w7;
+    if ((rc = ckks_tensor(key->plan, ct0, ct1, t3, n_cts, L, in_stride, 3 * poly))) return rc;
+    if ((rc = key_switch_core(key, t3 + 2 * poly, 3 * poly, t3, t3 + poly, 3 * poly, t2, 2 * poly, n_cts)))
+        return rc;
+    return ckks_rescale(key->plan, t2, out, n_cts, L, 2 * poly, out_stride);
+}
+
+}  // extern "C"

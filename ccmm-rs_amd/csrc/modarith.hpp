@@ -1,0 +1,111 @@
+// Modular arithmetic on 64-bit residues for gfx950.
+//
+// CDNA has no 64x64->128 multiply instruction; the compiler lowers the
+// 64-bit products below to v_mad_u64_u32 / v_mul_hi_u32 sequences.  The
+// same source also compiles for the host (g++) so the kernels' index logic
+// and arithmetic can be exercised on the CPU.
+//
+// Residue ranges: every kernel boundary holds canonical residues in [0, q).
+// Inside the NTT, butterflies keep values lazily in [0, 4q), which needs
+// q < 2^62 (all plans enforce it).
+#pragma once
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define CK_HD __host__ __device__ __forceinline__
+#else
+#define CK_HD inline
+#endif
+
+namespace ck {
+
+typedef uint64_t u64;
+typedef uint32_t u32;
+
+CK_HD u64 mulhi64(u64 a, u64 b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umul64hi(a, b);
+#else
+    return (u64)(((unsigned __int128)a * b) >> 64);
+#endif
+}
+
+// Full 128-bit product as (hi, lo).
+CK_HD void mul128(u64 a, u64 b, u64 &hi, u64 &lo) {
+    lo = a * b;
+    hi = mulhi64(a, b);
+}
+
+// Harvey/Shoup multiplication by a fixed w < q with wq = floor(w 2^64 / q).
+// For any x < 2^64 the result lies in [0, 2q).
+CK_HD u64 shoup_lazy(u64 x, u64 w, u64 wq, u64 q) {
+    return x * w - mulhi64(x, wq) * q;
+}
+
+CK_HD u64 shoup(u64 x, u64 w, u64 wq, u64 q) {
+    u64 r = shoup_lazy(x, w, wq, q);
+    return r >= q ? r - q : r;
+}
+
+CK_HD u64 add_mod(u64 a, u64 b, u64 q) {
+    u64 s = a + b;
+    return s >= q ? s - q : s;
+}
+
+CK_HD u64 sub_mod(u64 a, u64 b, u64 q) {
+    return a >= b ? a - b : a + q - b;
+}
+
+CK_HD u64 neg_mod(u64 a, u64 q) { return a == 0 ? 0 : q - a; }
+
+// Barrett constant for 128-bit inputs: r = floor(2^128 / q) split in two words.
+struct Barrett {
+    u64 q;
+    u64 r_hi, r_lo;
+};
+
+// Reduce a 128-bit value (hi, lo) with hi < q (true for any sum of a few
+// products of canonical residues when q < 2^62) to [0, q).
+// Quotient estimate floor(x * r / 2^128) is short by at most 2.
+CK_HD u64 barrett_reduce128(u64 hi, u64 lo, const Barrett &b) {
+    // x * r = (hi 2^64 + lo)(r_hi 2^64 + r_lo); need bits >= 128.
+    u64 p0h = mulhi64(lo, b.r_lo);
+    u64 p1l = lo * b.r_hi, p1h = mulhi64(lo, b.r_hi);
+    u64 p2l = hi * b.r_lo, p2h = mulhi64(hi, b.r_lo);
+    u64 p3l = hi * b.r_hi;
+    // middle column sum with carries
+    u64 m = p0h + p1l;
+    u64 c = m < p0h;
+    u64 m2 = m + p2l;
+    c += m2 < m;
+    u64 quot = p3l + p1h + p2h + c;
+    u64 r = lo - quot * b.q;  // true remainder < 3q fits in 64 bits
+    if (r >= b.q) r -= b.q;
+    if (r >= b.q) r -= b.q;
+    return r;
+}
+
+CK_HD u64 mul_mod(u64 a, u64 b, const Barrett &br) {
+    u64 hi, lo;
+    mul128(a, b, hi, lo);
+    return barrett_reduce128(hi, lo, br);
+}
+
+// 128-bit accumulator for sums of products before one reduction.
+struct Acc128 {
+    u64 hi, lo;
+    CK_HD void zero() { hi = lo = 0; }
+    CK_HD void mac(u64 a, u64 b) {
+        u64 h, l;
+        mul128(a, b, h, l);
+        lo += l;
+        hi += h + (lo < l);
+    }
+    CK_HD void add(u64 v) {
+        lo += v;
+        hi += (lo < v);
+    }
+};
+
+}  // namespace ck

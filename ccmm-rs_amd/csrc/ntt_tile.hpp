@@ -1,0 +1,258 @@
+// Negacyclic NTT tiles for 64-bit residues, shared by the gfx950 kernels and
+// a CPU build of the same code (used by the tests to check every index).
+//
+// Transform (ring/ntt.rs:100-127): X_k = sum_j a_j psi^((2k+1) j), natural
+// order in and out.  Internally it is the Cooley-Tukey negacyclic network
+// with twiddle table T[k] = psi^bitrev_n(k) (k in [1, N)), whose output at
+// position p is X_{bitrev_n(p)}; the inverse is the Gentleman-Sande network
+// with Tinv[k] = psi^-bitrev_n(k) followed by N^-1.  Because psi_{N/2} =
+// psi_N^2, the table for N/2 is the first N/2 entries of the table for N,
+// so one table per prime serves every supported degree.
+//
+// Decomposition: N = N1 * N2 (n = n1 + n2).
+//   column pass: stages 0..n1-1 act on columns {c + N2 t : t < N1};
+//   row pass:    stages n1..n-1 act on rows    {m N2 + r : r < N2}.
+// Forward = column pass then row pass; inverse = row pass then column pass.
+// A tile is 4096 elements (COLS sub-transforms of R = 2^L elements), held
+// as 16 residues per thread in 256 threads; radix-2^k rounds of up to 4
+// stages run in registers and exchange through LDS between rounds.  For
+// N <= 4096 a single "full" pass holds the whole transform in one tile.
+//
+// Twiddle index for a butterfly at local stage s of a sub-transform whose
+// element index is t (lower element of the pair):
+//   forward: (base << s) + (t >> (L - s))
+//   inverse: (base << (L - 1 - u)) + (t >> (u + 1))   (u = local GS stage)
+// with base = 1 for column / full passes and base = 2^n1 + m for row m.
+#pragma once
+#include "modarith.hpp"
+
+// Host builds may define CK_CHECK_BOUNDS to trap any out-of-range index of
+// the kernel code (the device build compiles the checks out).
+#if defined(CK_CHECK_BOUNDS) && !defined(__HIP_DEVICE_COMPILE__)
+#include <cstdio>
+#include <cstdlib>
+#define CK_BOUNDS(x, n)                                                        \
+    do {                                                                       \
+        if (!((x) < (n))) {                                                    \
+            std::fprintf(stderr, "index %u out of range %u at %s:%d\n",        \
+                         (unsigned)(x), (unsigned)(n), __FILE__, __LINE__);    \
+            std::abort();                                                      \
+        }                                                                      \
+    } while (0)
+#else
+#define CK_BOUNDS(x, n) ((void)0)
+#endif
+
+namespace ck {
+
+enum PassKind { PASS_COL = 0, PASS_ROW = 1, PASS_FULL = 2 };
+
+struct Tw {
+    u64 w, wq;
+};
+
+// Per-launch view of one limb transform, resolved by the kernel.
+struct TileJob {
+    const u64 *src;  // input residues of this limb transform
+    u64 *dst;        // output residues
+    const Tw *tw;    // forward or inverse twiddle table (index 1..N-1)
+    u64 q;
+    u64 ninv, ninv_q;  // N^-1 and its Shoup quotient (inverse only)
+    u32 logn;
+    u32 tile;  // tile index within the limb transform
+};
+
+CK_HD u32 bitrev(u32 x, u32 bits) {
+    u32 r = 0;
+    for (u32 i = 0; i < bits; ++i) r |= ((x >> i) & 1u) << (bits - 1 - i);
+    return r;
+}
+
+// ---- compile-time round geometry ------------------------------------------
+
+// Round sizes: split L stages into ceil(L/4) rounds as evenly as possible.
+constexpr int n_rounds(int L) { return (L + 3) / 4; }
+constexpr int round_k(int L, int i) {
+    // first (L % nr) rounds get one extra stage
+    return L / n_rounds(L) + (i < L % n_rounds(L) ? 1 : 0);
+}
+constexpr int round_start(int L, int i) {
+    int s = 0;
+    for (int j = 0; j < i; ++j) s += round_k(L, j);
+    return s;
+}
+
+// Bit layout of a tile coordinate e = t | (col << L), TB = L + CL bits
+// (TB = 12 for two-pass tiles).  A round keeps 4 "register" bits per thread
+// (the k butterfly bits plus 4-k extra bits) and maps the other TB-4 bits to
+// the thread id in a preference order that decides coalescing:
+//   col-first: lanes walk columns (contiguous in HBM for column passes and
+//              for the natural-order side of row passes);
+//   t-first:   lanes walk t (contiguous for the internal side of row passes).
+struct Geom {
+    int reg_pos[4];
+    int thr_pos[12];
+    int n_thr;
+};
+
+constexpr Geom make_geom(int L, int CL, int bfly_lo, int k, bool col_first) {
+    Geom g{};
+    int TB = L + CL;
+    // preference order of free bits
+    int order[16] = {};
+    int no = 0;
+    if (col_first) {
+        for (int b = L; b < TB; ++b) order[no++] = b;
+        for (int b = 0; b < L; ++b)
+            if (b < bfly_lo || b >= bfly_lo + k) order[no++] = b;
+    } else {
+        for (int b = 0; b < L; ++b)
+            if (b < bfly_lo || b >= bfly_lo + k) order[no++] = b;
+        for (int b = L; b < TB; ++b) order[no++] = b;
+    }
+    for (int b = 0; b < k; ++b) g.reg_pos[b] = bfly_lo + b;
+    // extra register bits: the least preferred free bits
+    int extra = 4 - k;
+    for (int i = 0; i < extra; ++i) g.reg_pos[k + i] = order[no - extra + i];
+    g.n_thr = no - extra;
+    for (int i = 0; i < g.n_thr; ++i) g.thr_pos[i] = order[i];
+    return g;
+}
+
+template <int L, int CL>
+struct TileShape {
+    static constexpr int TB = L + CL;
+    static constexpr int ELEMS = 1 << TB;
+    static constexpr int THREADS = ELEMS / 16;
+    static constexpr int COLS = 1 << CL;
+    static constexpr int R = 1 << L;
+};
+
+// Coordinate of register v for thread tid under geometry g.
+CK_HD u32 coord(const Geom &g, u32 tid, int v) {
+    u32 e = 0;
+    for (int i = 0; i < g.n_thr; ++i) e |= ((tid >> i) & 1u) << g.thr_pos[i];
+    for (int b = 0; b < 4; ++b) e |= (u32)((v >> b) & 1) << g.reg_pos[b];
+    return e;
+}
+
+// LDS placement with an XOR swizzle so both lane walks are conflict-free.
+template <int L, int CL>
+CK_HD u32 lds_index(u32 e) {
+    CK_BOUNDS(e, 1u << (L + CL));
+    u32 t = e & ((1u << L) - 1), col = e >> L;
+    if (CL == 0) {
+        // single sub-transform: swizzle the low 4 bits by bits 4..7
+        return t ^ ((t >> 4) & 15u);
+    }
+    u32 mask = (1u << CL) - 1;
+    return (t << CL) + (col ^ (t & mask));
+}
+
+// ---- butterflies ------------------------------------------------------------
+
+CK_HD void ct_bfly(u64 &x, u64 &y, const Tw &w, u64 q) {
+    u64 two_q = q << 1;
+    u64 a = x >= two_q ? x - two_q : x;
+    u64 t = shoup_lazy(y, w.w, w.wq, q);
+    x = a + t;
+    y = a - t + two_q;
+}
+
+CK_HD void gs_bfly(u64 &x, u64 &y, const Tw &w, u64 q) {
+    u64 two_q = q << 1;
+    u64 s = x + y;
+    s = s >= two_q ? s - two_q : s;
+    u64 d = x - y + two_q;
+    x = s;
+    y = shoup_lazy(d, w.w, w.wq, q);
+}
+
+// ---- per-pass element addressing ----------------------------------------------
+
+// Global element offsets (within the limb) for tile coordinate e.
+// Column pass: t indexes rows of a column, col indexes columns.
+// Row pass: col indexes rows u (natural side), t = r.
+struct PassAddr {
+    u32 logn, n1, n2;
+    u32 tile;
+};
+
+template <int L, int CL>
+CK_HD u32 col_pass_offset(const PassAddr &pa, u32 e) {
+    u32 t = e & ((1u << L) - 1), col = e >> L;
+    u32 c = pa.tile * (1u << CL) + col;
+    return c + (t << pa.n2);
+}
+
+// Row pass, internal side: position m * N2 + r where m = bitrev_n1(u).
+template <int L, int CL>
+CK_HD u32 row_internal_offset(const PassAddr &pa, u32 e) {
+    u32 r = e & ((1u << L) - 1), col = e >> L;
+    u32 u = pa.tile * (1u << CL) + col;
+    u32 m = bitrev(u, pa.n1);
+    return (m << pa.n2) + r;
+}
+
+// Row pass, natural side: bitrev_n2(r) * N1 + u.
+template <int L, int CL>
+CK_HD u32 row_natural_offset(const PassAddr &pa, u32 e) {
+    u32 r = e & ((1u << L) - 1), col = e >> L;
+    u32 u = pa.tile * (1u << CL) + col;
+    return (bitrev(r, pa.n2) << pa.n1) + u;
+}
+
+template <int L, int CL>
+CK_HD u32 row_twiddle_base(const PassAddr &pa, u32 e) {
+    u32 col = e >> L;
+    u32 u = pa.tile * (1u << CL) + col;
+    return (1u << pa.n1) + bitrev(u, pa.n1);
+}
+
+// ---- one round of butterflies on the 16 registers -------------------------------
+
+// Forward round: stages s0..s0+k-1 (butterfly bit of stage s is L-1-s).
+template <int L, int CL, int K>
+CK_HD void fwd_round(u64 (&v)[16], const Geom &g, u32 tid, int s0, u32 base_col_const,
+                     bool row_base, const PassAddr &pa, const Tw *tw, u64 q) {
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        const int s = s0 + j;
+        const int rb = K - 1 - j;  // register bit of this stage
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            if (i & (1 << rb)) continue;
+            u32 e = coord(g, tid, i);
+            u32 t = e & ((1u << L) - 1);
+            u32 base = row_base ? row_twiddle_base<L, CL>(pa, e) : base_col_const;
+            u32 idx = (base << s) + (t >> (L - s));
+            CK_BOUNDS(idx, 1u << pa.logn);
+            Tw w = tw[idx];
+            ct_bfly(v[i], v[i | (1 << rb)], w, q);
+        }
+    }
+}
+
+// Inverse round: GS stages u0..u0+k-1 (butterfly bit of stage u is u).
+template <int L, int CL, int K>
+CK_HD void inv_round(u64 (&v)[16], const Geom &g, u32 tid, int u0, u32 base_col_const,
+                     bool row_base, const PassAddr &pa, const Tw *tw, u64 q) {
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        const int u = u0 + j;
+        const int rb = j;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            if (i & (1 << rb)) continue;
+            u32 e = coord(g, tid, i);
+            u32 t = e & ((1u << L) - 1);
+            u32 base = row_base ? row_twiddle_base<L, CL>(pa, e) : base_col_const;
+            u32 idx = (base << (L - 1 - u)) + (t >> (u + 1));
+            CK_BOUNDS(idx, 1u << pa.logn);
+            Tw w = tw[idx];
+            gs_bfly(v[i], v[i | (1 << rb)], w, q);
+        }
+    }
+}
+
+}  // namespace ck

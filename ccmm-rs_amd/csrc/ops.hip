@@ -1,0 +1,366 @@
+// Element-wise ring kernels (gfx950): pointwise arithmetic, Galois
+// automorphisms in both domains, exact rescale, mod switch and the tensor.
+// All are HBM-bound streams; each thread owns 2 consecutive residues of one
+// limb so loads/stores are 16-byte and a block never straddles limbs.
+#include "host_math.hpp"
+#include "internal.hpp"
+
+namespace ck {
+
+constexpr int kEwThreads = 256;
+constexpr int kEwPerThread = 2;
+constexpr int kEwChunk = kEwThreads * kEwPerThread;  // residues per block
+
+enum EwOp { EW_MUL = 0, EW_MAC = 1, EW_ADD = 2, EW_SUB = 3, EW_NEG = 4 };
+
+struct EwArgs {
+    const u64 *x, *y;
+    u64 *out;
+    const DevPrime *primes;
+    u64 poly_stride;
+    u32 n_limbs, logn, chunks;  // chunks per limb
+};
+
+template <int OP>
+__global__ __launch_bounds__(kEwThreads) void k_elementwise(EwArgs a) {
+    const u32 b = blockIdx.x;
+    const u32 chunk = b % a.chunks;
+    const u32 pl = b / a.chunks;
+    const u32 limb = pl % a.n_limbs, poly = pl / a.n_limbs;
+    const DevPrime P = a.primes[limb];
+    Barrett br{P.q, P.br_hi, P.br_lo};
+    const size_t base = poly * a.poly_stride + ((size_t)limb << a.logn) +
+                        (size_t)chunk * kEwChunk + threadIdx.x * kEwPerThread;
+    const u64 q = P.q;
+#pragma unroll
+    for (int i = 0; i < kEwPerThread; ++i) {
+        u64 x = a.x[base + i];
+        u64 r;
+        if (OP == EW_NEG) {
+            r = neg_mod(x, q);
+        } else {
+            u64 y = a.y[base + i];
+            if (OP == EW_MUL) r = mul_mod(x, y, br);
+            else if (OP == EW_ADD) r = add_mod(x, y, q);
+            else if (OP == EW_SUB) r = sub_mod(x, y, q);
+            else r = add_mod(a.out[base + i], mul_mod(x, y, br), q);  // EW_MAC
+        }
+        a.out[base + i] = r;
+    }
+}
+
+static int elementwise(int op, const ckks_rns_plan *plan, const u64 *x, const u64 *y, u64 *out,
+                       size_t n_polys, size_t n_limbs, size_t poly_stride) {
+    int rc = check_plan_limbs(plan, n_limbs, "pointwise");
+    if (rc) return rc;
+    if (n_polys == 0) return CKKS_OK;
+    if (!x || !out || (op != EW_NEG && !y)) return fail(CKKS_ERR_INVALID, "null buffer");
+    if (n_polys > 1 && poly_stride < n_limbs * plan->degree)
+        return fail(CKKS_ERR_INVALID, "polynomial stride smaller than its limbs");
+    EwArgs a{};
+    a.x = x;
+    a.y = y;
+    a.out = out;
+    a.primes = plan->d_primes;
+    a.poly_stride = n_polys > 1 ? poly_stride : n_limbs * plan->degree;
+    a.n_limbs = (u32)n_limbs;
+    a.logn = plan->logn;
+    a.chunks = (u32)(plan->degree / kEwChunk);
+    dim3 grid((u32)(n_polys * n_limbs * a.chunks));
+    hipStream_t s = plan->ctx->stream;
+    switch (op) {
+        case EW_MUL: hipLaunchKernelGGL(k_elementwise<EW_MUL>, grid, dim3(kEwThreads), 0, s, a); break;
+        case EW_MAC: hipLaunchKernelGGL(k_elementwise<EW_MAC>, grid, dim3(kEwThreads), 0, s, a); break;
+        case EW_ADD: hipLaunchKernelGGL(k_elementwise<EW_ADD>, grid, dim3(kEwThreads), 0, s, a); break;
+        case EW_SUB: hipLaunchKernelGGL(k_elementwise<EW_SUB>, grid, dim3(kEwThreads), 0, s, a); break;
+        default: hipLaunchKernelGGL(k_elementwise<EW_NEG>, grid, dim3(kEwThreads), 0, s, a); break;
+    }
+    CK_HIP(hipGetLastError());
+    return CKKS_OK;
+}
+
+// ---- Galois automorphisms ------------------------------------------------------
+
+struct PermArgs {
+    const u64 *in;
+    u64 *out;
+    const DevPrime *primes;
+    u64 poly_stride;
+    u64 k;  // coefficient domain: k^-1 mod 2N; NTT domain: k mod 2N
+    u32 n_limbs, logn, chunks;
+};
+
+// Coefficient domain (ckks/automorphism.rs:8-46) as a gather: output j takes
+// source i' = j k^-1 mod 2N; i' < N gives +in[i'], otherwise -in[i' - N].
+__global__ __launch_bounds__(kEwThreads) void k_automorphism(PermArgs a) {
+    const u32 b = blockIdx.x;
+    const u32 chunk = b % a.chunks;
+    const u32 pl = b / a.chunks;
+    const u32 limb = pl % a.n_limbs, poly = pl / a.n_limbs;
+    const u64 q = a.primes[limb].q;
+    const u64 n = (u64)1 << a.logn, mask2 = 2 * n - 1;
+    const size_t base = poly * a.poly_stride + ((size_t)limb << a.logn);
+#pragma unroll
+    for (int i = 0; i < kEwPerThread; ++i) {
+        u64 j = (u64)chunk * kEwChunk + threadIdx.x * kEwPerThread + i;
+        u64 src = (j * a.k) & mask2;
+        u64 v = src < n ? a.in[base + src] : neg_mod(a.in[base + src - n], q);
+        a.out[base + j] = v;
+    }
+}
+
+// NTT domain (ring/ntt_galois_permutation.rs:5-19):
+// out[j] = in[((k (2j+1) mod 2N) - 1) / 2].
+__global__ __launch_bounds__(kEwThreads) void k_ntt_galois(PermArgs a) {
+    const u32 b = blockIdx.x;
+    const u32 chunk = b % a.chunks;
+    const u32 pl = b / a.chunks;
+    const u32 limb = pl % a.n_limbs, poly = pl / a.n_limbs;
+    const u64 n = (u64)1 << a.logn, mask2 = 2 * n - 1;
+    const size_t base = poly * a.poly_stride + ((size_t)limb << a.logn);
+#pragma unroll
+    for (int i = 0; i < kEwPerThread; ++i) {
+        u64 j = (u64)chunk * kEwChunk + threadIdx.x * kEwPerThread + i;
+        u64 rot = (a.k * (2 * j + 1)) & mask2;
+        a.out[base + j] = a.in[base + ((rot - 1) >> 1)];
+    }
+}
+
+static int permute(bool ntt_domain, const ckks_rns_plan *plan, const u64 *in, u64 *out,
+                   size_t n_polys, size_t n_limbs, size_t poly_stride, u64 exponent) {
+    int rc = check_plan_limbs(plan, n_limbs, "automorphism");
+    if (rc) return rc;
+    if (n_polys == 0) return CKKS_OK;
+    if (!in || !out) return fail(CKKS_ERR_INVALID, "null buffer");
+    if (in == out) return fail(CKKS_ERR_INVALID, "automorphism output must not alias its input");
+    const u64 two_n = 2 * (u64)plan->degree;
+    u64 k = exponent % two_n;
+    if ((k & 1) == 0)
+        return fail(CKKS_ERR_INVALID, ntt_domain ? "Galois exponent must be odd"
+                                                 : "CKKS automorphism exponent must be odd modulo 2N");
+    if (n_polys > 1 && poly_stride < n_limbs * plan->degree)
+        return fail(CKKS_ERR_INVALID, "polynomial stride smaller than its limbs");
+    PermArgs a{};
+    a.in = in;
+    a.out = out;
+    a.primes = plan->d_primes;
+    a.poly_stride = n_polys > 1 ? poly_stride : n_limbs * plan->degree;
+    if (ntt_domain) {
+        a.k = k;
+    } else {
+        // inverse of k modulo 2N (odd k, power-of-two modulus): Newton
+        u64 inv = k;
+        for (int it = 0; it < 6; ++it) inv = inv * (2 - k * inv);
+        a.k = inv & (two_n - 1);
+    }
+    a.n_limbs = (u32)n_limbs;
+    a.logn = plan->logn;
+    a.chunks = (u32)(plan->degree / kEwChunk);
+    dim3 grid((u32)(n_polys * n_limbs * a.chunks));
+    if (ntt_domain)
+        hipLaunchKernelGGL(k_ntt_galois, grid, dim3(kEwThreads), 0, plan->ctx->stream, a);
+    else
+        hipLaunchKernelGGL(k_automorphism, grid, dim3(kEwThreads), 0, plan->ctx->stream, a);
+    CK_HIP(hipGetLastError());
+    return CKKS_OK;
+}
+
+// ---- rescale / mod switch ------------------------------------------------------
+
+struct RescaleArgs {
+    const u64 *in;
+    u64 *out;
+    const DevPrime *primes;
+    const RescaleConst *rc;  // row of the dropped limb: rc[i] for target i
+    u64 in_stride, out_stride;
+    u32 n_limbs, logn;
+    u32 rescale;  // 0: mod switch (drop only)
+};
+
+// One thread per coefficient of one ciphertext component; loops over the
+// kept limbs so the dropped limb is read once.
+__global__ __launch_bounds__(kEwThreads) void k_rescale(RescaleArgs a) {
+    const u64 n = (u64)1 << a.logn;
+    const u64 chunks = n / kEwThreads;
+    const u64 b = blockIdx.x;
+    const u64 c = (b % chunks) * kEwThreads + threadIdx.x;
+    const u64 comp = (b / chunks) % 2, ct = (b / chunks) / 2;
+    const u32 L = a.n_limbs, last = L - 1;
+    const u64 *src = a.in + ct * a.in_stride + comp * (u64)L * n;
+    u64 *dst = a.out + ct * a.out_stride + comp * (u64)last * n;
+    if (!a.rescale) {
+        for (u32 i = 0; i < last; ++i) dst[(u64)i * n + c] = src[(u64)i * n + c];
+        return;
+    }
+    const u64 q_drop = a.primes[last].q;
+    const u64 rem = src[(u64)last * n + c];
+    const bool neg = rem > q_drop / 2;
+    for (u32 i = 0; i < last; ++i) {
+        const DevPrime &P = a.primes[i];
+        const RescaleConst k = a.rc[i];
+        u64 rq = rem >= P.q ? barrett_reduce128(0, rem, Barrett{P.q, P.br_hi, P.br_lo}) : rem;
+        if (neg) rq = sub_mod(rq, k.ql_mod, P.q);  // (rem - q_drop) mod q_i
+        u64 diff = sub_mod(src[(u64)i * n + c], rq, P.q);
+        dst[(u64)i * n + c] = shoup(diff, k.inv, k.inv_q, P.q);
+    }
+}
+
+static int rescale_like(bool rescale, const ckks_rns_plan *plan, const u64 *in, u64 *out,
+                        size_t n_cts, size_t n_limbs, size_t in_stride, size_t out_stride) {
+    int rc = check_plan_limbs(plan, n_limbs, "rescale");
+    if (rc) return rc;
+    if (n_limbs < 2) return fail(CKKS_ERR_INVALID, "RNS CKKS rescale requires at least two modulus limbs");
+    if (n_cts == 0) return CKKS_OK;
+    if (!in || !out) return fail(CKKS_ERR_INVALID, "null buffer");
+    const size_t n = plan->degree;
+    if (in_stride == 0) in_stride = 2 * n_limbs * n;
+    if (out_stride == 0) out_stride = 2 * (n_limbs - 1) * n;
+    if (in_stride < 2 * n_limbs * n || out_stride < 2 * (n_limbs - 1) * n)
+        return fail(CKKS_ERR_INVALID, "ciphertext stride smaller than its limbs");
+    RescaleArgs a{};
+    a.in = in;
+    a.out = out;
+    a.primes = plan->d_primes;
+    a.rc = plan->d_rescale + (n_limbs - 1) * plan->moduli.size();
+    a.in_stride = in_stride;
+    a.out_stride = out_stride;
+    a.n_limbs = (u32)n_limbs;
+    a.logn = plan->logn;
+    a.rescale = rescale ? 1u : 0u;
+    dim3 grid((u32)(n_cts * 2 * (n / kEwThreads)));
+    hipLaunchKernelGGL(k_rescale, grid, dim3(kEwThreads), 0, plan->ctx->stream, a);
+    CK_HIP(hipGetLastError());
+    return CKKS_OK;
+}
+
+// ---- tensor --------------------------------------------------------------------
+
+// NTT-domain tensor of (b0,a0) x (b1,a1): c0 = b0 b1, c1 = b0 a1 + a0 b1,
+// c2 = a0 a1, each reduced once from a 128-bit sum.
+struct TensorArgs {
+    const u64 *x;  // [ct][b0,a0,b1,a1][limb][N]
+    u64 *y;        // [ct][c0,c1,c2][limb][N]
+    const DevPrime *primes;
+    u32 n_limbs, logn, chunks;
+};
+
+__global__ __launch_bounds__(kEwThreads) void k_tensor(TensorArgs a) {
+    const u32 b = blockIdx.x;
+    const u32 chunk = b % a.chunks;
+    const u32 pl = b / a.chunks;
+    const u32 limb = pl % a.n_limbs, ct = pl / a.n_limbs;
+    const DevPrime P = a.primes[limb];
+    Barrett br{P.q, P.br_hi, P.br_lo};
+    const u64 n = (u64)1 << a.logn;
+    const u64 poly = (u64)a.n_limbs * n;
+    const u64 off = (u64)limb * n + (u64)chunk * kEwChunk + threadIdx.x * kEwPerThread;
+    const u64 *x = a.x + ct * 4 * poly + off;
+    u64 *y = a.y + ct * 3 * poly + off;
+#pragma unroll
+    for (int i = 0; i < kEwPerThread; ++i) {
+        u64 b0 = x[i], a0 = x[poly + i], b1 = x[2 * poly + i], a1 = x[3 * poly + i];
+        y[i] = mul_mod(b0, b1, br);
+        Acc128 acc;
+        acc.zero();
+        acc.mac(b0, a1);
+        acc.mac(a0, b1);
+        y[poly + i] = barrett_reduce128(acc.hi, acc.lo, br);
+        y[2 * poly + i] = mul_mod(a0, a1, br);
+    }
+}
+
+static int tensor(const ckks_rns_plan *plan, const u64 *ct0, const u64 *ct1, u64 *out, size_t n_cts,
+                  size_t n_limbs, size_t in_stride, size_t out_stride) {
+    int rc = check_plan_limbs(plan, n_limbs, "tensor");
+    if (rc) return rc;
+    if (n_cts == 0) return CKKS_OK;
+    if (!ct0 || !ct1 || !out) return fail(CKKS_ERR_INVALID, "null buffer");
+    const size_t n = plan->degree, poly = n_limbs * n;
+    if (in_stride == 0) in_stride = 2 * poly;
+    if (out_stride == 0) out_stride = 3 * poly;
+    if (in_stride < 2 * poly || out_stride < 3 * poly)
+        return fail(CKKS_ERR_INVALID, "ciphertext stride smaller than its limbs");
+    ckks_context *ctx = plan->ctx;
+    void *w1, *w2;
+    if ((rc = ctx->workspace(1, n_cts * 4 * poly * sizeof(u64), &w1))) return rc;
+    if ((rc = ctx->workspace(2, n_cts * 3 * poly * sizeof(u64), &w2))) return rc;
+    u64 *x = (u64 *)w1, *y = (u64 *)w2;
+    // x = [ct][b0, a0, b1, a1][limb][N] in the NTT domain
+    NttCall fwd[4] = {
+        {plan, ct0, x, n_cts, n_limbs, in_stride, 4 * poly},
+        {plan, ct0 + poly, x + poly, n_cts, n_limbs, in_stride, 4 * poly},
+        {plan, ct1, x + 2 * poly, n_cts, n_limbs, in_stride, 4 * poly},
+        {plan, ct1 + poly, x + 3 * poly, n_cts, n_limbs, in_stride, 4 * poly},
+    };
+    if ((rc = launch_ntt(ctx, fwd, 4, true))) return rc;
+    TensorArgs a{};
+    a.x = x;
+    a.y = y;
+    a.primes = plan->d_primes;
+    a.n_limbs = (u32)n_limbs;
+    a.logn = plan->logn;
+    a.chunks = (u32)(n / kEwChunk);
+    dim3 grid((u32)(n_cts * n_limbs * a.chunks));
+    hipLaunchKernelGGL(k_tensor, grid, dim3(kEwThreads), 0, ctx->stream, a);
+    CK_HIP(hipGetLastError());
+    NttCall inv[3] = {
+        {plan, y, out, n_cts, n_limbs, 3 * poly, out_stride},
+        {plan, y + poly, out + poly, n_cts, n_limbs, 3 * poly, out_stride},
+        {plan, y + 2 * poly, out + 2 * poly, n_cts, n_limbs, 3 * poly, out_stride},
+    };
+    return launch_ntt(ctx, inv, 3, false);
+}
+
+}  // namespace ck
+
+using namespace ck;
+
+extern "C" {
+
+int ckks_poly_mul(const ckks_rns_plan *plan, const uint64_t *x, const uint64_t *y, uint64_t *out,
+                  size_t n_polys, size_t n_limbs, size_t poly_stride) {
+    return elementwise(EW_MUL, plan, x, y, out, n_polys, n_limbs, poly_stride);
+}
+int ckks_poly_mac(const ckks_rns_plan *plan, uint64_t *acc, const uint64_t *x, const uint64_t *y,
+                  size_t n_polys, size_t n_limbs, size_t poly_stride) {
+    return elementwise(EW_MAC, plan, x, y, acc, n_polys, n_limbs, poly_stride);
+}
+int ckks_poly_add(const ckks_rns_plan *plan, const uint64_t *x, const uint64_t *y, uint64_t *out,
+                  size_t n_polys, size_t n_limbs, size_t poly_stride) {
+    return elementwise(EW_ADD, plan, x, y, out, n_polys, n_limbs, poly_stride);
+}
+int ckks_poly_sub(const ckks_rns_plan *plan, const uint64_t *x, const uint64_t *y, uint64_t *out,
+                  size_t n_polys, size_t n_limbs, size_t poly_stride) {
+    return elementwise(EW_SUB, plan, x, y, out, n_polys, n_limbs, poly_stride);
+}
+int ckks_poly_neg(const ckks_rns_plan *plan, const uint64_t *x, uint64_t *out, size_t n_polys,
+                  size_t n_limbs, size_t poly_stride) {
+    return elementwise(EW_NEG, plan, x, nullptr, out, n_polys, n_limbs, poly_stride);
+}
+
+int ckks_automorphism(const ckks_rns_plan *plan, const uint64_t *in, uint64_t *out, size_t n_polys,
+                      size_t n_limbs, size_t poly_stride, uint64_t exponent) {
+    return permute(false, plan, in, out, n_polys, n_limbs, poly_stride, exponent);
+}
+
+int ckks_ntt_galois_permute(const ckks_rns_plan *plan, const uint64_t *in, uint64_t *out,
+                            size_t n_polys, size_t n_limbs, size_t poly_stride, uint64_t exponent) {
+    return permute(true, plan, in, out, n_polys, n_limbs, poly_stride, exponent);
+}
+
+int ckks_rescale(const ckks_rns_plan *plan, const uint64_t *in, uint64_t *out, size_t n_cts,
+                 size_t n_limbs, size_t in_stride, size_t out_stride) {
+    return rescale_like(true, plan, in, out, n_cts, n_limbs, in_stride, out_stride);
+}
+
+int ckks_mod_switch(const ckks_rns_plan *plan, const uint64_t *in, uint64_t *out, size_t n_cts,
+                    size_t n_limbs, size_t in_stride, size_t out_stride) {
+    return rescale_like(false, plan, in, out, n_cts, n_limbs, in_stride, out_stride);
+}
+
+int ckks_tensor(const ckks_rns_plan *plan, const uint64_t *ct0, const uint64_t *ct1, uint64_t *out,
+                size_t n_cts, size_t n_limbs, size_t in_stride, size_t out_stride) {
+    return tensor(plan, ct0, ct1, out, n_cts, n_limbs, in_stride, out_stride);
+}
+
+}  // extern "C"

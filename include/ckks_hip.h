@@ -1,0 +1,208 @@
+/*
+ * ckks_hip.h -- C ABI of the MI355X (gfx950) backend for the leveled
+ * RNS-CKKS evaluation path of ccmm-rs ("FHE-rs").
+ *
+ * The reference crate is pure Rust with no FFI; every entry point below
+ * replaces one Rust function of the hot path (cited as file:line into the
+ * reference crate) and computes bit-identical residues.  A Rust binding
+ * (`ccmm-rs-hip-sys`, a separate crate because the reference is
+ * #![forbid(unsafe_code)]) maps the status codes back to the reference's
+ * panic messages, which ckks_last_error() reproduces.
+ *
+ * Conventions
+ *   - residues are canonical uint64_t in [0, q); every prime q < 2^62;
+ *   - device buffers are plain device pointers (hipMalloc'd or any other
+ *     device allocation); the caller owns host and device memory passed in,
+ *     the library owns only its plans, keys and workspace;
+ *   - polynomial batches are limb-major: polynomial p, limb i occupies
+ *     [p * poly_stride + i * N, ... + N) in elements;
+ *   - a ciphertext is (b, a) decrypting as b + a*s, stored as
+ *     [b limbs][a limbs]: component c, limb i at (c * n_limbs + i) * N
+ *     within the ciphertext, ciphertexts ct_stride elements apart;
+ *   - NTT-domain index k holds the evaluation at psi^(2k+1) (natural order,
+ *     ring/ntt.rs:100-127);
+ *   - calls are asynchronous on the context's stream; ckks_sync() waits;
+ *   - a context is single-stream and thread-compatible: use one context per
+ *     host thread.  Functions return CKKS_OK or an error code and never
+ *     unwind across the ABI.
+ */
+#ifndef CKKS_HIP_H
+#define CKKS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum ckks_status {
+    CKKS_OK = 0,
+    CKKS_ERR_INVALID = 1,     /* argument violates a reference assertion      */
+    CKKS_ERR_HIP = 2,         /* HIP runtime error                            */
+    CKKS_ERR_OOM = 3,         /* device allocation failed / budget exceeded   */
+    CKKS_ERR_UNSUPPORTED = 4, /* outside the GPU path (e.g. q >= 2^62)        */
+    CKKS_ERR_BUSY = 5         /* serving queue full                           */
+};
+
+/* Message of the last failed call on this host thread.  Where the reference
+ * panics with a fixed message the same text is used. */
+const char *ckks_last_error(void);
+
+/* Library and device identification. */
+const char *ckks_version(void);
+
+/* ---------------------------------------------------------------- context */
+typedef struct ckks_context ckks_context;
+
+/* stream: the hipStream_t to run on; NULL means the device's null
+ * (default) stream, the one torch uses by default.  The context owns every
+ * plan and key created from it: ckks_context_destroy releases those still
+ * alive, after which their handles must not be used. */
+int ckks_context_create(int device, void *stream, ckks_context **out);
+int ckks_context_destroy(ckks_context *ctx);
+int ckks_context_set_stream(ckks_context *ctx, void *stream);
+int ckks_sync(ckks_context *ctx);
+
+/* Device memory helpers for callers without their own allocator. */
+int ckks_malloc(ckks_context *ctx, size_t bytes, void **dev);
+int ckks_free(ckks_context *ctx, void *dev);
+int ckks_upload(ckks_context *ctx, void *dev, const void *host, size_t bytes);
+int ckks_download(ckks_context *ctx, void *host, const void *dev, size_t bytes);
+
+/* ---------------------------------------------------------------- ring */
+typedef struct ckks_rns_plan ckks_rns_plan;
+
+/* find_ntt_prime_below_bits (ring/ntt_prime.rs:67-94) continued downward:
+ * the first `count` primes q = k*2N + 1 below 2^bits (bits <= 62), skipping
+ * the n_skip values in skip.  Writes them to out and returns how many were
+ * found through *found.  Host-only; needs no context. */
+int ckks_find_ntt_primes(unsigned bits, size_t degree, size_t count, const uint64_t *skip,
+                         size_t n_skip, uint64_t *out, size_t *found);
+
+/* PreparedRnsNttPlan::new (ring/prepared_rns_ntt.rs:8-37) over
+ * RnsNttPlan::new (ring/rns_ntt.rs:4-40): one negacyclic plan per modulus,
+ * psi chosen as find_negacyclic_root (ring/ntt_params.rs:16-50).
+ * degree: 2^10 .. 2^17; every modulus prime, < 2^62, 2N | q-1. */
+int ckks_rns_plan_create(ckks_context *ctx, const uint64_t *moduli, size_t n_limbs,
+                         size_t degree, ckks_rns_plan **out);
+int ckks_rns_plan_destroy(ckks_rns_plan *plan);
+/* psi of every limb (NttPlan::psi, ring/ntt.rs:87-89). */
+int ckks_rns_plan_psi(const ckks_rns_plan *plan, uint64_t *psi_out);
+
+/* PreparedRnsNttPlan::forward / inverse (ring/prepared_rns_ntt.rs:39-84,
+ * per limb PreparedNttPlan::forward prepared_ntt.rs:200 and ::inverse
+ * prepared_ntt.rs:370) over n_polys polynomials of the first n_limbs limbs.
+ * in may equal out. */
+int ckks_ntt_forward(const ckks_rns_plan *plan, const uint64_t *in, uint64_t *out,
+                     size_t n_polys, size_t n_limbs, size_t poly_stride);
+int ckks_ntt_inverse(const ckks_rns_plan *plan, const uint64_t *in, uint64_t *out,
+                     size_t n_polys, size_t n_limbs, size_t poly_stride);
+
+/* Mixed-degree / mixed-level batch in one launch per pass (at most 16
+ * entries; every entry's plan must belong to the same context). */
+typedef struct ckks_ntt_batch {
+    const ckks_rns_plan *plan;
+    const uint64_t *in;
+    uint64_t *out;
+    size_t n_polys;
+    size_t n_limbs;
+    size_t poly_stride;
+} ckks_ntt_batch;
+int ckks_ntt_forward_batch(ckks_context *ctx, const ckks_ntt_batch *batch, size_t count);
+int ckks_ntt_inverse_batch(ckks_context *ctx, const ckks_ntt_batch *batch, size_t count);
+
+/* NttPolynomial::pointwise_mul / pointwise_mul_add_assign / add / sub
+ * (ring/ntt_polynomial.rs:160-270) and Polynomial::{add,sub,neg}
+ * (ring/polynomial.rs); the same arithmetic serves both domains.
+ * out may alias inputs. */
+int ckks_poly_mul(const ckks_rns_plan *plan, const uint64_t *x, const uint64_t *y,
+                  uint64_t *out, size_t n_polys, size_t n_limbs, size_t poly_stride);
+int ckks_poly_mac(const ckks_rns_plan *plan, uint64_t *acc, const uint64_t *x, const uint64_t *y,
+                  size_t n_polys, size_t n_limbs, size_t poly_stride);
+int ckks_poly_add(const ckks_rns_plan *plan, const uint64_t *x, const uint64_t *y,
+                  uint64_t *out, size_t n_polys, size_t n_limbs, size_t poly_stride);
+int ckks_poly_sub(const ckks_rns_plan *plan, const uint64_t *x, const uint64_t *y,
+                  uint64_t *out, size_t n_polys, size_t n_limbs, size_t poly_stride);
+int ckks_poly_neg(const ckks_rns_plan *plan, const uint64_t *x, uint64_t *out,
+                  size_t n_polys, size_t n_limbs, size_t poly_stride);
+
+/* apply_automorphism (ckks/automorphism.rs:8-46) per limb, coefficient
+ * domain; exponent odd modulo 2N.  out must not alias in. */
+int ckks_automorphism(const ckks_rns_plan *plan, const uint64_t *in, uint64_t *out,
+                      size_t n_polys, size_t n_limbs, size_t poly_stride, uint64_t exponent);
+/* permute_negacyclic_ntt_galois (ring/ntt_galois_permutation.rs:5-19),
+ * NTT domain.  out must not alias in. */
+int ckks_ntt_galois_permute(const ckks_rns_plan *plan, const uint64_t *in, uint64_t *out,
+                            size_t n_polys, size_t n_limbs, size_t poly_stride,
+                            uint64_t exponent);
+
+/* ------------------------------------------------------------ ciphertexts */
+
+/* rescale_rns_ckks_to_next -> rescale_rns_rlwe -> rescale_polynomial_limb
+ * (ckks/rns_rescale.rs:19-167): n_cts ciphertexts over the first n_limbs
+ * limbs (coefficient domain) to n_limbs-1 limbs.  in_stride/out_stride are
+ * the ciphertext strides in elements.  out must not alias in. */
+int ckks_rescale(const ckks_rns_plan *plan, const uint64_t *in, uint64_t *out, size_t n_cts,
+                 size_t n_limbs, size_t in_stride, size_t out_stride);
+
+/* mod_switch_rns_ckks_to_next (ckks/rns_mod_switch.rs:14-59): drop the
+ * trailing limb of both components. */
+int ckks_mod_switch(const ckks_rns_plan *plan, const uint64_t *in, uint64_t *out, size_t n_cts,
+                    size_t n_limbs, size_t in_stride, size_t out_stride);
+
+/* rns_tensor_with_prepared_ntt (grafting/rns_key_switch.rs:1724-1777):
+ * (b0,a0) x (b1,a1) -> (c0,c1,c2) coefficient domain, out laid out as
+ * [c0 limbs][c1 limbs][c2 limbs] per ciphertext (out_stride elements). */
+int ckks_tensor(const ckks_rns_plan *plan, const uint64_t *ct0, const uint64_t *ct1,
+                uint64_t *out, size_t n_cts, size_t n_limbs, size_t in_stride,
+                size_t out_stride);
+
+/* ------------------------------------------------------------ key switching */
+typedef struct ckks_ksk ckks_ksk;
+
+/* PreparedRnsKeySwitchKey::new / PreparedRnsMultiplicationKey
+ * (grafting/rns_key_switch.rs:1930-2013, 2347-2430): NTT-domain key
+ * entries, block-major [block][limb] pairs (b, a), uploaded from host
+ * arrays key_b, key_a of n_blocks * n_limbs * N residues each.
+ * RnsGadgetLayout::new (grafting/rns_gadget.rs:37-67) block sizes. */
+int ckks_rns_ksk_create(const ckks_rns_plan *plan, const size_t *block_sizes, size_t n_blocks,
+                        const uint64_t *key_b, const uint64_t *key_a, ckks_ksk **out);
+
+/* dnum-block hybrid key with special primes P.  qp_plan holds Q limbs
+ * followed by P limbs; q_limbs of them are Q.  Entry j encrypts
+ * P * E_j * s' over QP in the NTT domain, [block][qp limb] pairs. */
+int ckks_hybrid_ksk_create(const ckks_rns_plan *qp_plan, size_t q_limbs, const size_t *block_sizes,
+                           size_t n_blocks, const uint64_t *key_b, const uint64_t *key_a,
+                           ckks_ksk **out);
+int ckks_ksk_destroy(ckks_ksk *key);
+
+/* rns_key_switch_with_prepared_dynamic_ntt (grafting/rns_key_switch.rs:
+ * 2526-2608) for an RNS-gadget key, or the dnum hybrid key switch for a
+ * hybrid key (ModUp -> NTT -> key MAC -> iNTT -> ModDown by P).
+ * Coefficient-domain ciphertexts over the key's full Q basis in and out;
+ * out may equal in. */
+int ckks_key_switch(const ckks_ksk *key, const uint64_t *in, uint64_t *out, size_t n_cts,
+                    size_t in_stride, size_t out_stride);
+
+/* apply_rns_galois_automorphism_with_prepared_ntt (ckks/rns_galois_key.rs:
+ * 439-469) / rotate_left_rns_ckks_with_prepared_ntt (:730-751): sigma_k on
+ * both components then key switch with a key for sigma_k(s) -> s. */
+int ckks_apply_galois(const ckks_ksk *key, uint64_t exponent, const uint64_t *in, uint64_t *out,
+                      size_t n_cts, size_t in_stride, size_t out_stride);
+
+/* rns_relinearize_with_prepared_dynamic_ntt (grafting/rns_key_switch.rs:
+ * 2113-2259): (c0,c1,c2) -> (c0,c1) + switch(c2). */
+int ckks_relinearize(const ckks_ksk *key, const uint64_t *in3, uint64_t *out, size_t n_cts,
+                     size_t in_stride, size_t out_stride);
+
+/* multiply_relinearize_rescale_rns_ckks_with_prepared_ntt
+ * (ckks/rns_multiply.rs:161-215): tensor, relinearize, rescale. */
+int ckks_multiply_relinearize_rescale(const ckks_ksk *relin_key, const uint64_t *ct0,
+                                      const uint64_t *ct1, uint64_t *out, size_t n_cts,
+                                      size_t in_stride, size_t out_stride);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
