@@ -1,0 +1,291 @@
+"""Benchmark of the leveled RNS-CKKS rotation hot path on one MI355X.
+
+Workload (batched core primitives, N = 2^16):
+  Q = one 60-bit base prime + 24 40-bit primes (L = 24, 25 limbs), hybrid
+  key switching with dnum = 3 blocks [9, 8, 8] and 7 60-bit special primes;
+  a batch of 64 ciphertexts at N = 2^16 plus 16 ciphertexts at N = 2^15
+  (13 limbs, blocks [5, 4, 4], 4 special primes); one step rotates every
+  ciphertext by each of 2^0 .. 2^14 slots (15 Galois keys per degree).
+  Inputs, keys and outputs stay resident in HBM; residues are synthetic
+  (uniform, canonical) and keys are synthetic NTT-domain material: the work
+  done per rotation does not depend on the values.
+
+Reported (one JSON line):
+  value           batched hybrid-key-switched rotations per second
+  plain_*         the plain per-op GPU path: the reference's own rotation
+                  algorithm (RNS-gadget key switch with singleton blocks,
+                  rotate_left_rns_ckks_with_prepared_ntt) one ciphertext per
+                  call, timed on a sample of rotations
+  ntt_*           forward NTT over the batch's 96 digit limbs per ciphertext
+  p50/p99_ms      latency of one batched rotation call (64 ciphertexts)
+  roofline        the forward NTT (dominant kernel pair) against HBM peak
+  cpu_baseline    the CPU oracle's hybrid rotation on one core, bounded sample
+"""
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+HBM_PEAK_GBPS = 8000.0
+
+
+def _load_pkg():
+    name = "ccmm_rs_amd"
+    if name in sys.modules:
+        return sys.modules[name]
+    spec = importlib.util.spec_from_file_location(name, os.path.join(ROOT, "ccmm-rs_amd", "__init__.py"))
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def rotation_exponent_left(n, steps):
+    return pow(5, steps % (n // 2), 2 * n)
+
+
+def rand_residues(torch, shape, device):
+    # < 2^39 is canonical for every 40- and 60-bit prime of the chain
+    return torch.randint(0, 1 << 39, shape, dtype=torch.int64, device=device)
+
+
+def synthetic_key(torch, blocks_total, limbs, n):
+    g = torch.Generator().manual_seed(limbs * 1000 + blocks_total)
+    kb = torch.randint(0, 1 << 39, (blocks_total, limbs, n), dtype=torch.int64, generator=g).numpy()
+    ka = torch.randint(0, 1 << 39, (blocks_total, limbs, n), dtype=torch.int64, generator=g).numpy()
+    return kb.view(np.uint64), ka.view(np.uint64)
+
+
+class Timer:
+    """HIP events on torch's current stream, which the library runs on."""
+
+    def __init__(self, torch):
+        self.torch = torch
+
+    def __enter__(self):
+        self.e0 = self.torch.cuda.Event(enable_timing=True)
+        self.e1 = self.torch.cuda.Event(enable_timing=True)
+        self.e0.record()
+        return self
+
+    def __exit__(self, *a):
+        self.e1.record()
+        self.e1.synchronize()
+        self.ms = self.e0.elapsed_time(self.e1)
+
+
+def cpu_baseline(q_moduli, p_moduli, n, blocks, seconds=10.0):
+    """The oracle's rotation (automorphism + hybrid key switch) on one core."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O  # the checker, timed here as the CPU baseline
+
+    O.build()
+    rng = np.random.default_rng(0)
+    qp = q_moduli + p_moduli
+    plan = O.RnsPlan(qp, n)
+    L = len(q_moduli)
+    kb = rng.integers(0, 1 << 39, (len(blocks), len(qp), n), dtype=np.uint64)
+    ka = rng.integers(0, 1 << 39, (len(blocks), len(qp), n), dtype=np.uint64)
+    b = rng.integers(0, 1 << 39, (L, n), dtype=np.uint64)
+    a = rng.integers(0, 1 << 39, (L, n), dtype=np.uint64)
+    k = rotation_exponent_left(n, 1)
+    done = 0
+    t0 = time.perf_counter()
+    while True:
+        rb = np.stack([O.apply_automorphism(b[i], q, k) for i, q in enumerate(q_moduli)])
+        ra = np.stack([O.apply_automorphism(a[i], q, k) for i, q in enumerate(q_moduli)])
+        O.hybrid_key_switch(plan, L, blocks, rb, ra, kb, ka)
+        done += 1
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            break
+    return {"value": done / el, "unit": "rotations/s", "cores": 1, "kind": "port",
+            "sample": f"{done} hybrid rotations of one ciphertext at N=2^16, L=24, dnum=3, K=7 "
+                      f"({el:.1f} s, oracle in C, one thread)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--mixed", type=int, default=16, help="ciphertexts at N=2^15 in each step")
+    ap.add_argument("--plain-sample", type=int, default=6, help="plain-path rotations timed")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: every GPU box of this build has one; "
+              "sharding across GPUs is by ciphertext and is not run here", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    pkg = _load_pkg()
+    dev = "cuda"
+    torch.cuda.init()
+    ctx = pkg.Context(0)
+
+    N = 1 << 16
+    N2 = 1 << 15
+    q_moduli, p_moduli = pkg.chain_moduli(24, 7)
+    L, K = 25, 7
+    blocks = [9, 8, 8]
+    L2, K2, blocks2 = 13, 4, [5, 4, 4]
+    rot_steps = [1 << i for i in range(15)]
+
+    qp_plan = pkg.PreparedRnsNttPlan(ctx, q_moduli + p_moduli, N)
+    qp2 = q_moduli[:L2] + p_moduli[:K2]
+    qp_plan2 = pkg.PreparedRnsNttPlan(ctx, qp2, N2)
+
+    print(f"# building {len(rot_steps)} Galois keys per degree", file=sys.stderr, flush=True)
+    keys, keys2 = [], []
+    for r in rot_steps:
+        kb, ka = synthetic_key(torch, len(blocks), L + K, N)
+        keys.append((rotation_exponent_left(N, r), pkg.HybridKeySwitchKey(qp_plan, L, blocks, kb, ka)))
+        if args.mixed:
+            kb2, ka2 = synthetic_key(torch, len(blocks2), L2 + K2, N2)
+            keys2.append((rotation_exponent_left(N2, r),
+                          pkg.HybridKeySwitchKey(qp_plan2, L2, blocks2, kb2, ka2)))
+
+    B = args.batch
+    cts = rand_residues(torch, (B, 2, L, N), dev)
+    out = torch.empty_like(cts)
+    cts2 = rand_residues(torch, (args.mixed, 2, L2, N2), dev) if args.mixed else None
+    out2 = torch.empty_like(cts2) if args.mixed else None
+
+    def step(lat=None):
+        for idx, (k, key) in enumerate(keys):
+            if lat is not None:
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record()
+            key.apply_galois(cts, k, out)
+            if lat is not None:
+                e1.record()
+                lat.append((e0, e1))
+            if args.mixed:
+                k2, key2 = keys2[idx]
+                key2.apply_galois(cts2, k2, out2)
+
+    print("# warmup", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    lat_events = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    with Timer(torch) as tm:
+        for _ in range(args.steps):
+            step(lat_events)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_step = tm.ms / args.steps
+    rot_per_step = len(rot_steps) * (B + args.mixed)
+    value = rot_per_step / (ms_step / 1e3)
+    lats = sorted(e0.elapsed_time(e1) for e0, e1 in lat_events)
+    p50 = float(np.percentile(lats, 50))
+    p99 = float(np.percentile(lats, 99))
+    print(f"# batched: {ms_step:.1f} ms/step, {value:.0f} rot/s (wall {wall:.2f}s)", file=sys.stderr,
+          flush=True)
+
+    # ---- plain per-op path: reference algorithm (RNS gadget, singleton
+    # blocks), one ciphertext per call
+    q_plan = pkg.PreparedRnsNttPlan(ctx, q_moduli, N)
+    kbp, kap = synthetic_key(torch, L, L, N)
+    plain_key = pkg.RnsKeySwitchKey(q_plan, [1] * L, kbp, kap)
+    k1 = rotation_exponent_left(N, 1)
+    one = cts[0:1].contiguous()
+    one_out = torch.empty_like(one)
+    plain_key.apply_galois(one, k1, one_out)
+    torch.cuda.synchronize()
+    with Timer(torch) as tp:
+        for i in range(args.plain_sample):
+            plain_key.apply_galois(cts[i % B:i % B + 1], k1, one_out)
+    plain_value = args.plain_sample / (tp.ms / 1e3)
+    del plain_key
+    print(f"# plain: {tp.ms / args.plain_sample:.2f} ms/rotation, {plain_value:.0f} rot/s", file=sys.stderr,
+          flush=True)
+
+    # ---- dominant kernel: forward NTT of the digit limbs of the batch
+    # (dnum * (L+K) limb transforms per ciphertext), timed live
+    digits = rand_residues(torch, (B * len(blocks), L + K, N), dev)
+    qp_plan.forward(digits)
+    torch.cuda.synchronize()
+    reps = 5
+    with Timer(torch) as tn:
+        for _ in range(reps):
+            qp_plan.forward(digits)
+    ntt_ms = tn.ms / reps
+    limb_transforms = B * len(blocks) * (L + K)
+    ntt_bytes = 2 * N * 8 * limb_transforms
+    ntt_gbps = ntt_bytes / (ntt_ms / 1e3) / 1e9
+    with Timer(torch) as ti:
+        for _ in range(reps):
+            qp_plan.inverse(digits)
+    intt_gbps = ntt_bytes / (ti.ms / reps / 1e3) / 1e9
+    del digits
+    print(f"# NTT fwd {ntt_ms:.2f} ms over {limb_transforms} limbs: {ntt_gbps:.0f} GB/s; "
+          f"inv {intt_gbps:.0f} GB/s", file=sys.stderr, flush=True)
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "ntt_forward_traffic.json")
+    if os.path.exists(tpath):
+        try:
+            traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
+        except Exception:
+            traffic = None
+
+    result = {
+        "metric": "key-switched rotations/s at N=2^16 L=24 batched vs plain GPU; NTT GB/s; p99 lat",
+        "value": round(value, 1),
+        "unit": "rotations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"rotate {B} ct at N=2^16 L=24 (dnum=3 [9,8,8], K=7) + {args.mixed} ct at "
+                        f"N=2^15 L=12 (dnum=3 [5,4,4], K=4) by 2^0..2^14 slots",
+            "ring_degree": N, "limbs": L, "dnum": 3, "special_primes": K,
+            "batch": B, "mixed_batch": args.mixed, "rotations_per_step": rot_per_step,
+        },
+        "plain_rotations_per_s": round(plain_value, 1),
+        "speedup_vs_plain": round(value / plain_value, 2),
+        "ntt_forward_gbps": round(ntt_gbps, 1),
+        "ntt_inverse_gbps": round(intt_gbps, 1),
+        "p50_ms": round(p50, 3),
+        "p99_ms": round(p99, 3),
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "forward NTT (column + row pass), 2*N*8 bytes per limb transform",
+            "achieved": round(ntt_gbps, 1),
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": round(ntt_gbps / HBM_PEAK_GBPS, 4),
+            "traffic": traffic,
+        },
+    }
+    if not args.no_cpu_baseline:
+        print("# cpu baseline", file=sys.stderr, flush=True)
+        result["cpu_baseline"] = cpu_baseline(q_moduli, p_moduli, N, blocks, args.cpu_seconds)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()
