@@ -55,6 +55,7 @@ _SIGNATURES = {
     "ckks_context_destroy": (C.c_int, [_p]),
     "ckks_context_set_stream": (C.c_int, [_p, _p]),
     "ckks_sync": (C.c_int, [_p]),
+    "ckks_set_ntt_chunk_bytes": (C.c_int, [_p, _sz]),
     "ckks_malloc": (C.c_int, [_p, _sz, C.POINTER(_p)]),
     "ckks_free": (C.c_int, [_p, _p]),
     "ckks_upload": (C.c_int, [_p, _p, _p, _sz]),
@@ -188,6 +189,10 @@ class Context:
 
     def sync(self):
         _check(lib().ckks_sync(self._h))
+
+    def set_ntt_chunk_bytes(self, nbytes: int):
+        """NTT inter-pass buffer per chunk (tuning only; results unchanged)."""
+        _check(lib().ckks_set_ntt_chunk_bytes(self._h, int(nbytes)))
 
     def close(self):
         if getattr(self, "_h", None):

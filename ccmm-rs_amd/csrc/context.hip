@@ -98,6 +98,12 @@ int ckks_sync(ckks_context *ctx) {
     return CKKS_OK;
 }
 
+int ckks_set_ntt_chunk_bytes(ckks_context *ctx, size_t bytes) {
+    if (!ctx) return fail(CKKS_ERR_INVALID, "null context");
+    ctx->ntt_chunk_bytes = bytes;
+    return CKKS_OK;
+}
+
 int ckks_malloc(ckks_context *ctx, size_t bytes, void **dev) {
     if (!ctx || !dev) return fail(CKKS_ERR_INVALID, "null argument");
     hipError_t e = hipMalloc(dev, bytes ? bytes : 1);
@@ -179,8 +185,13 @@ int ckks_rns_plan_create(ckks_context *ctx, const uint64_t *moduli, size_t n_lim
             tt.q = q;
             tt.logn = logn;
             std::vector<uint64_t> fwd, inv;
-            host::twiddle_table(q, psi, logn, fwd);
-            host::twiddle_table(q, host::invmod(psi, q), logn, inv);
+            if (q < host::kFpModulusBound) {
+                host::twiddle_table_fp(q, psi, logn, fwd);
+                host::twiddle_table_fp(q, host::invmod(psi, q), logn, inv);
+            } else {
+                host::twiddle_table(q, psi, logn, fwd);
+                host::twiddle_table(q, host::invmod(psi, q), logn, inv);
+            }
             size_t bytes = fwd.size() * sizeof(uint64_t);
             CK_HIP(hipMalloc(&tt.fwd, bytes));
             CK_HIP(hipMalloc(&tt.inv, bytes));
@@ -196,6 +207,11 @@ int ckks_rns_plan_create(ckks_context *ctx, const uint64_t *moduli, size_t n_lim
         p.ninv_q = host::shoup_quotient(p.ninv, q);
         host::barrett(q, p.br_hi, p.br_lo);
         p.logn = logn;
+        p.fp = q < host::kFpModulusBound ? 1u : 0u;
+        p.qd = (double)q;
+        p.qinv = 1.0 / (double)q;
+        p.ninv_d = (double)p.ninv;
+        p.ninv_p = (double)p.ninv / (double)q;
         plan->moduli.push_back(q);
         plan->psi.push_back(psi);
         plan->h_primes.push_back(p);

@@ -73,6 +73,25 @@ inline void twiddle_table(uint64_t q, uint64_t root, uint32_t logn, std::vector<
     }
 }
 
+// Moduli below this bound run the NTT on the exact FP64 path.
+constexpr uint64_t kFpModulusBound = 1ull << 41;
+
+inline uint64_t f64_bits(double d) {
+    uint64_t v;
+    __builtin_memcpy(&v, &d, 8);
+    return v;
+}
+
+// FP64-path table: (double w, double fl(w / q)) as raw bits.
+inline void twiddle_table_fp(uint64_t q, uint64_t root, uint32_t logn, std::vector<uint64_t> &out) {
+    twiddle_table(q, root, logn, out);
+    for (size_t k = 0; k < out.size() / 2; ++k) {
+        double w = (double)out[2 * k];
+        out[2 * k] = f64_bits(w);
+        out[2 * k + 1] = f64_bits(w / (double)q);
+    }
+}
+
 // Deterministic Miller-Rabin for 64-bit values (witness set of
 // ring/ntt_prime.rs:8-33).
 inline bool is_prime(uint64_t v) {

@@ -21,7 +21,10 @@ struct DevPrime {
     const Tw *inv;  // psi^-bitrev(k)
     u64 ninv, ninv_q;
     u64 br_hi, br_lo;  // floor(2^128 / q)
-    u32 logn, pad;
+    double qd, qinv;      // FP64 path: q and 1/q
+    double ninv_d, ninv_p;  // FP64 path: N^-1 and N^-1 / q
+    u32 logn;
+    u32 fp;  // 1: q < 2^41, the NTT runs on the exact FP64 path
 };
 
 // Constants of the exact rescale by a dropped limb l into limb i < l
@@ -70,6 +73,8 @@ struct ckks_context {
     static constexpr int kSlots = 8;
     void *ws[kSlots] = {};
     size_t ws_bytes[kSlots] = {};
+    // NTT inter-pass buffer per chunk in bytes (0 = whole batch in one chunk)
+    size_t ntt_chunk_bytes = 96u << 20;
     std::vector<void *> owned;  // misc device allocations freed at destroy
 
     int workspace(int slot, size_t bytes, void **out);

@@ -83,19 +83,38 @@ CK_HD u64 reduce64(u64 x, const DevPrime &P) {
 struct ModUpArgs {
     const u64 *in;  // component being decomposed: limb i of ct c at in + c*in_stride + i*N
     u64 in_stride;
+    u64 kinv;  // nonzero: decompose sigma_k(in), k^-1 mod 2N (rotation fused in)
     u64 *digits;  // [ct][dnum][B][N]
     const ModUpBlock *blocks;
     const DevPrime *primes;  // key basis B
     u32 dnum, limbs, logn, chunks;
 };
 
+// Coefficient n of sigma_k(p) is +-p[g]: g = n k^-1 mod 2N, negated when
+// g >= N (apply_automorphism, ckks/automorphism.rs:8-46, as a gather).
+struct Gather {
+    u64 idx;
+    bool neg;
+};
+
+CK_HD Gather galois_gather(u64 n, u64 kinv, u32 logn) {
+    if (kinv == 0) return Gather{n, false};
+    const u64 N = (u64)1 << logn;
+    u64 g = (n * kinv) & (2 * N - 1);
+    return g < N ? Gather{g, false} : Gather{g - N, true};
+}
+
 template <int A>
 __device__ void modup_thread(const ModUpArgs &a, const ModUpBlock &blk, u32 ct, u32 j, u64 n) {
     const u64 N = (u64)1 << a.logn;
-    const u64 *src = a.in + ct * a.in_stride + (u64)blk.start * N + n;
+    const Gather gi = galois_gather(n, a.kinv, a.logn);
+    const u64 *src = a.in + ct * a.in_stride + (u64)blk.start * N + gi.idx;
     u64 x[A], c[A];
 #pragma unroll
-    for (int m = 0; m < A; ++m) x[m] = src[(u64)m * N];
+    for (int m = 0; m < A; ++m) {
+        u64 v = src[(u64)m * N];
+        x[m] = gi.neg ? neg_mod(v, a.primes[blk.start + m].q) : v;
+    }
     // mixed-radix digits
 #pragma unroll
     for (int k = 0; k < A; ++k) {
@@ -116,7 +135,10 @@ __device__ void modup_thread(const ModUpArgs &a, const ModUpBlock &blk, u32 ct, 
         u64 v;
         if (i >= blk.start && i < blk.start + A) {
             // inside the block the canonical digit reduces to the residue
-            v = src[(u64)(i - blk.start) * N];
+            v = x[0];
+#pragma unroll
+            for (int m = 1; m < A; ++m)
+                if (i == blk.start + m) v = x[m];
         } else {
             const DevPrime &Pi = a.primes[i];
             const u64 *w = blk.w + (u64)i * kMaxAlpha;
@@ -233,6 +255,7 @@ struct DownArgs {
     const u64 *acc;  // [ct][2][B][N] coefficient domain
     const u64 *add0, *add1;  // pass-through for component 0 / 1 (may be null)
     u64 add_stride;
+    u64 kinv;  // nonzero: component 0 pass-through is sigma_k(add0)
     u64 *out;  // [ct][2][q_limbs][N] with out_stride
     u64 out_stride;
     const ModDownTable *md;  // null: no special primes
@@ -250,12 +273,16 @@ __global__ __launch_bounds__(kKsThreads) void k_moddown(DownArgs a) {
     const u64 n = (u64)chunk * kKsThreads + threadIdx.x;
     const u64 *x = a.acc + ((u64)ct * 2 + comp) * a.limbs * N + n;
     const u64 *add = comp == 0 ? a.add0 : a.add1;
-    if (add) add += ct * a.add_stride + n;
+    const Gather ga = galois_gather(n, comp == 0 ? a.kinv : 0, a.logn);
+    if (add) add += ct * a.add_stride + ga.idx;
     u64 *o = a.out + ct * a.out_stride + (u64)comp * a.q_limbs * N + n;
     if (K == 0) {
         for (u32 i = 0; i < a.q_limbs; ++i) {
             u64 v = x[(u64)i * N];
-            if (add) v = add_mod(v, add[(u64)i * N], a.primes[i].q);
+            if (add) {
+                u64 ad = add[(u64)i * N];
+                v = ga.neg ? sub_mod(v, ad, a.primes[i].q) : add_mod(v, ad, a.primes[i].q);
+            }
             o[(u64)i * N] = v;
         }
         return;
@@ -297,7 +324,10 @@ __global__ __launch_bounds__(kKsThreads) void k_moddown(DownArgs a) {
         u64 r = reduce128(t, Pi);  // x_P mod q_i
         if (neg) r = sub_mod(r, md.pmod[i], Pi.q);  // (x_P - P) mod q_i
         u64 v = shoup(sub_mod(x[(u64)i * N], r, Pi.q), md.pinv[i], md.pinv_q[i], Pi.q);
-        if (add) v = add_mod(v, add[(u64)i * N], Pi.q);
+        if (add) {
+            u64 ad = add[(u64)i * N];
+            v = ga.neg ? sub_mod(v, ad, Pi.q) : add_mod(v, ad, Pi.q);
+        }
         o[(u64)i * N] = v;
     }
 }
@@ -463,7 +493,8 @@ static int create_key(const ckks_rns_plan *plan, size_t q_limbs, const size_t *b
 //   out[c][0] = add0 + KS_b(x),  out[c][1] = add1 + KS_a(x)
 // (add0/add1 may be null) over the key's Q limbs.
 static int key_switch_core(const ckks_ksk *k, const u64 *x, u64 x_stride, const u64 *add0,
-                           const u64 *add1, u64 add_stride, u64 *out, u64 out_stride, size_t n_cts) {
+                           const u64 *add1, u64 add_stride, u64 *out, u64 out_stride, size_t n_cts,
+                           u64 galois_kinv = 0) {
     ckks_context *ctx = k->ctx;
     const ckks_rns_plan *plan = k->plan;
     const u64 N = k->degree, B = k->limbs, D = k->dnum;
@@ -479,6 +510,7 @@ static int key_switch_core(const ckks_ksk *k, const u64 *x, u64 x_stride, const 
     ModUpArgs mu{};
     mu.in = x;
     mu.in_stride = x_stride;
+    mu.kinv = galois_kinv;
     mu.digits = digits;
     mu.blocks = k->d_modup;
     mu.primes = plan->d_primes;
@@ -521,6 +553,7 @@ static int key_switch_core(const ckks_ksk *k, const u64 *x, u64 x_stride, const 
     da.add0 = add0;
     da.add1 = add1;
     da.add_stride = add_stride;
+    da.kinv = galois_kinv;
     da.out = out;
     da.out_stride = out_stride;
     da.md = k->d_moddown;
@@ -602,21 +635,16 @@ int ckks_apply_galois(const ckks_ksk *key, uint64_t exponent, const uint64_t *in
                       size_t n_cts, size_t in_stride, size_t out_stride) {
     int rc = check_ct_args(key, in, out, n_cts, in_stride, out_stride, 2);
     if (rc || n_cts == 0) return rc;
+    const u64 two_n = 2 * (u64)key->degree;
+    const u64 k = exponent % two_n;
+    if ((k & 1) == 0) return fail(CKKS_ERR_INVALID, "CKKS automorphism exponent must be odd modulo 2N");
+    if (in == out) return fail(CKKS_ERR_INVALID, "rotation output must not alias its input");
+    // sigma_k is fused into the key switch as gathers with k^-1 mod 2N
+    u64 kinv = k;
+    for (int it = 0; it < 6; ++it) kinv = kinv * (2 - k * kinv);
+    kinv &= two_n - 1;
     const size_t poly = key->q_limbs * key->degree;
-    ckks_context *ctx = key->ctx;
-    void *w;
-    if ((rc = ctx->workspace(3, n_cts * 2 * poly * sizeof(u64), &w))) return rc;
-    u64 *rot = (u64 *)w;
-    // sigma_k on both components (apply_rns_automorphism, ckks/rns_automorphism.rs:12)
-    if (in_stride == 2 * poly) {
-        if ((rc = ckks_automorphism(key->plan, in, rot, 2 * n_cts, key->q_limbs, poly, exponent))) return rc;
-    } else {
-        for (size_t c = 0; c < n_cts; ++c)
-            if ((rc = ckks_automorphism(key->plan, in + c * in_stride, rot + c * 2 * poly, 2, key->q_limbs, poly,
-                                        exponent)))
-                return rc;
-    }
-    return key_switch_core(key, rot + poly, 2 * poly, rot, nullptr, 2 * poly, out, out_stride, n_cts);
+    return key_switch_core(key, in + poly, in_stride, in, nullptr, in_stride, out, out_stride, n_cts, kinv);
 }
 
 int ckks_relinearize(const ckks_ksk *key, const uint64_t *in3, uint64_t *out, size_t n_cts,

@@ -92,6 +92,54 @@ CK_HD u64 mul_mod(u64 a, u64 b, const Barrett &br) {
     return barrett_reduce128(hi, lo, br);
 }
 
+// ---- exact FP64 modular arithmetic for primes q < 2^42 --------------------
+//
+// CDNA4 runs FP64 FMA at full rate but 32-bit integer multiplies at a
+// fraction of it, so for the 40-bit primes residues are carried as doubles
+// (exact integers) in a signed lazy range.  For integer-valued doubles y, w
+// with |y| < 2^45 and 0 <= w < q < 2^42, and wp = fl(w / q):
+//   hi = fl(y w), lo = y w - hi exactly (TwoProduct by FMA),
+//   k = rint(fl(y wp)) is round(y w / q) or off by one (|error| < 2^-7),
+//   r = (hi - k q) + lo is then exact (every intermediate is an integer
+//       below 2^53) and r = y w (mod q) with |r| <= 0.51 q.
+// No rounding error ever reaches a residue: the result is the exact
+// integer, merely not yet canonical.
+CK_HD double fp_mulmod(double y, double w, double wp, double q) {
+#pragma clang fp contract(off)
+    double hi = y * w;
+    double lo = __builtin_fma(y, w, -hi);
+    double k = __builtin_rint(y * wp);
+    double r = __builtin_fma(-k, q, hi);
+    return r + lo;
+}
+
+// x (integer-valued, |x| < 2^45) -> x mod q in (-0.5q, 0.5q].
+CK_HD double fp_reduce(double x, double q, double qinv) {
+#pragma clang fp contract(off)
+    double k = __builtin_rint(x * qinv);
+    return __builtin_fma(-k, q, x);
+}
+
+// Canonical residue in [0, q) of an integer-valued double, |x| < 2^45.
+CK_HD u64 fp_canonical(double x, double q, double qinv) {
+    double r = fp_reduce(x, q, qinv);
+    r = r < 0.0 ? r + q : r;
+    r = r >= q ? r - q : r;
+    return (u64)r;
+}
+
+CK_HD double u64_as_f64(u64 v) {
+    double d;
+    __builtin_memcpy(&d, &v, 8);
+    return d;
+}
+
+CK_HD u64 f64_as_u64(double d) {
+    u64 v;
+    __builtin_memcpy(&v, &d, 8);
+    return v;
+}
+
 // 128-bit accumulator for sums of products before one reduction.
 struct Acc128 {
     u64 hi, lo;

@@ -1,9 +1,18 @@
 // Batched negacyclic NTT / iNTT kernels (gfx950) and their launch.
 //
-// One launch per pass covers every (polynomial, limb, tile) of up to 16
-// batch entries of possibly different degree and level.  Blocks are ordered
-// limb-major (limb outer, polynomial inner, tile innermost) so blocks in
-// flight share a prime and its twiddle table stays hot in each XCD's L2.
+// A launch covers every (polynomial, limb, tile) of up to 16 batch entries
+// of possibly different degree and level.  Blocks are ordered limb-major
+// (limb outer, polynomial inner, tile innermost) so the blocks in flight
+// share a prime and its twiddle table stays hot in each XCD's L2.
+//
+// Degrees above 4096 take two passes (column pass, row pass).  The batch is
+// cut into chunks of limb transforms whose inter-pass buffer (~96 MiB) is
+// reused chunk after chunk: the buffer stays resident in the 256 MiB
+// Infinity Cache, so HBM sees one read and one write per residue instead
+// of two of each.
+#include <algorithm>
+#include <cstdlib>
+
 #include "internal.hpp"
 #include "ntt_pass.hpp"
 
@@ -12,10 +21,11 @@ namespace ck {
 struct DevNttDesc {
     const u64 *src;
     u64 *dst;
-    u64 *tmp;
+    u64 *tmp;  // inter-pass buffer of this entry's first limb transform in the chunk
     const DevPrime *primes;
     u64 in_stride, out_stride;
     u32 n_polys, n_limbs, logn, block_start;
+    u32 lt_begin;  // first limb transform (limb-major index) covered
 };
 
 constexpr int kMaxNttDesc = 16;
@@ -35,7 +45,8 @@ __global__ __launch_bounds__(256) void k_ntt(NttLaunch launch) {
     const u32 tiles = ntt_tiles(D.logn);
     const u32 local = b - D.block_start;
     const u32 tile = local % tiles;
-    const u32 lt = local / tiles;
+    const u32 rel = local / tiles;
+    const u32 lt = D.lt_begin + rel;
     const u32 limb = lt / D.n_polys, poly = lt % D.n_polys;
     const DevPrime &P = D.primes[limb];
     const size_t N = (size_t)1 << D.logn;
@@ -45,11 +56,15 @@ __global__ __launch_bounds__(256) void k_ntt(NttLaunch launch) {
     job.q = P.q;
     job.ninv = P.ninv;
     job.ninv_q = P.ninv_q;
+    job.qd = P.qd;
+    job.qinv = P.qinv;
+    job.ninv_d = P.ninv_d;
+    job.ninv_p = P.ninv_p;
     job.logn = D.logn;
     job.tile = tile;
     const u64 *src = D.src + poly * D.in_stride + limb * N;
     u64 *dst = D.dst + poly * D.out_stride + limb * N;
-    u64 *tmp = D.tmp + ((size_t)poly * D.n_limbs + limb) * N;
+    u64 *tmp = D.tmp + (size_t)rel * N;
     const bool two = ntt_two_pass(D.logn);
     if (FIRST) {
         job.src = src;
@@ -61,15 +76,28 @@ __global__ __launch_bounds__(256) void k_ntt(NttLaunch launch) {
     u64 regs[16];
     BlockExec ex{256};
     auto regs_of = [&](u32) -> u64(&)[16] { return regs; };
-    ntt_dispatch<FWD, FIRST>(ex, regs_of, job, lds);
+    if (P.fp)
+        ntt_dispatch<FWD, FIRST, true>(ex, regs_of, job, lds);
+    else
+        ntt_dispatch<FWD, FIRST, false>(ex, regs_of, job, lds);
 }
+
+static void launch_pass(hipStream_t s, bool forward, bool first, const NttLaunch &l, u32 blocks) {
+    if (!blocks) return;
+    if (forward) {
+        if (first) hipLaunchKernelGGL((k_ntt<true, true>), dim3(blocks), dim3(256), 0, s, l);
+        else hipLaunchKernelGGL((k_ntt<true, false>), dim3(blocks), dim3(256), 0, s, l);
+    } else {
+        if (first) hipLaunchKernelGGL((k_ntt<false, true>), dim3(blocks), dim3(256), 0, s, l);
+        else hipLaunchKernelGGL((k_ntt<false, false>), dim3(blocks), dim3(256), 0, s, l);
+    }
+}
+
 
 int launch_ntt(ckks_context *ctx, const NttCall *calls, size_t count, bool forward) {
     if (count == 0) return CKKS_OK;
     if (count > (size_t)kMaxNttDesc) return fail(CKKS_ERR_INVALID, "NTT batch holds at most 16 entries");
-    NttLaunch first{}, second{};
-    size_t tmp_elems = 0;
-    u32 blocks1 = 0, blocks2 = 0;
+    size_t total_two = 0, max_n = 0;  // residues of all two-pass limb transforms
     for (size_t i = 0; i < count; ++i) {
         const NttCall &c = calls[i];
         int rc = check_plan_limbs(c.plan, c.n_limbs, "NTT");
@@ -80,19 +108,13 @@ int launch_ntt(ckks_context *ctx, const NttCall *calls, size_t count, bool forwa
         if (c.n_polys > 1 && (c.in_stride < c.n_limbs * c.plan->degree ||
                               c.out_stride < c.n_limbs * c.plan->degree))
             return fail(CKKS_ERR_INVALID, "polynomial stride smaller than its limbs");
-        if (ntt_two_pass(c.plan->logn)) tmp_elems += c.n_polys * c.n_limbs * c.plan->degree;
+        if (ntt_two_pass(c.plan->logn)) {
+            total_two += c.n_polys * c.n_limbs * c.plan->degree;
+            max_n = std::max(max_n, c.plan->degree);
+        }
     }
-    u64 *tmp = nullptr;
-    if (tmp_elems) {
-        void *ws;
-        int rc = ctx->workspace(0, tmp_elems * sizeof(u64), &ws);
-        if (rc) return rc;
-        tmp = (u64 *)ws;
-    }
-    size_t tmp_off = 0;
-    for (size_t i = 0; i < count; ++i) {
-        const NttCall &c = calls[i];
-        if (c.n_polys == 0) continue;
+    hipStream_t s = ctx->stream;
+    auto base_desc = [&](const NttCall &c) {
         DevNttDesc d{};
         d.src = c.in;
         d.dst = c.out;
@@ -102,27 +124,65 @@ int launch_ntt(ckks_context *ctx, const NttCall *calls, size_t count, bool forwa
         d.n_polys = (u32)c.n_polys;
         d.n_limbs = (u32)c.n_limbs;
         d.logn = c.plan->logn;
-        u32 tiles = ntt_tiles(d.logn);
-        u32 nblk = d.n_polys * d.n_limbs * tiles;
-        bool two = ntt_two_pass(d.logn);
-        d.tmp = two ? tmp + tmp_off : nullptr;
-        if (two) tmp_off += c.n_polys * c.n_limbs * c.plan->degree;
-        d.block_start = blocks1;
-        first.d[first.n++] = d;
-        blocks1 += nblk;
-        if (two) {
-            d.block_start = blocks2;
-            second.d[second.n++] = d;
-            blocks2 += nblk;
+        return d;
+    };
+    auto two_pass = [&](size_t i) { return calls[i].n_polys != 0 && ntt_two_pass(calls[i].plan->logn); };
+    // single-pass entries (N <= 4096): one launch
+    {
+        NttLaunch l{};
+        u32 blocks = 0;
+        for (size_t i = 0; i < count; ++i) {
+            const NttCall &c = calls[i];
+            if (c.n_polys == 0 || two_pass(i)) continue;
+            DevNttDesc d = base_desc(c);
+            d.block_start = blocks;
+            l.d[l.n++] = d;
+            blocks += d.n_polys * d.n_limbs;
         }
+        launch_pass(s, forward, true, l, blocks);
     }
-    hipStream_t s = ctx->stream;
-    if (forward) {
-        if (blocks1) hipLaunchKernelGGL((k_ntt<true, true>), dim3(blocks1), dim3(256), 0, s, first);
-        if (blocks2) hipLaunchKernelGGL((k_ntt<true, false>), dim3(blocks2), dim3(256), 0, s, second);
-    } else {
-        if (blocks1) hipLaunchKernelGGL((k_ntt<false, true>), dim3(blocks1), dim3(256), 0, s, first);
-        if (blocks2) hipLaunchKernelGGL((k_ntt<false, false>), dim3(blocks2), dim3(256), 0, s, second);
+    if (total_two) {
+        size_t cap = ctx->ntt_chunk_bytes / sizeof(u64);
+        if (cap == 0 || cap > total_two) cap = total_two;
+        if (cap < max_n) cap = max_n;
+        void *ws;
+        int rc = ctx->workspace(0, cap * sizeof(u64), &ws);
+        if (rc) return rc;
+        u64 *tmp = (u64 *)ws;
+        // walk the limb transforms of the two-pass entries in order, closing a
+        // chunk when its inter-pass buffer is full
+        size_t ci = 0, lt = 0;
+        while (ci < count) {
+            NttLaunch l{};
+            u32 blocks = 0;
+            size_t used = 0;
+            while (ci < count && l.n < (u32)kMaxNttDesc) {
+                if (!two_pass(ci)) {
+                    ++ci;
+                    lt = 0;
+                    continue;
+                }
+                const NttCall &c = calls[ci];
+                const size_t total_lt = c.n_polys * c.n_limbs, N = c.plan->degree;
+                const size_t room = (cap - used) / N;
+                if (room == 0) break;
+                const size_t take = std::min(room, total_lt - lt);
+                DevNttDesc d = base_desc(c);
+                d.block_start = blocks;
+                d.lt_begin = (u32)lt;
+                d.tmp = tmp + used;
+                l.d[l.n++] = d;
+                blocks += (u32)(take * ntt_tiles(d.logn));
+                used += take * N;
+                lt += take;
+                if (lt >= total_lt) {
+                    ++ci;
+                    lt = 0;
+                }
+            }
+            launch_pass(s, forward, true, l, blocks);
+            launch_pass(s, forward, false, l, blocks);
+        }
     }
     CK_HIP(hipGetLastError());
     return CKKS_OK;

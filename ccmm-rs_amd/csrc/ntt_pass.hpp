@@ -29,40 +29,39 @@ struct BlockExec {
 };
 #endif
 
-template <int L, int CL, PassKind KIND, bool FWD>
+template <int L, int CL, PassKind KIND, bool FWD, bool FP>
 struct NttPass {
     typedef TileShape<L, CL> S;
     static constexpr int NR = n_rounds(L);
 
-    // geometry of round i; first/last rounds choose the lane walk that makes
-    // their HBM side contiguous.
+    // geometry of round i.  Column passes: lanes walk columns (contiguous in
+    // HBM, and every column shares its twiddles).  Row passes and the full
+    // pass: lanes walk t, so the twiddles a wave loads are contiguous; the
+    // natural-order side of a row pass is staged through LDS (see run).
     static constexpr Geom geom(int i) {
         int k = round_k(L, i);
         int lo = FWD ? (L - round_start(L, i) - k) : round_start(L, i);
-        bool col_first = false;
-        if (KIND == PASS_COL) {
-            col_first = true;
-        } else if (KIND == PASS_ROW) {
-            // forward: internal side (read) is t-contiguous, natural side
-            // (write) col-contiguous; inverse the other way round.
-            if (FWD) col_first = (i == NR - 1);
-            else col_first = (i == 0);
-        } else {
-            col_first = false;  // PASS_FULL: single column, lanes walk t
-        }
-        return make_geom(L, CL, lo, k, col_first);
+        return make_geom(L, CL, lo, k, KIND == PASS_COL);
+    }
+
+    // Linear staging order for LDS <-> HBM copies: x = i * THREADS + tid.
+    // Row passes map it so lanes walk columns (the natural side is
+    // contiguous along columns); the full pass maps it to the natural index.
+    CK_HD static u32 stage_coord(u32 x) {
+        if (KIND == PASS_ROW) return (x >> CL) | ((x & ((1u << CL) - 1)) << L);
+        return x;
     }
 
     template <int I>
-    CK_HD static void do_round(u64 (&v)[16], u32 tid, const PassAddr &pa, const Tw *tw, u64 q) {
+    CK_HD static void do_round(u64 (&v)[16], u32 tid, const PassAddr &pa, const TileJob &job) {
         constexpr Geom g = geom(I);
         constexpr int K = round_k(L, I);
         constexpr int s0 = round_start(L, I);
         constexpr bool row = (KIND == PASS_ROW);
         if (FWD)
-            fwd_round<L, CL, K>(v, g, tid, s0, 1u, row, pa, tw, q);
+            fwd_round<L, CL, K, FP>(v, g, tid, s0, row, pa, job);
         else
-            inv_round<L, CL, K>(v, g, tid, s0, 1u, row, pa, tw, q);
+            inv_round<L, CL, K, FP>(v, g, tid, s0, row, pa, job);
     }
 
     // global offset (within the limb) of the element at tile coordinate e on
@@ -82,6 +81,14 @@ struct NttPass {
 
     // final pass of the whole transform: canonicalise (and scale by N^-1).
     static constexpr bool FINAL = (KIND == PASS_FULL) || (FWD ? KIND == PASS_ROW : KIND == PASS_COL);
+    // first pass: reads canonical residues (the FP64 path converts them;
+    // between passes the FP64 path keeps its lazy doubles as raw bits).
+    static constexpr bool INITIAL = (KIND == PASS_FULL) || (FWD ? KIND == PASS_COL : KIND == PASS_ROW);
+
+    CK_HD static u64 load_conv(u64 x) {
+        if (FP && INITIAL) return f64_as_u64((double)x);
+        return x;
+    }
 
     template <int I>
     CK_HD static void exchange_out(u64 (&v)[16], u32 tid, u64 *lds) {
@@ -97,6 +104,11 @@ struct NttPass {
     }
 
     CK_HD static u64 finish(u64 x, const TileJob &job) {
+        if (FP) {
+            double d = u64_as_f64(x);
+            if (!FWD) d = fp_mulmod(d, job.ninv_d, job.ninv_p, job.qd);
+            return fp_canonical(d, job.qd, job.qinv);
+        }
         u64 q = job.q;
         if (FWD) {
             u64 two_q = q << 1;
@@ -118,15 +130,26 @@ struct NttPass {
         const u32 n = job.logn;
 
         // ---- load
-        if (KIND == PASS_FULL && !FWD) {
-            // natural input x goes to internal position bitrev(x)
+        // staged: the full inverse pass (natural x -> internal bitrev(x)) and
+        // the inverse row pass (natural side, lanes walk columns)
+        constexpr bool STAGED_IN = !FWD && (KIND == PASS_FULL || KIND == PASS_ROW);
+        constexpr bool STAGED_OUT = FWD && (KIND == PASS_FULL || KIND == PASS_ROW);
+        if (STAGED_IN) {
             ex.run([&](u32 tid) {
                 if (tid >= T) return;
 #pragma unroll
                 for (int i = 0; i < 16; ++i) {
                     u32 x = (u32)i * T + tid;
-                    CK_BOUNDS(x, 1u << n);
-                    lds[lds_index<L, CL>(bitrev(x, n))] = job.src[x];
+                    u32 e, off;
+                    if (KIND == PASS_FULL) {
+                        e = bitrev(x, n);
+                        off = x;
+                    } else {
+                        e = stage_coord(x);
+                        off = row_natural_offset<L, CL>(pa, e);
+                    }
+                    CK_BOUNDS(off, 1u << n);
+                    lds[lds_index<L, CL>(e)] = load_conv(job.src[off]);
                 }
             });
             ex.barrier();
@@ -143,7 +166,7 @@ struct NttPass {
                 for (int i = 0; i < 16; ++i) {
                     u32 off = read_off(pa, coord(g, tid, i));
                     CK_BOUNDS(off, 1u << n);
-                    v[i] = job.src[off];
+                    v[i] = load_conv(job.src[off]);
                 }
             });
         }
@@ -152,16 +175,19 @@ struct NttPass {
         rounds<0>(ex, regs_of, pa, job, lds);
 
         // ---- store
-        if (KIND == PASS_FULL && FWD) {
-            // internal position p holds X_bitrev(p)
+        if (STAGED_OUT) {
             ex.barrier();
             ex.run([&](u32 tid) {
                 if (tid >= T) return;
                 constexpr Geom g = geom(NR - 1);
                 u64(&v)[16] = regs_of(tid);
 #pragma unroll
-                for (int i = 0; i < 16; ++i)
-                    lds[lds_index<L, CL>(bitrev(coord(g, tid, i), n))] = v[i];
+                for (int i = 0; i < 16; ++i) {
+                    u32 e = coord(g, tid, i);
+                    // full pass: internal position p holds X_bitrev(p)
+                    if (KIND == PASS_FULL) e = bitrev(e, n);
+                    lds[lds_index<L, CL>(e)] = v[i];
+                }
             });
             ex.barrier();
             ex.run([&](u32 tid) {
@@ -169,8 +195,10 @@ struct NttPass {
 #pragma unroll
                 for (int i = 0; i < 16; ++i) {
                     u32 x = (u32)i * T + tid;
-                    CK_BOUNDS(x, 1u << n);
-                    job.dst[x] = finish(lds[lds_index<L, CL>(x)], job);
+                    u32 e = stage_coord(x);
+                    u32 off = KIND == PASS_FULL ? x : row_natural_offset<L, CL>(pa, e);
+                    CK_BOUNDS(off, 1u << n);
+                    job.dst[off] = finish(lds[lds_index<L, CL>(e)], job);
                 }
             });
         } else {
@@ -208,7 +236,7 @@ struct NttPass {
             }
             ex.run([&](u32 tid) {
                 if (tid >= T) return;
-                do_round<I>(regs_of(tid), tid, pa, job.tw, job.q);
+                do_round<I>(regs_of(tid), tid, pa, job);
             });
             rounds<I + 1>(ex, regs_of, pa, job, lds);
         }
@@ -224,12 +252,12 @@ CK_HD u32 ntt_tiles(u32 logn) { return logn > 12 ? (1u << (logn - 12)) : 1u; }
 
 // Dispatch on log N to the instantiated pass.  first = first pass of the
 // transform (forward: column/full, inverse: row/full).
-template <bool FWD, bool FIRST, class Exec, class Regs>
+template <bool FWD, bool FIRST, bool FP, class Exec, class Regs>
 CK_HD void ntt_dispatch(Exec &ex, Regs &&regs_of, const TileJob &job, u64 *lds) {
     switch (job.logn) {
 #define CK_FULL(LG)                                                   \
     case LG:                                                          \
-        if (FIRST) NttPass<LG, 0, PASS_FULL, FWD>::run(ex, regs_of, job, lds); \
+        if (FIRST) NttPass<LG, 0, PASS_FULL, FWD, FP>::run(ex, regs_of, job, lds); \
         break;
         CK_FULL(10)
         CK_FULL(11)
@@ -239,9 +267,9 @@ CK_HD void ntt_dispatch(Exec &ex, Regs &&regs_of, const TileJob &job, u64 *lds) 
     case LG: {                                                                              \
         constexpr int N1 = LG - (LG + 1) / 2, N2 = (LG + 1) / 2;                            \
         if (FWD == FIRST)                                                                   \
-            NttPass<N1, 12 - N1, PASS_COL, FWD>::run(ex, regs_of, job, lds);                \
+            NttPass<N1, 12 - N1, PASS_COL, FWD, FP>::run(ex, regs_of, job, lds);            \
         else                                                                                \
-            NttPass<N2, 12 - N2, PASS_ROW, FWD>::run(ex, regs_of, job, lds);                \
+            NttPass<N2, 12 - N2, PASS_ROW, FWD, FP>::run(ex, regs_of, job, lds);            \
         break;                                                                              \
     }
         CK_TWO(13)

@@ -58,14 +58,21 @@ struct TileJob {
     const Tw *tw;    // forward or inverse twiddle table (index 1..N-1)
     u64 q;
     u64 ninv, ninv_q;  // N^-1 and its Shoup quotient (inverse only)
+    // FP64 path (q < 2^42): twiddle entries are (double w, double w/q)
+    double qd, qinv;    // q and 1/q as doubles
+    double ninv_d, ninv_p;  // N^-1 and N^-1/q as doubles
     u32 logn;
     u32 tile;  // tile index within the limb transform
 };
 
 CK_HD u32 bitrev(u32 x, u32 bits) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return bits ? __builtin_bitreverse32(x) >> (32 - bits) : 0u;  // v_bfrev_b32
+#else
     u32 r = 0;
     for (u32 i = 0; i < bits; ++i) r |= ((x >> i) & 1u) << (bits - 1 - i);
     return r;
+#endif
 }
 
 // ---- compile-time round geometry ------------------------------------------
@@ -168,6 +175,23 @@ CK_HD void gs_bfly(u64 &x, u64 &y, const Tw &w, u64 q) {
     y = shoup_lazy(d, w.w, w.wq, q);
 }
 
+// FP64 butterflies on bit-cast doubles (see fp_mulmod).  Forward: no
+// reduction; |x| grows by at most 0.51 q per stage, so after 17 stages it is
+// below 10 q < 2^46 (q < 2^42).  Inverse: the sum path is reduced each stage
+// so both outputs stay within 0.51 q.
+CK_HD void ct_bfly_fp(u64 &x, u64 &y, const Tw &w, double q) {
+    double a = u64_as_f64(x);
+    double t = fp_mulmod(u64_as_f64(y), u64_as_f64(w.w), u64_as_f64(w.wq), q);
+    x = f64_as_u64(a + t);
+    y = f64_as_u64(a - t);
+}
+
+CK_HD void gs_bfly_fp(u64 &x, u64 &y, const Tw &w, double q, double qinv) {
+    double a = u64_as_f64(x), b = u64_as_f64(y);
+    x = f64_as_u64(fp_reduce(a + b, q, qinv));
+    y = f64_as_u64(fp_mulmod(a - b, u64_as_f64(w.w), u64_as_f64(w.wq), q));
+}
+
 // ---- per-pass element addressing ----------------------------------------------
 
 // Global element offsets (within the limb) for tile coordinate e.
@@ -212,9 +236,9 @@ CK_HD u32 row_twiddle_base(const PassAddr &pa, u32 e) {
 // ---- one round of butterflies on the 16 registers -------------------------------
 
 // Forward round: stages s0..s0+k-1 (butterfly bit of stage s is L-1-s).
-template <int L, int CL, int K>
-CK_HD void fwd_round(u64 (&v)[16], const Geom &g, u32 tid, int s0, u32 base_col_const,
-                     bool row_base, const PassAddr &pa, const Tw *tw, u64 q) {
+template <int L, int CL, int K, bool FP>
+CK_HD void fwd_round(u64 (&v)[16], const Geom &g, u32 tid, int s0, bool row_base,
+                     const PassAddr &pa, const TileJob &job) {
 #pragma unroll
     for (int j = 0; j < K; ++j) {
         const int s = s0 + j;
@@ -224,19 +248,22 @@ CK_HD void fwd_round(u64 (&v)[16], const Geom &g, u32 tid, int s0, u32 base_col_
             if (i & (1 << rb)) continue;
             u32 e = coord(g, tid, i);
             u32 t = e & ((1u << L) - 1);
-            u32 base = row_base ? row_twiddle_base<L, CL>(pa, e) : base_col_const;
+            u32 base = row_base ? row_twiddle_base<L, CL>(pa, e) : 1u;
             u32 idx = (base << s) + (t >> (L - s));
             CK_BOUNDS(idx, 1u << pa.logn);
-            Tw w = tw[idx];
-            ct_bfly(v[i], v[i | (1 << rb)], w, q);
+            Tw w = job.tw[idx];
+            if (FP)
+                ct_bfly_fp(v[i], v[i | (1 << rb)], w, job.qd);
+            else
+                ct_bfly(v[i], v[i | (1 << rb)], w, job.q);
         }
     }
 }
 
 // Inverse round: GS stages u0..u0+k-1 (butterfly bit of stage u is u).
-template <int L, int CL, int K>
-CK_HD void inv_round(u64 (&v)[16], const Geom &g, u32 tid, int u0, u32 base_col_const,
-                     bool row_base, const PassAddr &pa, const Tw *tw, u64 q) {
+template <int L, int CL, int K, bool FP>
+CK_HD void inv_round(u64 (&v)[16], const Geom &g, u32 tid, int u0, bool row_base,
+                     const PassAddr &pa, const TileJob &job) {
 #pragma unroll
     for (int j = 0; j < K; ++j) {
         const int u = u0 + j;
@@ -246,11 +273,14 @@ CK_HD void inv_round(u64 (&v)[16], const Geom &g, u32 tid, int u0, u32 base_col_
             if (i & (1 << rb)) continue;
             u32 e = coord(g, tid, i);
             u32 t = e & ((1u << L) - 1);
-            u32 base = row_base ? row_twiddle_base<L, CL>(pa, e) : base_col_const;
+            u32 base = row_base ? row_twiddle_base<L, CL>(pa, e) : 1u;
             u32 idx = (base << (L - 1 - u)) + (t >> (u + 1));
             CK_BOUNDS(idx, 1u << pa.logn);
-            Tw w = tw[idx];
-            gs_bfly(v[i], v[i | (1 << rb)], w, q);
+            Tw w = job.tw[idx];
+            if (FP)
+                gs_bfly_fp(v[i], v[i | (1 << rb)], w, job.qd, job.qinv);
+            else
+                gs_bfly(v[i], v[i | (1 << rb)], w, job.q);
         }
     }
 }

@@ -64,6 +64,11 @@ int ckks_context_destroy(ckks_context *ctx);
 int ckks_context_set_stream(ckks_context *ctx, void *stream);
 int ckks_sync(ckks_context *ctx);
 
+/* Tuning: bytes of the NTT inter-pass buffer per chunk of a batched
+ * transform (default 96 MiB, sized to stay in the Infinity Cache; 0 = one
+ * chunk).  Results do not depend on it.  No reference counterpart. */
+int ckks_set_ntt_chunk_bytes(ckks_context *ctx, size_t bytes);
+
 /* Device memory helpers for callers without their own allocator. */
 int ckks_malloc(ckks_context *ctx, size_t bytes, void **dev);
 int ckks_free(ckks_context *ctx, void *dev);

@@ -11,7 +11,7 @@
 
 using namespace ck;
 
-template <bool FWD, bool FIRST>
+template <bool FWD, bool FIRST, bool FP>
 static void run_pass(const TileJob &base, u32 tiles) {
     static u64 regs[256][16];
     static u64 lds[4096];
@@ -21,7 +21,7 @@ static void run_pass(const TileJob &base, u32 tiles) {
         BlockExec ex{256};
         std::memset(lds, 0xA5, sizeof lds);
         auto regs_of = [&](u32 tid) -> u64(&)[16] { return regs[tid]; };
-        ntt_dispatch<FWD, FIRST>(ex, regs_of, job, lds);
+        ntt_dispatch<FWD, FIRST, FP>(ex, regs_of, job, lds);
     }
 }
 
@@ -31,7 +31,11 @@ extern "C" int emu_ntt(const uint64_t *in, uint64_t *out, uint64_t q, uint32_t l
     uint64_t psi = host::negacyclic_root(q, 1ull << logn);
     if (!psi) return -2;
     std::vector<uint64_t> tw;
-    host::twiddle_table(q, forward ? psi : host::invmod(psi, q), logn, tw);
+    const bool fp = q < host::kFpModulusBound;
+    if (fp)
+        host::twiddle_table_fp(q, forward ? psi : host::invmod(psi, q), logn, tw);
+    else
+        host::twiddle_table(q, forward ? psi : host::invmod(psi, q), logn, tw);
     uint32_t n = 1u << logn;
     std::vector<uint64_t> tmp(n, 0xDEADBEEFull), src(in, in + n);
     TileJob job{};
@@ -39,17 +43,26 @@ extern "C" int emu_ntt(const uint64_t *in, uint64_t *out, uint64_t q, uint32_t l
     job.q = q;
     job.ninv = host::invmod(n % q, q);
     job.ninv_q = host::shoup_quotient(job.ninv, q);
+    job.qd = (double)q;
+    job.qinv = 1.0 / (double)q;
+    job.ninv_d = (double)job.ninv;
+    job.ninv_p = (double)job.ninv / (double)q;
     job.logn = logn;
     bool two = ntt_two_pass(logn);
     job.src = src.data();
     job.dst = two ? tmp.data() : out;
-    if (forward) run_pass<true, true>(job, ntt_tiles(logn));
-    else run_pass<false, true>(job, ntt_tiles(logn));
+#define CK_RUN(F, S)                                              \
+    do {                                                          \
+        if (fp) run_pass<F, S, true>(job, ntt_tiles(logn));       \
+        else run_pass<F, S, false>(job, ntt_tiles(logn));         \
+    } while (0)
+    if (forward) CK_RUN(true, true);
+    else CK_RUN(false, true);
     if (two) {
         job.src = tmp.data();
         job.dst = out;
-        if (forward) run_pass<true, false>(job, ntt_tiles(logn));
-        else run_pass<false, false>(job, ntt_tiles(logn));
+        if (forward) CK_RUN(true, false);
+        else CK_RUN(false, false);
     }
     return 0;
 }

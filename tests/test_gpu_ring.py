@@ -92,6 +92,33 @@ def test_mixed_degree_batch_single_launch(pkg, ctx):
     assert (pkg.to_host(o12) == x12).all()
 
 
+@pytest.mark.parametrize("chunk_limbs", [1, 3, 7])
+def test_chunked_two_pass_batches(pkg, ctx, chunk_limbs):
+    """Inter-pass chunks that cut entries and limbs at arbitrary places
+    (tiny chunks force every boundary case); results must not change."""
+    rng = np.random.default_rng(20 + chunk_limbs)
+    q = MODULI[:6]
+    p16, p15, p13 = (_plan(pkg, ctx, 1 << k, q) for k in (16, 15, 13))
+    xs = [rand_residues(rng, q[:5], 1 << 16, polys=2), rand_residues(rng, q[:3], 1 << 15, polys=3),
+          rand_residues(rng, q[:6], 1 << 13, polys=2)]
+    plans = [(p16, 5, 1 << 16), (p15, 3, 1 << 15), (p13, 6, 1 << 13)]
+    ds = [pkg.to_device(x) for x in xs]
+    outs = [pkg.empty(x.shape) for x in xs]
+    try:
+        ctx.set_ntt_chunk_bytes(chunk_limbs * (1 << 16) * 8)
+        pkg.ntt_batch(ctx, [(p, d, o, L) for (p, L, _), d, o in zip(plans, ds, outs)], forward=True)
+        got = [pkg.to_host(o) for o in outs]
+        pkg.ntt_batch(ctx, [(p, o, None, L) for (p, L, _), o in zip(plans, outs)], forward=False)
+        back = [pkg.to_host(o) for o in outs]
+    finally:
+        ctx.set_ntt_chunk_bytes(96 << 20)
+    for x, g, b, (_, L, n) in zip(xs, got, back, plans):
+        orc = O.RnsPlan(q[:L], n)
+        for p in range(x.shape[0]):
+            assert (g[p] == orc.forward(x[p])).all()
+        assert (b == x).all()
+
+
 @pytest.mark.parametrize("logn", [12, 16])
 def test_pointwise_ops(pkg, ctx, logn):
     n = 1 << logn
