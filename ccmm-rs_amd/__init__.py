@@ -56,6 +56,7 @@ _SIGNATURES = {
     "ckks_context_set_stream": (C.c_int, [_p, _p]),
     "ckks_sync": (C.c_int, [_p]),
     "ckks_set_ntt_chunk_bytes": (C.c_int, [_p, _sz]),
+    "ckks_set_base_conversion": (C.c_int, [_p, C.c_int]),
     "ckks_malloc": (C.c_int, [_p, _sz, C.POINTER(_p)]),
     "ckks_free": (C.c_int, [_p, _p]),
     "ckks_upload": (C.c_int, [_p, _p, _p, _sz]),
@@ -193,6 +194,12 @@ class Context:
     def set_ntt_chunk_bytes(self, nbytes: int):
         """NTT inter-pass buffer per chunk (tuning only; results unchanged)."""
         _check(lib().ckks_set_ntt_chunk_bytes(self._h, int(nbytes)))
+
+    BASECONV_AUTO, BASECONV_VALU = 0, 1
+
+    def set_base_conversion(self, mode: int):
+        """Hybrid ModUp/ModDown engine: 0 = int8 MFMA where possible, 1 = VALU."""
+        _check(lib().ckks_set_base_conversion(self._h, int(mode)))
 
     def close(self):
         if getattr(self, "_h", None):

@@ -104,6 +104,14 @@ int ckks_set_ntt_chunk_bytes(ckks_context *ctx, size_t bytes) {
     return CKKS_OK;
 }
 
+int ckks_set_base_conversion(ckks_context *ctx, int mode) {
+    if (!ctx) return fail(CKKS_ERR_INVALID, "null context");
+    if (mode != CKKS_BASECONV_AUTO && mode != CKKS_BASECONV_VALU)
+        return fail(CKKS_ERR_INVALID, "unknown base-conversion mode");
+    ctx->base_conversion = mode;
+    return CKKS_OK;
+}
+
 int ckks_malloc(ckks_context *ctx, size_t bytes, void **dev) {
     if (!ctx || !dev) return fail(CKKS_ERR_INVALID, "null argument");
     hipError_t e = hipMalloc(dev, bytes ? bytes : 1);

@@ -73,8 +73,10 @@ struct ckks_context {
     static constexpr int kSlots = 8;
     void *ws[kSlots] = {};
     size_t ws_bytes[kSlots] = {};
-    // NTT inter-pass buffer per chunk in bytes (0 = whole batch in one chunk)
-    size_t ntt_chunk_bytes = 96u << 20;
+    // NTT inter-pass buffer per chunk in bytes (0 = whole batch in one chunk;
+    // measured on MI355X: chunks sized for the Infinity Cache gave no gain)
+    size_t ntt_chunk_bytes = 0;
+    int base_conversion = CKKS_BASECONV_AUTO;
     std::vector<void *> owned;  // misc device allocations freed at destroy
 
     int workspace(int slot, size_t bytes, void **out);

@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <cstring>
 
+#include "baseconv.hpp"
 #include "host_math.hpp"
 #include "internal.hpp"
 
@@ -64,6 +65,10 @@ struct ckks_ksk {
     std::vector<ck::ModUpBlock> h_modup;
     ck::u64 *d_consts = nullptr;
     ck::ModDownTable *d_moddown = nullptr;
+    // exact int8-MFMA base conversion: dnum ModUp plans then the ModDown plan
+    int mfma = 0;
+    ck::BcPlan *d_bc = nullptr;
+    signed char *d_bt = nullptr;
 };
 
 namespace ck {
@@ -332,7 +337,282 @@ __global__ __launch_bounds__(kKsThreads) void k_moddown(DownArgs a) {
     }
 }
 
+// ---- exact base conversion on the int8 matrix cores ----------------------------------
+//
+// ModUp and ModDown as one GEMM each (see baseconv.hpp): the VALU computes
+// the mixed-radix digits of every coefficient and writes their 7-bit chunks
+// (9 per digit, row k = 9m + a) to LDS; v_mfma_i32_32x32x32_i8 multiplies
+// them by the constant chunk matrix; the epilogue recombines
+// sum_b E_b 2^(7b) and reduces it once per target.  256 coefficients per
+// workgroup, each wave owns 64 of them (two 32-row M-tiles).
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+constexpr int bc_krows(int A) { return ((9 * A + 31) / 32) * 32; }
+
+// Chunk row of one coefficient: byte k = 9m + a is bits [7a, 7a+7) of c_m.
+template <int A>
+__device__ __forceinline__ void bc_store_row(const u64 (&c)[A], signed char *row) {
+    constexpr int KR = bc_krows(A);
+#pragma unroll
+    for (int w = 0; w < KR / 4; ++w) {
+        u32 word = 0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int k = 4 * w + i;
+            if (k < 9 * A) {
+                const int m = k / 9, sh = 7 * (k % 9);
+                word |= (u32)((c[m] >> sh) & 127u) << (8 * i);
+            }
+        }
+        reinterpret_cast<u32 *>(row)[w] = word;
+    }
+}
+
+// One column tile for the wave's 64 rows: E tile -> per-wave LDS staging.
+template <int KR>
+__device__ __forceinline__ void bc_gemm_tile(const signed char *lds_a, const signed char *bt, u32 tile,
+                                             u32 wave, u32 lane, int *lds_e) {
+    v16i acc0 = {}, acc1 = {};
+    const u32 r = lane & 31, kh = (lane >> 5) * 16;
+#pragma unroll
+    for (int ks = 0; ks < KR / 32; ++ks) {
+        const u32 k0 = (u32)ks * 32 + kh;
+        v4i a0 = *reinterpret_cast<const v4i *>(lds_a + (wave * 64 + r) * KR + k0);
+        v4i a1 = *reinterpret_cast<const v4i *>(lds_a + (wave * 64 + 32 + r) * KR + k0);
+        v4i b = *reinterpret_cast<const v4i *>(bt + (tile * kBcTileCols + r) * KR + k0);
+        acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b, acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b, acc1, 0, 0, 0);
+    }
+    // accumulator i of lane l holds row (i%4) + 8(i/4) + 4(l/32), column l%32
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const u32 row = (i & 3) + 8 * (i >> 2) + 4 * (lane >> 5);
+        lds_e[row * 33 + r] = acc0[i];
+        lds_e[(row + 32) * 33 + r] = acc1[i];
+    }
+}
+
+// sum_b E_b 2^(7b) for the chunk columns [col, col+nch) of one row, as a
+// 128-bit (hi, lo) with hi < 2^15.
+__device__ __forceinline__ void bc_combine(const int *e_row, u32 col, u32 nch, u64 &hi, u64 &lo) {
+    u64 g0 = 0, g1 = 0;
+    for (u32 b = 0; b < nch; ++b) {
+        u64 e = (u64)(u32)e_row[col + b];
+        if (b < 6) g0 += e << (7 * b);
+        else g1 += e << (7 * (b - 6));
+    }
+    const u64 add_lo = g1 << 42;
+    lo = g0 + add_lo;
+    hi = (g1 >> 22) + (lo < add_lo ? 1u : 0u);
+}
+
+struct ModUpMfmaArgs {
+    ModUpArgs base;
+    const BcPlan *plans;  // one per gadget block
+};
+
+template <int A>
+__device__ __forceinline__ void modup_mfma_body(const ModUpMfmaArgs &args, signed char *lds_a,
+                                                int (*lds_e)[64 * 33]) {
+    constexpr int KR = bc_krows(A);
+    const ModUpArgs &a = args.base;
+    const u32 N = 1u << a.logn;
+    const u32 chunks = N / kBcRows;
+    const u32 b = blockIdx.x;
+    const u32 chunk = b % chunks, cj = b / chunks;
+    const u32 j = cj % a.dnum, ct = cj / a.dnum;
+    const ModUpBlock &blk = a.blocks[j];
+    const BcPlan &bp = args.plans[j];
+    const u32 tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const u64 n = (u64)chunk * kBcRows + tid;
+    u64 *dst = a.digits + (((u64)ct * a.dnum + j) * a.limbs) * N;
+    // ---- digits (VALU): gather sigma_k, Garner, in-block limbs, chunk row
+    {
+        const Gather gi = galois_gather(n, a.kinv, a.logn);
+        const u64 *src = a.in + ct * a.in_stride + (u64)blk.start * N + gi.idx;
+        u64 x[A], c[A];
+#pragma unroll
+        for (int m = 0; m < A; ++m) {
+            u64 v = src[(u64)m * N];
+            x[m] = gi.neg ? neg_mod(v, a.primes[blk.start + m].q) : v;
+            dst[(u64)(blk.start + m) * N + n] = x[m];  // digit_j mod q_i = x_i inside the block
+        }
+#pragma unroll
+        for (int k = 0; k < A; ++k) {
+            const DevPrime &Pk = a.primes[blk.start + k];
+            if (k == 0) {
+                c[0] = x[0];
+            } else {
+                Acc128 t;
+                t.zero();
+#pragma unroll
+                for (int m = 0; m < k; ++m) t.mac(c[m], blk.pm[k][m]);
+                c[k] = shoup(sub_mod(x[k], reduce128(t, Pk), Pk.q), blk.inv[k], blk.inv_q[k], Pk.q);
+            }
+        }
+        bc_store_row<A>(c, lds_a + tid * KR);
+    }
+    __syncthreads();
+    // ---- GEMM per column tile, then the epilogue for its targets
+    int *e = lds_e[wave];
+    const u64 row_n = (u64)chunk * kBcRows + wave * 64 + lane;
+    for (u32 tile = 0; tile < bp.n_tiles; ++tile) {
+        bc_gemm_tile<KR>(lds_a, bp.bt, tile, wave, lane, e);
+        __syncthreads();
+        const BcTile &T = bp.tiles[tile];
+        for (u32 t = 0; t < T.n_targets; ++t) {
+            const BcTarget tg = T.t[t];
+            const DevPrime &Pt = a.primes[tg.limb];
+            u64 hi, lo;
+            bc_combine(e + lane * 33, tg.col, tg.nch, hi, lo);
+            dst[(u64)tg.limb * N + row_n] = barrett_reduce128(hi, lo, Barrett{Pt.q, Pt.br_hi, Pt.br_lo});
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_modup_mfma(ModUpMfmaArgs args) {
+    __shared__ __align__(16) signed char lds_a[kBcRows * kBcMaxK];
+    __shared__ int lds_e[4][64 * 33];
+    const u32 N = 1u << args.base.logn;
+    const u32 j = (blockIdx.x / (N / kBcRows)) % args.base.dnum;
+    switch (args.base.blocks[j].size) {
+#define CK_MM(A) \
+    case A: modup_mfma_body<A>(args, lds_a, lds_e); break;
+        CK_MM(2) CK_MM(3) CK_MM(4) CK_MM(5) CK_MM(6) CK_MM(7)
+        CK_MM(8) CK_MM(9)
+#undef CK_MM
+        default: break;
+    }
+}
+
+struct ModDownMfmaArgs {
+    DownArgs base;
+    const BcPlan *plan;  // P digits -> Q limbs
+};
+
+template <int K>
+__global__ __launch_bounds__(256) void k_moddown_mfma(ModDownMfmaArgs args) {
+    constexpr int KR = bc_krows(K);
+    __shared__ __align__(16) signed char lds_a[kBcRows * KR];
+    __shared__ int lds_e[4][64 * 33];
+    __shared__ unsigned char lds_neg[kBcRows];
+    const DownArgs &a = args.base;
+    const BcPlan &bp = *args.plan;
+    const ModDownTable &md = *a.md;
+    const u32 N = 1u << a.logn;
+    const u32 chunks = N / kBcRows;
+    const u32 b = blockIdx.x;
+    const u32 chunk = b % chunks, rest = b / chunks;
+    const u32 comp = rest % 2, ct = rest / 2;
+    const u32 tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const u64 *x = a.acc + ((u64)ct * 2 + comp) * a.limbs * N;
+    // ---- P-side mixed-radix digits and the centering decision (VALU)
+    {
+        const u64 n = (u64)chunk * kBcRows + tid;
+        u64 c[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const DevPrime &Pk = a.primes[a.q_limbs + k];
+            const u64 xk = x[(u64)(a.q_limbs + k) * N + n];
+            if (k == 0) {
+                c[0] = xk;
+            } else {
+                Acc128 t;
+                t.zero();
+#pragma unroll
+                for (int m = 0; m < k; ++m) t.mac(c[m], md.pm[k][m]);
+                c[k] = shoup(sub_mod(xk, reduce128(t, Pk), Pk.q), md.inv[k], md.inv_q[k], Pk.q);
+            }
+        }
+        int cmp = 0;
+#pragma unroll
+        for (int k = K - 1; k >= 0; --k)
+            if (cmp == 0 && c[k] != md.half[k]) cmp = c[k] > md.half[k] ? 1 : -1;
+        lds_neg[tid] = cmp > 0 ? 1 : 0;
+        bc_store_row<K>(c, lds_a + tid * KR);
+    }
+    __syncthreads();
+    const u64 *add = comp == 0 ? a.add0 : a.add1;
+    const u64 row_n = (u64)chunk * kBcRows + wave * 64 + lane;
+    const Gather ga = galois_gather(row_n, comp == 0 ? a.kinv : 0, a.logn);
+    if (add) add += ct * a.add_stride + ga.idx;
+    u64 *o = a.out + ct * a.out_stride + (u64)comp * a.q_limbs * N + row_n;
+    const bool neg = lds_neg[wave * 64 + lane] != 0;
+    int *e = lds_e[wave];
+    for (u32 tile = 0; tile < bp.n_tiles; ++tile) {
+        bc_gemm_tile<KR>(lds_a, bp.bt, tile, wave, lane, e);
+        __syncthreads();
+        const BcTile &T = bp.tiles[tile];
+        for (u32 t = 0; t < T.n_targets; ++t) {
+            const BcTarget tg = T.t[t];
+            const u32 i = tg.limb;
+            const DevPrime &Pi = a.primes[i];
+            u64 hi, lo;
+            bc_combine(e + lane * 33, tg.col, tg.nch, hi, lo);
+            u64 r = barrett_reduce128(hi, lo, Barrett{Pi.q, Pi.br_hi, Pi.br_lo});  // x_P mod q_i
+            if (neg) r = sub_mod(r, md.pmod[i], Pi.q);
+            u64 v = shoup(sub_mod(x[(u64)i * N + row_n], r, Pi.q), md.pinv[i], md.pinv_q[i], Pi.q);
+            if (add) {
+                u64 ad = add[(u64)i * N];
+                v = ga.neg ? sub_mod(v, ad, Pi.q) : add_mod(v, ad, Pi.q);
+            }
+            o[(u64)i * N] = v;
+        }
+        __syncthreads();
+    }
+}
+
 // ---- host-side setup ----------------------------------------------------------------
+
+// Constant side of an exact base conversion from `src` digits to `targets`
+// (pairs of destination limb index and modulus): targets packed whole into
+// 32-column tiles; B^T[column][k], k = 9m + a, column = tile*32 + col + b,
+// entry = chunk_b((2^(7a) M_m) mod p_t), M_m = prod_{l<m} src_l.
+static bool build_bc_plan(const std::vector<u64> &src, const std::vector<std::pair<u32, u64>> &targets,
+                          BcPlan &plan, std::vector<signed char> &bt) {
+    const int A = (int)src.size();
+    if (A < 1 || A > 12) return false;
+    plan = BcPlan{};
+    plan.k_rows = (u32)bc_krows(A);
+    u32 tile = 0, col = 0;
+    for (auto &t : targets) {
+        u32 nch = (u32)chunks7(t.second);
+        if (col + nch > (u32)kBcTileCols || plan.tiles[tile].n_targets == (u32)kBcMaxPerTile) {
+            ++tile;
+            col = 0;
+        }
+        if (tile >= (u32)kBcMaxTiles) return false;
+        BcTile &T = plan.tiles[tile];
+        T.t[T.n_targets++] = BcTarget{t.first, col, nch, 0};
+        col += nch;
+    }
+    plan.n_tiles = targets.empty() ? 0 : tile + 1;
+    const u32 KR = plan.k_rows;
+    bt.assign((size_t)plan.n_tiles * kBcTileCols * KR, 0);
+    for (u32 ti = 0; ti < plan.n_tiles; ++ti) {
+        const BcTile &T = plan.tiles[ti];
+        for (u32 e = 0; e < T.n_targets; ++e) {
+            u64 p = 0;
+            for (auto &t : targets)
+                if (t.first == T.t[e].limb) p = t.second;
+            u64 Mm = 1 % p;  // M_m mod p
+            for (int m = 0; m < A; ++m) {
+                u64 v = Mm;  // (2^(7a) M_m) mod p for a = 0
+                for (int a = 0; a < 9; ++a) {
+                    for (u32 b = 0; b < T.t[e].nch; ++b)
+                        bt[((size_t)ti * kBcTileCols + T.t[e].col + b) * KR + 9 * m + a] =
+                            (signed char)((v >> (7 * b)) & 127u);
+                    v = host::mulmod(v, 128 % p, p);
+                }
+                Mm = host::mulmod(Mm, src[m] % p, p);
+            }
+        }
+    }
+    return true;
+}
 
 static void fill_garner(const std::vector<u64> &mods, u64 (*pm)[kMaxAlpha], u64 *inv, u64 *inv_q,
                         int stride_dummy) {
@@ -470,6 +750,45 @@ static int create_key(const ckks_rns_plan *plan, size_t q_limbs, const size_t *b
         CK_HIP(hipMalloc(&k->d_moddown, sizeof(ModDownTable)));
         CK_HIP(hipMemcpy(k->d_moddown, &md, sizeof(ModDownTable), hipMemcpyHostToDevice));
     }
+    // exact int8-MFMA base conversion for the hybrid (blocks of >= 2 limbs)
+    bool mfma = hybrid;
+    // (blocks of 2..9 limbs; larger blocks keep the VALU kernel, whose
+    // register footprint does not limit the MFMA kernel's occupancy)
+    for (size_t j = 0; j < n_blocks; ++j) mfma &= block_sizes[j] >= 2 && block_sizes[j] <= 9;
+    if (mfma) {
+        std::vector<BcPlan> plans(n_blocks + 1);
+        std::vector<std::vector<signed char>> bts(n_blocks + 1);
+        for (size_t j = 0; j <= n_blocks && mfma; ++j) {
+            std::vector<u64> src;
+            std::vector<std::pair<u32, u64>> tg;
+            if (j < n_blocks) {  // ModUp of block j: every other limb of Q u P
+                src.assign(mods.begin() + k->block_start[j], mods.begin() + k->block_start[j] + block_sizes[j]);
+                for (size_t i = 0; i < B; ++i)
+                    if (i < k->block_start[j] || i >= k->block_start[j] + block_sizes[j])
+                        tg.emplace_back((u32)i, mods[i]);
+            } else {  // ModDown: P digits onto every Q limb
+                src.assign(mods.begin() + q_limbs, mods.end());
+                for (size_t i = 0; i < q_limbs; ++i) tg.emplace_back((u32)i, mods[i]);
+            }
+            mfma &= build_bc_plan(src, tg, plans[j], bts[j]);
+        }
+        if (mfma) {
+            size_t total = 0;
+            std::vector<size_t> off(n_blocks + 1);
+            for (size_t j = 0; j <= n_blocks; ++j) {
+                off[j] = total;
+                total += (bts[j].size() + 255) / 256 * 256;
+            }
+            CK_HIP(hipMalloc(&k->d_bt, std::max<size_t>(total, 256)));
+            for (size_t j = 0; j <= n_blocks; ++j) {
+                CK_HIP(hipMemcpy(k->d_bt + off[j], bts[j].data(), bts[j].size(), hipMemcpyHostToDevice));
+                plans[j].bt = k->d_bt + off[j];
+            }
+            CK_HIP(hipMalloc(&k->d_bc, plans.size() * sizeof(BcPlan)));
+            CK_HIP(hipMemcpy(k->d_bc, plans.data(), plans.size() * sizeof(BcPlan), hipMemcpyHostToDevice));
+            k->mfma = 1;
+        }
+    }
     // key material
     const size_t kelems = n_blocks * B * N;
     for (size_t e = 0; e < kelems; ++e) {
@@ -518,7 +837,13 @@ static int key_switch_core(const ckks_ksk *k, const u64 *x, u64 x_stride, const 
     mu.limbs = (u32)B;
     mu.logn = logn;
     mu.chunks = chunks;
-    hipLaunchKernelGGL(k_modup, dim3((u32)(n_cts * D * chunks)), dim3(kKsThreads), 0, s, mu);
+    const bool mfma = k->mfma && ctx->base_conversion != CKKS_BASECONV_VALU;
+    if (mfma) {
+        ModUpMfmaArgs mm{mu, k->d_bc};
+        hipLaunchKernelGGL(k_modup_mfma, dim3((u32)(n_cts * D * (N / kBcRows))), dim3(256), 0, s, mm);
+    } else {
+        hipLaunchKernelGGL(k_modup, dim3((u32)(n_cts * D * chunks)), dim3(kKsThreads), 0, s, mu);
+    }
     CK_HIP(hipGetLastError());
     // 2. forward NTT of every digit limb
     NttCall f{plan, digits, digits, n_cts * D, B, B * N, B * N};
@@ -563,6 +888,19 @@ static int key_switch_core(const ckks_ksk *k, const u64 *x, u64 x_stride, const 
     da.logn = logn;
     da.chunks = chunks;
     dim3 g((u32)(n_cts * 2 * chunks));
+    if (mfma) {
+        ModDownMfmaArgs dm{da, k->d_bc + D};
+        dim3 gm((u32)(n_cts * 2 * (N / kBcRows)));
+        switch (k->p_limbs) {
+#define CK_MDM(KK) \
+    case KK: hipLaunchKernelGGL(k_moddown_mfma<KK>, gm, dim3(256), 0, s, dm); break;
+            CK_MDM(1) CK_MDM(2) CK_MDM(3) CK_MDM(4) CK_MDM(5) CK_MDM(6) CK_MDM(7) CK_MDM(8)
+#undef CK_MDM
+            default: return fail(CKKS_ERR_UNSUPPORTED, "too many special primes");
+        }
+        CK_HIP(hipGetLastError());
+        return CKKS_OK;
+    }
     switch (k->p_limbs) {
 #define CK_MD(KK) \
     case KK: hipLaunchKernelGGL(k_moddown<KK>, g, dim3(kKsThreads), 0, s, da); break;
@@ -613,6 +951,8 @@ int ckks_ksk_destroy(ckks_ksk *k) {
     hipFree(k->d_modup);
     hipFree(k->d_consts);
     hipFree(k->d_moddown);
+    hipFree(k->d_bc);
+    hipFree(k->d_bt);
     delete k;
     return CKKS_OK;
 }

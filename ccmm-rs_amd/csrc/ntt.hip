@@ -5,11 +5,10 @@
 // (limb outer, polynomial inner, tile innermost) so the blocks in flight
 // share a prime and its twiddle table stays hot in each XCD's L2.
 //
-// Degrees above 4096 take two passes (column pass, row pass).  The batch is
-// cut into chunks of limb transforms whose inter-pass buffer (~96 MiB) is
-// reused chunk after chunk: the buffer stays resident in the 256 MiB
-// Infinity Cache, so HBM sees one read and one write per residue instead
-// of two of each.
+// Degrees above 4096 take two passes (column pass, row pass) through an
+// inter-pass buffer.  The batch may be cut into chunks whose buffer is
+// reused chunk after chunk (ckks_set_ntt_chunk_bytes); by default the whole
+// batch is one chunk, which measured fastest on MI355X.
 #include <algorithm>
 #include <cstdlib>
 

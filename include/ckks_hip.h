@@ -65,9 +65,17 @@ int ckks_context_set_stream(ckks_context *ctx, void *stream);
 int ckks_sync(ckks_context *ctx);
 
 /* Tuning: bytes of the NTT inter-pass buffer per chunk of a batched
- * transform (default 96 MiB, sized to stay in the Infinity Cache; 0 = one
- * chunk).  Results do not depend on it.  No reference counterpart. */
+ * transform (default 0 = the whole batch in one chunk).  Results do not
+ * depend on it.  No reference counterpart. */
 int ckks_set_ntt_chunk_bytes(ckks_context *ctx, size_t bytes);
+
+/* Tuning: engine of the exact ModUp/ModDown base conversion of hybrid keys.
+ * CKKS_BASECONV_AUTO uses the int8 matrix cores (7-bit chunks, int32
+ * accumulation) wherever every gadget block has >= 2 limbs;
+ * CKKS_BASECONV_VALU forces the vector-ALU kernels.  Both are exact and give
+ * identical residues.  No reference counterpart. */
+enum { CKKS_BASECONV_AUTO = 0, CKKS_BASECONV_VALU = 1 };
+int ckks_set_base_conversion(ckks_context *ctx, int mode);
 
 /* Device memory helpers for callers without their own allocator. */
 int ckks_malloc(ckks_context *ctx, size_t bytes, void **dev);

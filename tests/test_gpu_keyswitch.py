@@ -36,9 +36,12 @@ def test_rns_gadget_key_switch_bit_exact(pkg, ctx, logn, L, blocks):
         assert (out[c, 0] == bo).all() and (out[c, 1] == ao).all(), f"ct {c}"
 
 
+@pytest.mark.parametrize("engine", ["mfma", "valu"])
 @pytest.mark.parametrize("logn,L,K,blocks", [(12, 4, 1, [2, 2]), (12, 4, 2, [1, 3]), (13, 9, 4, [5, 4]),
-                                             (15, 13, 4, [5, 4, 4]), (16, 25, 7, [9, 8, 8])])
-def test_hybrid_key_switch_bit_exact(pkg, ctx, logn, L, K, blocks):
+                                             (15, 13, 4, [5, 4, 4]), (16, 25, 7, [9, 8, 8]),
+                                             (12, 12, 3, [3, 9])])
+def test_hybrid_key_switch_bit_exact(pkg, ctx, logn, L, K, blocks, engine):
+    """Both base-conversion engines (int8 MFMA and VALU) against the oracle."""
     n = 1 << logn
     q_moduli = [Q0[0]] + Q40[:L - 1]
     qp = q_moduli + P60[:K]
@@ -48,7 +51,14 @@ def test_hybrid_key_switch_bit_exact(pkg, ctx, logn, L, K, blocks):
     key = pkg.HybridKeySwitchKey(plan, L, blocks, kb, ka)
     n_cts = 2
     ct = _ct(rng, q_moduli, n, n_cts)
-    out = pkg.to_host(key.switch(pkg.to_device(ct)))
+    # residues at the extremes exercise the largest chunk sums
+    ct[0, 1, :, :8] = np.array(q_moduli, dtype=np.uint64)[:, None] - 1
+    ct[0, 1, :, 8:16] = 0
+    ctx.set_base_conversion(ctx.BASECONV_VALU if engine == "valu" else ctx.BASECONV_AUTO)
+    try:
+        out = pkg.to_host(key.switch(pkg.to_device(ct)))
+    finally:
+        ctx.set_base_conversion(ctx.BASECONV_AUTO)
     orc = O.RnsPlan(qp, n)
     for c in range(n_cts):
         bo, ao = O.hybrid_key_switch(orc, L, blocks, ct[c, 0], ct[c, 1], kb, ka)
@@ -69,7 +79,7 @@ def test_hybrid_mod_down_centering_boundaries(pkg, ctx):
     orc = O.RnsPlan(qp, n)
     # a = constant 1 polynomial -> every digit is the constant 1; the key
     # b-entry of block 0 encodes a constant c, so acc_b = c exactly
-    for cval in [Pv // 2, Pv // 2 + 1, Pv - 1, 1, 0]:
+    for cval in [Pv // 2, Pv // 2 + 1, Pv // 2 - 1, Pv - 1, 1, 0]:
         kb = np.zeros((2, len(qp), n), dtype=np.uint64)
         ka = np.zeros_like(kb)
         const = np.zeros((len(qp), n), dtype=np.uint64)

@@ -111,7 +111,7 @@ def test_chunked_two_pass_batches(pkg, ctx, chunk_limbs):
         pkg.ntt_batch(ctx, [(p, o, None, L) for (p, L, _), o in zip(plans, outs)], forward=False)
         back = [pkg.to_host(o) for o in outs]
     finally:
-        ctx.set_ntt_chunk_bytes(96 << 20)
+        ctx.set_ntt_chunk_bytes(0)
     for x, g, b, (_, L, n) in zip(xs, got, back, plans):
         orc = O.RnsPlan(q[:L], n)
         for p in range(x.shape[0]):
