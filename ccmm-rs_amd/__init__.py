@@ -79,6 +79,11 @@ _SIGNATURES = {
     "ckks_rescale": (C.c_int, [_p, _p, _p, _sz, _sz, _sz, _sz]),
     "ckks_mod_switch": (C.c_int, [_p, _p, _p, _sz, _sz, _sz, _sz]),
     "ckks_tensor": (C.c_int, [_p, _p, _p, _p, _sz, _sz, _sz, _sz]),
+    "ckks_multiply_plain": (C.c_int, [_p, _p, _p, _p, _sz, _sz, _sz, _sz, _sz]),
+    "ckks_graph_begin": (C.c_int, [_p]),
+    "ckks_graph_end": (C.c_int, [_p, C.POINTER(_p)]),
+    "ckks_graph_launch": (C.c_int, [_p]),
+    "ckks_graph_destroy": (C.c_int, [_p]),
     "ckks_rns_ksk_create": (C.c_int, [_p, _p, _sz, _p, _p, C.POINTER(_p)]),
     "ckks_hybrid_ksk_create": (C.c_int, [_p, _sz, _p, _sz, _p, _p, C.POINTER(_p)]),
     "ckks_ksk_destroy": (C.c_int, [_p]),
@@ -195,6 +200,15 @@ class Context:
         """NTT inter-pass buffer per chunk (tuning only; results unchanged)."""
         _check(lib().ckks_set_ntt_chunk_bytes(self._h, int(nbytes)))
 
+    def graph_begin(self):
+        """Start capturing library calls on this context's stream."""
+        _check(lib().ckks_graph_begin(self._h))
+
+    def graph_end(self) -> "Graph":
+        h = _p()
+        _check(lib().ckks_graph_end(self._h, C.byref(h)))
+        return Graph(self, h)
+
     BASECONV_AUTO, BASECONV_VALU = 0, 1
 
     def set_base_conversion(self, mode: int):
@@ -205,6 +219,28 @@ class Context:
         if getattr(self, "_h", None):
             lib().ckks_context_destroy(self._h)
             self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Graph:
+    """A captured schedule of library calls (hipGraph), replayed by launch()."""
+
+    def __init__(self, ctx: "Context", h):
+        self.ctx = ctx
+        self._h = h
+
+    def launch(self):
+        _check(lib().ckks_graph_launch(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None) and self.ctx.handle:
+            lib().ckks_graph_destroy(self._h)
+        self._h = None
 
     def __del__(self):
         try:
@@ -320,6 +356,19 @@ class PreparedRnsNttPlan:
         out = empty((n_cts, 2, n_limbs - 1, n)) if out is None else out
         _check(lib().ckks_mod_switch(self._h, _ptr(ct), _ptr(out), n_cts, n_limbs,
                                      2 * n_limbs * n, 2 * (n_limbs - 1) * n))
+        return out
+
+    def multiply_plain(self, ct, pt_ntt, n_limbs: int, out=None):
+        """multiply_plain_rns_ckks_with_prepared_ntt (ckks/rns_plain_multiply.rs:193):
+        ct [n][2][n_limbs][N] (coefficient domain) times NTT-domain plaintexts
+        pt_ntt ([n_limbs][N] shared, or [n][n_limbs][N] one per ciphertext)."""
+        n = self.degree
+        n_cts = ct.numel() // (2 * n_limbs * n)
+        per_ct = pt_ntt.numel() == n_cts * n_limbs * n and n_cts > 1
+        out = empty((n_cts, 2, n_limbs, n)) if out is None else out
+        _check(lib().ckks_multiply_plain(self._h, _ptr(ct), _ptr(pt_ntt), _ptr(out), n_cts, n_limbs,
+                                         2 * n_limbs * n, n_limbs * n if per_ct else 0,
+                                         2 * n_limbs * n))
         return out
 
     def tensor(self, ct0, ct1, n_limbs: int, out=None):

@@ -31,6 +31,9 @@ int check_plan_limbs(const ckks_rns_plan *plan, size_t n_limbs, const char *what
 using namespace ck;
 
 int ckks_context::workspace(int slot, size_t bytes, void **out) {
+    if (bytes > ws_bytes[slot] && capturing)
+        return fail(CKKS_ERR_INVALID,
+                    "workspace must be sized before graph capture: run the schedule once eagerly first");
     if (bytes > ws_bytes[slot]) {
         if (ws[slot]) {
             hipStreamSynchronize(stream);
@@ -72,6 +75,7 @@ int ckks_context_destroy(ckks_context *ctx) {
     if (!ctx) return CKKS_OK;
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
+    while (!ctx->graphs.empty()) ckks_graph_destroy(*ctx->graphs.begin());
     while (!ctx->keys.empty()) ckks_ksk_destroy(*ctx->keys.begin());
     while (!ctx->plans.empty()) ckks_rns_plan_destroy(*ctx->plans.begin());
     for (auto &kv : ctx->tables) {
@@ -101,6 +105,54 @@ int ckks_sync(ckks_context *ctx) {
 int ckks_set_ntt_chunk_bytes(ckks_context *ctx, size_t bytes) {
     if (!ctx) return fail(CKKS_ERR_INVALID, "null context");
     ctx->ntt_chunk_bytes = bytes;
+    return CKKS_OK;
+}
+
+}  // extern "C"
+
+struct ckks_graph {
+    ckks_context *ctx = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+extern "C" {
+
+int ckks_graph_begin(ckks_context *ctx) {
+    if (!ctx) return fail(CKKS_ERR_INVALID, "null context");
+    if (!ctx->stream) return fail(CKKS_ERR_INVALID, "graph capture needs a context on a non-null stream");
+    if (ctx->capturing) return fail(CKKS_ERR_INVALID, "a graph capture is already open");
+    CK_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    ctx->capturing = true;
+    return CKKS_OK;
+}
+
+int ckks_graph_end(ckks_context *ctx, ckks_graph **out) {
+    if (!ctx || !out) return fail(CKKS_ERR_INVALID, "null argument");
+    if (!ctx->capturing) return fail(CKKS_ERR_INVALID, "no graph capture is open");
+    ctx->capturing = false;
+    std::unique_ptr<ckks_graph> g(new ckks_graph());
+    g->ctx = ctx;
+    CK_HIP(hipStreamEndCapture(ctx->stream, &g->graph));
+    CK_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    ctx->graphs.insert(g.get());
+    *out = g.release();
+    return CKKS_OK;
+}
+
+int ckks_graph_launch(ckks_graph *g) {
+    if (!g) return fail(CKKS_ERR_INVALID, "null graph");
+    CK_HIP(hipGraphLaunch(g->exec, g->ctx->stream));
+    return CKKS_OK;
+}
+
+int ckks_graph_destroy(ckks_graph *g) {
+    if (!g) return CKKS_OK;
+    hipStreamSynchronize(g->ctx->stream);
+    g->ctx->graphs.erase(g);
+    if (g->exec) hipGraphExecDestroy(g->exec);
+    if (g->graph) hipGraphDestroy(g->graph);
+    delete g;
     return CKKS_OK;
 }
 

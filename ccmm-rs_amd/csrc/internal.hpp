@@ -59,6 +59,7 @@ struct TwiddleTables {
 
 struct ckks_rns_plan;
 struct ckks_ksk;
+struct ckks_graph;
 
 struct ckks_context {
     int device = 0;
@@ -66,6 +67,7 @@ struct ckks_context {
     // plans and keys created from this context, released with it
     std::set<ckks_rns_plan *> plans;
     std::set<ckks_ksk *> keys;
+    std::set<ckks_graph *> graphs;
     // (q, logn) -> device twiddle tables, shared by plans of this context
     std::map<std::pair<uint64_t, uint32_t>, ck::TwiddleTables> tables;
     // grow-only scratch slots: slot 0 is the NTT's inter-pass buffer, the
@@ -77,6 +79,7 @@ struct ckks_context {
     // measured on MI355X: chunks sized for the Infinity Cache gave no gain)
     size_t ntt_chunk_bytes = 0;
     int base_conversion = CKKS_BASECONV_AUTO;
+    bool capturing = false;  // a hipGraph capture of `stream` is open
     std::vector<void *> owned;  // misc device allocations freed at destroy
 
     int workspace(int slot, size_t bytes, void **out);

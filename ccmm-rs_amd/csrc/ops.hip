@@ -311,11 +311,80 @@ static int tensor(const ckks_rns_plan *plan, const u64 *ct0, const u64 *ct1, u64
     return launch_ntt(ctx, inv, 3, false);
 }
 
+// ---- plaintext multiply ----------------------------------------------------------
+
+// NTT-domain ciphertext components times a plaintext: x[c][comp][i] * pt[c][i].
+struct MulPlainArgs {
+    const u64 *x;   // [ct][2][limb][N] NTT domain (workspace)
+    const u64 *pt;  // plaintext of ct c at pt + c * pt_stride
+    u64 *y;         // [ct][2][limb][N]
+    const DevPrime *primes;
+    u64 pt_stride;
+    u32 n_limbs, logn, chunks;
+};
+
+__global__ __launch_bounds__(kEwThreads) void k_mul_plain(MulPlainArgs a) {
+    const u32 b = blockIdx.x;
+    const u32 chunk = b % a.chunks;
+    const u32 pl = b / a.chunks;
+    const u32 limb = pl % a.n_limbs, cc = pl / a.n_limbs;  // cc = ct * 2 + comp
+    const u64 ct = cc / 2;
+    const DevPrime P = a.primes[limb];
+    Barrett br{P.q, P.br_hi, P.br_lo};
+    const u64 n = (u64)1 << a.logn;
+    const u64 off = (u64)chunk * kEwChunk + threadIdx.x * kEwPerThread;
+    const u64 *x = a.x + ((u64)cc * a.n_limbs + limb) * n + off;
+    const u64 *p = a.pt + ct * a.pt_stride + (u64)limb * n + off;
+    u64 *y = a.y + ((u64)cc * a.n_limbs + limb) * n + off;
+#pragma unroll
+    for (int i = 0; i < kEwPerThread; ++i) y[i] = mul_mod(x[i], p[i], br);
+}
+
+static int multiply_plain(const ckks_rns_plan *plan, const u64 *in, const u64 *pt, u64 *out, size_t n_cts,
+                          size_t n_limbs, size_t in_stride, size_t pt_stride, size_t out_stride) {
+    int rc = check_plan_limbs(plan, n_limbs, "plaintext multiply");
+    if (rc) return rc;
+    if (n_cts == 0) return CKKS_OK;
+    if (!in || !pt || !out) return fail(CKKS_ERR_INVALID, "null buffer");
+    const size_t n = plan->degree, poly = n_limbs * n;
+    if (in_stride == 0) in_stride = 2 * poly;
+    if (out_stride == 0) out_stride = 2 * poly;
+    if (in_stride < 2 * poly || out_stride < 2 * poly || (pt_stride && pt_stride < poly))
+        return fail(CKKS_ERR_INVALID, "ciphertext stride smaller than its limbs");
+    ckks_context *ctx = plan->ctx;
+    void *w;
+    if ((rc = ctx->workspace(1, n_cts * 2 * poly * sizeof(u64), &w))) return rc;
+    u64 *x = (u64 *)w;
+    NttCall fwd[2] = {{plan, in, x, n_cts, n_limbs, in_stride, 2 * poly},
+                      {plan, in + poly, x + poly, n_cts, n_limbs, in_stride, 2 * poly}};
+    if ((rc = launch_ntt(ctx, fwd, 2, true))) return rc;
+    MulPlainArgs a{};
+    a.x = x;
+    a.pt = pt;
+    a.y = x;
+    a.primes = plan->d_primes;
+    a.pt_stride = pt_stride;
+    a.n_limbs = (u32)n_limbs;
+    a.logn = plan->logn;
+    a.chunks = (u32)(n / kEwChunk);
+    hipLaunchKernelGGL(k_mul_plain, dim3((u32)(n_cts * 2 * n_limbs * a.chunks)), dim3(kEwThreads), 0,
+                       ctx->stream, a);
+    CK_HIP(hipGetLastError());
+    NttCall inv[2] = {{plan, x, out, n_cts, n_limbs, 2 * poly, out_stride},
+                      {plan, x + poly, out + poly, n_cts, n_limbs, 2 * poly, out_stride}};
+    return launch_ntt(ctx, inv, 2, false);
+}
+
 }  // namespace ck
 
 using namespace ck;
 
 extern "C" {
+
+int ckks_multiply_plain(const ckks_rns_plan *plan, const uint64_t *in, const uint64_t *pt, uint64_t *out,
+                        size_t n_cts, size_t n_limbs, size_t in_stride, size_t pt_stride, size_t out_stride) {
+    return multiply_plain(plan, in, pt, out, n_cts, n_limbs, in_stride, pt_stride, out_stride);
+}
 
 int ckks_poly_mul(const ckks_rns_plan *plan, const uint64_t *x, const uint64_t *y, uint64_t *out,
                   size_t n_polys, size_t n_limbs, size_t poly_stride) {

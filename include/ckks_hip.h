@@ -164,6 +164,16 @@ int ckks_rescale(const ckks_rns_plan *plan, const uint64_t *in, uint64_t *out, s
 int ckks_mod_switch(const ckks_rns_plan *plan, const uint64_t *in, uint64_t *out, size_t n_cts,
                     size_t n_limbs, size_t in_stride, size_t out_stride);
 
+/* multiply_plain_rns_ckks_with_prepared_ntt (ckks/rns_plain_multiply.rs:
+ * 193-260) over PreparedRnsPlaintextNtt (:76-110): ciphertexts
+ * (coefficient domain) times NTT-domain plaintexts of n_limbs limbs;
+ * plaintext c is at pt + c * pt_stride (pt_stride 0: one plaintext for the
+ * whole batch).  The CKKS scale bookkeeping (scale product) stays with the
+ * caller.  out may equal in. */
+int ckks_multiply_plain(const ckks_rns_plan *plan, const uint64_t *in, const uint64_t *pt,
+                        uint64_t *out, size_t n_cts, size_t n_limbs, size_t in_stride,
+                        size_t pt_stride, size_t out_stride);
+
 /* rns_tensor_with_prepared_ntt (grafting/rns_key_switch.rs:1724-1777):
  * (b0,a0) x (b1,a1) -> (c0,c1,c2) coefficient domain, out laid out as
  * [c0 limbs][c1 limbs][c2 limbs] per ciphertext (out_stride elements). */
@@ -214,6 +224,21 @@ int ckks_relinearize(const ckks_ksk *key, const uint64_t *in3, uint64_t *out, si
 int ckks_multiply_relinearize_rescale(const ckks_ksk *relin_key, const uint64_t *ct0,
                                       const uint64_t *ct1, uint64_t *out, size_t n_cts,
                                       size_t in_stride, size_t out_stride);
+
+/* ------------------------------------------------------------ graphs */
+/* Rotation schedules (BSGS of CCMM / FFT) captured once as a hipGraph and
+ * replayed: ckks_graph_begin starts capturing the context's stream (which
+ * must not be the null stream); every library call until ckks_graph_end is
+ * recorded, not run.  Workspaces must already be large enough: run the
+ * schedule once eagerly first (a call that would have to grow a workspace
+ * during capture fails).  A replay reads and writes the buffers the calls
+ * named at capture time.  No reference counterpart (the reference runs the
+ * same schedule as a plain loop). */
+typedef struct ckks_graph ckks_graph;
+int ckks_graph_begin(ckks_context *ctx);
+int ckks_graph_end(ckks_context *ctx, ckks_graph **out);
+int ckks_graph_launch(ckks_graph *graph);
+int ckks_graph_destroy(ckks_graph *graph);
 
 #ifdef __cplusplus
 }
