@@ -121,6 +121,10 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=6, help="plain-path rotations timed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default NTT variant and base-conversion engine")
+    ap.add_argument("--base-conversion", choices=["auto", "valu", "mfma"], default="auto",
+                    help="ModUp/ModDown engine: autotuned per key (auto), VALU, or int8 MFMA")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -134,6 +138,8 @@ def main():
     dev = "cuda"
     torch.cuda.init()
     ctx = pkg.Context(0)
+    ctx.set_base_conversion({"auto": ctx.BASECONV_AUTO, "valu": ctx.BASECONV_VALU,
+                             "mfma": ctx.BASECONV_MFMA}[args.base_conversion])
 
     N = 1 << 16
     N2 = 1 << 15
@@ -156,6 +162,23 @@ def main():
             kb2, ka2 = synthetic_key(torch, len(blocks2), L2 + K2, N2)
             keys2.append((rotation_exponent_left(N2, r),
                           pkg.HybridKeySwitchKey(qp_plan2, L2, blocks2, kb2, ka2)))
+
+    # ---- autotuner: NTT kernel variant per degree, base-conversion engine per key
+    tuned = {}
+    if not args.no_autotune:
+        v16, ms16 = qp_plan.autotune_ntt(16)
+        v15, ms15 = qp_plan2.autotune_ntt(16)
+        tuned["ntt_variant"] = {"2^16": v16, "2^15": v15}
+        print(f"# autotune NTT: 2^16 variant {v16} {['%.2f' % m for m in ms16]} ms; "
+              f"2^15 variant {v15}", file=sys.stderr, flush=True)
+        if args.base_conversion == "auto":
+            engines = set()
+            for (_, key), n in [(kk, args.batch) for kk in keys] + [(kk, max(args.mixed, 1)) for kk in keys2]:
+                eng, mv, mm = key.autotune(n)
+                engines.add(eng)
+            tuned["base_conversion"] = sorted("mfma" if e == ctx.BASECONV_MFMA else "valu" for e in engines)
+            print(f"# autotune base conversion (last key): valu {mv:.2f} ms, mfma {mm:.2f} ms",
+                  file=sys.stderr, flush=True)
 
     B = args.batch
     cts = rand_residues(torch, (B, 2, L, N), dev)
@@ -269,6 +292,7 @@ def main():
         "speedup_vs_plain": round(value / plain_value, 2),
         "ntt_forward_gbps": round(ntt_gbps, 1),
         "ntt_inverse_gbps": round(intt_gbps, 1),
+        "autotune": tuned,
         "p50_ms": round(p50, 3),
         "p99_ms": round(p99, 3),
         "roofline": {

@@ -57,6 +57,11 @@ _SIGNATURES = {
     "ckks_sync": (C.c_int, [_p]),
     "ckks_set_ntt_chunk_bytes": (C.c_int, [_p, _sz]),
     "ckks_set_base_conversion": (C.c_int, [_p, C.c_int]),
+    "ckks_set_ntt_variant": (C.c_int, [_p, C.c_uint32, C.c_int]),
+    "ckks_get_ntt_variant": (C.c_int, [_p, C.c_uint32, C.POINTER(C.c_int)]),
+    "ckks_autotune_ntt": (C.c_int, [_p, _sz, _sz, C.POINTER(C.c_int), C.POINTER(C.c_float)]),
+    "ckks_autotune_key_switch": (C.c_int, [_p, _sz, C.POINTER(C.c_int), C.POINTER(C.c_float),
+                                           C.POINTER(C.c_float)]),
     "ckks_malloc": (C.c_int, [_p, _sz, C.POINTER(_p)]),
     "ckks_free": (C.c_int, [_p, _p]),
     "ckks_upload": (C.c_int, [_p, _p, _p, _sz]),
@@ -209,11 +214,23 @@ class Context:
         _check(lib().ckks_graph_end(self._h, C.byref(h)))
         return Graph(self, h)
 
-    BASECONV_AUTO, BASECONV_VALU = 0, 1
+    BASECONV_AUTO, BASECONV_VALU, BASECONV_MFMA = 0, 1, 2
+    NTT_VARIANT_BLOCKED, NTT_VARIANT_TILE8K, NTT_VARIANT_COUNT = 1, 2, 4
 
     def set_base_conversion(self, mode: int):
-        """Hybrid ModUp/ModDown engine: 0 = int8 MFMA where possible, 1 = VALU."""
+        """Hybrid ModUp/ModDown engine: AUTO (the key's tuned engine, VALU
+        when untuned), VALU, or MFMA (int8 matrix cores where the key allows)."""
         _check(lib().ckks_set_base_conversion(self._h, int(mode)))
+
+    def set_ntt_variant(self, logn: int, variant: int):
+        """Two-pass NTT kernel variant for degree 2^logn (bit set of
+        NTT_VARIANT_BLOCKED / NTT_VARIANT_TILE8K); results unchanged."""
+        _check(lib().ckks_set_ntt_variant(self._h, int(logn), int(variant)))
+
+    def ntt_variant(self, logn: int) -> int:
+        v = C.c_int()
+        _check(lib().ckks_get_ntt_variant(self._h, int(logn), C.byref(v)))
+        return v.value
 
     def close(self):
         if getattr(self, "_h", None):
@@ -380,6 +397,15 @@ class PreparedRnsNttPlan:
                                  2 * n_limbs * n, 3 * n_limbs * n))
         return out
 
+    def autotune_ntt(self, n_polys: int, n_limbs: int | None = None):
+        """Times every NTT kernel variant on n_polys x n_limbs limbs, keeps
+        the fastest for this degree in the context; returns (best, ms list)."""
+        n_limbs = len(self.moduli) if n_limbs is None else n_limbs
+        best = C.c_int()
+        ms = (C.c_float * 4)()
+        _check(lib().ckks_autotune_ntt(self._h, int(n_polys), int(n_limbs), C.byref(best), ms))
+        return best.value, list(ms)
+
     def close(self):
         # the context releases its plans itself when it is closed first
         if getattr(self, "_h", None) and self.ctx.handle:
@@ -443,6 +469,16 @@ class _KeySwitchKey:
         _check(lib().ckks_multiply_relinearize_rescale(self._h, _ptr(ct0), _ptr(ct1), _ptr(out), n,
                                                        0, 0))
         return out
+
+    def autotune(self, n_cts: int):
+        """Times the key switch of n_cts ciphertexts with each base-conversion
+        engine and keeps the faster for BASECONV_AUTO; returns
+        (engine, ms_valu, ms_mfma) with -1 for an engine that does not apply."""
+        best = C.c_int()
+        mv, mm = C.c_float(), C.c_float()
+        _check(lib().ckks_autotune_key_switch(self._h, int(n_cts), C.byref(best), C.byref(mv),
+                                              C.byref(mm)))
+        return best.value, mv.value, mm.value
 
     def close(self):
         if getattr(self, "_h", None) and self.ctx.handle:

@@ -158,9 +158,23 @@ int ckks_graph_destroy(ckks_graph *g) {
 
 int ckks_set_base_conversion(ckks_context *ctx, int mode) {
     if (!ctx) return fail(CKKS_ERR_INVALID, "null context");
-    if (mode != CKKS_BASECONV_AUTO && mode != CKKS_BASECONV_VALU)
+    if (mode != CKKS_BASECONV_AUTO && mode != CKKS_BASECONV_VALU && mode != CKKS_BASECONV_MFMA)
         return fail(CKKS_ERR_INVALID, "unknown base-conversion mode");
     ctx->base_conversion = mode;
+    return CKKS_OK;
+}
+
+int ckks_set_ntt_variant(ckks_context *ctx, uint32_t logn, int variant) {
+    if (!ctx) return fail(CKKS_ERR_INVALID, "null context");
+    if (logn >= 32 || variant < 0 || variant >= CKKS_NTT_VARIANT_COUNT)
+        return fail(CKKS_ERR_INVALID, "unknown NTT variant");
+    ctx->ntt_variant[logn] = (uint8_t)variant;
+    return CKKS_OK;
+}
+
+int ckks_get_ntt_variant(const ckks_context *ctx, uint32_t logn, int *variant) {
+    if (!ctx || !variant || logn >= 32) return fail(CKKS_ERR_INVALID, "invalid argument");
+    *variant = ctx->ntt_variant[logn];
     return CKKS_OK;
 }
 

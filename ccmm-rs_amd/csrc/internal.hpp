@@ -79,6 +79,7 @@ struct ckks_context {
     // measured on MI355X: chunks sized for the Infinity Cache gave no gain)
     size_t ntt_chunk_bytes = 0;
     int base_conversion = CKKS_BASECONV_AUTO;
+    uint8_t ntt_variant[32] = {};  // two-pass NTT kernel variant per log2 N
     bool capturing = false;  // a hipGraph capture of `stream` is open
     std::vector<void *> owned;  // misc device allocations freed at destroy
 
@@ -107,6 +108,7 @@ struct NttCall {
     size_t in_stride, out_stride;  // elements between consecutive polynomials
 };
 int launch_ntt(ckks_context *ctx, const NttCall *calls, size_t count, bool forward);
+int ntt_variant_for(const ckks_context *ctx, u32 logn);
 
 int check_plan_limbs(const ckks_rns_plan *plan, size_t n_limbs, const char *what);
 

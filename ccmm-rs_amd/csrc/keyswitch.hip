@@ -67,6 +67,7 @@ struct ckks_ksk {
     ck::ModDownTable *d_moddown = nullptr;
     // exact int8-MFMA base conversion: dnum ModUp plans then the ModDown plan
     int mfma = 0;
+    int tuned_engine = CKKS_BASECONV_VALU;  // engine of CKKS_BASECONV_AUTO for this key
     ck::BcPlan *d_bc = nullptr;
     signed char *d_bt = nullptr;
 };
@@ -837,7 +838,8 @@ static int key_switch_core(const ckks_ksk *k, const u64 *x, u64 x_stride, const 
     mu.limbs = (u32)B;
     mu.logn = logn;
     mu.chunks = chunks;
-    const bool mfma = k->mfma && ctx->base_conversion != CKKS_BASECONV_VALU;
+    const int engine = ctx->base_conversion == CKKS_BASECONV_AUTO ? k->tuned_engine : ctx->base_conversion;
+    const bool mfma = k->mfma && engine == CKKS_BASECONV_MFMA;
     if (mfma) {
         ModUpMfmaArgs mm{mu, k->d_bc};
         hipLaunchKernelGGL(k_modup_mfma, dim3((u32)(n_cts * D * (N / kBcRows))), dim3(256), 0, s, mm);
@@ -954,6 +956,65 @@ int ckks_ksk_destroy(ckks_ksk *k) {
     hipFree(k->d_bc);
     hipFree(k->d_bt);
     delete k;
+    return CKKS_OK;
+}
+
+int ckks_autotune_key_switch(ckks_ksk *k, size_t n_cts, int *best_engine, float *ms_valu,
+                             float *ms_mfma) {
+    if (!k || !best_engine) return fail(CKKS_ERR_INVALID, "null argument");
+    ckks_context *ctx = k->ctx;
+    if (ctx->capturing) return fail(CKKS_ERR_INVALID, "autotuning is not allowed during graph capture");
+    if (n_cts == 0) return fail(CKKS_ERR_INVALID, "autotune needs at least one ciphertext");
+    if (ms_valu) *ms_valu = -1.f;
+    if (ms_mfma) *ms_mfma = -1.f;
+    const size_t poly = k->q_limbs * k->degree, per = 2 * poly;
+    CK_HIP(hipSetDevice(ctx->device));
+    u64 *buf = nullptr;
+    CK_HIP(hipMalloc(&buf, 2 * n_cts * per * sizeof(u64)));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const int saved_mode = ctx->base_conversion;
+    int best = CKKS_BASECONV_VALU;
+    float best_ms = 0.f;
+    auto body = [&]() -> int {
+        CK_HIP(hipMemsetAsync(buf, 0, 2 * n_cts * per * sizeof(u64), ctx->stream));
+        CK_HIP(hipEventCreate(&e0));
+        CK_HIP(hipEventCreate(&e1));
+        u64 *in = buf, *out = buf + n_cts * per;
+        const int engines[2] = {CKKS_BASECONV_VALU, CKKS_BASECONV_MFMA};
+        for (int engine : engines) {
+            if (engine == CKKS_BASECONV_MFMA && !k->mfma) continue;
+            ctx->base_conversion = engine;
+            int r = key_switch_core(k, in + poly, per, in, nullptr, per, out, per, n_cts);
+            if (r) return r;
+            const int reps = 2;
+            CK_HIP(hipEventRecord(e0, ctx->stream));
+            for (int i = 0; i < reps; ++i) {
+                r = key_switch_core(k, in + poly, per, in, nullptr, per, out, per, n_cts);
+                if (r) return r;
+            }
+            CK_HIP(hipEventRecord(e1, ctx->stream));
+            CK_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            CK_HIP(hipEventElapsedTime(&ms, e0, e1));
+            ms /= reps;
+            if (engine == CKKS_BASECONV_VALU && ms_valu) *ms_valu = ms;
+            if (engine == CKKS_BASECONV_MFMA && ms_mfma) *ms_mfma = ms;
+            if (best_ms == 0.f || ms < best_ms) {
+                best_ms = ms;
+                best = engine;
+            }
+        }
+        return CKKS_OK;
+    };
+    int rc = body();
+    ctx->base_conversion = saved_mode;
+    hipStreamSynchronize(ctx->stream);
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    hipFree(buf);
+    if (rc) return rc;
+    k->tuned_engine = best;
+    *best_engine = best;
     return CKKS_OK;
 }
 

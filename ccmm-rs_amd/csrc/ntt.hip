@@ -34,14 +34,16 @@ struct NttLaunch {
     u32 n;
 };
 
-template <bool FWD, bool FIRST>
-__global__ __launch_bounds__(256) void k_ntt(NttLaunch launch) {
-    __shared__ u64 lds[4096];
+// TB = log2 of the two-pass tile (12: 256 threads, 13: 512 threads);
+// BLK = blocked inter-pass layout.  Single-pass launches use <.., 12, false>.
+template <bool FWD, bool FIRST, int TB, bool BLK>
+__global__ __launch_bounds__(1 << (TB - 4)) void k_ntt(NttLaunch launch) {
+    __shared__ u64 lds[1 << TB];
     const u32 b = blockIdx.x;
     u32 di = 0;
     while (di + 1 < launch.n && b >= launch.d[di + 1].block_start) ++di;
     const DevNttDesc &D = launch.d[di];
-    const u32 tiles = ntt_tiles(D.logn);
+    const u32 tiles = ntt_tiles(D.logn, TB);
     const u32 local = b - D.block_start;
     const u32 tile = local % tiles;
     const u32 rel = local / tiles;
@@ -73,23 +75,44 @@ __global__ __launch_bounds__(256) void k_ntt(NttLaunch launch) {
         job.dst = dst;
     }
     u64 regs[16];
-    BlockExec ex{256};
+    BlockExec ex{1u << (TB - 4)};
     auto regs_of = [&](u32) -> u64(&)[16] { return regs; };
     if (P.fp)
-        ntt_dispatch<FWD, FIRST, true>(ex, regs_of, job, lds);
+        ntt_dispatch<FWD, FIRST, true, TB, BLK>(ex, regs_of, job, lds);
     else
-        ntt_dispatch<FWD, FIRST, false>(ex, regs_of, job, lds);
+        ntt_dispatch<FWD, FIRST, false, TB, BLK>(ex, regs_of, job, lds);
 }
 
-static void launch_pass(hipStream_t s, bool forward, bool first, const NttLaunch &l, u32 blocks) {
-    if (!blocks) return;
+template <int TB, bool BLK>
+static void launch_variant(hipStream_t s, bool forward, bool first, const NttLaunch &l, u32 blocks) {
+    const dim3 g(blocks), t(1u << (TB - 4));
     if (forward) {
-        if (first) hipLaunchKernelGGL((k_ntt<true, true>), dim3(blocks), dim3(256), 0, s, l);
-        else hipLaunchKernelGGL((k_ntt<true, false>), dim3(blocks), dim3(256), 0, s, l);
+        if (first) hipLaunchKernelGGL((k_ntt<true, true, TB, BLK>), g, t, 0, s, l);
+        else hipLaunchKernelGGL((k_ntt<true, false, TB, BLK>), g, t, 0, s, l);
     } else {
-        if (first) hipLaunchKernelGGL((k_ntt<false, true>), dim3(blocks), dim3(256), 0, s, l);
-        else hipLaunchKernelGGL((k_ntt<false, false>), dim3(blocks), dim3(256), 0, s, l);
+        if (first) hipLaunchKernelGGL((k_ntt<false, true, TB, BLK>), g, t, 0, s, l);
+        else hipLaunchKernelGGL((k_ntt<false, false, TB, BLK>), g, t, 0, s, l);
     }
+}
+
+static void launch_pass(hipStream_t s, bool forward, bool first, const NttLaunch &l, u32 blocks,
+                        int variant = 0) {
+    if (!blocks) return;
+    switch (variant) {
+    case CKKS_NTT_VARIANT_BLOCKED: launch_variant<12, true>(s, forward, first, l, blocks); break;
+    case CKKS_NTT_VARIANT_TILE8K: launch_variant<13, false>(s, forward, first, l, blocks); break;
+    case CKKS_NTT_VARIANT_TILE8K | CKKS_NTT_VARIANT_BLOCKED:
+        launch_variant<13, true>(s, forward, first, l, blocks);
+        break;
+    default: launch_variant<12, false>(s, forward, first, l, blocks); break;
+    }
+}
+
+int ntt_variant_for(const ckks_context *ctx, u32 logn) {
+    if (!ntt_two_pass(logn) || logn >= 32) return 0;
+    int v = ctx->ntt_variant[logn];
+    if ((v & CKKS_NTT_VARIANT_TILE8K) && logn < 14) v &= ~CKKS_NTT_VARIANT_TILE8K;
+    return v;
 }
 
 
@@ -148,15 +171,25 @@ int launch_ntt(ckks_context *ctx, const NttCall *calls, size_t count, bool forwa
         int rc = ctx->workspace(0, cap * sizeof(u64), &ws);
         if (rc) return rc;
         u64 *tmp = (u64 *)ws;
-        // walk the limb transforms of the two-pass entries in order, closing a
-        // chunk when its inter-pass buffer is full
+        // one sweep per kernel variant present (the variant is chosen per
+        // degree); each sweep walks the limb transforms of its entries in
+        // order, closing a chunk when the inter-pass buffer is full
+        bool seen[CKKS_NTT_VARIANT_COUNT] = {};
+        for (size_t i = 0; i < count; ++i)
+            if (two_pass(i)) seen[ntt_variant_for(ctx, calls[i].plan->logn)] = true;
+        for (int var = 0; var < CKKS_NTT_VARIANT_COUNT; ++var) {
+        if (!seen[var]) continue;
+        const u32 tb = (var & CKKS_NTT_VARIANT_TILE8K) ? 13 : 12;
+        auto in_sweep = [&](size_t i) {
+            return two_pass(i) && ntt_variant_for(ctx, calls[i].plan->logn) == var;
+        };
         size_t ci = 0, lt = 0;
         while (ci < count) {
             NttLaunch l{};
             u32 blocks = 0;
             size_t used = 0;
             while (ci < count && l.n < (u32)kMaxNttDesc) {
-                if (!two_pass(ci)) {
+                if (!in_sweep(ci)) {
                     ++ci;
                     lt = 0;
                     continue;
@@ -171,7 +204,7 @@ int launch_ntt(ckks_context *ctx, const NttCall *calls, size_t count, bool forwa
                 d.lt_begin = (u32)lt;
                 d.tmp = tmp + used;
                 l.d[l.n++] = d;
-                blocks += (u32)(take * ntt_tiles(d.logn));
+                blocks += (u32)(take * ntt_tiles(d.logn, tb));
                 used += take * N;
                 lt += take;
                 if (lt >= total_lt) {
@@ -179,8 +212,9 @@ int launch_ntt(ckks_context *ctx, const NttCall *calls, size_t count, bool forwa
                     lt = 0;
                 }
             }
-            launch_pass(s, forward, true, l, blocks);
-            launch_pass(s, forward, false, l, blocks);
+            launch_pass(s, forward, true, l, blocks, var);
+            launch_pass(s, forward, false, l, blocks, var);
+        }
         }
     }
     CK_HIP(hipGetLastError());
@@ -226,3 +260,67 @@ int ckks_ntt_inverse_batch(ckks_context *ctx, const ckks_ntt_batch *batch, size_
 }
 
 }  // extern "C"
+
+extern "C" int ckks_autotune_ntt(const ckks_rns_plan *plan, size_t n_polys, size_t n_limbs,
+                                 int *best_variant, float *ms_per_variant) {
+    if (!plan || !best_variant) return fail(CKKS_ERR_INVALID, "null argument");
+    ckks_context *ctx = plan->ctx;
+    if (ctx->capturing) return fail(CKKS_ERR_INVALID, "autotuning is not allowed during graph capture");
+    int rc = check_plan_limbs(plan, n_limbs, "NTT autotune");
+    if (rc) return rc;
+    if (n_polys == 0) return fail(CKKS_ERR_INVALID, "autotune needs at least one polynomial");
+    if (ms_per_variant)
+        for (int v = 0; v < CKKS_NTT_VARIANT_COUNT; ++v) ms_per_variant[v] = -1.f;
+    const u32 logn = plan->logn;
+    *best_variant = 0;
+    if (!ntt_two_pass(logn)) return CKKS_OK;  // one kernel serves N <= 4096
+    const size_t per = n_limbs * plan->degree, elems = n_polys * per;
+    CK_HIP(hipSetDevice(ctx->device));
+    u64 *buf = nullptr;
+    CK_HIP(hipMalloc(&buf, 2 * elems * sizeof(u64)));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const int saved = ctx->ntt_variant[logn];
+    float best_ms = 0.f;
+    int best = saved;
+    rc = CKKS_OK;
+    auto body = [&]() -> int {
+        CK_HIP(hipMemsetAsync(buf, 0, 2 * elems * sizeof(u64), ctx->stream));
+        CK_HIP(hipEventCreate(&e0));
+        CK_HIP(hipEventCreate(&e1));
+        const NttCall fwd{plan, buf, buf + elems, n_polys, n_limbs, per, per};
+        const NttCall inv{plan, buf + elems, buf, n_polys, n_limbs, per, per};
+        for (int v = 0; v < CKKS_NTT_VARIANT_COUNT; ++v) {
+            if ((v & CKKS_NTT_VARIANT_TILE8K) && logn < 14) continue;
+            ctx->ntt_variant[logn] = (uint8_t)v;
+            int r = launch_ntt(ctx, &fwd, 1, true);
+            if (!r) r = launch_ntt(ctx, &inv, 1, false);
+            if (r) return r;
+            const int reps = 3;
+            CK_HIP(hipEventRecord(e0, ctx->stream));
+            for (int i = 0; i < reps && !r; ++i) {
+                r = launch_ntt(ctx, &fwd, 1, true);
+                if (!r) r = launch_ntt(ctx, &inv, 1, false);
+            }
+            if (r) return r;
+            CK_HIP(hipEventRecord(e1, ctx->stream));
+            CK_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            CK_HIP(hipEventElapsedTime(&ms, e0, e1));
+            ms /= reps;
+            if (ms_per_variant) ms_per_variant[v] = ms;
+            if (best_ms == 0.f || ms < best_ms) {
+                best_ms = ms;
+                best = v;
+            }
+        }
+        return CKKS_OK;
+    };
+    rc = body();
+    hipStreamSynchronize(ctx->stream);
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    hipFree(buf);
+    ctx->ntt_variant[logn] = (uint8_t)(rc ? saved : best);
+    if (!rc) *best_variant = best;
+    return rc;
+}

@@ -29,7 +29,8 @@ struct BlockExec {
 };
 #endif
 
-template <int L, int CL, PassKind KIND, bool FWD, bool FP>
+// BLK: the inter-pass buffer uses the blocked layout (internal_offset).
+template <int L, int CL, PassKind KIND, bool FWD, bool FP, bool BLK = false>
 struct NttPass {
     typedef TileShape<L, CL> S;
     static constexpr int NR = n_rounds(L);
@@ -67,15 +68,17 @@ struct NttPass {
     // global offset (within the limb) of the element at tile coordinate e on
     // the read side and the write side of this pass.
     CK_HD static u32 read_off(const PassAddr &pa, u32 e) {
-        if (KIND == PASS_COL) return col_pass_offset<L, CL>(pa, e);
+        if (KIND == PASS_COL)
+            return FWD ? col_pass_offset<L, CL>(pa, e) : col_internal_offset<L, CL, BLK>(pa, e);
         if (KIND == PASS_ROW)
-            return FWD ? row_internal_offset<L, CL>(pa, e) : row_natural_offset<L, CL>(pa, e);
+            return FWD ? row_internal_offset<L, CL, BLK>(pa, e) : row_natural_offset<L, CL>(pa, e);
         return e;  // PASS_FULL reads are staged linearly (see run)
     }
     CK_HD static u32 write_off(const PassAddr &pa, u32 e) {
-        if (KIND == PASS_COL) return col_pass_offset<L, CL>(pa, e);
+        if (KIND == PASS_COL)
+            return FWD ? col_internal_offset<L, CL, BLK>(pa, e) : col_pass_offset<L, CL>(pa, e);
         if (KIND == PASS_ROW)
-            return FWD ? row_natural_offset<L, CL>(pa, e) : row_internal_offset<L, CL>(pa, e);
+            return FWD ? row_natural_offset<L, CL>(pa, e) : row_internal_offset<L, CL, BLK>(pa, e);
         return e;
     }
 
@@ -248,11 +251,14 @@ struct NttPass {
 CK_HD u32 ntt_n2(u32 logn) { return (logn + 1) / 2; }
 CK_HD u32 ntt_n1(u32 logn) { return logn - ntt_n2(logn); }
 CK_HD bool ntt_two_pass(u32 logn) { return logn > 12; }
-CK_HD u32 ntt_tiles(u32 logn) { return logn > 12 ? (1u << (logn - 12)) : 1u; }
+// tiles per limb transform for two-pass tiles of 2^tb residues
+CK_HD u32 ntt_tiles(u32 logn, u32 tb = 12) { return logn > 12 ? (1u << (logn - tb)) : 1u; }
 
 // Dispatch on log N to the instantiated pass.  first = first pass of the
-// transform (forward: column/full, inverse: row/full).
-template <bool FWD, bool FIRST, bool FP, class Exec, class Regs>
+// transform (forward: column/full, inverse: row/full).  TB = log2 of the
+// two-pass tile size (12: 256 threads, 13: 512 threads; the full pass for
+// N <= 4096 always uses 256 threads); BLK = blocked inter-pass layout.
+template <bool FWD, bool FIRST, bool FP, int TB = 12, bool BLK = false, class Exec, class Regs>
 CK_HD void ntt_dispatch(Exec &ex, Regs &&regs_of, const TileJob &job, u64 *lds) {
     switch (job.logn) {
 #define CK_FULL(LG)                                                   \
@@ -267,9 +273,9 @@ CK_HD void ntt_dispatch(Exec &ex, Regs &&regs_of, const TileJob &job, u64 *lds) 
     case LG: {                                                                              \
         constexpr int N1 = LG - (LG + 1) / 2, N2 = (LG + 1) / 2;                            \
         if (FWD == FIRST)                                                                   \
-            NttPass<N1, 12 - N1, PASS_COL, FWD, FP>::run(ex, regs_of, job, lds);            \
+            NttPass<N1, TB - N1, PASS_COL, FWD, FP, BLK>::run(ex, regs_of, job, lds);       \
         else                                                                                \
-            NttPass<N2, 12 - N2, PASS_ROW, FWD, FP>::run(ex, regs_of, job, lds);            \
+            NttPass<N2, TB - N2, PASS_ROW, FWD, FP, BLK>::run(ex, regs_of, job, lds);       \
         break;                                                                              \
     }
         CK_TWO(13)

@@ -209,13 +209,38 @@ CK_HD u32 col_pass_offset(const PassAddr &pa, u32 e) {
     return c + (t << pa.n2);
 }
 
-// Row pass, internal side: position m * N2 + r where m = bitrev_n1(u).
-template <int L, int CL>
+// Inter-pass (internal) position of element (m, c): row m of the row pass,
+// column c of the column pass.  Plain layout: m * N2 + c.  Blocked layout
+// (BLK): the intersection of one column tile (2^CLc columns) and one row
+// tile (2^CLr rows m sharing their low lt = log2(tiles) bits) is stored as
+// one contiguous chunk, so both passes move whole chunks of
+// 2^(CLr + CLc) residues (2 KiB at N = 2^16 with 4096-residue tiles) instead
+// of 2^CL-residue segments.  TB = log2 of the tile size.
+template <int TB, bool BLK>
+CK_HD u32 internal_offset(const PassAddr &pa, u32 m, u32 c) {
+    if (!BLK) return (m << pa.n2) + c;
+    const u32 lt = pa.logn - TB;
+    const u32 clc = TB - pa.n1, clr = TB - pa.n2;
+    const u32 m_lo = m & ((1u << lt) - 1), m_hi = m >> lt;
+    const u32 c_hi = c >> clc, c_lo = c & ((1u << clc) - 1);
+    return ((((m_lo << lt) + c_hi) << (clr + clc)) + (m_hi << clc)) + c_lo;
+}
+
+// Row pass, internal side: element (m, r) with m = bitrev_n1(u).
+template <int L, int CL, bool BLK>
 CK_HD u32 row_internal_offset(const PassAddr &pa, u32 e) {
     u32 r = e & ((1u << L) - 1), col = e >> L;
     u32 u = pa.tile * (1u << CL) + col;
     u32 m = bitrev(u, pa.n1);
-    return (m << pa.n2) + r;
+    return internal_offset<L + CL, BLK>(pa, m, r);
+}
+
+// Column pass, internal side: element (m = t, c).
+template <int L, int CL, bool BLK>
+CK_HD u32 col_internal_offset(const PassAddr &pa, u32 e) {
+    u32 t = e & ((1u << L) - 1), col = e >> L;
+    u32 c = pa.tile * (1u << CL) + col;
+    return internal_offset<L + CL, BLK>(pa, t, c);
 }
 
 // Row pass, natural side: bitrev_n2(r) * N1 + u.

@@ -69,12 +69,28 @@ int ckks_sync(ckks_context *ctx);
  * depend on it.  No reference counterpart. */
 int ckks_set_ntt_chunk_bytes(ckks_context *ctx, size_t bytes);
 
+/* Tuning: kernel variant of the two-pass NTT (N > 4096) for one degree
+ * 2^logn, a bit set: CKKS_NTT_VARIANT_BLOCKED stores the inter-pass buffer
+ * in chunks shared by one column tile and one row tile;
+ * CKKS_NTT_VARIANT_TILE8K uses 8192-residue tiles (512 threads; N >= 2^14,
+ * ignored below).  Every variant gives identical residues.  Default 0.
+ * No reference counterpart. */
+enum {
+    CKKS_NTT_VARIANT_BLOCKED = 1,
+    CKKS_NTT_VARIANT_TILE8K = 2,
+    CKKS_NTT_VARIANT_COUNT = 4
+};
+int ckks_set_ntt_variant(ckks_context *ctx, uint32_t logn, int variant);
+int ckks_get_ntt_variant(const ckks_context *ctx, uint32_t logn, int *variant);
+
 /* Tuning: engine of the exact ModUp/ModDown base conversion of hybrid keys.
- * CKKS_BASECONV_AUTO uses the int8 matrix cores (7-bit chunks, int32
- * accumulation) wherever every gadget block has >= 2 limbs;
- * CKKS_BASECONV_VALU forces the vector-ALU kernels.  Both are exact and give
- * identical residues.  No reference counterpart. */
-enum { CKKS_BASECONV_AUTO = 0, CKKS_BASECONV_VALU = 1 };
+ * CKKS_BASECONV_MFMA uses the int8 matrix cores (7-bit chunks, int32
+ * accumulation) wherever every gadget block has 2..9 limbs;
+ * CKKS_BASECONV_VALU uses the vector-ALU kernels; CKKS_BASECONV_AUTO (the
+ * default) uses the engine ckks_autotune_key_switch picked for the key, or
+ * VALU for an untuned key.  All engines are exact and give identical
+ * residues.  No reference counterpart. */
+enum { CKKS_BASECONV_AUTO = 0, CKKS_BASECONV_VALU = 1, CKKS_BASECONV_MFMA = 2 };
 int ckks_set_base_conversion(ckks_context *ctx, int mode);
 
 /* Device memory helpers for callers without their own allocator. */
@@ -124,6 +140,16 @@ typedef struct ckks_ntt_batch {
 } ckks_ntt_batch;
 int ckks_ntt_forward_batch(ckks_context *ctx, const ckks_ntt_batch *batch, size_t count);
 int ckks_ntt_inverse_batch(ckks_context *ctx, const ckks_ntt_batch *batch, size_t count);
+
+/* Autotuner: times every NTT variant on a forward + inverse transform of
+ * n_polys x n_limbs limbs of `plan` (scratch buffers allocated and freed
+ * here; synchronises the context stream), keeps the fastest for the plan's
+ * degree in the context and returns it.  ms_per_variant (optional, length
+ * CKKS_NTT_VARIANT_COUNT) receives the measured time of each variant in
+ * milliseconds, -1 for a variant that does not apply.  Not callable during
+ * graph capture.  No reference counterpart. */
+int ckks_autotune_ntt(const ckks_rns_plan *plan, size_t n_polys, size_t n_limbs,
+                      int *best_variant, float *ms_per_variant);
 
 /* NttPolynomial::pointwise_mul / pointwise_mul_add_assign / add / sub
  * (ring/ntt_polynomial.rs:160-270) and Polynomial::{add,sub,neg}
@@ -224,6 +250,15 @@ int ckks_relinearize(const ckks_ksk *key, const uint64_t *in3, uint64_t *out, si
 int ckks_multiply_relinearize_rescale(const ckks_ksk *relin_key, const uint64_t *ct0,
                                       const uint64_t *ct1, uint64_t *out, size_t n_cts,
                                       size_t in_stride, size_t out_stride);
+
+/* Autotuner: times one key switch of n_cts ciphertexts with each
+ * base-conversion engine the key supports (scratch ciphertexts allocated
+ * and freed here; synchronises the context stream) and records the faster
+ * one in the key for CKKS_BASECONV_AUTO.  ms_valu / ms_mfma (optional)
+ * receive the times, -1 where an engine does not apply.  Not callable
+ * during graph capture.  No reference counterpart. */
+int ckks_autotune_key_switch(ckks_ksk *key, size_t n_cts, int *best_engine, float *ms_valu,
+                             float *ms_mfma);
 
 /* ------------------------------------------------------------ graphs */
 /* Rotation schedules (BSGS of CCMM / FFT) captured once as a hipGraph and

@@ -11,23 +11,39 @@
 
 using namespace ck;
 
-template <bool FWD, bool FIRST, bool FP>
+template <bool FWD, bool FIRST, bool FP, int TB, bool BLK>
 static void run_pass(const TileJob &base, u32 tiles) {
-    static u64 regs[256][16];
-    static u64 lds[4096];
+    static u64 regs[1 << (TB - 4)][16];
+    static u64 lds[1 << TB];
     for (u32 tile = 0; tile < tiles; ++tile) {
         TileJob job = base;
         job.tile = tile;
-        BlockExec ex{256};
+        BlockExec ex{1u << (TB - 4)};
         std::memset(lds, 0xA5, sizeof lds);
         auto regs_of = [&](u32 tid) -> u64(&)[16] { return regs[tid]; };
-        ntt_dispatch<FWD, FIRST, FP>(ex, regs_of, job, lds);
+        ntt_dispatch<FWD, FIRST, FP, TB, BLK>(ex, regs_of, job, lds);
     }
 }
 
-// Transforms one limb of 2^logn residues modulo q (< 2^62).
-extern "C" int emu_ntt(const uint64_t *in, uint64_t *out, uint64_t q, uint32_t logn, int forward) {
-    if (logn < 10 || logn > 17) return -1;
+template <bool FWD, bool FIRST, bool FP>
+static void run_variant(const TileJob &job, u32 logn, int tb, int blk) {
+    const bool two = ntt_two_pass(logn);
+    if (two && tb == 13) {
+        if (blk) run_pass<FWD, FIRST, FP, 13, true>(job, ntt_tiles(logn, 13));
+        else run_pass<FWD, FIRST, FP, 13, false>(job, ntt_tiles(logn, 13));
+    } else {
+        if (blk && two) run_pass<FWD, FIRST, FP, 12, true>(job, ntt_tiles(logn));
+        else run_pass<FWD, FIRST, FP, 12, false>(job, ntt_tiles(logn));
+    }
+}
+
+// Transforms one limb of 2^logn residues modulo q (< 2^62) with kernel
+// variant (tb = log2 two-pass tile size 12 or 13, blk = blocked inter-pass
+// layout).
+extern "C" int emu_ntt_variant(const uint64_t *in, uint64_t *out, uint64_t q, uint32_t logn,
+                               int forward, int tb, int blk) {
+    if (logn < 10 || logn > 17 || (tb != 12 && tb != 13)) return -1;
+    if (tb == 13 && logn < 14) return -1;
     uint64_t psi = host::negacyclic_root(q, 1ull << logn);
     if (!psi) return -2;
     std::vector<uint64_t> tw;
@@ -53,8 +69,8 @@ extern "C" int emu_ntt(const uint64_t *in, uint64_t *out, uint64_t q, uint32_t l
     job.dst = two ? tmp.data() : out;
 #define CK_RUN(F, S)                                              \
     do {                                                          \
-        if (fp) run_pass<F, S, true>(job, ntt_tiles(logn));       \
-        else run_pass<F, S, false>(job, ntt_tiles(logn));         \
+        if (fp) run_variant<F, S, true>(job, logn, tb, blk);      \
+        else run_variant<F, S, false>(job, logn, tb, blk);        \
     } while (0)
     if (forward) CK_RUN(true, true);
     else CK_RUN(false, true);
@@ -65,4 +81,8 @@ extern "C" int emu_ntt(const uint64_t *in, uint64_t *out, uint64_t q, uint32_t l
         else CK_RUN(false, false);
     }
     return 0;
+}
+
+extern "C" int emu_ntt(const uint64_t *in, uint64_t *out, uint64_t q, uint32_t logn, int forward) {
+    return emu_ntt_variant(in, out, q, logn, forward, 12, 0);
 }

@@ -44,3 +44,26 @@ def test_emulated_kernel_ntt_matches_oracle(emu, logn):
         y = rng.integers(0, q, n, dtype=np.uint64)
         assert emu.emu_ntt(y.ctypes.data, back.ctypes.data, q, logn, 0) == 0
         assert (back == plan.inverse(y)).all()
+
+
+@pytest.mark.parametrize("logn", [13, 14, 15, 16, 17])
+@pytest.mark.parametrize("tb,blk", [(12, 1), (13, 0), (13, 1)])
+def test_emulated_kernel_variants_match_oracle(emu, logn, tb, blk):
+    """Every NTT kernel variant the autotuner may pick (tile of 2^tb
+    residues, blocked inter-pass layout) gives the oracle's residues."""
+    if tb == 13 and logn < 14:
+        pytest.skip("8192-residue tiles serve N >= 2^14")
+    emu.emu_ntt_variant.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32, C.c_int,
+                                    C.c_int, C.c_int]
+    n = 1 << logn
+    rng = np.random.default_rng(100 + logn)
+    primes = [Q40[1], P60[1]] if logn < 17 else [O.ntt_primes(60, 1 << 17, 1)[0]]
+    for q in primes:
+        plan = O.Plan(q, n)
+        x = rng.integers(0, q, n, dtype=np.uint64)
+        out = np.zeros(n, dtype=np.uint64)
+        assert emu.emu_ntt_variant(x.ctypes.data, out.ctypes.data, q, logn, 1, tb, blk) == 0
+        assert (out == plan.forward(x)).all()
+        back = np.zeros(n, dtype=np.uint64)
+        assert emu.emu_ntt_variant(x.ctypes.data, back.ctypes.data, q, logn, 0, tb, blk) == 0
+        assert (back == plan.inverse(x)).all()

@@ -54,7 +54,7 @@ def test_hybrid_key_switch_bit_exact(pkg, ctx, logn, L, K, blocks, engine):
     # residues at the extremes exercise the largest chunk sums
     ct[0, 1, :, :8] = np.array(q_moduli, dtype=np.uint64)[:, None] - 1
     ct[0, 1, :, 8:16] = 0
-    ctx.set_base_conversion(ctx.BASECONV_VALU if engine == "valu" else ctx.BASECONV_AUTO)
+    ctx.set_base_conversion(ctx.BASECONV_VALU if engine == "valu" else ctx.BASECONV_MFMA)
     try:
         out = pkg.to_host(key.switch(pkg.to_device(ct)))
     finally:

@@ -1,0 +1,94 @@
+"""Autotuner: every NTT kernel variant and every key-switch engine it can
+pick gives the oracle's residues bit-exactly, and a tuned context keeps
+giving them."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import O, P60, Q0, Q40, rand_residues
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("logn", [13, 14, 15, 16, 17])
+@pytest.mark.parametrize("variant", [1, 2, 3])
+def test_ntt_variant_bit_exact(pkg, ctx, logn, variant):
+    n = 1 << logn
+    moduli = [Q0[0], Q40[0], Q40[1], P60[0]] if logn < 17 else O.ntt_primes(60, 1 << 17, 2)
+    rng = np.random.default_rng(10 * logn + variant)
+    x = rand_residues(rng, moduli, n, polys=3)
+    x[1, :, :4] = np.array(moduli, dtype=np.uint64)[:, None] - 1
+    plan = pkg.PreparedRnsNttPlan(ctx, moduli, n)
+    orc = O.RnsPlan(moduli, n)
+    saved = ctx.ntt_variant(logn)
+    ctx.set_ntt_variant(logn, variant)
+    try:
+        out = pkg.empty(x.shape)
+        plan.forward(pkg.to_device(x), out)
+        got = pkg.to_host(out)
+        back = pkg.to_host(plan.inverse(pkg.to_device(x)))
+    finally:
+        ctx.set_ntt_variant(logn, saved)
+    for p in range(3):
+        assert (got[p] == orc.forward(x[p])).all(), f"forward poly {p}"
+        assert (back[p] == orc.inverse(x[p])).all(), f"inverse poly {p}"
+
+
+def test_mixed_degree_batch_across_variants(pkg, ctx):
+    """One batched call whose entries run different kernel variants."""
+    rng = np.random.default_rng(5)
+    moduli = [Q0[0], Q40[0], Q40[1]]
+    entries, want = [], []
+    for logn, var in ((13, 1), (14, 2), (15, 0), (16, 3)):
+        ctx.set_ntt_variant(logn, var)
+        n = 1 << logn
+        plan = pkg.PreparedRnsNttPlan(ctx, moduli, n)
+        x = rand_residues(rng, moduli, n, polys=2)
+        orc = O.RnsPlan(moduli, n)
+        entries.append((plan, pkg.to_device(x), pkg.empty(x.shape), None))
+        want.append(np.stack([orc.forward(x[p]) for p in range(2)]))
+    try:
+        pkg.ntt_batch(ctx, entries, forward=True)
+        for e, w in zip(entries, want):
+            assert (pkg.to_host(e[2]) == w).all()
+    finally:
+        for logn in (13, 14, 15, 16):
+            ctx.set_ntt_variant(logn, 0)
+
+
+def test_autotune_ntt_then_bit_exact(pkg, ctx):
+    n = 1 << 16
+    moduli = [Q0[0]] + Q40[:5] + P60[:2]
+    plan = pkg.PreparedRnsNttPlan(ctx, moduli, n)
+    best, ms = plan.autotune_ntt(4)
+    assert 0 <= best < 4 and ms[best] > 0
+    assert all(m > 0 for m in ms)  # all four variants apply at 2^16
+    assert ctx.ntt_variant(16) == best
+    rng = np.random.default_rng(6)
+    x = rand_residues(rng, moduli, n, polys=2)
+    got = pkg.to_host(plan.forward(pkg.to_device(x)))
+    orc = O.RnsPlan(moduli, n)
+    for p in range(2):
+        assert (got[p] == orc.forward(x[p])).all()
+    ctx.set_ntt_variant(16, 0)
+
+
+def test_autotune_key_switch_engine(pkg, ctx):
+    n, L, K, blocks = 1 << 13, 6, 3, [3, 3]
+    q_moduli = [Q0[0]] + Q40[:L - 1]
+    qp = q_moduli + P60[:K]
+    rng = np.random.default_rng(8)
+    kb = rand_residues(rng, qp, n, len(blocks))
+    ka = rand_residues(rng, qp, n, len(blocks))
+    plan = pkg.PreparedRnsNttPlan(ctx, qp, n)
+    key = pkg.HybridKeySwitchKey(plan, L, blocks, kb, ka)
+    ct = rand_residues(rng, q_moduli, n, polys=4).reshape(2, 2, L, n)
+    ctx.set_base_conversion(ctx.BASECONV_VALU)
+    want = pkg.to_host(key.switch(pkg.to_device(ct)))
+    ctx.set_base_conversion(ctx.BASECONV_AUTO)
+    engine, ms_valu, ms_mfma = key.autotune(2)
+    assert engine in (ctx.BASECONV_VALU, ctx.BASECONV_MFMA)
+    assert ms_valu > 0 and ms_mfma > 0
+    got = pkg.to_host(key.switch(pkg.to_device(ct)))
+    assert (got == want).all()
