@@ -111,6 +111,47 @@ def cpu_baseline(q_moduli, p_moduli, n, blocks, seconds=10.0):
                       f"({el:.1f} s, oracle in C, one thread)"}
 
 
+def measure_ntt(torch, plan, polys, limbs, n, reps=5):
+    """Forward and inverse NTT of `polys` x `limbs` limbs (the digit limbs of
+    a key-switch batch), timed live on the library's stream (the null
+    stream, which torch's events also use)."""
+    digits = rand_residues(torch, (polys, limbs, n), "cuda")
+    plan.forward(digits)
+    plan.inverse(digits)
+    torch.cuda.synchronize()
+    with Timer(torch) as tn:
+        for _ in range(reps):
+            plan.forward(digits)
+    ntt_ms = tn.ms / reps
+    limb_transforms = polys * limbs
+    ntt_bytes = 2 * n * 8 * limb_transforms
+    ntt_gbps = ntt_bytes / (ntt_ms / 1e3) / 1e9
+    with Timer(torch) as ti:
+        for _ in range(reps):
+            plan.inverse(digits)
+    intt_gbps = ntt_bytes / (ti.ms / reps / 1e3) / 1e9
+    del digits
+    print(f"# NTT fwd {ntt_ms:.2f} ms over {limb_transforms} limbs: {ntt_gbps:.0f} GB/s; "
+          f"inv {intt_gbps:.0f} GB/s", file=sys.stderr, flush=True)
+    return limb_transforms, ntt_ms, ntt_gbps, intt_gbps
+
+
+def copy_calibration(torch, nbytes=1 << 30, reps=5):
+    """A plain device copy of known size: the reference point for reading
+    the HBM counters (FETCH_SIZE / WRITE_SIZE) of the NTT kernels."""
+    src = torch.empty(nbytes // 8, dtype=torch.int64, device="cuda")
+    src.fill_(1)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    with Timer(torch) as t:
+        for _ in range(reps):
+            dst.copy_(src)
+    gbps = 2 * nbytes / (t.ms / reps / 1e3) / 1e9
+    print(f"# copy {nbytes >> 20} MiB: {gbps:.0f} GB/s (read + write)", file=sys.stderr, flush=True)
+    return gbps
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,6 +162,11 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=6, help="plain-path rotations timed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--fusion", choices=["auto", "off", "xcd", "linear"], default="auto",
+                    help="digits' forward row pass fused with the key MAC: autotuned per key, "
+                         "off, or fused with XCD-contiguous / linear block order")
+    ap.add_argument("--only", choices=["all", "ntt"], default="all",
+                    help="ntt: only the NTT measurement and a copy of known size (for counter runs)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default NTT variant and base-conversion engine")
     ap.add_argument("--base-conversion", choices=["auto", "valu", "mfma"], default="auto",
@@ -138,6 +184,8 @@ def main():
     dev = "cuda"
     torch.cuda.init()
     ctx = pkg.Context(0)
+    ctx.set_key_switch_fusion({"auto": ctx.KS_FUSION_AUTO, "off": ctx.KS_FUSION_OFF,
+                               "xcd": ctx.KS_FUSION_XCD, "linear": ctx.KS_FUSION_LINEAR}[args.fusion])
     ctx.set_base_conversion({"auto": ctx.BASECONV_AUTO, "valu": ctx.BASECONV_VALU,
                              "mfma": ctx.BASECONV_MFMA}[args.base_conversion])
 
@@ -152,6 +200,17 @@ def main():
     qp_plan = pkg.PreparedRnsNttPlan(ctx, q_moduli + p_moduli, N)
     qp2 = q_moduli[:L2] + p_moduli[:K2]
     qp_plan2 = pkg.PreparedRnsNttPlan(ctx, qp2, N2)
+
+    if args.only == "ntt":
+        if not args.no_autotune:
+            v16, _ = qp_plan.autotune_ntt(16)
+            print(f"# autotune NTT: 2^16 variant {v16}", file=sys.stderr, flush=True)
+        lt, ntt_ms, fwd, inv = measure_ntt(torch, qp_plan, args.batch * len(blocks), L + K, N)
+        cp = copy_calibration(torch)
+        print(json.dumps({"ntt_limb_transforms": lt, "ntt_forward_ms": round(ntt_ms, 3),
+                          "ntt_forward_gbps": round(fwd, 1), "ntt_inverse_gbps": round(inv, 1),
+                          "copy_gbps": round(cp, 1)}), flush=True)
+        return
 
     print(f"# building {len(rot_steps)} Galois keys per degree", file=sys.stderr, flush=True)
     keys, keys2 = [], []
@@ -171,14 +230,19 @@ def main():
         tuned["ntt_variant"] = {"2^16": v16, "2^15": v15}
         print(f"# autotune NTT: 2^16 variant {v16} {['%.2f' % m for m in ms16]} ms; "
               f"2^15 variant {v15}", file=sys.stderr, flush=True)
-        if args.base_conversion == "auto":
-            engines = set()
-            for (_, key), n in [(kk, args.batch) for kk in keys] + [(kk, max(args.mixed, 1)) for kk in keys2]:
-                eng, mv, mm = key.autotune(n)
-                engines.add(eng)
-            tuned["base_conversion"] = sorted("mfma" if e == ctx.BASECONV_MFMA else "valu" for e in engines)
-            print(f"# autotune base conversion (last key): valu {mv:.2f} ms, mfma {mm:.2f} ms",
-                  file=sys.stderr, flush=True)
+        if args.base_conversion == "auto" and args.fusion == "auto":
+            picks = {}
+            for deg, kl, n in (("2^16", keys, args.batch), ("2^15", keys2, max(args.mixed, 1))):
+                for _, key in kl:
+                    eng, fus, times = key.autotune(n)
+                    name = ("mfma" if eng == ctx.BASECONV_MFMA else "valu") + "/" + \
+                        ["off", "xcd", "linear"][fus]
+                    picks.setdefault(deg, {}).setdefault(name, 0)
+                    picks[deg][name] += 1
+                if kl:
+                    print(f"# autotune key switch {deg} (last key, ms per (engine, fusion)): "
+                          f"{ {k: round(v, 2) for k, v in times.items()} }", file=sys.stderr, flush=True)
+            tuned["key_switch"] = picks
 
     B = args.batch
     cts = rand_residues(torch, (B, 2, L, N), dev)
@@ -240,26 +304,7 @@ def main():
     print(f"# plain: {tp.ms / args.plain_sample:.2f} ms/rotation, {plain_value:.0f} rot/s", file=sys.stderr,
           flush=True)
 
-    # ---- dominant kernel: forward NTT of the digit limbs of the batch
-    # (dnum * (L+K) limb transforms per ciphertext), timed live
-    digits = rand_residues(torch, (B * len(blocks), L + K, N), dev)
-    qp_plan.forward(digits)
-    torch.cuda.synchronize()
-    reps = 5
-    with Timer(torch) as tn:
-        for _ in range(reps):
-            qp_plan.forward(digits)
-    ntt_ms = tn.ms / reps
-    limb_transforms = B * len(blocks) * (L + K)
-    ntt_bytes = 2 * N * 8 * limb_transforms
-    ntt_gbps = ntt_bytes / (ntt_ms / 1e3) / 1e9
-    with Timer(torch) as ti:
-        for _ in range(reps):
-            qp_plan.inverse(digits)
-    intt_gbps = ntt_bytes / (ti.ms / reps / 1e3) / 1e9
-    del digits
-    print(f"# NTT fwd {ntt_ms:.2f} ms over {limb_transforms} limbs: {ntt_gbps:.0f} GB/s; "
-          f"inv {intt_gbps:.0f} GB/s", file=sys.stderr, flush=True)
+    limb_transforms, ntt_ms, ntt_gbps, intt_gbps = measure_ntt(torch, qp_plan, B * len(blocks), L + K, N)
 
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "ntt_forward_traffic.json")

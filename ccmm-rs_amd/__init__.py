@@ -58,9 +58,10 @@ _SIGNATURES = {
     "ckks_set_ntt_chunk_bytes": (C.c_int, [_p, _sz]),
     "ckks_set_base_conversion": (C.c_int, [_p, C.c_int]),
     "ckks_set_ntt_variant": (C.c_int, [_p, C.c_uint32, C.c_int]),
+    "ckks_set_key_switch_fusion": (C.c_int, [_p, C.c_int]),
     "ckks_get_ntt_variant": (C.c_int, [_p, C.c_uint32, C.POINTER(C.c_int)]),
     "ckks_autotune_ntt": (C.c_int, [_p, _sz, _sz, C.POINTER(C.c_int), C.POINTER(C.c_float)]),
-    "ckks_autotune_key_switch": (C.c_int, [_p, _sz, C.POINTER(C.c_int), C.POINTER(C.c_float),
+    "ckks_autotune_key_switch": (C.c_int, [_p, _sz, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                            C.POINTER(C.c_float)]),
     "ckks_malloc": (C.c_int, [_p, _sz, C.POINTER(_p)]),
     "ckks_free": (C.c_int, [_p, _p]),
@@ -226,6 +227,14 @@ class Context:
         """Two-pass NTT kernel variant for degree 2^logn (bit set of
         NTT_VARIANT_BLOCKED / NTT_VARIANT_TILE8K); results unchanged."""
         _check(lib().ckks_set_ntt_variant(self._h, int(logn), int(variant)))
+
+    KS_FUSION_AUTO, KS_FUSION_OFF, KS_FUSION_XCD, KS_FUSION_LINEAR = -1, 0, 1, 2
+
+    def set_key_switch_fusion(self, mode: int):
+        """Key switch: digits' forward row pass fused with the key MAC
+        (XCD or LINEAR block order), separate kernels (OFF), or the key's
+        tuned choice (AUTO, the default)."""
+        _check(lib().ckks_set_key_switch_fusion(self._h, int(mode)))
 
     def ntt_variant(self, logn: int) -> int:
         v = C.c_int()
@@ -471,14 +480,14 @@ class _KeySwitchKey:
         return out
 
     def autotune(self, n_cts: int):
-        """Times the key switch of n_cts ciphertexts with each base-conversion
-        engine and keeps the faster for BASECONV_AUTO; returns
-        (engine, ms_valu, ms_mfma) with -1 for an engine that does not apply."""
-        best = C.c_int()
-        mv, mm = C.c_float(), C.c_float()
-        _check(lib().ckks_autotune_key_switch(self._h, int(n_cts), C.byref(best), C.byref(mv),
-                                              C.byref(mm)))
-        return best.value, mv.value, mm.value
+        """Times the key switch of n_cts ciphertexts for every (base-conversion
+        engine, fusion mode) pair and keeps the fastest for the AUTO modes;
+        returns (engine, fusion, {(engine, fusion): ms}) over the pairs that apply."""
+        be, bf = C.c_int(), C.c_int()
+        ms = (C.c_float * 6)()
+        _check(lib().ckks_autotune_key_switch(self._h, int(n_cts), C.byref(be), C.byref(bf), ms))
+        times = {(e, f): ms[(e - 1) * 3 + f] for e in (1, 2) for f in range(3) if ms[(e - 1) * 3 + f] >= 0}
+        return be.value, bf.value, times
 
     def close(self):
         if getattr(self, "_h", None) and self.ctx.handle:

@@ -172,6 +172,14 @@ int ckks_set_ntt_variant(ckks_context *ctx, uint32_t logn, int variant) {
     return CKKS_OK;
 }
 
+int ckks_set_key_switch_fusion(ckks_context *ctx, int mode) {
+    if (!ctx) return fail(CKKS_ERR_INVALID, "null context");
+    if (mode < CKKS_KS_FUSION_AUTO || mode > CKKS_KS_FUSION_LINEAR)
+        return fail(CKKS_ERR_INVALID, "unknown key-switch fusion mode");
+    ctx->ks_fusion = mode;
+    return CKKS_OK;
+}
+
 int ckks_get_ntt_variant(const ckks_context *ctx, uint32_t logn, int *variant) {
     if (!ctx || !variant || logn >= 32) return fail(CKKS_ERR_INVALID, "invalid argument");
     *variant = ctx->ntt_variant[logn];

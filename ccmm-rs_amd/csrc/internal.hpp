@@ -80,6 +80,7 @@ struct ckks_context {
     size_t ntt_chunk_bytes = 0;
     int base_conversion = CKKS_BASECONV_AUTO;
     uint8_t ntt_variant[32] = {};  // two-pass NTT kernel variant per log2 N
+    int ks_fusion = CKKS_KS_FUSION_AUTO;  // digits' row pass fused with the key MAC
     bool capturing = false;  // a hipGraph capture of `stream` is open
     std::vector<void *> owned;  // misc device allocations freed at destroy
 
@@ -109,6 +110,11 @@ struct NttCall {
 };
 int launch_ntt(ckks_context *ctx, const NttCall *calls, size_t count, bool forward);
 int ntt_variant_for(const ckks_context *ctx, u32 logn);
+// Forward NTT of key-switch digits [ct][dnum][limbs][N] fused with the key
+// inner product into acc [ct][2][limbs][N] (two-pass degrees only).
+int launch_ntt_mac(ckks_context *ctx, const ckks_rns_plan *plan, const u64 *digits, size_t n_cts,
+                   size_t dnum, size_t limbs, const u64 *kb, const u64 *ka, u64 *acc,
+                   bool xcd_order);
 
 int check_plan_limbs(const ckks_rns_plan *plan, size_t n_limbs, const char *what);
 

@@ -68,6 +68,7 @@ struct ckks_ksk {
     // exact int8-MFMA base conversion: dnum ModUp plans then the ModDown plan
     int mfma = 0;
     int tuned_engine = CKKS_BASECONV_VALU;  // engine of CKKS_BASECONV_AUTO for this key
+    int tuned_fusion = CKKS_KS_FUSION_OFF;  // mode of CKKS_KS_FUSION_AUTO for this key
     ck::BcPlan *d_bc = nullptr;
     signed char *d_bt = nullptr;
 };
@@ -847,10 +848,15 @@ static int key_switch_core(const ckks_ksk *k, const u64 *x, u64 x_stride, const 
         hipLaunchKernelGGL(k_modup, dim3((u32)(n_cts * D * chunks)), dim3(kKsThreads), 0, s, mu);
     }
     CK_HIP(hipGetLastError());
-    // 2. forward NTT of every digit limb
+    // 2+3. forward NTT of every digit limb, key MAC
+    const int fusion = ctx->ks_fusion == CKKS_KS_FUSION_AUTO ? k->tuned_fusion : ctx->ks_fusion;
+    if (fusion != CKKS_KS_FUSION_OFF && logn > 12 && logn <= 17) {
+        if ((rc = launch_ntt_mac(ctx, plan, digits, n_cts, D, B, k->d_kb, k->d_ka, acc,
+                                 fusion == CKKS_KS_FUSION_XCD)))
+            return rc;
+    } else {
     NttCall f{plan, digits, digits, n_cts * D, B, B * N, B * N};
     if ((rc = launch_ntt(ctx, &f, 1, true))) return rc;
-    // 3. MAC
     MacArgs ma{};
     ma.digits = digits;
     ma.kb = k->d_kb;
@@ -871,6 +877,7 @@ static int key_switch_core(const ckks_ksk *k, const u64 *x, u64 x_stride, const 
             hipLaunchKernelGGL(k_ks_mac_any, dim3((u32)(n_cts * B * chunks)), dim3(kKsThreads), 0, s, ma);
     }
     CK_HIP(hipGetLastError());
+    }
     // 4. inverse NTT of both accumulators
     NttCall iv{plan, acc, acc, n_cts * 2, B, B * N, B * N};
     if ((rc = launch_ntt(ctx, &iv, 1, false))) return rc;
@@ -959,62 +966,69 @@ int ckks_ksk_destroy(ckks_ksk *k) {
     return CKKS_OK;
 }
 
-int ckks_autotune_key_switch(ckks_ksk *k, size_t n_cts, int *best_engine, float *ms_valu,
-                             float *ms_mfma) {
-    if (!k || !best_engine) return fail(CKKS_ERR_INVALID, "null argument");
+int ckks_autotune_key_switch(ckks_ksk *k, size_t n_cts, int *best_engine, int *best_fusion,
+                             float *ms) {
+    if (!k || !best_engine || !best_fusion) return fail(CKKS_ERR_INVALID, "null argument");
     ckks_context *ctx = k->ctx;
     if (ctx->capturing) return fail(CKKS_ERR_INVALID, "autotuning is not allowed during graph capture");
     if (n_cts == 0) return fail(CKKS_ERR_INVALID, "autotune needs at least one ciphertext");
-    if (ms_valu) *ms_valu = -1.f;
-    if (ms_mfma) *ms_mfma = -1.f;
+    if (ms)
+        for (int i = 0; i < 6; ++i) ms[i] = -1.f;
     const size_t poly = k->q_limbs * k->degree, per = 2 * poly;
     CK_HIP(hipSetDevice(ctx->device));
     u64 *buf = nullptr;
     CK_HIP(hipMalloc(&buf, 2 * n_cts * per * sizeof(u64)));
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    const int saved_mode = ctx->base_conversion;
-    int best = CKKS_BASECONV_VALU;
+    const int saved_mode = ctx->base_conversion, saved_fusion = ctx->ks_fusion;
+    int best_e = CKKS_BASECONV_VALU, best_f = CKKS_KS_FUSION_OFF;
     float best_ms = 0.f;
+    const bool can_fuse = k->plan->logn > 12 && k->plan->logn <= 17;
     auto body = [&]() -> int {
         CK_HIP(hipMemsetAsync(buf, 0, 2 * n_cts * per * sizeof(u64), ctx->stream));
         CK_HIP(hipEventCreate(&e0));
         CK_HIP(hipEventCreate(&e1));
         u64 *in = buf, *out = buf + n_cts * per;
-        const int engines[2] = {CKKS_BASECONV_VALU, CKKS_BASECONV_MFMA};
-        for (int engine : engines) {
+        for (int engine = CKKS_BASECONV_VALU; engine <= CKKS_BASECONV_MFMA; ++engine) {
             if (engine == CKKS_BASECONV_MFMA && !k->mfma) continue;
-            ctx->base_conversion = engine;
-            int r = key_switch_core(k, in + poly, per, in, nullptr, per, out, per, n_cts);
-            if (r) return r;
-            const int reps = 2;
-            CK_HIP(hipEventRecord(e0, ctx->stream));
-            for (int i = 0; i < reps; ++i) {
-                r = key_switch_core(k, in + poly, per, in, nullptr, per, out, per, n_cts);
+            for (int fusion = CKKS_KS_FUSION_OFF; fusion <= CKKS_KS_FUSION_LINEAR; ++fusion) {
+                if (fusion != CKKS_KS_FUSION_OFF && !can_fuse) continue;
+                ctx->base_conversion = engine;
+                ctx->ks_fusion = fusion;
+                int r = key_switch_core(k, in + poly, per, in, nullptr, per, out, per, n_cts);
                 if (r) return r;
-            }
-            CK_HIP(hipEventRecord(e1, ctx->stream));
-            CK_HIP(hipEventSynchronize(e1));
-            float ms = 0.f;
-            CK_HIP(hipEventElapsedTime(&ms, e0, e1));
-            ms /= reps;
-            if (engine == CKKS_BASECONV_VALU && ms_valu) *ms_valu = ms;
-            if (engine == CKKS_BASECONV_MFMA && ms_mfma) *ms_mfma = ms;
-            if (best_ms == 0.f || ms < best_ms) {
-                best_ms = ms;
-                best = engine;
+                const int reps = 2;
+                CK_HIP(hipEventRecord(e0, ctx->stream));
+                for (int i = 0; i < reps; ++i) {
+                    r = key_switch_core(k, in + poly, per, in, nullptr, per, out, per, n_cts);
+                    if (r) return r;
+                }
+                CK_HIP(hipEventRecord(e1, ctx->stream));
+                CK_HIP(hipEventSynchronize(e1));
+                float t = 0.f;
+                CK_HIP(hipEventElapsedTime(&t, e0, e1));
+                t /= reps;
+                if (ms) ms[(engine - 1) * 3 + fusion] = t;
+                if (best_ms == 0.f || t < best_ms) {
+                    best_ms = t;
+                    best_e = engine;
+                    best_f = fusion;
+                }
             }
         }
         return CKKS_OK;
     };
     int rc = body();
     ctx->base_conversion = saved_mode;
+    ctx->ks_fusion = saved_fusion;
     hipStreamSynchronize(ctx->stream);
     if (e0) hipEventDestroy(e0);
     if (e1) hipEventDestroy(e1);
     hipFree(buf);
     if (rc) return rc;
-    k->tuned_engine = best;
-    *best_engine = best;
+    k->tuned_engine = best_e;
+    k->tuned_fusion = best_f;
+    *best_engine = best_e;
+    *best_fusion = best_f;
     return CKKS_OK;
 }
 

@@ -221,6 +221,174 @@ int launch_ntt(ckks_context *ctx, const NttCall *calls, size_t count, bool forwa
     return CKKS_OK;
 }
 
+// ---- forward NTT fused with the key-switch inner product -----------------------
+//
+// The digits of a key switch ([ct][dnum][B][N], coefficient domain) go
+// through the column pass as usual; the row pass of every digit j of one
+// (ciphertext, limb, tile) then runs in the same block, and instead of
+// storing the NTT-domain digit it multiplies it with the key entries
+// (kb_j, ka_j) at the same positions and accumulates both products in
+// registers.  Only the two accumulators reach HBM: the NTT-domain digits
+// are never written or read back.
+//
+// Block order: (limb, tile) outer, ciphertext inner, remapped so that each
+// XCD (blocks are dispatched round-robin over the 8 XCDs) runs a contiguous
+// range: the blocks sharing one key tile meet in the same L2.
+
+struct RowMacArgs {
+    const u64 *tmp;      // column-pass output, limb-major limb transforms
+    const u64 *kb, *ka;  // [dnum][B][N] NTT domain
+    u64 *acc;            // [ct][2][B][N]
+    const DevPrime *primes;
+    u32 n_cts, dnum, limbs, logn;
+    u32 xcd_order;  // 1: remap blocks so each XCD runs a contiguous range
+};
+
+constexpr u32 kXcds = 8;
+
+__device__ __forceinline__ u32 xcd_contiguous_block(u32 b, u32 nb) {
+    const u32 full = nb - nb % kXcds;
+    return b < full ? (b % kXcds) * (full / kXcds) + b / kXcds : b;
+}
+
+// Reloads the thread id as a value the compiler cannot see through, so index
+// math derived from it is recomputed where it is used.
+__device__ __forceinline__ void opaque_tid(BlockExec &ex) {
+    ex.tid = threadIdx.x;
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" : "+v"(ex.tid));
+#endif
+}
+
+template <int TB, bool BLK, bool FP, int LG>
+__device__ void row_mac_body(const RowMacArgs &a, const DevPrime &P, u32 ct, u32 tile, u32 i,
+                             u64 *lds) {
+    constexpr int N1 = LG - (LG + 1) / 2, N2 = (LG + 1) / 2;
+    typedef NttPass<N2, TB - N2, PASS_ROW, true, FP, BLK> Pass;
+    constexpr u32 T = 1u << (TB - 4);
+    (void)N1;
+    const u64 N = (u64)1 << LG;
+    TileJob job;
+    job.tw = P.fwd;
+    job.q = P.q;
+    job.ninv = P.ninv;
+    job.ninv_q = P.ninv_q;
+    job.qd = P.qd;
+    job.qinv = P.qinv;
+    job.ninv_d = P.ninv_d;
+    job.ninv_p = P.ninv_p;
+    job.logn = LG;
+    job.tile = tile;
+    job.dst = nullptr;
+    const PassAddr pa = Pass::pass_addr(job);
+    const Barrett br{P.q, P.br_hi, P.br_lo};
+    u64 regs[16];
+    BlockExec ex{T};
+    auto off = [&](int r) { return row_natural_offset<N2, TB - N2>(pa, Pass::stage_coord((u32)r * T + ex.tid)); };
+    u64 sb[16], sa[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) sb[r] = sa[r] = 0;
+    auto regs_of = [&](u32) -> u64(&)[16] { return regs; };
+    const u64 polys = (u64)a.n_cts * a.dnum;
+#pragma unroll 1
+    for (u32 j = 0; j < a.dnum; ++j) {
+        job.src = a.tmp + ((u64)i * polys + (u64)ct * a.dnum + j) * N;
+        // opaque per digit: stops the index math of 16 elements x every
+        // butterfly from being hoisted out of the loop (and kept in VGPRs)
+        opaque_tid(ex);
+        Pass::run_to_lds(ex, regs_of, job, lds);
+        const u64 *kb = a.kb + ((u64)j * a.limbs + i) * N;
+        const u64 *ka = a.ka + ((u64)j * a.limbs + i) * N;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const u32 e = Pass::stage_coord((u32)r * T + ex.tid);
+            const u64 d = Pass::finish(lds[lds_index<N2, TB - N2>(e)], job);
+            const u32 o = off(r);
+            sb[r] = add_mod(sb[r], mul_mod(d, kb[o], br), P.q);
+            sa[r] = add_mod(sa[r], mul_mod(d, ka[o], br), P.q);
+            // keep the scheduler from hoisting all 48 loads of the epilogue
+            if ((r & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+        }
+        __syncthreads();  // the next digit's pass reuses lds
+    }
+    opaque_tid(ex);
+    u64 *ob = a.acc + ((u64)ct * 2 * a.limbs + i) * N;
+    u64 *oa = ob + (u64)a.limbs * N;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const u32 o = off(r);
+        ob[o] = sb[r];
+        oa[o] = sa[r];
+    }
+}
+
+template <int TB, bool BLK, int LG>
+__global__ __launch_bounds__(1 << (TB - 4)) void k_ntt_row_mac(RowMacArgs a) {
+    __shared__ u64 lds[1 << TB];
+    const u32 tiles = 1u << (LG - TB);
+    const u32 lb = a.xcd_order ? xcd_contiguous_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    const u32 ct = lb % a.n_cts;
+    const u32 rest = lb / a.n_cts;
+    const u32 tile = rest % tiles, i = rest / tiles;
+    const DevPrime &P = a.primes[i];
+    if (P.fp) row_mac_body<TB, BLK, true, LG>(a, P, ct, tile, i, lds);
+    else row_mac_body<TB, BLK, false, LG>(a, P, ct, tile, i, lds);
+}
+
+template <int TB, bool BLK>
+static int launch_row_mac(const RowMacArgs &a, dim3 g, hipStream_t s) {
+    const dim3 t(1u << (TB - 4));
+    switch (a.logn) {
+#define CK_RM(LG) \
+    case LG: hipLaunchKernelGGL((k_ntt_row_mac<TB, BLK, LG>), g, t, 0, s, a); return CKKS_OK;
+        CK_RM(13) CK_RM(14) CK_RM(15) CK_RM(16) CK_RM(17)
+#undef CK_RM
+        default: break;
+    }
+    return fail(CKKS_ERR_UNSUPPORTED, "fused NTT-MAC: unsupported degree");
+}
+
+int launch_ntt_mac(ckks_context *ctx, const ckks_rns_plan *plan, const u64 *digits, size_t n_cts,
+                   size_t dnum, size_t limbs, const u64 *kb, const u64 *ka, u64 *acc,
+                   bool xcd_order) {
+    const u32 logn = plan->logn;
+    if (!ntt_two_pass(logn) || logn > 17) return fail(CKKS_ERR_UNSUPPORTED, "fused NTT-MAC needs 2^13 <= N <= 2^17");
+    // 4096-residue tiles: the 8192-residue fused kernel needs more registers
+    // than a CU can give two waves per SIMD without spilling
+    const int var = ntt_variant_for(ctx, logn) & CKKS_NTT_VARIANT_BLOCKED;
+    const u32 tb = 12;
+    const size_t N = plan->degree, polys = n_cts * dnum, lts = polys * limbs;
+    const size_t tiles = ntt_tiles(logn, tb);
+    if (lts * tiles >= (1ull << 31) || n_cts * limbs * tiles >= (1ull << 31))
+        return fail(CKKS_ERR_INVALID, "key-switch batch too large for one launch");
+    void *ws;
+    int rc = ctx->workspace(0, lts * N * sizeof(u64), &ws);
+    if (rc) return rc;
+    // column pass of every digit limb into the inter-pass buffer
+    NttLaunch l{};
+    DevNttDesc &d = l.d[0];
+    d.src = digits;
+    d.dst = nullptr;
+    d.tmp = (u64 *)ws;
+    d.primes = plan->d_primes;
+    d.in_stride = limbs * N;
+    d.out_stride = limbs * N;
+    d.n_polys = (u32)polys;
+    d.n_limbs = (u32)limbs;
+    d.logn = logn;
+    d.block_start = 0;
+    d.lt_begin = 0;
+    l.n = 1;
+    launch_pass(ctx->stream, true, true, l, (u32)(lts * tiles), var);
+    RowMacArgs a{(const u64 *)ws, kb, ka, acc, plan->d_primes, (u32)n_cts, (u32)dnum, (u32)limbs, logn,
+                 xcd_order ? 1u : 0u};
+    const dim3 g((u32)(n_cts * limbs * tiles));
+    rc = var ? launch_row_mac<12, true>(a, g, ctx->stream) : launch_row_mac<12, false>(a, g, ctx->stream);
+    if (rc) return rc;
+    CK_HIP(hipGetLastError());
+    return CKKS_OK;
+}
+
 }  // namespace ck
 
 using namespace ck;

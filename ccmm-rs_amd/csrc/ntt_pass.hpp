@@ -12,15 +12,20 @@ namespace ck {
 #if defined(__HIP_DEVICE_COMPILE__)
 struct BlockExec {
     u32 threads;
+    // thread id handed to the phases; a caller that runs several passes in a
+    // loop can make it opaque per iteration (see k_ntt_row_mac) so the
+    // compiler recomputes the index math instead of keeping it live
+    u32 tid = threadIdx.x;
     template <class F>
     __device__ __forceinline__ void run(F &&phase) {
-        phase((u32)threadIdx.x);
+        phase(tid);
     }
     __device__ __forceinline__ void barrier() { __syncthreads(); }
 };
 #else
 struct BlockExec {
     u32 threads;
+    u32 tid = 0;  // unused on the host: run() walks every thread
     template <class F>
     void run(F &&phase) {
         for (u32 t = 0; t < threads; ++t) phase(t);
@@ -93,17 +98,21 @@ struct NttPass {
         return x;
     }
 
+    // lds_index is linear: thread part (computed once) ^ register part
+    // (a compile-time constant per unrolled i).
     template <int I>
     CK_HD static void exchange_out(u64 (&v)[16], u32 tid, u64 *lds) {
         constexpr Geom g = geom(I);
+        const u32 base = lds_index<L, CL>(coord(g, tid, 0));
 #pragma unroll
-        for (int i = 0; i < 16; ++i) lds[lds_index<L, CL>(coord(g, tid, i))] = v[i];
+        for (int i = 0; i < 16; ++i) lds[base ^ lds_index<L, CL>(coord(g, 0, i))] = v[i];
     }
     template <int I>
     CK_HD static void exchange_in(u64 (&v)[16], u32 tid, const u64 *lds) {
         constexpr Geom g = geom(I);
+        const u32 base = lds_index<L, CL>(coord(g, tid, 0));
 #pragma unroll
-        for (int i = 0; i < 16; ++i) v[i] = lds[lds_index<L, CL>(coord(g, tid, i))];
+        for (int i = 0; i < 16; ++i) v[i] = lds[base ^ lds_index<L, CL>(coord(g, 0, i))];
     }
 
     CK_HD static u64 finish(u64 x, const TileJob &job) {
@@ -122,13 +131,35 @@ struct NttPass {
         return x >= q ? x - q : x;
     }
 
-    template <class Exec, class Regs>
-    CK_HD static void run(Exec &ex, Regs &&regs_of, const TileJob &job, u64 *lds) {
+    CK_HD static PassAddr pass_addr(const TileJob &job) {
         PassAddr pa;
         pa.logn = job.logn;
         pa.n2 = (job.logn + 1) / 2;
         pa.n1 = job.logn - pa.n2;
         pa.tile = job.tile;
+        return pa;
+    }
+
+    template <class Exec, class Regs>
+    CK_HD static void run(Exec &ex, Regs &&regs_of, const TileJob &job, u64 *lds) {
+        run_impl<false>(ex, regs_of, job, lds);
+    }
+
+    // Forward row pass (and forward full pass) that stops with the tile's
+    // lazy results staged in LDS (after a barrier): element x of the linear
+    // staging order (x = i * THREADS + tid) sits at lds[lds_index(e)] with
+    // e = stage_coord(x) (full pass: e = x), destined for
+    // row_natural_offset(pass_addr(job), e) (full pass: x); finish() makes it
+    // canonical.  Callers fuse their own epilogue on the NTT output.
+    template <class Exec, class Regs>
+    CK_HD static void run_to_lds(Exec &ex, Regs &&regs_of, const TileJob &job, u64 *lds) {
+        static_assert(FWD && KIND != PASS_COL, "only passes that stage their output");
+        run_impl<true>(ex, regs_of, job, lds);
+    }
+
+    template <bool TO_LDS, class Exec, class Regs>
+    CK_HD static void run_impl(Exec &ex, Regs &&regs_of, const TileJob &job, u64 *lds) {
+        const PassAddr pa = pass_addr(job);
         const u32 T = S::THREADS;
         const u32 n = job.logn;
 
@@ -193,6 +224,7 @@ struct NttPass {
                 }
             });
             ex.barrier();
+            if (TO_LDS) return;
             ex.run([&](u32 tid) {
                 if (tid >= T) return;
 #pragma unroll

@@ -143,17 +143,32 @@ CK_HD u32 coord(const Geom &g, u32 tid, int v) {
     return e;
 }
 
-// LDS placement with an XOR swizzle so both lane walks are conflict-free.
+// LDS placement.  The linear address a = (t << CL) | col is swizzled by
+// XORing its bits 0..4 with M * a[5..12] over GF(2), M's columns below.
+// Every LDS access of the passes moves 64 lanes whose addresses differ in
+// six bit positions (a fixed bit permutation of the lane id); M was picked
+// by checking all 22 such position sets of the column, row and full passes
+// (N = 2^10 .. 2^17, 4096- and 8192-residue tiles): each set maps onto five
+// independent low bits, so a wave's 64 u64 cover each 8-byte bank slot
+// exactly twice (and each of 16 slots four times with 32 banks).  The map is
+// a bijection and linear: lds_index(a ^ b) = lds_index(a) ^ lds_index(b) for
+// disjoint bit sets, which lets the kernels add a per-register constant to a
+// per-thread base.
+constexpr u32 kLdsSwizzle[8] = {20, 7, 9, 29, 31, 25, 22, 19};
+
+CK_HD u32 lds_swizzle(u32 a) {
+    u32 x = a;
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        if ((a >> (5 + k)) & 1u) x ^= kLdsSwizzle[k];
+    return x;
+}
+
 template <int L, int CL>
 CK_HD u32 lds_index(u32 e) {
     CK_BOUNDS(e, 1u << (L + CL));
     u32 t = e & ((1u << L) - 1), col = e >> L;
-    if (CL == 0) {
-        // single sub-transform: swizzle the low 4 bits by bits 4..7
-        return t ^ ((t >> 4) & 15u);
-    }
-    u32 mask = (1u << CL) - 1;
-    return (t << CL) + (col ^ (t & mask));
+    return lds_swizzle((t << CL) | col);
 }
 
 // ---- butterflies ------------------------------------------------------------

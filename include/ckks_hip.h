@@ -83,6 +83,22 @@ enum {
 int ckks_set_ntt_variant(ckks_context *ctx, uint32_t logn, int variant);
 int ckks_get_ntt_variant(const ckks_context *ctx, uint32_t logn, int *variant);
 
+/* Tuning: how a key switch runs the forward NTT of its digits and the key
+ * inner product.  CKKS_KS_FUSION_OFF: separate NTT and MAC kernels.
+ * CKKS_KS_FUSION_XCD / CKKS_KS_FUSION_LINEAR: the digits' row pass is
+ * fused with the MAC (the NTT-domain digits never reach HBM), with the
+ * blocks sharing a key tile placed on one XCD or in plain order.
+ * CKKS_KS_FUSION_AUTO (default): the choice ckks_autotune_key_switch made
+ * for the key, OFF for an untuned key.  Identical residues in every mode.
+ * No reference counterpart. */
+enum {
+    CKKS_KS_FUSION_AUTO = -1,
+    CKKS_KS_FUSION_OFF = 0,
+    CKKS_KS_FUSION_XCD = 1,
+    CKKS_KS_FUSION_LINEAR = 2
+};
+int ckks_set_key_switch_fusion(ckks_context *ctx, int mode);
+
 /* Tuning: engine of the exact ModUp/ModDown base conversion of hybrid keys.
  * CKKS_BASECONV_MFMA uses the int8 matrix cores (7-bit chunks, int32
  * accumulation) wherever every gadget block has 2..9 limbs;
@@ -252,13 +268,15 @@ int ckks_multiply_relinearize_rescale(const ckks_ksk *relin_key, const uint64_t 
                                       size_t in_stride, size_t out_stride);
 
 /* Autotuner: times one key switch of n_cts ciphertexts with each
- * base-conversion engine the key supports (scratch ciphertexts allocated
- * and freed here; synchronises the context stream) and records the faster
- * one in the key for CKKS_BASECONV_AUTO.  ms_valu / ms_mfma (optional)
- * receive the times, -1 where an engine does not apply.  Not callable
- * during graph capture.  No reference counterpart. */
-int ckks_autotune_key_switch(ckks_ksk *key, size_t n_cts, int *best_engine, float *ms_valu,
-                             float *ms_mfma);
+ * base-conversion engine the key supports and each fusion mode (scratch
+ * ciphertexts allocated and freed here; synchronises the context stream)
+ * and records the fastest pair in the key for CKKS_BASECONV_AUTO and
+ * CKKS_KS_FUSION_AUTO.  ms (optional, 2 x 3 floats) receives the times as
+ * ms[engine - 1][fusion mode], engine VALU = 1 / MFMA = 2, -1 where a pair
+ * does not apply.  Not callable during graph capture.  No reference
+ * counterpart. */
+int ckks_autotune_key_switch(ckks_ksk *key, size_t n_cts, int *best_engine, int *best_fusion,
+                             float *ms);
 
 /* ------------------------------------------------------------ graphs */
 /* Rotation schedules (BSGS of CCMM / FFT) captured once as a hipGraph and

@@ -87,8 +87,37 @@ def test_autotune_key_switch_engine(pkg, ctx):
     ctx.set_base_conversion(ctx.BASECONV_VALU)
     want = pkg.to_host(key.switch(pkg.to_device(ct)))
     ctx.set_base_conversion(ctx.BASECONV_AUTO)
-    engine, ms_valu, ms_mfma = key.autotune(2)
+    engine, fusion, times = key.autotune(2)
     assert engine in (ctx.BASECONV_VALU, ctx.BASECONV_MFMA)
-    assert ms_valu > 0 and ms_mfma > 0
+    assert fusion in (ctx.KS_FUSION_OFF, ctx.KS_FUSION_XCD, ctx.KS_FUSION_LINEAR)
+    assert len(times) == 6 and all(t > 0 for t in times.values())
+    assert times[(engine, fusion)] == min(times.values())
     got = pkg.to_host(key.switch(pkg.to_device(ct)))
     assert (got == want).all()
+
+
+@pytest.mark.parametrize("logn,variant", [(13, 0), (14, 1), (15, 2), (16, 3)])
+def test_fused_ntt_mac_matches_unfused(pkg, ctx, logn, variant):
+    """The key switch with the digits' row pass fused into the key MAC gives
+    the residues of the separate NTT + MAC kernels, for every NTT variant."""
+    n, L, K, blocks = 1 << logn, 5, 2, [3, 2]
+    q_moduli = [Q0[0]] + Q40[:L - 1]
+    qp = q_moduli + P60[:K]
+    rng = np.random.default_rng(logn)
+    plan = pkg.PreparedRnsNttPlan(ctx, qp, n)
+    key = pkg.HybridKeySwitchKey(plan, L, blocks, rand_residues(rng, qp, n, 2), rand_residues(rng, qp, n, 2))
+    ct = pkg.to_device(rand_residues(rng, q_moduli, n, polys=6).reshape(3, 2, L, n))
+    k = O.rotation_exponent_left(n, 3)
+    ctx.set_ntt_variant(logn, variant)
+    try:
+        ctx.set_key_switch_fusion(ctx.KS_FUSION_OFF)
+        want = pkg.to_host(key.apply_galois(ct, k))
+        got = {}
+        for mode in (ctx.KS_FUSION_XCD, ctx.KS_FUSION_LINEAR):
+            ctx.set_key_switch_fusion(mode)
+            got[mode] = pkg.to_host(key.apply_galois(ct, k))
+    finally:
+        ctx.set_key_switch_fusion(ctx.KS_FUSION_AUTO)
+        ctx.set_ntt_variant(logn, 0)
+    for mode, g in got.items():
+        assert (g == want).all(), f"fusion mode {mode}"
