@@ -92,6 +92,9 @@ _SIGNATURES = {
     "ckks_graph_destroy": (C.c_int, [_p]),
     "ckks_rns_ksk_create": (C.c_int, [_p, _p, _sz, _p, _p, C.POINTER(_p)]),
     "ckks_hybrid_ksk_create": (C.c_int, [_p, _sz, _p, _sz, _p, _p, C.POINTER(_p)]),
+    "ckks_bounded_ksk_create": (C.c_int, [_p, _sz, C.c_uint, _p, _p, C.POINTER(_p)]),
+    "ckks_bounded_digit_count": (C.c_int, [_p, _sz, C.c_uint, C.POINTER(_sz)]),
+    "ckks_bounded_decompose": (C.c_int, [_p, _sz, C.c_uint, _p, _p, _sz, _sz]),
     "ckks_ksk_destroy": (C.c_int, [_p]),
     "ckks_key_switch": (C.c_int, [_p, _p, _p, _sz, _sz, _sz]),
     "ckks_apply_galois": (C.c_int, [_p, _u64, _p, _p, _sz, _sz, _sz]),
@@ -534,6 +537,48 @@ class HybridKeySwitchKey(_KeySwitchKey):
         _check(lib().ckks_hybrid_ksk_create(plan.handle, self.q_limbs, bs.ctypes.data, len(bs),
                                             kb.ctypes.data, ka.ctypes.data, C.byref(h)))
         self._finish_init(plan, h)
+
+
+class BoundedKeySwitchKey(_KeySwitchKey):
+    """Bounded balanced base-2^w gadget key (grafting/bounded_gadget.rs):
+    without special primes (q_limbs = plan limbs) the bounded multiplication
+    key of grafting/bounded_key_switch.rs, with them the hybrid-QP Galois key
+    of ckks/hybrid_rns_galois_key.rs.  Entries [digit_count][plan limbs][N],
+    NTT domain."""
+
+    def __init__(self, plan: PreparedRnsNttPlan, q_limbs: int, base_log: int, key_b, key_a):
+        self.q_limbs = int(q_limbs)
+        self.base_log = int(base_log)
+        kb = np.ascontiguousarray(key_b, dtype=np.uint64)
+        ka = np.ascontiguousarray(key_a, dtype=np.uint64)
+        count = bounded_digit_count(plan, self.q_limbs, self.base_log)
+        want = (count, len(plan.moduli), plan.degree)
+        if kb.shape != want or ka.shape != want:
+            raise CkksError(f"bounded key entries must have shape {want}")
+        h = _p()
+        _check(lib().ckks_bounded_ksk_create(plan.handle, self.q_limbs, self.base_log, kb.ctypes.data,
+                                             ka.ctypes.data, C.byref(h)))
+        self.digit_count = count
+        self._finish_init(plan, h)
+
+
+def bounded_digit_count(plan: PreparedRnsNttPlan, q_limbs: int, base_log: int) -> int:
+    """BoundedGadgetLayout::new digit count over the first q_limbs limbs."""
+    c = _sz()
+    _check(lib().ckks_bounded_digit_count(plan.handle, int(q_limbs), int(base_log), C.byref(c)))
+    return c.value
+
+
+def bounded_decompose(plan: PreparedRnsNttPlan, q_limbs: int, base_log: int, x, out=None):
+    """BoundedGadgetLayout::decompose + lift_digit on the device: x
+    [polys][q_limbs][N] -> [polys][digit_count][plan limbs][N]."""
+    n = plan.degree
+    polys = x.numel() // (q_limbs * n)
+    count = bounded_digit_count(plan, q_limbs, base_log)
+    out = empty((polys, count, len(plan.moduli), n)) if out is None else out
+    _check(lib().ckks_bounded_decompose(plan.handle, int(q_limbs), int(base_log), _ptr(x), _ptr(out),
+                                        polys, 0))
+    return out
 
 
 class NttBatchEntry(C.Structure):

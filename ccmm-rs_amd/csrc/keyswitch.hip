@@ -67,6 +67,10 @@ struct ckks_ksk {
     ck::ModDownTable *d_moddown = nullptr;
     // exact int8-MFMA base conversion: dnum ModUp plans then the ModDown plan
     int mfma = 0;
+    // bounded (balanced base-2^w) gadget: dnum = digit count, no blocks
+    int bounded = 0;
+    unsigned base_log = 0, bounded_words = 0;
+    ck::u64 *d_bounded = nullptr;  // table blob (BoundedTable points into it)
     int tuned_engine = CKKS_BASECONV_VALU;  // engine of CKKS_BASECONV_AUTO for this key
     int tuned_fusion = CKKS_KS_FUSION_OFF;  // mode of CKKS_KS_FUSION_AUTO for this key
     ck::BcPlan *d_bc = nullptr;
@@ -181,6 +185,135 @@ __global__ __launch_bounds__(kKsThreads) void k_modup(ModUpArgs a) {
 }
 
 // ---- 3. key MAC -------------------------------------------------------------------
+
+// ---- 1b. bounded gadget decomposition ---------------------------------------------
+//
+// BoundedGadgetLayout::decompose + lift_digit (grafting/bounded_gadget.rs:
+// 59-122, 382-402): the centered value of every coefficient over Q, written
+// in balanced base-2^w digits d_j in [-2^(w-1), 2^(w-1)), each lifted to
+// every limb of the key basis.  One thread per coefficient; the canonical
+// value X in [0, Q) is built in W 64-bit words held in registers:
+//   v_i = (x_i - X mod q_i) * (q_0 .. q_{i-1})^-1 mod q_i,
+//   X  += v_i * (q_0 .. q_{i-1})                        (Garner, limb by limb)
+// with X mod q_i = sum_k X[k] * (2^(64k) mod q_i).  X > floor(Q/2) is
+// replaced by X - Q in two's complement, and the digits are read off w bits
+// at a time with a carry (r + c >= 2^(w-1) -> digit r + c - 2^w, carry 1),
+// which is the reference's repeated (value - digit) / 2^w.
+
+struct BoundedTable {
+    const u64 *pow64;   // [q_limbs][W]: 2^(64k) mod q_i
+    const u64 *prefix;  // [q_limbs][W]: q_0 .. q_{i-1} (1 for i = 0)
+    const u64 *inv;     // [q_limbs]: (q_0 .. q_{i-1})^-1 mod q_i
+    const u64 *half;    // [W]: floor(Q / 2)
+    const u64 *qw;      // [W]: Q
+};
+
+struct BoundedArgs {
+    const u64 *in;  // component: limb i of ct c at in + c*in_stride + i*N
+    u64 in_stride;
+    u64 kinv;       // nonzero: decompose sigma_k(in) (rotation fused in)
+    u64 *digits;    // [ct][D][B][N]
+    BoundedTable t;
+    const DevPrime *primes;  // key basis B
+    u32 q_limbs, limbs, logn, digit_count, base_log;
+    u64 total;  // n_cts * N
+};
+
+CK_HD u64 lift_signed(int64_t d, const DevPrime &P) {
+    const Barrett br{P.q, P.br_hi, P.br_lo};
+    if (d >= 0) return barrett_reduce128(0, (u64)d, br);
+    const u64 m = barrett_reduce128(0, (u64)(-(d + 1)) + 1, br);
+    return m ? P.q - m : 0;
+}
+
+template <int W>
+__global__ __launch_bounds__(kKsThreads) void k_bounded_decompose(BoundedArgs a) {
+    const u64 gid = (u64)blockIdx.x * kKsThreads + threadIdx.x;
+    if (gid >= a.total) return;
+    const u64 N = (u64)1 << a.logn;
+    const u64 ct = gid >> a.logn, n = gid & (N - 1);
+    const Gather gi = galois_gather(n, a.kinv, a.logn);
+    const u64 *x = a.in + ct * a.in_stride + gi.idx;
+    u64 X[W];
+#pragma unroll
+    for (int k = 0; k < W; ++k) X[k] = 0;
+    for (u32 i = 0; i < a.q_limbs; ++i) {
+        const DevPrime &P = a.primes[i];
+        const Barrett br{P.q, P.br_hi, P.br_lo};
+        u64 xi = x[(u64)i * N];
+        if (gi.neg) xi = neg_mod(xi, P.q);
+        const u64 *pw = a.t.pow64 + (u64)i * W;
+        u64 r = 0;
+#pragma unroll
+        for (int k = 0; k < W; ++k) r = add_mod(r, mul_mod(X[k], pw[k], br), P.q);
+        const u64 v = mul_mod(sub_mod(xi, r, P.q), a.t.inv[i], br);
+        const u64 *pre = a.t.prefix + (u64)i * W;
+        u64 carry = 0;
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            u64 hi, lo;
+            mul128(v, pre[k], hi, lo);
+            const u64 l1 = lo + carry;
+            hi += l1 < lo;
+            const u64 l2 = l1 + X[k];
+            hi += l2 < l1;
+            X[k] = l2;
+            carry = hi;
+        }
+    }
+    // centered representative: X > floor(Q/2) -> X - Q (two's complement)
+    int cmp = 0;
+#pragma unroll
+    for (int k = W - 1; k >= 0; --k)
+        if (cmp == 0 && X[k] != a.t.half[k]) cmp = X[k] > a.t.half[k] ? 1 : -1;
+    const bool negative = cmp > 0;
+    if (negative) {
+        u64 borrow = 0;
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            const u64 q = a.t.qw[k];
+            const u64 d1 = X[k] - q;
+            const u64 b1 = X[k] < q;
+            const u64 d2 = d1 - borrow;
+            const u64 b2 = d1 < borrow;
+            X[k] = d2;
+            borrow = b1 | b2;
+        }
+    }
+    // balanced digits, w <= 63 bits at a time, from a 128-bit bit buffer
+    const u32 w = a.base_log;
+    const u64 mask = ((u64)1 << w) - 1, half_base = (u64)1 << (w - 1);
+    const u64 fill = negative ? ~(u64)0 : 0;
+    u64 *out = a.digits + ct * (u64)a.digit_count * a.limbs * N + n;
+    u64 lo = 0, hi = 0, carry = 0;
+    u32 nb = 0, j = 0;
+#pragma unroll
+    for (int k = 0; k < W + 2; ++k) {
+        const u64 word = k < W ? X[k] : fill;
+        // nb < w <= 63 bits are buffered; append 64 more
+        lo |= nb ? (word << nb) : word;
+        hi = nb ? (word >> (64 - nb)) : 0;
+        nb += 64;
+        while (nb >= w && j < a.digit_count) {
+            const u64 r = lo & mask;
+            lo = (lo >> w) | (hi << (64 - w));
+            hi >>= w;
+            nb -= w;
+            const u64 t = r + carry;
+            int64_t d;
+            if (t >= half_base) {
+                d = (int64_t)t - (int64_t)mask - 1;
+                carry = 1;
+            } else {
+                d = (int64_t)t;
+                carry = 0;
+            }
+            u64 *o = out + (u64)j * a.limbs * N;
+            for (u32 t2 = 0; t2 < a.limbs; ++t2) o[(u64)t2 * N] = lift_signed(d, a.primes[t2]);
+            ++j;
+        }
+    }
+}
 
 struct MacArgs {
     const u64 *digits;  // [ct][dnum][B][N] NTT domain
@@ -807,6 +940,158 @@ static int create_key(const ckks_rns_plan *plan, size_t q_limbs, const size_t *b
     return CKKS_OK;
 }
 
+// ---- bounded gadget tables ---------------------------------------------------------
+
+// Words W of the register-resident value: the smallest instantiated size
+// holding bits(Q) + 1 bits (the sign of X - Q).
+static const unsigned kBoundedWords[] = {4, 8, 12, 16, 24, 32};
+
+struct BoundedHost {
+    unsigned words = 0, digit_count = 0;
+    std::vector<u64> blob;  // pow64 | prefix | inv | half | qw
+};
+
+static void words_mul_small(std::vector<u64> &x, u64 m) {
+    unsigned __int128 carry = 0;
+    for (auto &w : x) {
+        unsigned __int128 t = (unsigned __int128)w * m + carry;
+        w = (u64)t;
+        carry = t >> 64;
+    }
+}
+
+static int build_bounded(const std::vector<u64> &mods, size_t q_limbs, unsigned base_log, BoundedHost &h) {
+    if (base_log < 1 || base_log > 63) return fail(CKKS_ERR_INVALID, "bounded gadget base_log must be in 1..=63");
+    std::vector<u64> q(40, 0);
+    q[0] = 1;
+    for (size_t i = 0; i < q_limbs; ++i) words_mul_small(q, mods[i]);
+    unsigned bits = 0;
+    for (size_t k = q.size(); k-- > 0;)
+        if (q[k]) {
+            bits = (unsigned)(64 * k + 64 - __builtin_clzll(q[k]));
+            break;
+        }
+    h.digit_count = (bits + 1 + base_log - 1) / base_log;
+    h.words = 0;
+    for (unsigned w : kBoundedWords)
+        if (64 * w >= bits + 1) {
+            h.words = w;
+            break;
+        }
+    if (!h.words) return fail(CKKS_ERR_UNSUPPORTED, "bounded gadget: Q above 2047 bits");
+    const unsigned W = h.words;
+    h.blob.assign((size_t)q_limbs * W * 2 + q_limbs + 2 * W, 0);
+    u64 *pow64 = h.blob.data(), *prefix = pow64 + q_limbs * W, *inv = prefix + q_limbs * W;
+    u64 *half = inv + q_limbs, *qw = half + W;
+    std::vector<u64> pre(W, 0);
+    pre[0] = 1;
+    for (size_t i = 0; i < q_limbs; ++i) {
+        const u64 qi = mods[i];
+        const u64 two64 = (u64)((((unsigned __int128)1) << 64) % qi);
+        u64 p = 1 % qi;
+        for (unsigned k = 0; k < W; ++k) {
+            pow64[i * W + k] = p;
+            p = host::mulmod(p, two64, qi);
+        }
+        u64 pm = 0;  // prefix mod q_i
+        for (unsigned k = W; k-- > 0;) pm = host::mulmod(pm, two64, qi), pm = (pm + pre[k] % qi) % qi;
+        inv[i] = host::invmod(pm, qi);
+        for (unsigned k = 0; k < W; ++k) prefix[i * W + k] = pre[k];
+        words_mul_small(pre, qi);
+    }
+    for (unsigned k = 0; k < W; ++k) {
+        qw[k] = q[k];
+        half[k] = (q[k] >> 1) | (k + 1 < q.size() ? q[k + 1] << 63 : 0);
+    }
+    return CKKS_OK;
+}
+
+static BoundedTable bounded_table(const u64 *d_blob, size_t q_limbs, unsigned W) {
+    BoundedTable t;
+    t.pow64 = d_blob;
+    t.prefix = t.pow64 + q_limbs * W;
+    t.inv = t.prefix + q_limbs * W;
+    t.half = t.inv + q_limbs;
+    t.qw = t.half + W;
+    return t;
+}
+
+static int launch_bounded(hipStream_t s, const BoundedArgs &a, unsigned W) {
+    const dim3 g((u32)((a.total + kKsThreads - 1) / kKsThreads));
+    switch (W) {
+#define CK_BD(WW) \
+    case WW: hipLaunchKernelGGL(k_bounded_decompose<WW>, g, dim3(kKsThreads), 0, s, a); break;
+        CK_BD(4) CK_BD(8) CK_BD(12) CK_BD(16) CK_BD(24) CK_BD(32)
+#undef CK_BD
+        default: return fail(CKKS_ERR_UNSUPPORTED, "bounded gadget word count");
+    }
+    CK_HIP(hipGetLastError());
+    return CKKS_OK;
+}
+
+// Key of the bounded family: no special primes (bounded relinearization,
+// grafting/bounded_key_switch.rs) or with them (hybrid-QP Galois key,
+// ckks/hybrid_rns_galois_key.rs).  Entries [digit][limb] pairs (b, a).
+static int create_bounded_key(const ckks_rns_plan *plan, size_t q_limbs, unsigned base_log,
+                              const u64 *key_b, const u64 *key_a, ckks_ksk **out) {
+    if (!plan || !key_b || !key_a || !out) return fail(CKKS_ERR_INVALID, "null argument");
+    const size_t B = plan->moduli.size();
+    if (q_limbs == 0 || q_limbs > B) return fail(CKKS_ERR_INVALID, "Q limb count must be within the key basis");
+    const size_t K = B - q_limbs;
+    if (K > (size_t)kMaxSpecial) return fail(CKKS_ERR_UNSUPPORTED, "at most 8 special primes");
+    BoundedHost h;
+    int rc = build_bounded(plan->moduli, q_limbs, base_log, h);
+    if (rc) return rc;
+    // the ModDown tables come from the block-key builder (its block layout
+    // and placeholder key entries are replaced below; the bounded key never
+    // runs the block ModUp)
+    ckks_ksk *k = nullptr;
+    if (K > 0) {
+        std::vector<size_t> blocks;
+        for (size_t left = q_limbs; left;) {
+            size_t b = std::min<size_t>(left, kMaxAlpha);
+            blocks.push_back(b);
+            left -= b;
+        }
+        // key material of the digit count is uploaded below; pass a dummy of
+        // matching size for validation
+        std::vector<u64> zeros(blocks.size() * B * plan->degree, 0);
+        rc = create_key(plan, q_limbs, blocks.data(), blocks.size(), zeros.data(), zeros.data(), true, &k);
+    } else {
+        std::vector<size_t> blocks(q_limbs, 1);
+        std::vector<u64> zeros(q_limbs * B * plan->degree, 0);
+        rc = create_key(plan, q_limbs, blocks.data(), blocks.size(), zeros.data(), zeros.data(), false, &k);
+    }
+    if (rc) return rc;
+    std::unique_ptr<ckks_ksk, int (*)(ckks_ksk *)> guard(k, ckks_ksk_destroy);
+    hipFree(k->d_kb);
+    hipFree(k->d_ka);
+    k->d_kb = k->d_ka = nullptr;
+    hipFree(k->d_bc);
+    hipFree(k->d_bt);
+    k->d_bc = nullptr;
+    k->d_bt = nullptr;
+    k->mfma = 0;
+    k->bounded = 1;
+    k->base_log = base_log;
+    k->bounded_words = h.words;
+    k->dnum = h.digit_count;
+    const size_t N = plan->degree, kelems = (size_t)h.digit_count * B * N;
+    const auto &mods = plan->moduli;
+    for (size_t e = 0; e < kelems; ++e) {
+        u64 q = mods[(e / N) % B];
+        if (key_b[e] >= q || key_a[e] >= q) return fail(CKKS_ERR_INVALID, "key residues must be canonical");
+    }
+    CK_HIP(hipMalloc(&k->d_bounded, h.blob.size() * sizeof(u64)));
+    CK_HIP(hipMemcpy(k->d_bounded, h.blob.data(), h.blob.size() * sizeof(u64), hipMemcpyHostToDevice));
+    CK_HIP(hipMalloc(&k->d_kb, kelems * sizeof(u64)));
+    CK_HIP(hipMalloc(&k->d_ka, kelems * sizeof(u64)));
+    CK_HIP(hipMemcpy(k->d_kb, key_b, kelems * sizeof(u64), hipMemcpyHostToDevice));
+    CK_HIP(hipMemcpy(k->d_ka, key_a, kelems * sizeof(u64), hipMemcpyHostToDevice));
+    *out = guard.release();
+    return CKKS_OK;
+}
+
 // ---- the pipeline --------------------------------------------------------------------
 
 // Switches the key of component `x` (coefficient domain, q_limbs limbs per
@@ -841,7 +1126,22 @@ static int key_switch_core(const ckks_ksk *k, const u64 *x, u64 x_stride, const 
     mu.chunks = chunks;
     const int engine = ctx->base_conversion == CKKS_BASECONV_AUTO ? k->tuned_engine : ctx->base_conversion;
     const bool mfma = k->mfma && engine == CKKS_BASECONV_MFMA;
-    if (mfma) {
+    if (k->bounded) {
+        BoundedArgs ba{};
+        ba.in = x;
+        ba.in_stride = x_stride;
+        ba.kinv = galois_kinv;
+        ba.digits = digits;
+        ba.t = bounded_table(k->d_bounded, k->q_limbs, k->bounded_words);
+        ba.primes = plan->d_primes;
+        ba.q_limbs = (u32)k->q_limbs;
+        ba.limbs = (u32)B;
+        ba.logn = logn;
+        ba.digit_count = (u32)D;
+        ba.base_log = k->base_log;
+        ba.total = (u64)n_cts * N;
+        if ((rc = launch_bounded(s, ba, k->bounded_words))) return rc;
+    } else if (mfma) {
         ModUpMfmaArgs mm{mu, k->d_bc};
         hipLaunchKernelGGL(k_modup_mfma, dim3((u32)(n_cts * D * (N / kBcRows))), dim3(256), 0, s, mm);
     } else {
@@ -950,6 +1250,57 @@ int ckks_hybrid_ksk_create(const ckks_rns_plan *qp_plan, size_t q_limbs, const s
     return create_key(qp_plan, q_limbs, block_sizes, n_blocks, key_b, key_a, true, out);
 }
 
+int ckks_bounded_ksk_create(const ckks_rns_plan *plan, size_t q_limbs, unsigned base_log,
+                            const uint64_t *key_b, const uint64_t *key_a, ckks_ksk **out) {
+    return create_bounded_key(plan, q_limbs, base_log, key_b, key_a, out);
+}
+
+int ckks_bounded_digit_count(const ckks_rns_plan *plan, size_t q_limbs, unsigned base_log,
+                             size_t *count) {
+    if (!plan || !count) return fail(CKKS_ERR_INVALID, "null argument");
+    if (q_limbs == 0 || q_limbs > plan->moduli.size())
+        return fail(CKKS_ERR_INVALID, "Q limb count must be within the plan basis");
+    BoundedHost h;
+    int rc = build_bounded(plan->moduli, q_limbs, base_log, h);
+    if (rc) return rc;
+    *count = h.digit_count;
+    return CKKS_OK;
+}
+
+int ckks_bounded_decompose(const ckks_rns_plan *plan, size_t q_limbs, unsigned base_log,
+                           const uint64_t *in, uint64_t *out, size_t n_polys, size_t in_stride) {
+    if (!plan || (n_polys && (!in || !out))) return fail(CKKS_ERR_INVALID, "null argument");
+    if (q_limbs == 0 || q_limbs > plan->moduli.size())
+        return fail(CKKS_ERR_INVALID, "Q limb count must be within the plan basis");
+    ckks_context *ctx = plan->ctx;
+    if (ctx->capturing) return fail(CKKS_ERR_INVALID, "ckks_bounded_decompose uploads its tables: not capturable");
+    BoundedHost h;
+    int rc = build_bounded(plan->moduli, q_limbs, base_log, h);
+    if (rc || n_polys == 0) return rc;
+    const size_t N = plan->degree;
+    if (in_stride == 0) in_stride = q_limbs * N;
+    if (in_stride < q_limbs * N) return fail(CKKS_ERR_INVALID, "polynomial stride smaller than its limbs");
+    void *ws;
+    if ((rc = ctx->workspace(3, h.blob.size() * sizeof(u64), &ws))) return rc;
+    CK_HIP(hipSetDevice(ctx->device));
+    CK_HIP(hipStreamSynchronize(ctx->stream));
+    CK_HIP(hipMemcpy(ws, h.blob.data(), h.blob.size() * sizeof(u64), hipMemcpyHostToDevice));
+    BoundedArgs ba{};
+    ba.in = in;
+    ba.in_stride = in_stride;
+    ba.kinv = 0;
+    ba.digits = out;
+    ba.t = bounded_table((const u64 *)ws, q_limbs, h.words);
+    ba.primes = plan->d_primes;
+    ba.q_limbs = (u32)q_limbs;
+    ba.limbs = (u32)plan->moduli.size();
+    ba.logn = plan->logn;
+    ba.digit_count = h.digit_count;
+    ba.base_log = base_log;
+    ba.total = (u64)n_polys * N;
+    return launch_bounded(ctx->stream, ba, h.words);
+}
+
 int ckks_ksk_destroy(ckks_ksk *k) {
     if (!k) return CKKS_OK;
     hipSetDevice(k->ctx->device);
@@ -962,6 +1313,7 @@ int ckks_ksk_destroy(ckks_ksk *k) {
     hipFree(k->d_moddown);
     hipFree(k->d_bc);
     hipFree(k->d_bt);
+    hipFree(k->d_bounded);
     delete k;
     return CKKS_OK;
 }

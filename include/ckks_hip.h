@@ -240,6 +240,32 @@ int ckks_rns_ksk_create(const ckks_rns_plan *plan, const size_t *block_sizes, si
 int ckks_hybrid_ksk_create(const ckks_rns_plan *qp_plan, size_t q_limbs, const size_t *block_sizes,
                            size_t n_blocks, const uint64_t *key_b, const uint64_t *key_a,
                            ckks_ksk **out);
+/* Bounded balanced base-2^w gadget key (BoundedGadgetLayout,
+ * grafting/bounded_gadget.rs:18-37; digit_count = ceil((bits(Q)+1)/w),
+ * base_log w in 1..63).  plan holds the Q limbs (q_limbs of them) followed
+ * by the special primes P, if any.  Without P: BoundedRnsMultiplicationKey
+ * (grafting/bounded_key_switch.rs:27-103), entry j encrypting B^j * s^2
+ * (or any B^j * t).  With P: PreparedHybridRnsGaloisKey
+ * (ckks/hybrid_rns_galois_key.rs:61-170), entry j encrypting
+ * P * B^j * sigma(s).  Entries are NTT-domain [digit][limb] pairs (b, a) of
+ * digit_count * plan limbs * N residues each.  The key switch decomposes
+ * the centered value of its input into the digits, lifts each to every limb
+ * (lift_digit, :382-402), and runs NTT, key MAC, iNTT and (with P) the
+ * ModDown; the P * b base term of the reference is added after the ModDown,
+ * which gives the same residues. */
+int ckks_bounded_ksk_create(const ckks_rns_plan *plan, size_t q_limbs, unsigned base_log,
+                            const uint64_t *key_b, const uint64_t *key_a, ckks_ksk **out);
+int ckks_bounded_digit_count(const ckks_rns_plan *plan, size_t q_limbs, unsigned base_log,
+                             size_t *count);
+
+/* BoundedGadgetLayout::decompose + lift_digit (grafting/bounded_gadget.rs:
+ * 59-122, 382-402): n_polys polynomials over the first q_limbs limbs of
+ * plan (poly p at in + p * in_stride, in_stride 0 = q_limbs * N) to
+ * out[p][digit][limb][N] over every limb of the plan.  Synchronises the
+ * context stream to upload its tables; not capturable. */
+int ckks_bounded_decompose(const ckks_rns_plan *plan, size_t q_limbs, unsigned base_log,
+                           const uint64_t *in, uint64_t *out, size_t n_polys, size_t in_stride);
+
 int ckks_ksk_destroy(ckks_ksk *key);
 
 /* rns_key_switch_with_prepared_dynamic_ntt (grafting/rns_key_switch.rs:

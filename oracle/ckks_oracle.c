@@ -678,6 +678,146 @@ int orc_rns_gadget_decompose(const uint64_t *in, const uint64_t *moduli, size_t 
 }
 
 /* ======================================================================
+ * G2  grafting/bounded_gadget.rs:18-37 (layout), :75-122
+ *     (decompose_canonical), :382-402 (lift_digit)
+ * Signed multiword values are kept in two's complement over BIG_WORDS words.
+ * ==================================================================== */
+static unsigned big_bits(const big *x) {
+    for (int i = BIG_WORDS - 1; i >= 0; --i)
+        if (x->w[i]) return (unsigned)(64 * i + 64 - __builtin_clzll(x->w[i]));
+    return 0;
+}
+
+static void big_sub(big *x, const big *y) { /* x -= y mod 2^(64 BIG_WORDS) */
+    uint64_t borrow = 0;
+    for (int i = 0; i < BIG_WORDS; ++i) {
+        uint64_t a = x->w[i], b = y->w[i];
+        uint64_t d = a - b - borrow;
+        borrow = (a < b) || (a - b < borrow);
+        x->w[i] = d;
+    }
+}
+
+static void big_add_signed_small(big *x, int64_t v) { /* x += v, two's complement */
+    uint64_t add = (uint64_t)v, ext = v < 0 ? ~0ull : 0;
+    u128 carry = 0;
+    for (int i = 0; i < BIG_WORDS; ++i) {
+        u128 t = (u128)x->w[i] + (i == 0 ? add : ext) + carry;
+        x->w[i] = (uint64_t)t;
+        carry = t >> 64;
+    }
+}
+
+static void big_sar(big *x, unsigned s) { /* arithmetic shift right, 0 < s < 64 */
+    uint64_t sign = (x->w[BIG_WORDS - 1] >> 63) ? ~0ull : 0;
+    for (int i = 0; i < BIG_WORDS; ++i) {
+        uint64_t next = i + 1 < BIG_WORDS ? x->w[i + 1] : sign;
+        x->w[i] = (x->w[i] >> s) | (next << (64 - s));
+    }
+}
+
+static int big_is_zero(const big *x) {
+    for (int i = 0; i < BIG_WORDS; ++i)
+        if (x->w[i]) return 0;
+    return 1;
+}
+
+size_t orc_bounded_digit_count(const uint64_t *moduli, size_t limbs, unsigned base_log) {
+    big q;
+    big_product(&q, moduli, limbs);
+    unsigned bits = big_bits(&q);
+    return (bits + 1 + base_log - 1) / base_log;
+}
+
+int orc_bounded_decompose(const uint64_t *in, const uint64_t *moduli, size_t limbs, size_t n,
+                          unsigned base_log, int64_t *digits) {
+    const size_t count = orc_bounded_digit_count(moduli, limbs, base_log);
+    big q, half, x;
+    big_product(&q, moduli, limbs);
+    half = q;
+    big_shr1(&half);
+    uint64_t *invt = mixed_radix_inverses(moduli, limbs);
+    uint64_t *res = malloc(limbs * 8), *dig = malloc(limbs * 8);
+    const uint64_t mask = (base_log == 64) ? ~0ull : ((1ull << base_log) - 1);
+    const uint64_t half_base = 1ull << (base_log - 1);
+    int rc = 0;
+    for (size_t c = 0; c < n && !rc; ++c) {
+        for (size_t i = 0; i < limbs; ++i) res[i] = in[i * n + c];
+        mixed_radix_digits_inv(res, moduli, limbs, invt, dig);
+        big_from_mixed_radix(&x, dig, moduli, limbs);
+        /* centered_representative_big: value > floor(Q/2) -> value - Q */
+        if (big_cmp(&x, &half) > 0) big_sub(&x, &q);
+        for (size_t j = 0; j < count; ++j) {
+            /* residue = value mod base (Euclidean): the low bits in two's complement */
+            uint64_t r = x.w[0] & mask;
+            int64_t d = r >= half_base ? (int64_t)r - (int64_t)(mask) - 1 : (int64_t)r;
+            digits[j * n + c] = d;
+            big_add_signed_small(&x, -d); /* value - digit, divisible by base */
+            big_sar(&x, base_log);
+        }
+        if (!big_is_zero(&x)) rc = -1; /* "bounded gadget digit count is insufficient" */
+    }
+    free(invt); free(res); free(dig);
+    return rc;
+}
+
+void orc_lift_signed(const int64_t *digits, size_t n, uint64_t q, uint64_t *out) {
+    for (size_t c = 0; c < n; ++c) {
+        int64_t v = digits[c];
+        if (v >= 0) out[c] = (uint64_t)v % q;
+        else {
+            uint64_t m = ((uint64_t)(-(v + 1)) + 1) % q;
+            out[c] = m == 0 ? 0 : q - m;
+        }
+    }
+}
+
+/* K6 / C3, following bounded_key_switch.rs:139-219 and
+ * hybrid_rns_galois_key.rs:627-780 term by term. */
+int orc_bounded_key_switch(const orc_plan *const *plans, const uint64_t *qp_moduli,
+                           size_t q_limbs, size_t p_limbs, unsigned base_log,
+                           const uint64_t *b, const uint64_t *a,
+                           const uint64_t *key_b, const uint64_t *key_a,
+                           uint64_t *b_out, uint64_t *a_out) {
+    const size_t n = plans[0]->n, qp = q_limbs + p_limbs;
+    const size_t count = orc_bounded_digit_count(qp_moduli, q_limbs, base_log);
+    int64_t *digits = malloc(count * n * 8);
+    int rc = orc_bounded_decompose(a, qp_moduli, q_limbs, n, base_log, digits);
+    if (rc) { free(digits); return rc; }
+    uint64_t *acc_b = malloc(qp * n * 8), *acc_a = calloc(qp * n, 8);
+    uint64_t *t = malloc(n * 8), *dn = malloc(n * 8);
+    for (size_t i = 0; i < qp; ++i) {
+        const uint64_t qi = qp_moduli[i];
+        /* base term: b (K6) or P*b on Q limbs and 0 on P limbs (C3, :627-663) */
+        if (i >= q_limbs) memset(t, 0, n * 8);
+        else if (p_limbs == 0) memcpy(t, b + i * n, n * 8);
+        else {
+            uint64_t pf = 1;
+            for (size_t k = 0; k < p_limbs; ++k) pf = orc_mul(pf, qp_moduli[q_limbs + k] % qi, qi);
+            for (size_t c = 0; c < n; ++c) t[c] = orc_mul(b[i * n + c], pf, qi);
+        }
+        orc_ntt_forward(plans[i], t, acc_b + i * n);
+        for (size_t j = 0; j < count; ++j) {
+            orc_lift_signed(digits + j * n, n, qi, t);
+            orc_ntt_forward(plans[i], t, dn);
+            orc_pointwise_mac(acc_b + i * n, dn, key_b + (j * qp + i) * n, n, qi);
+            orc_pointwise_mac(acc_a + i * n, dn, key_a + (j * qp + i) * n, n, qi);
+        }
+        orc_ntt_inverse(plans[i], acc_b + i * n, acc_b + i * n);
+        orc_ntt_inverse(plans[i], acc_a + i * n, acc_a + i * n);
+    }
+    if (p_limbs == 0) {
+        memcpy(b_out, acc_b, q_limbs * n * 8);
+        memcpy(a_out, acc_a, q_limbs * n * 8);
+    } else {
+        orc_mod_down(acc_b, qp_moduli, q_limbs, qp_moduli + q_limbs, p_limbs, n, b_out);
+        orc_mod_down(acc_a, qp_moduli, q_limbs, qp_moduli + q_limbs, p_limbs, n, a_out);
+    }
+    free(digits); free(acc_b); free(acc_a); free(t); free(dn);
+    return 0;
+}
+
+/* ======================================================================
  * K2  grafting/rns_key_switch.rs:1724-1777  rns_tensor_with_prepared_ntt
  * ==================================================================== */
 void orc_tensor(const orc_plan *const *plans, size_t limbs, const uint64_t *b0,

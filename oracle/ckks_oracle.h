@@ -107,6 +107,31 @@ int orc_rns_gadget_decompose(const uint64_t *in, const uint64_t *moduli, size_t 
                              const size_t *block_sizes, size_t n_blocks, size_t n,
                              uint64_t *out);
 
+/* ---- G2: bounded balanced gadget (grafting/bounded_gadget.rs) ----------- */
+/* digit_count = ceil((bits(Q) + 1) / base_log), Q = product of moduli
+ * (BoundedGadgetLayout::new, :18-37).  base_log in 1..63. */
+size_t orc_bounded_digit_count(const uint64_t *moduli, size_t limbs, unsigned base_log);
+/* decompose / decompose_canonical (:59-122): balanced base-2^w digits of the
+ * centered value of each coefficient (limbs x n input), digit in
+ * [-2^(w-1), 2^(w-1)); out: digit_count x n.  Returns 0, or -1 if the digit
+ * count is insufficient (the reference's assert). */
+int orc_bounded_decompose(const uint64_t *in, const uint64_t *moduli, size_t limbs, size_t n,
+                          unsigned base_log, int64_t *digits);
+/* lift_digit (:382-402): signed digit rem_euclid q. */
+void orc_lift_signed(const int64_t *digits, size_t n, uint64_t q, uint64_t *out);
+
+/* ---- K6 / C3: bounded key switch (grafting/bounded_key_switch.rs:139-219;
+ * ckks/hybrid_rns_galois_key.rs:550-780).  plans/qp_moduli: q_limbs Q limbs
+ * then p_limbs P limbs (p_limbs = 0: no P, K6 form).  Digits of a over Q;
+ * key: digit_count x (q_limbs+p_limbs) x n NTT domain.  K6: b_out = b +
+ * sum d_j kb_j, a_out = sum d_j ka_j.  C3: the QP sums start from P*b on the
+ * Q limbs (0 on P), then both components are moved down by P. */
+int orc_bounded_key_switch(const orc_plan *const *plans, const uint64_t *qp_moduli,
+                           size_t q_limbs, size_t p_limbs, unsigned base_log,
+                           const uint64_t *b, const uint64_t *a,
+                           const uint64_t *key_b, const uint64_t *key_a,
+                           uint64_t *b_out, uint64_t *a_out);
+
 /* ---- K2: tensor (grafting/rns_key_switch.rs rns_tensor_with_prepared_ntt) */
 void orc_tensor(const orc_plan *const *plans, size_t limbs,
                 const uint64_t *b0, const uint64_t *a0, const uint64_t *b1, const uint64_t *a1,

@@ -79,6 +79,10 @@ def lib():
             "orc_tensor": (None, [P, sz, P, P, P, P, P, P, P]),
             "orc_rns_key_switch": (C.c_int, [P, P, sz, P, sz, P, P, P, P, P, P]),
             "orc_hybrid_key_switch": (C.c_int, [P, P, sz, sz, P, sz, P, P, P, P, P, P]),
+            "orc_bounded_digit_count": (sz, [P, sz, C.c_uint]),
+            "orc_bounded_decompose": (C.c_int, [P, P, sz, sz, C.c_uint, P]),
+            "orc_lift_signed": (None, [P, sz, C.c_uint64, P]),
+            "orc_bounded_key_switch": (C.c_int, [P, P, sz, sz, C.c_uint, P, P, P, P, P, P]),
             "orc_encrypt": (None, [P, P, sz, P, P, P, P, P]),
             "orc_decrypt": (None, [P, P, sz, P, P, P, P]),
             "orc_rng_seed": (None, [P, u64]),
@@ -321,6 +325,45 @@ def hybrid_key_switch(qp_plan: RnsPlan, q_limbs, block_sizes, b, a, key_b, key_a
                                      _p(bo), _p(ao))
     if rc:
         raise ValueError(f"hybrid key switch failed rc={rc}")
+    return bo, ao
+
+
+def bounded_digit_count(moduli, base_log):
+    """BoundedGadgetLayout::new digit count (grafting/bounded_gadget.rs:18-37)."""
+    m = np.array(moduli, dtype=np.uint64)
+    return int(lib().orc_bounded_digit_count(_p(m), len(moduli), base_log))
+
+
+def bounded_decompose(x, moduli, base_log):
+    """Balanced base-2^w digits of the centered coefficients of x (limbs x n):
+    BoundedGadgetLayout::decompose (grafting/bounded_gadget.rs:59-122)."""
+    x = _u64(x); m = np.array(moduli, dtype=np.uint64)
+    n = x.shape[-1]
+    out = np.empty((bounded_digit_count(moduli, base_log), n), dtype=np.int64)
+    if lib().orc_bounded_decompose(_p(x), _p(m), len(moduli), n, base_log, _p(out)):
+        raise ValueError("bounded gadget digit count is insufficient for centered coefficient")
+    return out
+
+
+def lift_signed(d, q):
+    """lift_digit (grafting/bounded_gadget.rs:382-402): rem_euclid q."""
+    d = np.ascontiguousarray(d, dtype=np.int64)
+    out = np.empty(d.shape, dtype=np.uint64)
+    lib().orc_lift_signed(_p(d), d.size, q, _p(out))
+    return out
+
+
+def bounded_key_switch(qp_plan: RnsPlan, q_limbs, base_log, b, a, key_b, key_a):
+    """Bounded-digit key switch: no P (bounded_key_switch.rs:139-219) when the
+    plan has only Q limbs, else the hybrid-QP form of
+    hybrid_rns_galois_key.rs:550-780."""
+    QP = len(qp_plan.plans); n = qp_plan.n
+    b = _u64(b); a = _u64(a); kb = _u64(key_b); ka = _u64(key_a)
+    bo = np.empty((q_limbs, n), dtype=np.uint64); ao = np.empty_like(bo)
+    rc = lib().orc_bounded_key_switch(qp_plan.handle, _p(qp_plan.mod_arr()), q_limbs, QP - q_limbs,
+                                      base_log, _p(b), _p(a), _p(kb), _p(ka), _p(bo), _p(ao))
+    if rc:
+        raise ValueError(f"bounded key switch failed rc={rc}")
     return bo, ao
 
 

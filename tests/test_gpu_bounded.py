@@ -1,0 +1,106 @@
+"""GPU parity of the bounded (balanced base-2^w) gadget family against the
+oracle, residue for residue: the decomposition with its lifted digits
+(grafting/bounded_gadget.rs), bounded relinearization without special primes
+(grafting/bounded_key_switch.rs) and the hybrid-QP Galois key switch
+(ckks/hybrid_rns_galois_key.rs)."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from conftest import O, P60, Q0, Q40, rand_residues
+
+pytestmark = pytest.mark.gpu
+
+POOL65536 = [0x7fffffffba0001, 0x7fffffffaa0001, 0x7fffffff7e0001, 0x7fffffff380001,
+             0x7ffffffef00001, 0x7ffffffeba0001, 0x7ffffffeac0001, 0x7ffffffe700001,
+             0x7ffffffe600001, 0x7ffffffe4c0001, 0x7ffffffe220001, 0x7ffffffe160001,
+             0x7ffffffdec0001, 0x7ffffffdd00001, 0x7ffffffdce0001]
+
+
+@pytest.mark.parametrize("logn,L,K,w", [(12, 4, 1, 61), (12, 4, 0, 20), (13, 9, 2, 32),
+                                         (15, 13, 4, 45), (16, 25, 7, 63), (14, 25, 0, 7)])
+def test_bounded_decomposition_bit_exact(pkg, ctx, logn, L, K, w):
+    n = 1 << logn
+    moduli = [Q0[0]] + Q40[:L - 1] + P60[:K]
+    rng = np.random.default_rng(logn * 100 + w)
+    x = rand_residues(rng, moduli[:L], n, polys=2)
+    Q = math.prod(moduli[:L])
+    # exact boundaries: 0, floor(Q/2), floor(Q/2)+1, Q-1
+    for c, v in enumerate((0, Q // 2, Q // 2 + 1, Q - 1, 1)):
+        x[0, :, c] = [v % q for q in moduli[:L]]
+    plan = pkg.PreparedRnsNttPlan(ctx, moduli, n)
+    count = pkg.bounded_digit_count(plan, L, w)
+    assert count == O.bounded_digit_count(moduli[:L], w)
+    got = pkg.to_host(pkg.bounded_decompose(plan, L, w, pkg.to_device(x)))
+    assert got.shape == (2, count, len(moduli), n)
+    for p in range(2):
+        d = O.bounded_decompose(x[p], moduli[:L], w)
+        for t, q in enumerate(moduli):
+            assert (got[p, :, t] == O.lift_signed(d, q)).all(), f"poly {p} limb {t}"
+
+
+def _key(rng, plan_moduli, n, count):
+    return rand_residues(rng, plan_moduli, n, count), rand_residues(rng, plan_moduli, n, count)
+
+
+@pytest.mark.parametrize("logn,L,w", [(12, 4, 20), (13, 6, 61), (16, 9, 40)])
+def test_bounded_relinearize_bit_exact(pkg, ctx, logn, L, w):
+    n = 1 << logn
+    moduli = [Q0[0]] + Q40[:L - 1]
+    rng = np.random.default_rng(7 * logn + w)
+    plan = pkg.PreparedRnsNttPlan(ctx, moduli, n)
+    count = O.bounded_digit_count(moduli, w)
+    kb, ka = _key(rng, moduli, n, count)
+    key = pkg.BoundedKeySwitchKey(plan, L, w, kb, ka)
+    ct3 = rand_residues(rng, moduli, n, polys=6).reshape(2, 3, L, n)
+    got = pkg.to_host(key.relinearize(pkg.to_device(ct3)))
+    orc = O.RnsPlan(moduli, n)
+    for c in range(2):
+        bo, ao = O.bounded_key_switch(orc, L, w, ct3[c, 0], ct3[c, 2], kb, ka)
+        ao = np.stack([(ao[i].astype(object) + ct3[c, 1, i]) % q for i, q in enumerate(moduli)]).astype(np.uint64)
+        assert (got[c, 0] == bo).all() and (got[c, 1] == ao).all(), f"ct {c}"
+
+
+@pytest.mark.parametrize("logn,L,K,w,steps", [(12, 4, 1, 61, 1), (13, 9, 2, 32, 5), (15, 13, 4, 45, -3)])
+def test_hybrid_qp_rotation_bit_exact(pkg, ctx, logn, L, K, w, steps):
+    n = 1 << logn
+    q_moduli = [Q0[0]] + Q40[:L - 1]
+    qp = q_moduli + P60[:K]
+    rng = np.random.default_rng(logn + w)
+    plan = pkg.PreparedRnsNttPlan(ctx, qp, n)
+    count = O.bounded_digit_count(q_moduli, w)
+    kb, ka = _key(rng, qp, n, count)
+    key = pkg.BoundedKeySwitchKey(plan, L, w, kb, ka)
+    k = O.rotation_exponent_left(n, steps) if steps > 0 else O.rotation_exponent_right(n, -steps)
+    ct = rand_residues(rng, q_moduli, n, polys=4).reshape(2, 2, L, n)
+    got = pkg.to_host(key.apply_galois(pkg.to_device(ct), k))
+    orc = O.RnsPlan(qp, n)
+    for c in range(2):
+        sb = np.stack([O.apply_automorphism(ct[c, 0, i], q, k) for i, q in enumerate(q_moduli)])
+        sa = np.stack([O.apply_automorphism(ct[c, 1, i], q, k) for i, q in enumerate(q_moduli)])
+        bo, ao = O.bounded_key_switch(orc, L, w, sb, sa, kb, ka)
+        assert (got[c, 0] == bo).all() and (got[c, 1] == ao).all(), f"ct {c}"
+
+
+def test_reference_rotation_benchmark_shape(pkg, ctx):
+    """The reference's UHD rotation configuration (hybrid_rns_galois_key.rs:
+    1152-1200): N = 2^16, Q = 13 limbs of the 55-bit pool, P = 2 limbs,
+    base_log 32 -> 23 digits; one ciphertext against the oracle."""
+    n = 1 << 16
+    q_moduli, p_moduli = POOL65536[:13], POOL65536[13:15]
+    qp = q_moduli + p_moduli
+    rng = np.random.default_rng(65536)
+    plan = pkg.PreparedRnsNttPlan(ctx, qp, n)
+    count = O.bounded_digit_count(q_moduli, 32)
+    assert count == 23
+    kb, ka = _key(rng, qp, n, count)
+    key = pkg.BoundedKeySwitchKey(plan, 13, 32, kb, ka)
+    ct = rand_residues(rng, q_moduli, n, polys=2).reshape(1, 2, 13, n)
+    got = pkg.to_host(key.apply_galois(pkg.to_device(ct), 3))
+    sb = np.stack([O.apply_automorphism(ct[0, 0, i], q, 3) for i, q in enumerate(q_moduli)])
+    sa = np.stack([O.apply_automorphism(ct[0, 1, i], q, 3) for i, q in enumerate(q_moduli)])
+    bo, ao = O.bounded_key_switch(O.RnsPlan(qp, n), 13, 32, sb, sa, kb, ka)
+    assert (got[0, 0] == bo).all() and (got[0, 1] == ao).all()
