@@ -93,6 +93,36 @@ def _keygen_hybrid_target(qp_plan: O.RnsPlan, q_limbs: int, blocks, s, rng: O.Rn
     return kb, ka
 
 
+def _keygen_bounded_target(plan: O.RnsPlan, q_limbs: int, w: int, s, rng: O.Rng, target):
+    """Entry j encrypts P * B^j * target under s (P = 1 without special
+    primes), NTT domain: BoundedRnsMultiplicationKey (grafting/
+    bounded_key_switch.rs:27-103) / PreparedHybridRnsGaloisKey
+    (ckks/hybrid_rns_galois_key.rs:61-170)."""
+    n, moduli = plan.n, plan.moduli
+    Pv = math.prod(moduli[q_limbs:])
+    count = O.bounded_digit_count(moduli[:q_limbs], w)
+    kb = np.zeros((count, len(moduli), n), dtype=np.uint64)
+    ka = np.zeros_like(kb)
+    for j in range(count):
+        msg = np.stack([(target(i, q).astype(object) * (Pv * (1 << (w * j)) % q) % q).astype(np.uint64)
+                        for i, q in enumerate(moduli)])
+        e = rng.gaussian(SIGMA, n)
+        a = np.stack([rng.uniform(q, n) for q in moduli])
+        b = O.encrypt(plan, msg, e, s, a)
+        kb[j], ka[j] = plan.forward(b), plan.forward(a)
+    return kb, ka
+
+
+def keygen_relin_bounded(plan: O.RnsPlan, w: int, s, rng: O.Rng):
+    s2 = secret_square(plan, s)
+    return _keygen_bounded_target(plan, len(plan.moduli), w, s, rng, lambda i, q: s2[i])
+
+
+def keygen_galois_bounded(qp_plan: O.RnsPlan, q_limbs: int, w: int, s, k, rng: O.Rng):
+    s_rot = sigma_ternary(s, k, qp_plan.n)
+    return _keygen_bounded_target(qp_plan, q_limbs, w, s, rng, lambda i, q: ternary_to_mod(s_rot, q))
+
+
 def encrypt_slots(plan: O.RnsPlan, slots, s, rng: O.Rng, emb: E.Embedding):
     m = E.encode_rns(slots, emb, plan.moduli, SCALE)
     e = rng.gaussian(SIGMA, plan.n)
@@ -126,8 +156,17 @@ def setup(seed: int = 0):
     # after the rescale the ciphertext has 3 limbs; hybrid key over them + P
     qp_plan = O.RnsPlan(Q[:3] + P, N)
     gk = keygen_galois_hybrid(qp_plan, 3, [1, 1, 1], s, k, rng)
+    # the reference's own algorithms: bounded relinearization (base 2^20,
+    # as bin/bounded_ckks_4096.rs) and the hybrid-QP rotation key with base
+    # 2^61 (3 digits over the 3 limbs left after the rescale)
+    brk = keygen_relin_bounded(plan, BOUNDED_RELIN_LOG, s, rng)
+    bgk = keygen_galois_bounded(qp_plan, 3, BOUNDED_ROT_LOG, s, k, rng)
     return dict(rng=rng, emb=emb, plan=plan, s=s, z0=z0, z1=z1, ct0=ct0, ct1=ct1, rk=rk, gk=gk, k=k,
-                qp_plan=qp_plan, qp4=qp4)
+                qp_plan=qp_plan, qp4=qp4, brk=brk, bgk=bgk)
+
+
+BOUNDED_RELIN_LOG = 20
+BOUNDED_ROT_LOG = 61
 
 
 def expected_slots(d):
