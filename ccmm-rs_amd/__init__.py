@@ -93,6 +93,8 @@ _SIGNATURES = {
     "ckks_rns_ksk_create": (C.c_int, [_p, _p, _sz, _p, _p, C.POINTER(_p)]),
     "ckks_hybrid_ksk_create": (C.c_int, [_p, _sz, _p, _sz, _p, _p, C.POINTER(_p)]),
     "ckks_bounded_ksk_create": (C.c_int, [_p, _sz, C.c_uint, _p, _p, C.POINTER(_p)]),
+    "ckks_encrypt_raw": (C.c_int, [_p, _p, _p, _p, _p, _p, _sz, _sz, _sz]),
+    "ckks_decrypt_raw": (C.c_int, [_p, _p, _p, _p, _p, _sz, _sz, _sz, _sz]),
     "ckks_bounded_digit_count": (C.c_int, [_p, _sz, C.c_uint, C.POINTER(_sz)]),
     "ckks_bounded_decompose": (C.c_int, [_p, _sz, C.c_uint, _p, _p, _sz, _sz]),
     "ckks_ksk_destroy": (C.c_int, [_p]),
@@ -409,6 +411,24 @@ class PreparedRnsNttPlan:
                                  2 * n_limbs * n, 3 * n_limbs * n))
         return out
 
+    def encrypt_raw(self, message, error, secret, a, out=None, n_limbs=None):
+        """b = m + e - a*s (grafting/rns_key_switch.rs:245-383) for messages
+        [p][limbs][N], errors [p][N] (int64), secret [N] (int8), uniform a
+        [p][limbs][N]; returns b."""
+        n_polys, n_limbs, per = self._shape(message, n_limbs)
+        out = empty(message.shape) if out is None else out
+        _check(lib().ckks_encrypt_raw(self._h, _ptr(message), _ptr(error), _ptr(secret), _ptr(a), _ptr(out),
+                                      n_polys, n_limbs, per))
+        return out
+
+    def decrypt_raw(self, b, a, secret, out=None, n_limbs=None):
+        """b + a*s (grafting/rns_key_switch.rs:906-947)."""
+        n_polys, n_limbs, per = self._shape(b, n_limbs)
+        out = empty(b.shape) if out is None else out
+        _check(lib().ckks_decrypt_raw(self._h, _ptr(b), _ptr(a), _ptr(secret), _ptr(out), n_polys, n_limbs,
+                                      per, per))
+        return out
+
     def autotune_ntt(self, n_polys: int, n_limbs: int | None = None):
         """Times every NTT kernel variant on n_polys x n_limbs limbs, keeps
         the fastest for this degree in the context; returns (best, ms list)."""
@@ -603,3 +623,6 @@ def load():
     import sys
 
     return sys.modules[__name__]
+
+
+from . import ckks  # noqa: E402  (level-aware evaluator over the classes above)

@@ -375,11 +375,156 @@ static int multiply_plain(const ckks_rns_plan *plan, const u64 *in, const u64 *p
     return launch_ntt(ctx, inv, 2, false);
 }
 
+// ---- RLWE raw encryption / decryption ---------------------------------------
+//
+// encrypt_rns_raw_with_distribution_ntt_rng (grafting/rns_key_switch.rs:
+// 245-383) once its randomness is drawn on the host: b = m + e - a*s per
+// limb, with the one logical error e and the ternary secret s projected to
+// every limb (project_error, rlwe/error.rs:150-166), and a*s through the
+// NTT.  decrypt_rns_raw_with_ntt (:906-947): b + a*s.
+
+struct ProjectArgs {
+    const int8_t *s;  // N signed secret coefficients
+    u64 *out;         // [limb][N]
+    const DevPrime *primes;
+    u32 n_limbs, logn;
+};
+
+__global__ __launch_bounds__(kEwThreads) void k_project_secret(ProjectArgs a) {
+    const u64 N = (u64)1 << a.logn;
+    const u64 gid = (u64)blockIdx.x * kEwThreads + threadIdx.x;
+    if (gid >= (u64)a.n_limbs * N) return;
+    const u32 limb = (u32)(gid >> a.logn);
+    const u64 n = gid & (N - 1);
+    const u64 q = a.primes[limb].q;
+    const int v = a.s[n];
+    a.out[gid] = v >= 0 ? (u64)v % q : neg_mod((u64)(-v) % q, q);
+}
+
+struct RlweArgs {
+    const u64 *x;       // [p][limb][N]: a (then a*s in place)
+    const u64 *s_ntt;   // [limb][N]
+    u64 *y;             // [p][limb][N]
+    const u64 *m;       // encrypt: message; decrypt: b
+    const int64_t *e;   // encrypt: [p][N] errors; decrypt: null
+    u64 *out;
+    const DevPrime *primes;
+    u64 m_stride, out_stride;  // elements between polynomials of m / out
+    u32 n_limbs, logn, chunks, encrypt;
+};
+
+__global__ __launch_bounds__(kEwThreads) void k_mul_secret(RlweArgs a) {
+    const u32 b = blockIdx.x;
+    const u32 chunk = b % a.chunks;
+    const u32 pl = b / a.chunks;
+    const u32 limb = pl % a.n_limbs, poly = pl / a.n_limbs;
+    const DevPrime P = a.primes[limb];
+    const Barrett br{P.q, P.br_hi, P.br_lo};
+    const u64 off = ((u64)limb << a.logn) + (u64)chunk * kEwChunk + threadIdx.x * kEwPerThread;
+    const u64 base = (u64)poly * ((u64)a.n_limbs << a.logn) + off;
+#pragma unroll
+    for (int i = 0; i < kEwPerThread; ++i) a.y[base + i] = mul_mod(a.x[base + i], a.s_ntt[off + i], br);
+}
+
+__global__ __launch_bounds__(kEwThreads) void k_rlwe_finish(RlweArgs a) {
+    const u32 b = blockIdx.x;
+    const u32 chunk = b % a.chunks;
+    const u32 pl = b / a.chunks;
+    const u32 limb = pl % a.n_limbs, poly = pl / a.n_limbs;
+    const u64 q = a.primes[limb].q;
+    const u64 N = (u64)1 << a.logn;
+    const u64 n0 = (u64)chunk * kEwChunk + threadIdx.x * kEwPerThread;
+    const u64 as_base = (u64)poly * ((u64)a.n_limbs << a.logn) + ((u64)limb << a.logn) + n0;
+    const u64 m_base = (u64)poly * a.m_stride + ((u64)limb << a.logn) + n0;
+    const u64 out_base = (u64)poly * a.out_stride + ((u64)limb << a.logn) + n0;
+#pragma unroll
+    for (int i = 0; i < kEwPerThread; ++i) {
+        const u64 as = a.y[as_base + i];
+        u64 r;
+        if (a.encrypt) {
+            const int64_t ev = a.e[(u64)poly * N + n0 + i];
+            u64 ep;
+            if (ev >= 0) {
+                ep = (u64)ev % q;
+            } else {
+                const u64 mag = ((u64)(-(ev + 1)) + 1) % q;  // |ev| mod q, no overflow at INT64_MIN
+                ep = mag ? q - mag : 0;
+            }
+            r = sub_mod(add_mod(a.m[m_base + i], ep, q), as, q);
+        } else {
+            r = add_mod(a.m[m_base + i], as, q);
+        }
+        a.out[out_base + i] = r;
+    }
+}
+
+static int rlwe_raw(const ckks_rns_plan *plan, const u64 *m, size_t m_stride, const int64_t *e,
+                    const int8_t *secret, const u64 *a_in, size_t a_stride, u64 *out, size_t out_stride,
+                    size_t n_polys, size_t n_limbs, bool encrypt) {
+    int rc = check_plan_limbs(plan, n_limbs, encrypt ? "encrypt" : "decrypt");
+    if (rc || n_polys == 0) return rc;
+    if (!m || !secret || !a_in || !out || (encrypt && !e)) return fail(CKKS_ERR_INVALID, "null buffer");
+    const size_t N = plan->degree, poly = n_limbs * N;
+    if (m_stride == 0) m_stride = poly;
+    if (a_stride == 0) a_stride = poly;
+    if (out_stride == 0) out_stride = poly;
+    if (m_stride < poly || a_stride < poly || out_stride < poly)
+        return fail(CKKS_ERR_INVALID, "polynomial stride smaller than its limbs");
+    ckks_context *ctx = plan->ctx;
+    void *w;
+    if ((rc = ctx->workspace(1, (1 + n_polys) * poly * sizeof(u64), &w))) return rc;
+    u64 *s_ntt = (u64 *)w, *as = s_ntt + poly;
+    ProjectArgs pa{secret, s_ntt, plan->d_primes, (u32)n_limbs, plan->logn};
+    hipLaunchKernelGGL(k_project_secret, dim3((u32)((poly + kEwThreads - 1) / kEwThreads)), dim3(kEwThreads), 0,
+                       ctx->stream, pa);
+    CK_HIP(hipGetLastError());
+    NttCall f[2] = {{plan, s_ntt, s_ntt, 1, n_limbs, poly, poly},
+                    {plan, a_in, as, n_polys, n_limbs, a_stride, poly}};
+    if ((rc = launch_ntt(ctx, f, 2, true))) return rc;
+    RlweArgs ra{};
+    ra.x = as;
+    ra.s_ntt = s_ntt;
+    ra.y = as;
+    ra.m = m;
+    ra.e = e;
+    ra.out = out;
+    ra.primes = plan->d_primes;
+    ra.m_stride = m_stride;
+    ra.out_stride = out_stride;
+    ra.n_limbs = (u32)n_limbs;
+    ra.logn = plan->logn;
+    ra.chunks = (u32)(N / kEwChunk);
+    ra.encrypt = encrypt ? 1u : 0u;
+    const dim3 grid((u32)(n_polys * n_limbs * ra.chunks));
+    hipLaunchKernelGGL(k_mul_secret, grid, dim3(kEwThreads), 0, ctx->stream, ra);
+    CK_HIP(hipGetLastError());
+    NttCall iv{plan, as, as, n_polys, n_limbs, poly, poly};
+    if ((rc = launch_ntt(ctx, &iv, 1, false))) return rc;
+    hipLaunchKernelGGL(k_rlwe_finish, grid, dim3(kEwThreads), 0, ctx->stream, ra);
+    CK_HIP(hipGetLastError());
+    return CKKS_OK;
+}
+
 }  // namespace ck
 
 using namespace ck;
 
 extern "C" {
+
+int ckks_encrypt_raw(const ckks_rns_plan *plan, const uint64_t *message, const int64_t *error,
+                     const int8_t *secret, const uint64_t *a, uint64_t *b, size_t n_polys, size_t n_limbs,
+                     size_t poly_stride) {
+    if (!plan) return fail(CKKS_ERR_INVALID, "null plan");
+    return rlwe_raw(plan, message, poly_stride, error, secret, a, poly_stride, b, poly_stride, n_polys,
+                    n_limbs, true);
+}
+
+int ckks_decrypt_raw(const ckks_rns_plan *plan, const uint64_t *b, const uint64_t *a, const int8_t *secret,
+                     uint64_t *out, size_t n_polys, size_t n_limbs, size_t poly_stride, size_t out_stride) {
+    if (!plan) return fail(CKKS_ERR_INVALID, "null plan");
+    return rlwe_raw(plan, b, poly_stride, nullptr, secret, a, poly_stride, out, out_stride, n_polys,
+                    n_limbs, false);
+}
 
 int ckks_multiply_plain(const ckks_rns_plan *plan, const uint64_t *in, const uint64_t *pt, uint64_t *out,
                         size_t n_cts, size_t n_limbs, size_t in_stride, size_t pt_stride, size_t out_stride) {

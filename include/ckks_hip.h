@@ -216,6 +216,24 @@ int ckks_multiply_plain(const ckks_rns_plan *plan, const uint64_t *in, const uin
                         uint64_t *out, size_t n_cts, size_t n_limbs, size_t in_stride,
                         size_t pt_stride, size_t out_stride);
 
+/* encrypt_rns_raw_with_distribution_ntt_rng (grafting/rns_key_switch.rs:
+ * 245-383) after its sampling, which stays on the host (ChaCha20, Gaussian
+ * table): b = m + e - a*s over n_limbs limbs for n_polys messages.  message,
+ * a and b are [p][limb][N] with poly_stride elements between polynomials (0:
+ * n_limbs * N); error is [p][N] signed (one logical error per message,
+ * projected to every limb as rlwe/error.rs:150-166); secret is N signed
+ * coefficients (ternary).  All device pointers. */
+int ckks_encrypt_raw(const ckks_rns_plan *plan, const uint64_t *message, const int64_t *error,
+                     const int8_t *secret, const uint64_t *a, uint64_t *b, size_t n_polys,
+                     size_t n_limbs, size_t poly_stride);
+
+/* decrypt_rns_raw_with_ntt (grafting/rns_key_switch.rs:906-947):
+ * out = b + a*s, [p][limb][N]; b and a with poly_stride, out with
+ * out_stride (0: n_limbs * N). */
+int ckks_decrypt_raw(const ckks_rns_plan *plan, const uint64_t *b, const uint64_t *a,
+                     const int8_t *secret, uint64_t *out, size_t n_polys, size_t n_limbs,
+                     size_t poly_stride, size_t out_stride);
+
 /* rns_tensor_with_prepared_ntt (grafting/rns_key_switch.rs:1724-1777):
  * (b0,a0) x (b1,a1) -> (c0,c1,c2) coefficient domain, out laid out as
  * [c0 limbs][c1 limbs][c2 limbs] per ciphertext (out_stride elements). */
