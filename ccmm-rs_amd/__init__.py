@@ -94,6 +94,15 @@ _SIGNATURES = {
     "ckks_hybrid_ksk_create": (C.c_int, [_p, _sz, _p, _sz, _p, _p, C.POINTER(_p)]),
     "ckks_bounded_ksk_create": (C.c_int, [_p, _sz, C.c_uint, _p, _p, C.POINTER(_p)]),
     "ckks_encrypt_raw": (C.c_int, [_p, _p, _p, _p, _p, _p, _sz, _sz, _sz]),
+    "ckks_encode_coefficients": (C.c_int, [_p, _p, C.c_double, _p, _sz, _sz]),
+    "ckks_decode_coefficients": (C.c_int, [_p, _p, C.c_double, _p, _sz, _sz]),
+    "ckks_server_create": (C.c_int, [_p, _sz, _sz, C.POINTER(_p)]),
+    "ckks_server_submit": (C.c_int, [_p, _p, C.POINTER(C.c_uint64)]),
+    "ckks_server_step": (C.c_int, [_p, C.POINTER(_sz)]),
+    "ckks_server_poll": (C.c_int, [_p, C.c_uint64, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+    "ckks_server_drain": (C.c_int, [_p]),
+    "ckks_server_stats": (C.c_int, [_p, _p]),
+    "ckks_server_destroy": (C.c_int, [_p]),
     "ckks_decrypt_raw": (C.c_int, [_p, _p, _p, _p, _p, _sz, _sz, _sz, _sz]),
     "ckks_bounded_digit_count": (C.c_int, [_p, _sz, C.c_uint, C.POINTER(_sz)]),
     "ckks_bounded_decompose": (C.c_int, [_p, _sz, C.c_uint, _p, _p, _sz, _sz]),
@@ -626,3 +635,6 @@ def load():
 
 
 from . import ckks  # noqa: E402  (level-aware evaluator over the classes above)
+from . import serving  # noqa: E402  (continuous-batching serving loop)
+from . import encoding  # noqa: E402  (CKKS encode / decode on the device)
+from . import linear  # noqa: E402  (rotation-schedule programs of the linear-algebra workloads)

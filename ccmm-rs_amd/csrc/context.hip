@@ -75,6 +75,7 @@ int ckks_context_destroy(ckks_context *ctx) {
     if (!ctx) return CKKS_OK;
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
+    while (!ctx->servers.empty()) ckks_server_destroy(*ctx->servers.begin());
     while (!ctx->graphs.empty()) ckks_graph_destroy(*ctx->graphs.begin());
     while (!ctx->keys.empty()) ckks_ksk_destroy(*ctx->keys.begin());
     while (!ctx->plans.empty()) ckks_rns_plan_destroy(*ctx->plans.begin());
@@ -325,6 +326,7 @@ int ckks_rns_plan_destroy(ckks_rns_plan *plan) {
     plan->ctx->plans.erase(plan);
     hipFree(plan->d_primes);
     hipFree(plan->d_rescale);
+    for (auto &kv : plan->crt) hipFree(kv.second.first);
     delete plan;
     return CKKS_OK;
 }

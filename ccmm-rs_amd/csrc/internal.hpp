@@ -60,6 +60,7 @@ struct TwiddleTables {
 struct ckks_rns_plan;
 struct ckks_ksk;
 struct ckks_graph;
+struct ckks_server;
 
 struct ckks_context {
     int device = 0;
@@ -68,6 +69,7 @@ struct ckks_context {
     std::set<ckks_rns_plan *> plans;
     std::set<ckks_ksk *> keys;
     std::set<ckks_graph *> graphs;
+    std::set<ckks_server *> servers;  // destroyed before the keys and plans they use
     // (q, logn) -> device twiddle tables, shared by plans of this context
     std::map<std::pair<uint64_t, uint32_t>, ck::TwiddleTables> tables;
     // grow-only scratch slots: slot 0 is the NTT's inter-pass buffer, the
@@ -96,6 +98,8 @@ struct ckks_rns_plan {
     ck::DevPrime *d_primes = nullptr;  // device array, one per limb
     std::vector<ck::DevPrime> h_primes;
     ck::RescaleConst *d_rescale = nullptr;  // [dropped l][target i], L x L
+    // exact-CRT tables of the decoder per limb count: device blob, words
+    std::map<size_t, std::pair<uint64_t *, unsigned>> crt;
 };
 
 namespace ck {

@@ -21,6 +21,7 @@
 #include <cstring>
 
 #include "baseconv.hpp"
+#include "crt.hpp"
 #include "host_math.hpp"
 #include "internal.hpp"
 
@@ -200,13 +201,7 @@ __global__ __launch_bounds__(kKsThreads) void k_modup(ModUpArgs a) {
 // at a time with a carry (r + c >= 2^(w-1) -> digit r + c - 2^w, carry 1),
 // which is the reference's repeated (value - digit) / 2^w.
 
-struct BoundedTable {
-    const u64 *pow64;   // [q_limbs][W]: 2^(64k) mod q_i
-    const u64 *prefix;  // [q_limbs][W]: q_0 .. q_{i-1} (1 for i = 0)
-    const u64 *inv;     // [q_limbs]: (q_0 .. q_{i-1})^-1 mod q_i
-    const u64 *half;    // [W]: floor(Q / 2)
-    const u64 *qw;      // [W]: Q
-};
+using BoundedTable = CrtTable;
 
 struct BoundedArgs {
     const u64 *in;  // component: limb i of ct c at in + c*in_stride + i*N
@@ -235,51 +230,7 @@ __global__ __launch_bounds__(kKsThreads) void k_bounded_decompose(BoundedArgs a)
     const Gather gi = galois_gather(n, a.kinv, a.logn);
     const u64 *x = a.in + ct * a.in_stride + gi.idx;
     u64 X[W];
-#pragma unroll
-    for (int k = 0; k < W; ++k) X[k] = 0;
-    for (u32 i = 0; i < a.q_limbs; ++i) {
-        const DevPrime &P = a.primes[i];
-        const Barrett br{P.q, P.br_hi, P.br_lo};
-        u64 xi = x[(u64)i * N];
-        if (gi.neg) xi = neg_mod(xi, P.q);
-        const u64 *pw = a.t.pow64 + (u64)i * W;
-        u64 r = 0;
-#pragma unroll
-        for (int k = 0; k < W; ++k) r = add_mod(r, mul_mod(X[k], pw[k], br), P.q);
-        const u64 v = mul_mod(sub_mod(xi, r, P.q), a.t.inv[i], br);
-        const u64 *pre = a.t.prefix + (u64)i * W;
-        u64 carry = 0;
-#pragma unroll
-        for (int k = 0; k < W; ++k) {
-            u64 hi, lo;
-            mul128(v, pre[k], hi, lo);
-            const u64 l1 = lo + carry;
-            hi += l1 < lo;
-            const u64 l2 = l1 + X[k];
-            hi += l2 < l1;
-            X[k] = l2;
-            carry = hi;
-        }
-    }
-    // centered representative: X > floor(Q/2) -> X - Q (two's complement)
-    int cmp = 0;
-#pragma unroll
-    for (int k = W - 1; k >= 0; --k)
-        if (cmp == 0 && X[k] != a.t.half[k]) cmp = X[k] > a.t.half[k] ? 1 : -1;
-    const bool negative = cmp > 0;
-    if (negative) {
-        u64 borrow = 0;
-#pragma unroll
-        for (int k = 0; k < W; ++k) {
-            const u64 q = a.t.qw[k];
-            const u64 d1 = X[k] - q;
-            const u64 b1 = X[k] < q;
-            const u64 d2 = d1 - borrow;
-            const u64 b2 = d1 < borrow;
-            X[k] = d2;
-            borrow = b1 | b2;
-        }
-    }
+    const bool negative = crt_centered<W>(x, N, gi.neg, a.q_limbs, a.primes, a.t, X);
     // balanced digits, w <= 63 bits at a time, from a 128-bit bit buffer
     const u32 w = a.base_log;
     const u64 mask = ((u64)1 << w) - 1, half_base = (u64)1 << (w - 1);
@@ -942,78 +893,22 @@ static int create_key(const ckks_rns_plan *plan, size_t q_limbs, const size_t *b
 
 // ---- bounded gadget tables ---------------------------------------------------------
 
-// Words W of the register-resident value: the smallest instantiated size
-// holding bits(Q) + 1 bits (the sign of X - Q).
-static const unsigned kBoundedWords[] = {4, 8, 12, 16, 24, 32};
-
 struct BoundedHost {
+    CrtHost crt;
     unsigned words = 0, digit_count = 0;
-    std::vector<u64> blob;  // pow64 | prefix | inv | half | qw
+    std::vector<u64> &blob = crt.blob;
 };
-
-static void words_mul_small(std::vector<u64> &x, u64 m) {
-    unsigned __int128 carry = 0;
-    for (auto &w : x) {
-        unsigned __int128 t = (unsigned __int128)w * m + carry;
-        w = (u64)t;
-        carry = t >> 64;
-    }
-}
 
 static int build_bounded(const std::vector<u64> &mods, size_t q_limbs, unsigned base_log, BoundedHost &h) {
     if (base_log < 1 || base_log > 63) return fail(CKKS_ERR_INVALID, "bounded gadget base_log must be in 1..=63");
-    std::vector<u64> q(40, 0);
-    q[0] = 1;
-    for (size_t i = 0; i < q_limbs; ++i) words_mul_small(q, mods[i]);
-    unsigned bits = 0;
-    for (size_t k = q.size(); k-- > 0;)
-        if (q[k]) {
-            bits = (unsigned)(64 * k + 64 - __builtin_clzll(q[k]));
-            break;
-        }
-    h.digit_count = (bits + 1 + base_log - 1) / base_log;
-    h.words = 0;
-    for (unsigned w : kBoundedWords)
-        if (64 * w >= bits + 1) {
-            h.words = w;
-            break;
-        }
-    if (!h.words) return fail(CKKS_ERR_UNSUPPORTED, "bounded gadget: Q above 2047 bits");
-    const unsigned W = h.words;
-    h.blob.assign((size_t)q_limbs * W * 2 + q_limbs + 2 * W, 0);
-    u64 *pow64 = h.blob.data(), *prefix = pow64 + q_limbs * W, *inv = prefix + q_limbs * W;
-    u64 *half = inv + q_limbs, *qw = half + W;
-    std::vector<u64> pre(W, 0);
-    pre[0] = 1;
-    for (size_t i = 0; i < q_limbs; ++i) {
-        const u64 qi = mods[i];
-        const u64 two64 = (u64)((((unsigned __int128)1) << 64) % qi);
-        u64 p = 1 % qi;
-        for (unsigned k = 0; k < W; ++k) {
-            pow64[i * W + k] = p;
-            p = host::mulmod(p, two64, qi);
-        }
-        u64 pm = 0;  // prefix mod q_i
-        for (unsigned k = W; k-- > 0;) pm = host::mulmod(pm, two64, qi), pm = (pm + pre[k] % qi) % qi;
-        inv[i] = host::invmod(pm, qi);
-        for (unsigned k = 0; k < W; ++k) prefix[i * W + k] = pre[k];
-        words_mul_small(pre, qi);
-    }
-    for (unsigned k = 0; k < W; ++k) {
-        qw[k] = q[k];
-        half[k] = (q[k] >> 1) | (k + 1 < q.size() ? q[k + 1] << 63 : 0);
-    }
+    if (!build_crt(mods, q_limbs, h.crt)) return fail(CKKS_ERR_UNSUPPORTED, "bounded gadget: Q above 2047 bits");
+    h.words = h.crt.words;
+    h.digit_count = (h.crt.bits + 1 + base_log - 1) / base_log;
     return CKKS_OK;
 }
 
 static BoundedTable bounded_table(const u64 *d_blob, size_t q_limbs, unsigned W) {
-    BoundedTable t;
-    t.pow64 = d_blob;
-    t.prefix = t.pow64 + q_limbs * W;
-    t.inv = t.prefix + q_limbs * W;
-    t.half = t.inv + q_limbs;
-    t.qw = t.half + W;
-    return t;
+    return crt_table(d_blob, q_limbs, W);
 }
 
 static int launch_bounded(hipStream_t s, const BoundedArgs &a, unsigned W) {
@@ -1218,6 +1113,13 @@ static int key_switch_core(const ckks_ksk *k, const u64 *x, u64 x_stride, const 
         default: return fail(CKKS_ERR_UNSUPPORTED, "too many special primes");
     }
     CK_HIP(hipGetLastError());
+    return CKKS_OK;
+}
+
+int ksk_info(const ckks_ksk *k, size_t *q_limbs, size_t *degree) {
+    if (!k) return CKKS_ERR_INVALID;
+    *q_limbs = k->q_limbs;
+    *degree = k->degree;
     return CKKS_OK;
 }
 

@@ -1,0 +1,258 @@
+"""Plaintext-diagonal x rotation programs: the linear-algebra callers of the
+rotation hot path (the BSGS / radix-4 FFT and GEMV / CCMM schedules of the
+reference's eblas layer, eblas/prepared_radix4_fft256.rs, eblas/fft.rs,
+and the structured CCMM of bin/structured_ccmm_4096.rs).
+
+Slot vectors have S = N/2 slots and hold data of period d (d | S), repeated
+S/d times.  A matrix M acting on the period is applied by its diagonals
+diag_i[j] = M[j][(j + i) mod d]:  M x = sum_i diag_i * rot_i(x), rot_i the
+left rotation by i slots (Galois exponent 5^i).  Dense matrices use
+baby-step / giant-step:  i = g k + b,
+    M x = sum_k rot_{g k}( sum_b rot_{-g k}(diag_{g k + b}) * rot_b(x) ),
+which takes g - 1 + K - 1 rotations instead of d - 1.  Sparse matrices (FFT
+stages, the CCMM masks) use their few diagonals directly.
+
+Builders emit serving.Program ops.  `key_for(limbs, steps)` returns (key,
+Galois exponent) for a left rotation by `steps` at a level of `limbs` limbs
+(None keys build cleartext-only programs for serving.Program.simulate), and
+`encode(values, limbs)` returns the NTT-domain device plaintext of a slot
+vector at that level (None likewise).  Every plaintext multiply is followed
+by one rescale of the sum, so a transform uses one level.
+"""
+from __future__ import annotations
+
+import math
+from typing import Callable, Optional
+
+import numpy as np
+
+from .serving import Program
+
+
+def tile(values, period: int, slots: int) -> np.ndarray:
+    v = np.asarray(values, dtype=np.complex128)
+    assert v.shape == (period,) and slots % period == 0
+    return np.tile(v, slots // period)
+
+
+def matrix_diagonals(M: np.ndarray) -> dict:
+    """diag_i[j] = M[j][(j + i) mod d] for the nonzero diagonals of M."""
+    d = M.shape[0]
+    j = np.arange(d)
+    out = {}
+    for i in range(d):
+        v = M[j, (j + i) % d]
+        if np.any(v != 0):
+            out[i] = v.astype(np.complex128)
+    return out
+
+
+def permutation_diagonals(cols: np.ndarray) -> dict:
+    """Diagonals of the permutation y[j] = x[cols[j]] on a period of len(cols)."""
+    d = len(cols)
+    off = (np.asarray(cols) - np.arange(d)) % d
+    out = {}
+    for i in np.unique(off):
+        v = np.zeros(d, dtype=np.complex128)
+        v[off == i] = 1.0
+        out[int(i)] = v
+    return out
+
+
+def compose_diagonals(A: dict, B: dict, d: int) -> dict:
+    """Diagonals of A @ B: (AB)_i[j] = sum_a A_a[j] B_{i-a}[(j + a) mod d]."""
+    out: dict = {}
+    for a, va in A.items():
+        for b, vb in B.items():
+            i = (a + b) % d
+            term = va * np.roll(vb, -a)
+            out[i] = out.get(i, 0) + term
+    return {i: v for i, v in out.items() if np.any(np.abs(v) > 1e-300)}
+
+
+class Builder:
+    def __init__(self, slots: int, key_for: Callable, encode: Callable, moduli):
+        self.S = slots
+        self.key_for = key_for
+        self.encode = encode
+        self.moduli = list(moduli)
+
+    def _rot(self, prog: Program, dst, src, steps, limbs):
+        steps %= self.S
+        if steps == 0:
+            prog.copy(dst, src)
+            return
+        key, exponent = self.key_for(limbs, steps)
+        prog.rotate(dst, src, key, exponent, steps=steps)
+
+    def _pt(self, values, limbs):
+        return self.encode(values, limbs)
+
+    def transform_direct(self, prog: Program, x, y, tmp, diags: dict, period: int, limbs: int):
+        """y = M x from M's (few) diagonals, then one rescale (in place on y)."""
+        first = True
+        for i, v in sorted(diags.items()):
+            vals = tile(v, period, self.S)
+            if i % period == 0:
+                src = x
+            else:
+                self._rot(prog, tmp, x, i, limbs)
+                src = tmp
+            if first:
+                prog.mul_plain(y, src, self._pt(vals, limbs), values=vals)
+                first = False
+            else:
+                prog.mul_plain(tmp, src, self._pt(vals, limbs), values=vals)
+                prog.add(y, y, tmp)
+        prog.rescale(y, y)
+
+    def transform_bsgs(self, prog: Program, x, y, regs: list, diags: dict, period: int, limbs: int,
+                       unit: int = 1, g: Optional[int] = None):
+        """y = M x by baby-step / giant-step over diagonal offsets that are
+        multiples of `unit`; regs: scratch registers (g + 2 of them)."""
+        offs = sorted(diags)
+        assert all(i % unit == 0 for i in offs)
+        span = max(offs) // unit + 1
+        g = g or max(1, int(math.ceil(math.sqrt(span))))
+        assert len(regs) >= g + 2
+        baby, acc, tmp = regs[:g], regs[g], regs[g + 1]
+        used_b = sorted({(i // unit) % g for i in offs})
+        for b in used_b:
+            self._rot(prog, baby[b], x, b * unit, limbs)
+        first_y = True
+        for k in range((span + g - 1) // g):
+            terms = [(b, diags[(g * k + b) * unit]) for b in range(g) if (g * k + b) * unit in diags]
+            if not terms:
+                continue
+            for t, (b, v) in enumerate(terms):
+                # rot_{-gk}(diag) so the giant rotation lands it on diag
+                vals = np.roll(tile(v, period, self.S), g * k * unit)
+                if t == 0:
+                    prog.mul_plain(acc, baby[b], self._pt(vals, limbs), values=vals)
+                else:
+                    prog.mul_plain(tmp, baby[b], self._pt(vals, limbs), values=vals)
+                    prog.add(acc, acc, tmp)
+            if k:
+                self._rot(prog, tmp, acc, g * k * unit, limbs)
+                src = tmp
+            else:
+                src = acc
+            if first_y:
+                prog.copy(y, src)
+                first_y = False
+            else:
+                prog.add(y, y, src)
+        prog.rescale(y, y)
+
+
+# ---------------------------------------------------------------- workloads
+
+def gemv_program(b: Builder, M: np.ndarray, limbs: int) -> Program:
+    """Public d x d matrix times an encrypted d-periodic vector (Halevi-Shoup
+    diagonals, BSGS)."""
+    d = M.shape[0]
+    diags = matrix_diagonals(M)
+    span = max(diags) + 1
+    g = max(1, int(math.ceil(math.sqrt(span))))
+    prog = Program(n_regs=g + 4, out_reg=1)
+    b.transform_bsgs(prog, 0, 1, list(range(2, g + 4)), diags, d, limbs, g=g)
+    return prog
+
+
+def bitrev_permutation(d: int) -> np.ndarray:
+    bits = d.bit_length() - 1
+    return np.array([int(format(i, f"0{bits}b")[::-1], 2) if bits else 0 for i in range(d)])
+
+
+def dft_stage_diagonals(m: int, d: int) -> dict:
+    """Radix-2 decimation-in-time stage of span m on a period of d:
+    out[s+p] = in[s+p] + w^p in[s+p+m], out[s+p+m] = in[s+p] - w^p in[s+p+m],
+    w = exp(-2 pi i / 2m) (numpy's forward sign)."""
+    j = np.arange(d)
+    p = j % (2 * m)
+    low = p < m
+    w = np.exp(-2j * np.pi * (np.where(low, p, p - m)) / (2 * m))
+    d0 = np.where(low, 1.0, -w)
+    dm = np.where(low, w, 0.0)
+    dneg = np.where(low, 0.0, 1.0)
+    out = {0: d0.astype(np.complex128)}
+    if m % d:
+        out[m % d] = dm.astype(np.complex128)
+        out[(d - m) % d] = out.get((d - m) % d, 0) + dneg.astype(np.complex128)
+    return out
+
+
+def fft_stages(d: int, radix: int = 4) -> list:
+    """Diagonal sets of the DFT of length d as a product of radix-4 stages
+    (pairs of radix-2 stages; a final radix-2 stage when log2 d is odd).
+    Input in bit-reversed order, output in natural order."""
+    spans = [1 << s for s in range(d.bit_length() - 1)]
+    stages = []
+    i = 0
+    while i < len(spans):
+        st = dft_stage_diagonals(spans[i], d)
+        if radix == 4 and i + 1 < len(spans):
+            st = compose_diagonals(dft_stage_diagonals(spans[i + 1], d), st, d)
+            i += 2
+        else:
+            i += 1
+        stages.append(st)
+    return stages
+
+
+def fft_program(b: Builder, d: int, limbs: int) -> Program:
+    """Encrypted length-d DFT (input slots in bit-reversed order, as the
+    caller encodes them): one level per radix-4 stage."""
+    stages = fft_stages(d)
+    prog = Program(n_regs=3, out_reg=0)
+    for t, st in enumerate(stages):
+        b.transform_direct(prog, 0, 1, 2, st, d, limbs - t)
+        prog.copy(0, 1)
+    return prog
+
+
+def ccmm_permutations(d: int):
+    r, c = np.divmod(np.arange(d * d), d)
+    sigma = r * d + (r + c) % d
+    tau = ((r + c) % d) * d + c
+    phi = [r * d + (c + k) % d for k in range(d)]
+    return sigma, tau, phi
+
+
+def ccmm_program(b: Builder, d: int, limbs: int, relin_key) -> Program:
+    """Encrypted d x d matrix product A B (Jiang-Kim-Lauter-Song): A, B
+    row-major in registers 0 and 1;
+      A B = sum_k phi^k(sigma(A)) * psi^k(tau(B)),
+    sigma, tau by BSGS (one level), phi^k by its two diagonals (one level),
+    psi^k a rotation by d k then a mod-switch, the products relinearized and
+    rescaled (one level): three levels in all."""
+    dd = d * d
+    sigma, tau, phi = ccmm_permutations(d)
+    sd = permutation_diagonals(sigma)
+    td = permutation_diagonals(tau)
+    g_s = max(1, int(math.ceil(math.sqrt(2 * d - 1))))
+    g_t = max(1, int(math.ceil(math.sqrt(d))))
+    g = max(g_s, g_t)
+    # registers: 0 A, 1 B, 2 sA, 3 tB, 4 Ak, 5 Bk, 6 prod, 7 C, 8 tmp, 9.. scratch
+    prog = Program(n_regs=10 + g + 2, out_reg=7, n_inputs=2)
+    scratch = list(range(10, 10 + g + 2))
+    # sigma's offsets are in (-d, d): rotate A right by d-1 first so they are 0..2d-2
+    shifted = {(i + d - 1) % dd: v for i, v in sd.items()}
+    b._rot(prog, 8, 0, -(d - 1), limbs)
+    b.transform_bsgs(prog, 8, 2, scratch, shifted, dd, limbs, g=g_s)
+    b.transform_bsgs(prog, 1, 3, scratch, td, dd, limbs, unit=d, g=g_t)
+    lv = limbs - 1
+    for k in range(d):
+        b.transform_direct(prog, 2, 4, 8, permutation_diagonals(phi[k]), dd, lv)
+        b._rot(prog, 5, 3, d * k, lv)
+        prog.mod_switch(5, 5)
+        if k == 0:
+            prog.mul_relin_rescale(7, 4, 5, relin_key)
+        else:
+            prog.mul_relin_rescale(6, 4, 5, relin_key)
+            prog.add(7, 7, 6)
+    return prog
+
+
+def rotation_steps(prog: Program) -> set:
+    return {int(m) for (code, *_), m in zip(prog.ops, prog.meta) if code == 1}

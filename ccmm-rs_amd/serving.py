@@ -1,0 +1,165 @@
+"""Continuous-batching serving loop (libckks_hip.so ckks_server_*) and the
+program format its jobs use.
+
+A program is a list of ops over ciphertext registers [2][limbs][N]
+(coefficient domain).  Registers 0 .. n_inputs-1 start as the job's inputs;
+`out_reg` is copied to the job's output at the end.  The server validates a
+program at submission, admits jobs by register footprint against an HBM
+budget, and batches the same op of different jobs into one launch.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Optional
+
+from . import PreparedRnsNttPlan, _check, _p, _ptr, _sz, lib
+
+OP_ROTATE, OP_MUL_PLAIN, OP_ADD, OP_RESCALE, OP_MUL_RELIN_RESCALE, OP_COPY, OP_MOD_SWITCH = 1, 2, 3, 4, 5, 6, 7
+
+
+class _Op(C.Structure):
+    _fields_ = [("code", C.c_uint32), ("dst", C.c_uint32), ("a", C.c_uint32), ("b", C.c_uint32),
+                ("key", _p), ("exponent", C.c_uint64), ("plaintext", _p)]
+
+
+class _Job(C.Structure):
+    _fields_ = [("plan", _p), ("n_limbs", C.c_uint32), ("n_regs", C.c_uint32), ("out_reg", C.c_uint32),
+                ("n_inputs", C.c_uint32), ("input", _p), ("output", _p), ("ops", _p), ("n_ops", _sz)]
+
+
+class _Stats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("submitted", "completed", "rejected", "waiting", "active",
+                                         "peak_active", "launches", "steps", "peak_bytes")] + \
+               [(n, C.c_double) for n in ("p50_ms", "p99_ms", "mean_ms", "max_ms")]
+
+
+def _fn(name):
+    return getattr(lib(), name)
+
+
+@dataclass
+class Program:
+    """Straight-line op list; `keep` holds device plaintexts alive."""
+    n_regs: int
+    out_reg: int
+    n_inputs: int = 1
+    ops: list = field(default_factory=list)
+    keep: list = field(default_factory=list)
+
+    meta: list = field(default_factory=list)  # per op: slot rotation or plaintext slot values
+
+    def rotate(self, dst, a, key, exponent, steps=None):
+        self.ops.append((OP_ROTATE, dst, a, 0, key, int(exponent), None))
+        self.meta.append(steps)
+
+    def mul_plain(self, dst, a, pt, values=None):
+        if pt is not None:
+            self.keep.append(pt)
+        self.ops.append((OP_MUL_PLAIN, dst, a, 0, None, 0, pt))
+        self.meta.append(values)
+
+    def add(self, dst, a, b):
+        self.ops.append((OP_ADD, dst, a, b, None, 0, None))
+        self.meta.append(None)
+
+    def rescale(self, dst, a):
+        self.ops.append((OP_RESCALE, dst, a, 0, None, 0, None))
+        self.meta.append(None)
+
+    def mul_relin_rescale(self, dst, a, b, key):
+        self.ops.append((OP_MUL_RELIN_RESCALE, dst, a, b, key, 0, None))
+        self.meta.append(None)
+
+    def copy(self, dst, a):
+        self.ops.append((OP_COPY, dst, a, 0, None, 0, None))
+        self.meta.append(None)
+
+    def mod_switch(self, dst, a):
+        self.ops.append((OP_MOD_SWITCH, dst, a, 0, None, 0, None))
+        self.meta.append(None)
+
+    def simulate(self, inputs):
+        """Runs the program on cleartext slot vectors (FP64): rotations are
+        cyclic shifts, plaintext products elementwise, rescale and
+        mod-switch identities.  The FP64 reference of a program's result."""
+        import numpy as np
+
+        regs = [None] * self.n_regs
+        for i, v in enumerate(inputs):
+            regs[i] = np.asarray(v, dtype=np.complex128)
+        for (code, dst, a, b, _k, _e, _pt), meta in zip(self.ops, self.meta):
+            if code == OP_ROTATE:
+                regs[dst] = np.roll(regs[a], -int(meta))
+            elif code == OP_MUL_PLAIN:
+                regs[dst] = regs[a] * meta
+            elif code == OP_ADD:
+                regs[dst] = regs[a] + regs[b]
+            elif code == OP_MUL_RELIN_RESCALE:
+                regs[dst] = regs[a] * regs[b]
+            else:
+                regs[dst] = regs[a].copy()
+        return regs[self.out_reg]
+
+    def counts(self) -> dict:
+        names = {OP_ROTATE: "rotate", OP_MUL_PLAIN: "mul_plain", OP_ADD: "add", OP_RESCALE: "rescale",
+                 OP_MUL_RELIN_RESCALE: "mul_relin_rescale", OP_COPY: "copy", OP_MOD_SWITCH: "mod_switch"}
+        out: dict = {}
+        for op in self.ops:
+            out[names[op[0]]] = out.get(names[op[0]], 0) + 1
+        return out
+
+
+class Server:
+    WAITING, ADMITTED, DONE = 0, 1, 2
+
+    def __init__(self, ctx, hbm_budget_bytes: int, max_active: int = 0):
+        self.ctx = ctx
+        h = _p()
+        _check(_fn("ckks_server_create")(ctx.handle, int(hbm_budget_bytes), int(max_active), C.byref(h)))
+        self._h = h
+        self._jobs = {}  # ticket -> (job struct, op array, program, tensors) kept alive until done
+
+    def submit(self, plan: PreparedRnsNttPlan, n_limbs: int, inputs, output, program: Program) -> int:
+        ops = (_Op * max(len(program.ops), 1))()
+        for i, (code, dst, a, b, key, exp, pt) in enumerate(program.ops):
+            ops[i] = _Op(code, dst, a, b, key.handle if key is not None else None, exp,
+                         _ptr(pt) if pt is not None else None)
+        job = _Job(plan.handle, int(n_limbs), program.n_regs, program.out_reg, program.n_inputs,
+                   _ptr(inputs), _ptr(output), C.cast(ops, _p), len(program.ops))
+        t = C.c_uint64()
+        _check(_fn("ckks_server_submit")(self._h, C.addressof(job), C.byref(t)))
+        self._jobs[t.value] = (job, ops, program, inputs, output)
+        return t.value
+
+    def step(self) -> int:
+        n = _sz()
+        _check(_fn("ckks_server_step")(self._h, C.byref(n)))
+        return n.value
+
+    def poll(self, ticket: int):
+        st, lat = C.c_int(), C.c_double()
+        _check(_fn("ckks_server_poll")(self._h, int(ticket), C.byref(st), C.byref(lat)))
+        if st.value == self.DONE:
+            self._jobs.pop(ticket, None)
+        return st.value, (lat.value if st.value == self.DONE else None)
+
+    def drain(self):
+        _check(_fn("ckks_server_drain")(self._h))
+
+    def stats(self) -> dict:
+        s = _Stats()
+        _check(_fn("ckks_server_stats")(self._h, C.addressof(s)))
+        return {n: getattr(s, n) for n, _ in _Stats._fields_}
+
+    def close(self):
+        # the context destroys its servers itself when it is closed first
+        if getattr(self, "_h", None) and self.ctx.handle:
+            _fn("ckks_server_destroy")(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

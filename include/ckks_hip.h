@@ -234,6 +234,22 @@ int ckks_decrypt_raw(const ckks_rns_plan *plan, const uint64_t *b, const uint64_
                      const int8_t *secret, uint64_t *out, size_t n_polys, size_t n_limbs,
                      size_t poly_stride, size_t out_stride);
 
+/* encode_rns (application_support/ckks.rs:39-66), integer half: real
+ * coefficients [p][N] (device doubles, from the canonical embedding) times
+ * scale, rounded half away from zero (f64::round), rem_euclid every limb ->
+ * out [p][limb][N].  CKKS_ERR_INVALID if a rounded value reaches 2^63.
+ * Synchronises the context stream (overflow check). */
+int ckks_encode_coefficients(const ckks_rns_plan *plan, const double *coeffs, double scale,
+                             uint64_t *out, size_t n_polys, size_t n_limbs);
+
+/* Decode (bin/r30_13b_ckks_primitives.rs:55-80), integer half: exact CRT of
+ * [p][limb][N] residues over the first n_limbs limbs to the centered integer
+ * (value > floor(Q/2) -> negative, ring/wide_crt.rs), converted to FP64 and
+ * divided by scale -> out [p][N] doubles.  The CRT tables are built on the
+ * first call per limb count (not capturable) and kept by the plan. */
+int ckks_decode_coefficients(const ckks_rns_plan *plan, const uint64_t *in, double scale,
+                             double *out, size_t n_polys, size_t n_limbs);
+
 /* rns_tensor_with_prepared_ntt (grafting/rns_key_switch.rs:1724-1777):
  * (b0,a0) x (b1,a1) -> (c0,c1,c2) coefficient domain, out laid out as
  * [c0 limbs][c1 limbs][c2 limbs] per ciphertext (out_stride elements). */
@@ -321,6 +337,60 @@ int ckks_multiply_relinearize_rescale(const ckks_ksk *relin_key, const uint64_t 
  * counterpart. */
 int ckks_autotune_key_switch(ckks_ksk *key, size_t n_cts, int *best_engine, int *best_fusion,
                              float *ms);
+
+/* ------------------------------------------------------------ serving */
+/* Continuous-batching request loop on one GPU.  No reference counterpart
+ * (the reference evaluates one job at a time on the CPU).  A job is a
+ * straight-line program over ciphertext registers [2][limbs][N] (registers
+ * 0.. n_inputs-1 start as copies of the inputs; `output` receives register
+ * `out_reg` at the end), validated at submission (register use, levels, key bases).  Each
+ * ckks_server_step retires finished requests, admits waiting ones in
+ * arrival order while their register footprint (n_regs * 2 * n_limbs * N *
+ * 8 bytes) fits the HBM budget and the active cap, then runs the next op of
+ * every active request, batching ops that share (op, key, exponent, plan,
+ * level) into one launch.  Evaluation keys and plans stay resident and are
+ * owned by the caller; they must outlive the jobs that name them.  All work
+ * runs on the context stream. */
+enum {
+    CKKS_OP_ROTATE = 1,             /* dst = keyswitch(sigma_exponent(a)), Galois key */
+    CKKS_OP_MUL_PLAIN = 2,          /* dst = a * plaintext (NTT domain, a's limbs) */
+    CKKS_OP_ADD = 3,                /* dst = a + b */
+    CKKS_OP_RESCALE = 4,            /* dst = rescale(a) */
+    CKKS_OP_MUL_RELIN_RESCALE = 5,  /* dst = rescale(relin(a x b)), relinearization key */
+    CKKS_OP_COPY = 6,               /* dst = a */
+    CKKS_OP_MOD_SWITCH = 7          /* dst = a without its last limb */
+};
+typedef struct ckks_op {
+    uint32_t code, dst, a, b;
+    const ckks_ksk *key;
+    uint64_t exponent;
+    const uint64_t *plaintext;
+} ckks_op;
+typedef struct ckks_job {
+    const ckks_rns_plan *plan; /* basis whose first n_limbs limbs hold the input */
+    uint32_t n_limbs, n_regs, out_reg;
+    uint32_t n_inputs;         /* registers 0.. n_inputs-1 <- input[i] (0 means 1) */
+    const uint64_t *input;     /* [n_inputs][2][n_limbs][N] */
+    uint64_t *output;
+    const ckks_op *ops;
+    size_t n_ops;
+} ckks_job;
+typedef struct ckks_server_stats {
+    uint64_t submitted, completed, rejected, waiting, active, peak_active, launches, steps;
+    uint64_t peak_bytes;
+    double p50_ms, p99_ms, mean_ms, max_ms; /* submit -> completion observed */
+} ckks_server_statistics;
+typedef struct ckks_server ckks_server;
+int ckks_server_create(ckks_context *ctx, size_t hbm_budget_bytes, size_t max_active, ckks_server **out);
+/* CKKS_ERR_INVALID for a malformed program, CKKS_ERR_OOM for a job whose
+ * footprint exceeds the whole budget; the op list is copied. */
+int ckks_server_submit(ckks_server *server, const ckks_job *job, uint64_t *ticket);
+int ckks_server_step(ckks_server *server, size_t *launched);
+/* state: 0 waiting, 1 admitted, 2 done (latency_ms set, else -1). */
+int ckks_server_poll(ckks_server *server, uint64_t ticket, int *state, double *latency_ms);
+int ckks_server_drain(ckks_server *server);
+int ckks_server_stats(ckks_server *server, ckks_server_statistics *out);
+int ckks_server_destroy(ckks_server *server);
 
 /* ------------------------------------------------------------ graphs */
 /* Rotation schedules (BSGS of CCMM / FFT) captured once as a hipGraph and

@@ -5,8 +5,8 @@ TEST INFRASTRUCTURE ONLY.  numpy restatement of
     exp(i pi (2j+1)/N); logical slot j is the root with exponent 5^j mod 2N;
     evaluation = phase twist zeta^k then a positive-sign DFT; interpolation =
     negative-sign DFT, 1/N and the inverse twist;
-  encode_rns (application_support/ckks.rs:39-66): round(coef * scale) then
-    rem_euclid per limb;
+  encode_rns (application_support/ckks.rs:39-66): round(coef * scale), half
+    away from zero as f64::round, then rem_euclid per limb;
   decode (bin/r30_13b_ckks_primitives.rs:55-80): exact CRT, centered,
     divided by the scale, then coefficients_to_slots.
 FP64 results agree with the reference's radix-2 FFT to rounding (not bit for
@@ -47,9 +47,17 @@ class Embedding:
         return coef.real
 
 
+def round_half_away(x: float) -> int:
+    """f64::round: halfway cases away from zero (x - trunc(x) is exact)."""
+    t = math.trunc(x)
+    if abs(x - t) >= 0.5:
+        t += 1 if x > 0 else -1
+    return int(t)
+
+
 def encode_rns(slots, emb: Embedding, moduli, scale: float) -> np.ndarray:
     coeffs = emb.slots_to_coefficients(slots)
-    signed = [int(round(v * scale)) for v in coeffs]
+    signed = [round_half_away(v * scale) for v in coeffs]
     return np.array([[s % q for s in signed] for q in moduli], dtype=np.uint64)
 
 

@@ -1,0 +1,136 @@
+// CPU build of the serving scheduler (ccmm-rs_amd/csrc/serve_core.hpp) with
+// a recording executor: launches are counted instead of run, completion
+// markers are reached `lag` scheduler steps after a request finishes, and
+// an op code can be made to fail.  Test infrastructure only.
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../ccmm-rs_amd/csrc/serve_core.hpp"
+
+using namespace ck::serve;
+
+struct FakeExec {
+    int lag = 1, fail_code = 0;
+    double clock = 0;
+    long launches = 0, group_members = 0;
+    std::vector<uint64_t> done_order;
+    std::vector<std::pair<Request *, int>> pending;  // request, steps left
+    char pool[1 << 16];
+    bool alloc(Request &r) {
+        r.regs = pool;
+        return true;
+    }
+    void release(Request &) {}
+    int load_input(Request &) { return 0; }
+    int run(Group &g) {
+        if ((int)g.code == fail_code) return 7;
+        ++launches;
+        group_members += (long)g.members.size();
+        return 0;
+    }
+    int finish(Request &r) {
+        pending.emplace_back(&r, lag);
+        return 0;
+    }
+    bool done(Request &r) {
+        for (auto &p : pending)
+            if (p.first == &r && p.second <= 0) {
+                done_order.push_back(r.ticket);
+                return true;
+            }
+        return false;
+    }
+    void tick() {
+        clock += 1.0;
+        for (auto &p : pending) --p.second;
+    }
+    double now_ms() const { return clock; }
+};
+
+static KeyInfo kKey{3, 8};
+static const KeyInfo *lookup(const void *k) { return k == (const void *)&kKey ? &kKey : nullptr; }
+static uint64_t dummy_in[1], dummy_out[1];
+
+static Job make_job(int kind, size_t regs) {
+    Job j{};
+    j.plan = &kKey;
+    j.degree = 8;
+    j.n_limbs = 3;
+    j.n_regs = regs;
+    j.out_reg = regs - 1;
+    j.input = dummy_in;
+    j.output = dummy_out;
+    // kind 0: rotate, mul_plain, add, rescale (4 ops); kind 1: add, copy (2 ops)
+    if (kind == 0) {
+        j.ops.push_back(Op{OP_ROTATE, 1, 0, 0, &kKey, 5, nullptr});
+        j.ops.push_back(Op{OP_MUL_PLAIN, 1, 1, 0, nullptr, 0, dummy_in});
+        j.ops.push_back(Op{OP_ADD, 1, 1, 0, nullptr, 0, nullptr});
+        j.ops.push_back(Op{OP_RESCALE, (uint32_t)(regs - 1), 1, 0, nullptr, 0, nullptr});
+    } else {
+        j.ops.push_back(Op{OP_ADD, 1, 0, 0, nullptr, 0, nullptr});
+        j.ops.push_back(Op{OP_COPY, (uint32_t)(regs - 1), 1, 0, nullptr, 0, nullptr});
+    }
+    return j;
+}
+
+extern "C" {
+
+// Runs n_jobs jobs (kinds[i], submitted at step arrive[i]) under a budget
+// of `budget_jobs` 2-register footprints and an active cap; fills
+// out[0..7] = completed, peak_active, launches, group_members, steps,
+// peak_bytes / footprint, last error code, and done_order into order[].
+int serve_emu_run(int n_jobs, const int *kinds, const int *arrive, int budget_jobs, int max_active, int lag,
+                  int fail_code, long *out, uint64_t *order) {
+    FakeExec ex;
+    ex.lag = lag;
+    ex.fail_code = fail_code;
+    const size_t foot = footprint(make_job(0, 2));
+    Scheduler<FakeExec> s(ex, (size_t)budget_jobs * foot, (size_t)max_active);
+    int submitted = 0, rc = 0, step = 0;
+    while ((submitted < n_jobs || !s.idle()) && step < 10000) {
+        for (int i = 0; i < n_jobs; ++i)
+            if (arrive[i] == step) {
+                Job j = make_job(kinds[i], 2);
+                if (!validate(j, lookup).empty()) return -1;
+                s.submit(std::move(j));
+                ++submitted;
+            }
+        rc = s.step();
+        if (rc) break;
+        ex.tick();
+        ++step;
+    }
+    auto &st = s.stats();
+    out[0] = (long)st.completed;
+    out[1] = (long)st.peak_active;
+    out[2] = ex.launches;
+    out[3] = ex.group_members;
+    out[4] = (long)st.steps;
+    out[5] = (long)(st.peak_bytes / foot);
+    out[6] = rc;
+    for (size_t i = 0; i < ex.done_order.size(); ++i) order[i] = ex.done_order[i];
+    return 0;
+}
+
+// Validation message of a malformed program, case 0..6.
+int serve_emu_validate(int which, char *msg, int cap) {
+    Job j = make_job(0, 2);
+    switch (which) {
+        case 0: j.ops[0].key = nullptr; break;                   // rotation without key
+        case 1: j.ops[0].exponent = 4; break;                    // even exponent
+        case 2: j.ops.insert(j.ops.begin(), Op{OP_ADD, 1, 0, 1, nullptr, 0, nullptr}); break;  // reads empty reg
+        case 3: j.out_reg = 5; break;                            // outside the file
+        case 4: j.ops.push_back(Op{OP_RESCALE, 1, 1, 0, nullptr, 0, nullptr});
+                j.ops.push_back(Op{OP_RESCALE, 1, 1, 0, nullptr, 0, nullptr}); break;  // past the last level
+        case 5: j.ops.push_back(Op{OP_ROTATE, 1, 1, 0, &kKey, 5, nullptr}); break;  // key level mismatch
+        case 6: break;                                            // valid
+        default: return -1;
+    }
+    std::string m = validate(j, lookup);
+    std::strncpy(msg, m.c_str(), (size_t)cap - 1);
+    msg[cap - 1] = 0;
+    return (int)m.size();
+}
+
+}

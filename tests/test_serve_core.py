@@ -1,0 +1,78 @@
+"""The serving scheduler (serve_core.hpp) compiled for the CPU with a
+recording executor: admission against the byte budget and the active cap,
+arrival-order fairness, batching of equal ops across requests, program
+validation messages and error propagation, without a GPU."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "emu")
+
+
+@pytest.fixture(scope="module")
+def emu():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    lib = C.CDLL(os.path.join(HERE, "libserve_emu.so"))
+    lib.serve_emu_run.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int,
+                                  C.c_void_p, C.c_void_p]
+    lib.serve_emu_validate.argtypes = [C.c_int, C.c_char_p, C.c_int]
+    return lib
+
+
+def run(emu, kinds, arrive, budget, cap=0, lag=1, fail=0):
+    kinds = np.array(kinds, dtype=np.int32)
+    arrive = np.array(arrive, dtype=np.int32)
+    out = np.zeros(8, dtype=np.int64)
+    order = np.zeros(len(kinds), dtype=np.uint64)
+    assert emu.serve_emu_run(len(kinds), kinds.ctypes.data, arrive.ctypes.data, budget, cap, lag, fail,
+                             out.ctypes.data, order.ctypes.data) == 0
+    return dict(completed=out[0], peak_active=out[1], launches=out[2], members=out[3], steps=out[4],
+                peak_jobs=out[5], rc=out[6], order=order.tolist())
+
+
+def test_same_program_batches_into_one_launch_per_op(emu):
+    r = run(emu, [0, 0, 0, 0], [0, 0, 0, 0], budget=8)
+    assert r["completed"] == 4 and r["peak_active"] == 4
+    assert r["launches"] == 4 and r["members"] == 16  # 4 ops, each one launch for all 4 jobs
+
+
+def test_budget_and_cap_bound_admission_and_keep_arrival_order(emu):
+    r = run(emu, [0] * 6, [0] * 6, budget=2)
+    assert r["completed"] == 6 and r["peak_active"] <= 2 and r["peak_jobs"] <= 2
+    assert r["order"] == sorted(r["order"])  # first come, first served
+    r = run(emu, [0] * 6, [0] * 6, budget=6, cap=3)
+    assert r["peak_active"] == 3 and r["completed"] == 6
+
+
+def test_late_arrivals_join_requests_in_flight(emu):
+    # job 2 (add, copy) arrives at step 2 while job 0 is at its add: the two
+    # adds (same op, same level) go out as one launch
+    r = run(emu, [0, 0, 1], [0, 1, 2], budget=8)
+    assert r["completed"] == 3
+    assert r["members"] == 4 + 4 + 2
+    assert r["launches"] == r["members"] - 1
+
+
+def test_executor_error_stops_the_step(emu):
+    r = run(emu, [0, 0], [0, 0], budget=4, fail=2)  # plaintext multiply fails
+    assert r["rc"] == 7 and r["completed"] == 0
+
+
+@pytest.mark.parametrize("which,msg", [
+    (0, "rotation needs a key and an odd exponent"),
+    (1, "rotation needs a key and an odd exponent"),
+    (2, "addition needs two ciphertexts at one level"),
+    (3, "output register outside the register file"),
+    (4, "cannot leave the final level"),
+    (5, "key basis must match the ciphertext level"),
+    (6, ""),
+])
+def test_validation_messages(emu, which, msg):
+    buf = C.create_string_buffer(256)
+    emu.serve_emu_validate(which, buf, 256)
+    assert buf.value.decode().startswith(msg)
