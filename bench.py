@@ -136,6 +136,110 @@ def measure_ntt(torch, plan, polys, limbs, n, reps=5):
     return limb_transforms, ntt_ms, ntt_gbps, intt_gbps
 
 
+def serving_benchmark(torch, pkg, ctx, n_requests=40, rate=4.0, seed=0):
+    """Continuous-batching serving of a Poisson stream (seeded) of encrypted
+    jobs on one GPU: 40% public 1024x1024 matrix x encrypted vector (N=2^14,
+    L=8), 40% 64x64 encrypted CCMM (N=2^13, L=9), 20% length-4096 encrypted
+    FFT1 (N=2^16, L=8).  Evaluation keys, matrix diagonals and DFT stage
+    diagonals stay resident; admission by HBM footprint.  Synthetic
+    ciphertexts and keys (the work does not depend on their values)."""
+    lin, srv, enc = pkg.linear, pkg.serving, pkg.encoding
+    q_all, p_all = pkg.chain_moduli(24, 7)
+    key_cache = {}
+
+    def blocks_for(limbs):
+        return [limbs - limbs // 2, limbs // 2] if limbs > 1 else [1]
+
+    def setup(n, L):
+        q = q_all[:L]
+        plan = pkg.PreparedRnsNttPlan(ctx, q, n)
+        emb = enc.CanonicalEmbedding(n)
+        qp_plans = {}
+        keys = {}
+
+        def key_for(limbs, steps):
+            k = pow(5, steps, 2 * n)
+            if (limbs, k) not in keys:
+                if limbs not in qp_plans:
+                    qp_plans[limbs] = pkg.PreparedRnsNttPlan(ctx, q[:limbs] + p_all[:4], n)
+                blocks = blocks_for(limbs)
+                shape = (len(blocks), limbs + 4, n)
+                if shape not in key_cache:
+                    key_cache[shape] = synthetic_key(torch, len(blocks), limbs + 4, n)
+                keys[(limbs, k)] = pkg.HybridKeySwitchKey(qp_plans[limbs], limbs, blocks, *key_cache[shape])
+            return keys[(limbs, k)], k
+
+        def encode(values, limbs):
+            pt = enc.encode(plan, emb, torch.from_numpy(np.asarray(values)), float(q[limbs - 1]), limbs)
+            return plan.forward(pt, n_limbs=limbs)
+
+        return plan, lin.Builder(n // 2, key_for, encode, q), key_for, qp_plans, keys
+
+    rng = np.random.default_rng(seed)
+    t0 = time.perf_counter()
+    kinds = {}
+    # GEMV
+    plan, b, key_for, qp, keys = setup(1 << 14, 8)
+    M = rng.uniform(-1, 1, (1024, 1024))
+    kinds["gemv"] = (plan, 8, 1, lin.gemv_program(b, M, 8), 7)
+    # CCMM
+    plan, b, key_for, qp, keys = setup(1 << 13, 9)
+    key_for(7, 1)
+    relin = pkg.HybridKeySwitchKey(qp[7], 7, blocks_for(7), *key_cache[(2, 11, 1 << 13)])
+    kinds["ccmm"] = (plan, 9, 2, lin.ccmm_program(b, 64, 9, relin), 6)
+    kinds["ccmm"][3].keep.append(relin)
+    # FFT1
+    plan, b, key_for, qp, keys = setup(1 << 16, 8)
+    kinds["fft"] = (plan, 8, 1, lin.fft_program(b, 4096, 8), 2)
+    setup_s = time.perf_counter() - t0
+    inputs = {name: rand_residues(torch, (n_in, 2, L, pl.degree), "cuda")
+              for name, (pl, L, n_in, _, _) in kinds.items()}
+    mix = rng.choice(["gemv", "ccmm", "fft"], size=n_requests, p=[0.4, 0.4, 0.2])
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n_requests))
+    foot = max(pr.n_regs * 2 * L * pl.degree * 8 for pl, L, _, pr, _ in kinds.values())
+    server = srv.Server(ctx, hbm_budget_bytes=16 * foot, max_active=16)
+    torch.cuda.synchronize()
+    outs, tickets = [], []
+    start = time.perf_counter()
+    i = 0
+    while i < n_requests or server.stats()["active"] or server.stats()["waiting"]:
+        now = time.perf_counter() - start
+        while i < n_requests and arrivals[i] <= now:
+            name = str(mix[i])
+            pl, L, n_in, prog, out_l = kinds[name]
+            out = torch.empty((2, out_l, pl.degree), dtype=torch.int64, device="cuda")
+            tickets.append((name, server.submit(pl, L, inputs[name], out, prog)))
+            outs.append(out)
+            i += 1
+        if server.stats()["active"] or server.stats()["waiting"]:
+            server.step()
+        elif i < n_requests:
+            time.sleep(max(0.0, min(0.01, arrivals[i] - (time.perf_counter() - start))))
+    server.drain()
+    makespan = time.perf_counter() - start
+    st = server.stats()
+    lat = {}
+    for name, t in tickets:
+        state, ms = server.poll(t)
+        lat.setdefault(name, []).append(ms)
+    server.close()
+    return {
+        "requests": int(st["completed"]),
+        "requests_per_s": round(st["completed"] / makespan, 2),
+        "offered_rate_per_s": rate,
+        "p50_ms": round(st["p50_ms"], 2),
+        "p99_ms": round(st["p99_ms"], 2),
+        "per_kind_p50_ms": {k: round(float(np.percentile(v, 50)), 2) for k, v in lat.items()},
+        "peak_active": int(st["peak_active"]),
+        "peak_register_bytes": int(st["peak_bytes"]),
+        "launches": int(st["launches"]),
+        "program_ops": {k: v[3].counts() for k, v in kinds.items()},
+        "setup_s": round(setup_s, 1),
+        "workload": "Poisson (seeded) mix: 40% GEMV 1024x1024 N=2^14 L=8, 40% CCMM 64x64 N=2^13 L=9, "
+                    "20% FFT1 4096 N=2^16 L=8; synthetic ciphertexts and keys",
+    }
+
+
 def copy_calibration(torch, nbytes=1 << 30, reps=5):
     """A plain device copy of known size: the reference point for reading
     the HBM counters (FETCH_SIZE / WRITE_SIZE) of the NTT kernels."""
@@ -165,8 +269,12 @@ def main():
     ap.add_argument("--fusion", choices=["auto", "off", "xcd", "linear"], default="auto",
                     help="digits' forward row pass fused with the key MAC: autotuned per key, "
                          "off, or fused with XCD-contiguous / linear block order")
-    ap.add_argument("--only", choices=["all", "ntt"], default="all",
-                    help="ntt: only the NTT measurement and a copy of known size (for counter runs)")
+    ap.add_argument("--only", choices=["all", "ntt", "serve"], default="all",
+                    help="ntt: only the NTT measurement and a copy of known size (for counter runs); "
+                         "serve: only the serving loop")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop measurement")
+    ap.add_argument("--serve-requests", type=int, default=40)
+    ap.add_argument("--serve-rate", type=float, default=4.0)
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default NTT variant and base-conversion engine")
     ap.add_argument("--base-conversion", choices=["auto", "valu", "mfma"], default="auto",
@@ -200,6 +308,10 @@ def main():
     qp_plan = pkg.PreparedRnsNttPlan(ctx, q_moduli + p_moduli, N)
     qp2 = q_moduli[:L2] + p_moduli[:K2]
     qp_plan2 = pkg.PreparedRnsNttPlan(ctx, qp2, N2)
+
+    if args.only == "serve":
+        print(json.dumps(serving_benchmark(torch, pkg, ctx, args.serve_requests, args.serve_rate)), flush=True)
+        return
 
     if args.only == "ntt":
         if not args.no_autotune:
@@ -350,6 +462,11 @@ def main():
             "traffic": traffic,
         },
     }
+    if not args.no_serving:
+        print("# serving loop", file=sys.stderr, flush=True)
+        result["serving"] = serving_benchmark(torch, pkg, ctx, args.serve_requests, args.serve_rate)
+        print(f"# serving: {result['serving']['requests_per_s']} req/s, p50 {result['serving']['p50_ms']} ms, "
+              f"p99 {result['serving']['p99_ms']} ms", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         print("# cpu baseline", file=sys.stderr, flush=True)
         result["cpu_baseline"] = cpu_baseline(q_moduli, p_moduli, N, blocks, args.cpu_seconds)
