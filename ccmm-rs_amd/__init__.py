@@ -95,6 +95,7 @@ _SIGNATURES = {
     "ckks_bounded_ksk_create": (C.c_int, [_p, _sz, C.c_uint, _p, _p, C.POINTER(_p)]),
     "ckks_encrypt_raw": (C.c_int, [_p, _p, _p, _p, _p, _p, _sz, _sz, _sz]),
     "ckks_encode_coefficients": (C.c_int, [_p, _p, C.c_double, _p, _sz, _sz]),
+    "ckks_mod_down": (C.c_int, [_p, _sz, _p, _p, _sz]),
     "ckks_decode_coefficients": (C.c_int, [_p, _p, C.c_double, _p, _sz, _sz]),
     "ckks_server_create": (C.c_int, [_p, _sz, _sz, C.POINTER(_p)]),
     "ckks_server_submit": (C.c_int, [_p, _p, C.POINTER(C.c_uint64)]),
@@ -420,6 +421,15 @@ class PreparedRnsNttPlan:
                                  2 * n_limbs * n, 3 * n_limbs * n))
         return out
 
+    def mod_down(self, x, q_limbs: int, out=None):
+        """auxiliary_mod_down_rns_candidate (ring/auxiliary_modulus.rs:89): x
+        [p][plan limbs][N] (Q then P) -> [p][q_limbs][N]."""
+        n = self.degree
+        polys = x.numel() // (len(self.moduli) * n)
+        out = empty((polys, q_limbs, n)) if out is None else out
+        _check(lib().ckks_mod_down(self._h, int(q_limbs), _ptr(x), _ptr(out), polys))
+        return out
+
     def encrypt_raw(self, message, error, secret, a, out=None, n_limbs=None):
         """b = m + e - a*s (grafting/rns_key_switch.rs:245-383) for messages
         [p][limbs][N], errors [p][N] (int64), secret [N] (int8), uniform a
@@ -638,3 +648,4 @@ from . import ckks  # noqa: E402  (level-aware evaluator over the classes above)
 from . import serving  # noqa: E402  (continuous-batching serving loop)
 from . import encoding  # noqa: E402  (CKKS encode / decode on the device)
 from . import linear  # noqa: E402  (rotation-schedule programs of the linear-algebra workloads)
+from . import wire  # noqa: E402  (versioned binary layout of RNS data)

@@ -327,6 +327,10 @@ int ckks_rns_plan_destroy(ckks_rns_plan *plan) {
     hipFree(plan->d_primes);
     hipFree(plan->d_rescale);
     for (auto &kv : plan->crt) hipFree(kv.second.first);
+    for (auto &kv : plan->moddown) {
+        hipFree(kv.second.first);
+        hipFree(kv.second.second);
+    }
     delete plan;
     return CKKS_OK;
 }

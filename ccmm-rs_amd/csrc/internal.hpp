@@ -100,6 +100,8 @@ struct ckks_rns_plan {
     ck::RescaleConst *d_rescale = nullptr;  // [dropped l][target i], L x L
     // exact-CRT tables of the decoder per limb count: device blob, words
     std::map<size_t, std::pair<uint64_t *, unsigned>> crt;
+    // standalone ModDown tables per Q limb count: constants, device table
+    std::map<size_t, std::pair<void *, void *>> moddown;
 };
 
 namespace ck {

@@ -718,6 +718,64 @@ static void fill_garner(const std::vector<u64> &mods, u64 (*pm)[kMaxAlpha], u64 
     }
 }
 
+// ModDown constants of Q u P (the last mods.size() - q_limbs limbs are P):
+// Garner over P into md; per Q limb the mixed-radix weights, P mod q_i,
+// P^-1 mod q_i and its Shoup quotient appended to consts (the table's
+// device pointers are set once consts is uploaded: md.w at the offset
+// consts.size() had on entry).
+static void build_moddown(const std::vector<u64> &mods, size_t q_limbs, std::vector<u64> &consts,
+                          ModDownTable &md) {
+    const size_t K = mods.size() - q_limbs;
+    const size_t md_w_off = consts.size();
+    md.q_limbs = (u32)q_limbs;
+    md.p_limbs = (u32)K;
+    std::vector<u64> pmods(mods.begin() + q_limbs, mods.end());
+    u64 tmp_pm[kMaxAlpha][kMaxAlpha] = {};
+    u64 tmp_inv[kMaxAlpha] = {}, tmp_inv_q[kMaxAlpha] = {};
+    fill_garner(pmods, tmp_pm, tmp_inv, tmp_inv_q, 0);
+    for (size_t a = 0; a < K; ++a) {
+        for (size_t b = 0; b < K; ++b) md.pm[a][b] = tmp_pm[a][b];
+        md.inv[a] = tmp_inv[a];
+        md.inv_q[a] = tmp_inv_q[a];
+    }
+    // mixed-radix digits of floor(P/2): P odd, so floor(P/2) = (P-1)/2.
+    // Digits of P-1 are (p_0 - 1, p_1 - 1, ...) ... computed by halving
+    // the positional representation from the top digit down.
+    {
+        std::vector<u64> d(K);
+        for (size_t m = 0; m < K; ++m) d[m] = pmods[m] - 1;  // P - 1 = sum (p_m - 1) M_m
+        // divide by 2 in mixed radix: process from most significant digit
+        unsigned __int128 rem = 0;
+        for (size_t m = K; m-- > 0;) {
+            unsigned __int128 cur = rem * pmods[m] + d[m];
+            md.half[m] = (u64)(cur / 2);
+            rem = cur % 2;
+        }
+    }
+    for (size_t i = 0; i < q_limbs; ++i) {
+        u64 prod = 1 % mods[i];
+        for (int m = 0; m < kMaxSpecial; ++m) {
+            consts.push_back(m < (int)K ? prod : 0);
+            if (m < (int)K) prod = host::mulmod(prod, pmods[m] % mods[i], mods[i]);
+        }
+    }
+    // P mod q_i, P^-1 mod q_i
+    for (size_t i = 0; i < q_limbs; ++i) {
+        u64 p = 1 % mods[i];
+        for (size_t m = 0; m < K; ++m) p = host::mulmod(p, pmods[m] % mods[i], mods[i]);
+        consts.push_back(p);
+    }
+    for (size_t i = 0; i < q_limbs; ++i) {
+        u64 p = 1 % mods[i];
+        for (size_t m = 0; m < K; ++m) p = host::mulmod(p, pmods[m] % mods[i], mods[i]);
+        u64 inv = host::invmod(p, mods[i]);
+        consts.push_back(inv);
+    }
+    for (size_t i = 0; i < q_limbs; ++i)
+        consts.push_back(host::shoup_quotient(consts[md_w_off + q_limbs * kMaxSpecial + q_limbs + i],
+                                              mods[i]));
+}
+
 static int create_key(const ckks_rns_plan *plan, size_t q_limbs, const size_t *block_sizes,
                       size_t n_blocks, const u64 *key_b, const u64 *key_a, bool hybrid,
                       ckks_ksk **out) {
@@ -774,55 +832,7 @@ static int create_key(const ckks_rns_plan *plan, size_t q_limbs, const size_t *b
     }
     size_t md_w_off = consts.size();
     ModDownTable md{};
-    if (hybrid) {
-        md.q_limbs = (u32)q_limbs;
-        md.p_limbs = (u32)K;
-        std::vector<u64> pmods(mods.begin() + q_limbs, mods.end());
-        u64 tmp_pm[kMaxAlpha][kMaxAlpha] = {};
-        u64 tmp_inv[kMaxAlpha] = {}, tmp_inv_q[kMaxAlpha] = {};
-        fill_garner(pmods, tmp_pm, tmp_inv, tmp_inv_q, 0);
-        for (size_t a = 0; a < K; ++a) {
-            for (size_t b = 0; b < K; ++b) md.pm[a][b] = tmp_pm[a][b];
-            md.inv[a] = tmp_inv[a];
-            md.inv_q[a] = tmp_inv_q[a];
-        }
-        // mixed-radix digits of floor(P/2): P odd, so floor(P/2) = (P-1)/2.
-        // Digits of P-1 are (p_0 - 1, p_1 - 1, ...) ... computed by halving
-        // the positional representation from the top digit down.
-        {
-            std::vector<u64> d(K);
-            for (size_t m = 0; m < K; ++m) d[m] = pmods[m] - 1;  // P - 1 = sum (p_m - 1) M_m
-            // divide by 2 in mixed radix: process from most significant digit
-            unsigned __int128 rem = 0;
-            for (size_t m = K; m-- > 0;) {
-                unsigned __int128 cur = rem * pmods[m] + d[m];
-                md.half[m] = (u64)(cur / 2);
-                rem = cur % 2;
-            }
-        }
-        for (size_t i = 0; i < q_limbs; ++i) {
-            u64 prod = 1 % mods[i];
-            for (int m = 0; m < kMaxSpecial; ++m) {
-                consts.push_back(m < (int)K ? prod : 0);
-                if (m < (int)K) prod = host::mulmod(prod, pmods[m] % mods[i], mods[i]);
-            }
-        }
-        // P mod q_i, P^-1 mod q_i
-        for (size_t i = 0; i < q_limbs; ++i) {
-            u64 p = 1 % mods[i];
-            for (size_t m = 0; m < K; ++m) p = host::mulmod(p, pmods[m] % mods[i], mods[i]);
-            consts.push_back(p);
-        }
-        for (size_t i = 0; i < q_limbs; ++i) {
-            u64 p = 1 % mods[i];
-            for (size_t m = 0; m < K; ++m) p = host::mulmod(p, pmods[m] % mods[i], mods[i]);
-            u64 inv = host::invmod(p, mods[i]);
-            consts.push_back(inv);
-        }
-        for (size_t i = 0; i < q_limbs; ++i)
-            consts.push_back(host::shoup_quotient(consts[md_w_off + q_limbs * kMaxSpecial + q_limbs + i],
-                                                  mods[i]));
-    }
+    if (hybrid) build_moddown(mods, q_limbs, consts, md);
     CK_HIP(hipMalloc(&k->d_consts, std::max<size_t>(consts.size(), 1) * sizeof(u64)));
     CK_HIP(hipMemcpy(k->d_consts, consts.data(), consts.size() * sizeof(u64), hipMemcpyHostToDevice));
     for (size_t j = 0; j < n_blocks; ++j) k->h_modup[j].w = k->d_consts + j * B * kMaxAlpha;
@@ -1201,6 +1211,56 @@ int ckks_bounded_decompose(const ckks_rns_plan *plan, size_t q_limbs, unsigned b
     ba.base_log = base_log;
     ba.total = (u64)n_polys * N;
     return launch_bounded(ctx->stream, ba, h.words);
+}
+
+int ckks_mod_down(const ckks_rns_plan *qp_plan, size_t q_limbs, const uint64_t *in, uint64_t *out,
+                  size_t n_polys) {
+    if (!qp_plan || (n_polys && (!in || !out))) return fail(CKKS_ERR_INVALID, "null argument");
+    const size_t B = qp_plan->moduli.size();
+    if (q_limbs == 0 || q_limbs >= B) return fail(CKKS_ERR_INVALID, "mod-down needs Q limbs and special primes");
+    const size_t K = B - q_limbs;
+    if (K > (size_t)kMaxSpecial) return fail(CKKS_ERR_UNSUPPORTED, "at most 8 special primes");
+    if (n_polys == 0) return CKKS_OK;
+    ckks_context *ctx = qp_plan->ctx;
+    auto *mp = const_cast<ckks_rns_plan *>(qp_plan);
+    auto it = mp->moddown.find(q_limbs);
+    if (it == mp->moddown.end()) {
+        if (ctx->capturing) return fail(CKKS_ERR_INVALID, "mod-down tables cannot be built during graph capture");
+        std::vector<u64> consts;
+        ModDownTable md{};
+        build_moddown(qp_plan->moduli, q_limbs, consts, md);
+        void *dc = nullptr, *dm = nullptr;
+        CK_HIP(hipSetDevice(ctx->device));
+        CK_HIP(hipMalloc(&dc, consts.size() * sizeof(u64)));
+        CK_HIP(hipMemcpy(dc, consts.data(), consts.size() * sizeof(u64), hipMemcpyHostToDevice));
+        md.w = (const u64 *)dc;
+        md.pmod = md.w + q_limbs * kMaxSpecial;
+        md.pinv = md.pmod + q_limbs;
+        md.pinv_q = md.pinv + q_limbs;
+        CK_HIP(hipMalloc(&dm, sizeof(ModDownTable)));
+        CK_HIP(hipMemcpy(dm, &md, sizeof(ModDownTable), hipMemcpyHostToDevice));
+        it = mp->moddown.emplace(q_limbs, std::make_pair(dc, dm)).first;
+    }
+    const size_t N = qp_plan->degree;
+    DownArgs da{};
+    da.acc = in;
+    da.out = out;
+    da.out_stride = 2 * q_limbs * N;  // polynomials are consecutive [p][q_limbs][N]
+    da.md = (const ModDownTable *)it->second.second;
+    da.primes = qp_plan->d_primes;
+    da.limbs = (u32)B;
+    da.q_limbs = (u32)q_limbs;
+    da.logn = qp_plan->logn;
+    da.chunks = (u32)(N / kKsThreads);
+    const dim3 g((u32)(n_polys * da.chunks));
+    switch (K) {
+#define CK_MDS(KK) \
+    case KK: hipLaunchKernelGGL(k_moddown<KK>, g, dim3(kKsThreads), 0, ctx->stream, da); break;
+        CK_MDS(1) CK_MDS(2) CK_MDS(3) CK_MDS(4) CK_MDS(5) CK_MDS(6) CK_MDS(7) CK_MDS(8)
+#undef CK_MDS
+    }
+    CK_HIP(hipGetLastError());
+    return CKKS_OK;
 }
 
 int ckks_ksk_destroy(ckks_ksk *k) {

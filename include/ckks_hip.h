@@ -234,6 +234,14 @@ int ckks_decrypt_raw(const ckks_rns_plan *plan, const uint64_t *b, const uint64_
                      const int8_t *secret, uint64_t *out, size_t n_polys, size_t n_limbs,
                      size_t poly_stride, size_t out_stride);
 
+/* auxiliary_mod_down_rns_candidate (ring/auxiliary_modulus.rs:89-171):
+ * n_polys consecutive polynomials [p][qp limbs][N] over qp_plan (q_limbs Q
+ * limbs then the special primes P) to [p][q_limbs][N]:
+ * round(centered(x) / P) mod Q, centered by x_P > floor(P/2).  Tables are
+ * built on the first call per q_limbs (not capturable). */
+int ckks_mod_down(const ckks_rns_plan *qp_plan, size_t q_limbs, const uint64_t *in, uint64_t *out,
+                  size_t n_polys);
+
 /* encode_rns (application_support/ckks.rs:39-66), integer half: real
  * coefficients [p][N] (device doubles, from the canonical embedding) times
  * scale, rounded half away from zero (f64::round), rem_euclid every limb ->
