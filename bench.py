@@ -275,6 +275,8 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop measurement")
     ap.add_argument("--serve-requests", type=int, default=40)
     ap.add_argument("--serve-rate", type=float, default=4.0)
+    ap.add_argument("--split", choices=["auto", "off", "on"], default="auto",
+                    help="key switches split over two streams: autotuned per key, off, on")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default NTT variant and base-conversion engine")
     ap.add_argument("--base-conversion", choices=["auto", "valu", "mfma"], default="auto",
@@ -292,6 +294,8 @@ def main():
     dev = "cuda"
     torch.cuda.init()
     ctx = pkg.Context(0)
+    ctx.set_key_switch_split({"auto": ctx.KS_SPLIT_AUTO, "off": ctx.KS_SPLIT_OFF,
+                              "on": ctx.KS_SPLIT_ON}[args.split])
     ctx.set_key_switch_fusion({"auto": ctx.KS_FUSION_AUTO, "off": ctx.KS_FUSION_OFF,
                                "xcd": ctx.KS_FUSION_XCD, "linear": ctx.KS_FUSION_LINEAR}[args.fusion])
     ctx.set_base_conversion({"auto": ctx.BASECONV_AUTO, "valu": ctx.BASECONV_VALU,

@@ -59,6 +59,7 @@ _SIGNATURES = {
     "ckks_set_base_conversion": (C.c_int, [_p, C.c_int]),
     "ckks_set_ntt_variant": (C.c_int, [_p, C.c_uint32, C.c_int]),
     "ckks_set_key_switch_fusion": (C.c_int, [_p, C.c_int]),
+    "ckks_set_key_switch_split": (C.c_int, [_p, C.c_int]),
     "ckks_get_ntt_variant": (C.c_int, [_p, C.c_uint32, C.POINTER(C.c_int)]),
     "ckks_autotune_ntt": (C.c_int, [_p, _sz, _sz, C.POINTER(C.c_int), C.POINTER(C.c_float)]),
     "ckks_autotune_key_switch": (C.c_int, [_p, _sz, C.POINTER(C.c_int), C.POINTER(C.c_int),
@@ -250,6 +251,13 @@ class Context:
         (XCD or LINEAR block order), separate kernels (OFF), or the key's
         tuned choice (AUTO, the default)."""
         _check(lib().ckks_set_key_switch_fusion(self._h, int(mode)))
+
+    KS_SPLIT_AUTO, KS_SPLIT_OFF, KS_SPLIT_ON = -1, 0, 1
+
+    def set_key_switch_split(self, mode: int):
+        """Key switches of >= 8 ciphertexts split over two streams (ON), not
+        (OFF), or the key's tuned choice (AUTO, the default)."""
+        _check(lib().ckks_set_key_switch_split(self._h, int(mode)))
 
     def ntt_variant(self, logn: int) -> int:
         v = C.c_int()

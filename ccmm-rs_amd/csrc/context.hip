@@ -30,7 +30,8 @@ int check_plan_limbs(const ckks_rns_plan *plan, size_t n_limbs, const char *what
 
 using namespace ck;
 
-int ckks_context::workspace(int slot, size_t bytes, void **out) {
+int ckks_context::workspace(int slot_in_lane, size_t bytes, void **out) {
+    const int slot = slot_in_lane + kSlots * lane;
     if (bytes > ws_bytes[slot] && capturing)
         return fail(CKKS_ERR_INVALID,
                     "workspace must be sized before graph capture: run the schedule once eagerly first");
@@ -84,8 +85,11 @@ int ckks_context_destroy(ckks_context *ctx) {
         hipFree(kv.second.inv);
     }
     for (void *p : ctx->owned) hipFree(p);
-    for (int i = 0; i < ckks_context::kSlots; ++i)
+    for (int i = 0; i < 2 * ckks_context::kSlots; ++i)
         if (ctx->ws[i]) hipFree(ctx->ws[i]);
+    if (ctx->ev_fork) hipEventDestroy(ctx->ev_fork);
+    if (ctx->ev_join) hipEventDestroy(ctx->ev_join);
+    if (ctx->aux) hipStreamDestroy(ctx->aux);
     delete ctx;
     return CKKS_OK;
 }
@@ -178,6 +182,14 @@ int ckks_set_key_switch_fusion(ckks_context *ctx, int mode) {
     if (mode < CKKS_KS_FUSION_AUTO || mode > CKKS_KS_FUSION_LINEAR)
         return fail(CKKS_ERR_INVALID, "unknown key-switch fusion mode");
     ctx->ks_fusion = mode;
+    return CKKS_OK;
+}
+
+int ckks_set_key_switch_split(ckks_context *ctx, int mode) {
+    if (!ctx) return fail(CKKS_ERR_INVALID, "null context");
+    if (mode != CKKS_KS_SPLIT_AUTO && mode != CKKS_KS_SPLIT_OFF && mode != CKKS_KS_SPLIT_ON)
+        return fail(CKKS_ERR_INVALID, "unknown key-switch split mode");
+    ctx->ks_split = mode;
     return CKKS_OK;
 }
 

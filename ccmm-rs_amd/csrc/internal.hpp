@@ -74,9 +74,15 @@ struct ckks_context {
     std::map<std::pair<uint64_t, uint32_t>, ck::TwiddleTables> tables;
     // grow-only scratch slots: slot 0 is the NTT's inter-pass buffer, the
     // others belong to the composite operations (tensor, key switch).
+    // two lanes of slots: lane 1 belongs to the auxiliary stream of a key
+    // switch whose batch is split in two (see key_switch_core)
     static constexpr int kSlots = 8;
-    void *ws[kSlots] = {};
-    size_t ws_bytes[kSlots] = {};
+    void *ws[2 * kSlots] = {};
+    size_t ws_bytes[2 * kSlots] = {};
+    int lane = 0;
+    hipStream_t aux = nullptr;  // default priority, non-blocking
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    int ks_split = CKKS_KS_SPLIT_AUTO;
     // NTT inter-pass buffer per chunk in bytes (0 = whole batch in one chunk;
     // measured on MI355X: chunks sized for the Infinity Cache gave no gain)
     size_t ntt_chunk_bytes = 0;

@@ -74,6 +74,7 @@ struct ckks_ksk {
     ck::u64 *d_bounded = nullptr;  // table blob (BoundedTable points into it)
     int tuned_engine = CKKS_BASECONV_VALU;  // engine of CKKS_BASECONV_AUTO for this key
     int tuned_fusion = CKKS_KS_FUSION_OFF;  // mode of CKKS_KS_FUSION_AUTO for this key
+    int tuned_split = CKKS_KS_SPLIT_OFF;    // mode of CKKS_KS_SPLIT_AUTO for this key
     ck::BcPlan *d_bc = nullptr;
     signed char *d_bt = nullptr;
 };
@@ -168,21 +169,36 @@ __device__ void modup_thread(const ModUpArgs &a, const ModUpBlock &blk, u32 ct, 
     }
 }
 
-__global__ __launch_bounds__(kKsThreads) void k_modup(ModUpArgs a) {
+// One instantiation per block size A, so each gets its own register budget;
+// a launch covers the consecutive digits [j_first, j_first + n_j) of one size.
+template <int A>
+__global__ __launch_bounds__(kKsThreads) void k_modup(ModUpArgs a, u32 j_first, u32 n_j) {
     const u32 b = blockIdx.x;
     const u32 chunk = b % a.chunks;
     const u32 cj = b / a.chunks;
-    const u32 j = cj % a.dnum, ct = cj / a.dnum;
-    const ModUpBlock &blk = a.blocks[j];
+    const u32 j = j_first + cj % n_j, ct = cj / n_j;
     const u64 n = (u64)chunk * kKsThreads + threadIdx.x;
-    switch (blk.size) {
+    modup_thread<A>(a, a.blocks[j], ct, j, n);
+}
+
+static int launch_modup(hipStream_t s, const ModUpArgs &mu, const std::vector<size_t> &sizes, size_t n_cts) {
+    for (size_t j0 = 0; j0 < sizes.size();) {
+        size_t j1 = j0 + 1;
+        while (j1 < sizes.size() && sizes[j1] == sizes[j0]) ++j1;
+        const dim3 g((u32)(n_cts * (j1 - j0) * mu.chunks));
+        const u32 jf = (u32)j0, nj = (u32)(j1 - j0);
+        switch (sizes[j0]) {
 #define CK_MU(A) \
-    case A: modup_thread<A>(a, blk, ct, j, n); break;
-        CK_MU(1) CK_MU(2) CK_MU(3) CK_MU(4) CK_MU(5) CK_MU(6)
-        CK_MU(7) CK_MU(8) CK_MU(9) CK_MU(10) CK_MU(11) CK_MU(12)
+    case A: hipLaunchKernelGGL(k_modup<A>, g, dim3(kKsThreads), 0, s, mu, jf, nj); break;
+            CK_MU(1) CK_MU(2) CK_MU(3) CK_MU(4) CK_MU(5) CK_MU(6)
+            CK_MU(7) CK_MU(8) CK_MU(9) CK_MU(10) CK_MU(11) CK_MU(12)
 #undef CK_MU
-        default: break;
+            default: return fail(CKKS_ERR_UNSUPPORTED, "gadget blocks of at most 12 limbs are supported");
+        }
+        j0 = j1;
     }
+    CK_HIP(hipGetLastError());
+    return CKKS_OK;
 }
 
 // ---- 3. key MAC -------------------------------------------------------------------
@@ -1003,9 +1019,47 @@ static int create_bounded_key(const ckks_rns_plan *plan, size_t q_limbs, unsigne
 // ct at x + c*x_stride) and writes
 //   out[c][0] = add0 + KS_b(x),  out[c][1] = add1 + KS_a(x)
 // (add0/add1 may be null) over the key's Q limbs.
+static int key_switch_lane(const ckks_ksk *k, const u64 *x, u64 x_stride, const u64 *add0,
+                           const u64 *add1, u64 add_stride, u64 *out, u64 out_stride, size_t n_cts,
+                           u64 galois_kinv);
+
+// Runs the key switch on one lane, or splits the batch over the context
+// stream (first half) and the auxiliary stream (second half): fork with an
+// event recorded on the context stream, join by making the context stream
+// wait for the auxiliary one.  Each lane has its own workspaces.
 static int key_switch_core(const ckks_ksk *k, const u64 *x, u64 x_stride, const u64 *add0,
                            const u64 *add1, u64 add_stride, u64 *out, u64 out_stride, size_t n_cts,
                            u64 galois_kinv = 0) {
+    ckks_context *ctx = k->ctx;
+    const int split = ctx->ks_split == CKKS_KS_SPLIT_AUTO ? k->tuned_split : ctx->ks_split;
+    if (split != CKKS_KS_SPLIT_ON || n_cts < 8 || ctx->capturing || ctx->lane != 0)
+        return key_switch_lane(k, x, x_stride, add0, add1, add_stride, out, out_stride, n_cts, galois_kinv);
+    if (!ctx->aux) {
+        CK_HIP(hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking));
+        CK_HIP(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
+        CK_HIP(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
+    }
+    const size_t n1 = n_cts / 2, n2 = n_cts - n1;
+    CK_HIP(hipEventRecord(ctx->ev_fork, ctx->stream));
+    CK_HIP(hipStreamWaitEvent(ctx->aux, ctx->ev_fork, 0));
+    hipStream_t main = ctx->stream;
+    ctx->stream = ctx->aux;
+    ctx->lane = 1;
+    int rc = key_switch_lane(k, x + n1 * x_stride, x_stride, add0 ? add0 + n1 * add_stride : nullptr,
+                             add1 ? add1 + n1 * add_stride : nullptr, add_stride, out + n1 * out_stride,
+                             out_stride, n2, galois_kinv);
+    ctx->stream = main;
+    ctx->lane = 0;
+    if (rc) return rc;
+    rc = key_switch_lane(k, x, x_stride, add0, add1, add_stride, out, out_stride, n1, galois_kinv);
+    CK_HIP(hipEventRecord(ctx->ev_join, ctx->aux));
+    CK_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
+    return rc;
+}
+
+static int key_switch_lane(const ckks_ksk *k, const u64 *x, u64 x_stride, const u64 *add0,
+                           const u64 *add1, u64 add_stride, u64 *out, u64 out_stride, size_t n_cts,
+                           u64 galois_kinv) {
     ckks_context *ctx = k->ctx;
     const ckks_rns_plan *plan = k->plan;
     const u64 N = k->degree, B = k->limbs, D = k->dnum;
@@ -1050,7 +1104,7 @@ static int key_switch_core(const ckks_ksk *k, const u64 *x, u64 x_stride, const 
         ModUpMfmaArgs mm{mu, k->d_bc};
         hipLaunchKernelGGL(k_modup_mfma, dim3((u32)(n_cts * D * (N / kBcRows))), dim3(256), 0, s, mm);
     } else {
-        hipLaunchKernelGGL(k_modup, dim3((u32)(n_cts * D * chunks)), dim3(kKsThreads), 0, s, mu);
+        if ((rc = launch_modup(s, mu, k->block_size, n_cts))) return rc;
     }
     CK_HIP(hipGetLastError());
     // 2+3. forward NTT of every digit limb, key MAC
@@ -1331,9 +1385,34 @@ int ckks_autotune_key_switch(ckks_ksk *k, size_t n_cts, int *best_engine, int *b
         }
         return CKKS_OK;
     };
+    const int saved_split = ctx->ks_split;
+    ctx->ks_split = CKKS_KS_SPLIT_OFF;
     int rc = body();
+    // then the best pair with the batch split over two streams
+    float split_ms = -1.f;
+    if (!rc && n_cts >= 8) {
+        auto timed_split = [&]() -> int {
+            ctx->base_conversion = best_e;
+            ctx->ks_fusion = best_f;
+            ctx->ks_split = CKKS_KS_SPLIT_ON;
+            u64 *in = buf, *out = buf + n_cts * per;
+            int r = key_switch_core(k, in + poly, per, in, nullptr, per, out, per, n_cts);
+            if (r) return r;
+            CK_HIP(hipEventRecord(e0, ctx->stream));
+            for (int i = 0; i < 2; ++i)
+                if ((r = key_switch_core(k, in + poly, per, in, nullptr, per, out, per, n_cts))) return r;
+            CK_HIP(hipEventRecord(e1, ctx->stream));
+            CK_HIP(hipEventSynchronize(e1));
+            CK_HIP(hipEventElapsedTime(&split_ms, e0, e1));
+            split_ms /= 2;
+            return CKKS_OK;
+        };
+        rc = timed_split();
+    }
+    ctx->ks_split = saved_split;
     ctx->base_conversion = saved_mode;
     ctx->ks_fusion = saved_fusion;
+    if (!rc) k->tuned_split = (split_ms > 0 && split_ms < best_ms) ? CKKS_KS_SPLIT_ON : CKKS_KS_SPLIT_OFF;
     hipStreamSynchronize(ctx->stream);
     if (e0) hipEventDestroy(e0);
     if (e1) hipEventDestroy(e1);

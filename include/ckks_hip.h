@@ -99,6 +99,16 @@ enum {
 };
 int ckks_set_key_switch_fusion(ckks_context *ctx, int mode);
 
+/* Tuning: a key switch of at least 8 ciphertexts may run its two halves on
+ * the context stream and on an auxiliary stream of the context (default
+ * priority), joined before the call returns, so the VALU-bound base
+ * conversions of one half overlap the HBM-bound NTT and MAC of the other.
+ * CKKS_KS_SPLIT_AUTO (default): the choice ckks_autotune_key_switch made
+ * for the key, OFF for an untuned key.  Never during graph capture.
+ * Identical residues either way.  No reference counterpart. */
+enum { CKKS_KS_SPLIT_AUTO = -1, CKKS_KS_SPLIT_OFF = 0, CKKS_KS_SPLIT_ON = 1 };
+int ckks_set_key_switch_split(ckks_context *ctx, int mode);
+
 /* Tuning: engine of the exact ModUp/ModDown base conversion of hybrid keys.
  * CKKS_BASECONV_MFMA uses the int8 matrix cores (7-bit chunks, int32
  * accumulation) wherever every gadget block has 2..9 limbs;
@@ -339,7 +349,8 @@ int ckks_multiply_relinearize_rescale(const ckks_ksk *relin_key, const uint64_t 
  * base-conversion engine the key supports and each fusion mode (scratch
  * ciphertexts allocated and freed here; synchronises the context stream)
  * and records the fastest pair in the key for CKKS_BASECONV_AUTO and
- * CKKS_KS_FUSION_AUTO.  ms (optional, 2 x 3 floats) receives the times as
+ * CKKS_KS_FUSION_AUTO; then times that pair with the batch split over two
+ * streams and records the faster for CKKS_KS_SPLIT_AUTO.  ms (optional, 2 x 3 floats) receives the times as
  * ms[engine - 1][fusion mode], engine VALU = 1 / MFMA = 2, -1 where a pair
  * does not apply.  Not callable during graph capture.  No reference
  * counterpart. */

@@ -121,3 +121,26 @@ def test_fused_ntt_mac_matches_unfused(pkg, ctx, logn, variant):
         ctx.set_ntt_variant(logn, 0)
     for mode, g in got.items():
         assert (g == want).all(), f"fusion mode {mode}"
+
+
+@pytest.mark.parametrize("n_cts", [8, 9, 16])
+def test_split_key_switch_matches_single_stream(pkg, ctx, n_cts):
+    """Halves of the batch on two streams give the single-stream residues,
+    for rotations and for relinearization (pass-through terms offset too)."""
+    n, L, K, blocks = 1 << 13, 5, 2, [3, 2]
+    q_moduli = [Q0[0]] + Q40[:L - 1]
+    qp = q_moduli + P60[:K]
+    rng = np.random.default_rng(n_cts)
+    plan = pkg.PreparedRnsNttPlan(ctx, qp, n)
+    key = pkg.HybridKeySwitchKey(plan, L, blocks, rand_residues(rng, qp, n, 2), rand_residues(rng, qp, n, 2))
+    ct = pkg.to_device(rand_residues(rng, q_moduli, n, polys=2 * n_cts).reshape(n_cts, 2, L, n))
+    ct3 = pkg.to_device(rand_residues(rng, q_moduli, n, polys=3 * n_cts).reshape(n_cts, 3, L, n))
+    k = O.rotation_exponent_left(n, 7)
+    try:
+        ctx.set_key_switch_split(ctx.KS_SPLIT_OFF)
+        want = pkg.to_host(key.apply_galois(ct, k)), pkg.to_host(key.relinearize(ct3))
+        ctx.set_key_switch_split(ctx.KS_SPLIT_ON)
+        got = pkg.to_host(key.apply_galois(ct, k)), pkg.to_host(key.relinearize(ct3))
+    finally:
+        ctx.set_key_switch_split(ctx.KS_SPLIT_AUTO)
+    assert (got[0] == want[0]).all() and (got[1] == want[1]).all()
