@@ -117,52 +117,61 @@ CK_HD Gather galois_gather(u64 n, u64 kinv, u32 logn) {
     return g < N ? Gather{g, false} : Gather{g - N, true};
 }
 
+__device__ __forceinline__ Barrett barrett_at(const CK_CONST_AS DevPrime *p, u32 i) {
+    return Barrett{p[i].q, p[i].br_hi, p[i].br_lo};
+}
+
+// The block and prime tables are read through the scalar unit (const_table):
+// every address below except the residues' is uniform over the wave.
 template <int A>
-__device__ void modup_thread(const ModUpArgs &a, const ModUpBlock &blk, u32 ct, u32 j, u64 n) {
+__device__ void modup_thread(const ModUpArgs &a, const CK_CONST_AS ModUpBlock *blk, u32 ct, u32 j, u64 n) {
     const u64 N = (u64)1 << a.logn;
+    const CK_CONST_AS DevPrime *primes = const_table(a.primes);
+    const u32 start = blk->start;
     const Gather gi = galois_gather(n, a.kinv, a.logn);
-    const u64 *src = a.in + ct * a.in_stride + (u64)blk.start * N + gi.idx;
+    const u64 *src = a.in + ct * a.in_stride + (u64)start * N + gi.idx;
     u64 x[A], c[A];
 #pragma unroll
     for (int m = 0; m < A; ++m) {
         u64 v = src[(u64)m * N];
-        x[m] = gi.neg ? neg_mod(v, a.primes[blk.start + m].q) : v;
+        x[m] = gi.neg ? neg_mod(v, primes[start + m].q) : v;
     }
     // mixed-radix digits
 #pragma unroll
     for (int k = 0; k < A; ++k) {
-        const DevPrime &Pk = a.primes[blk.start + k];
         if (k == 0) {
             c[0] = x[0];
         } else {
+            const Barrett bk = barrett_at(primes, start + k);
             Acc128 t;
             t.zero();
 #pragma unroll
-            for (int m = 0; m < k; ++m) t.mac(c[m], blk.pm[k][m]);
-            u64 tr = reduce128(t, Pk);
-            c[k] = shoup(sub_mod(x[k], tr, Pk.q), blk.inv[k], blk.inv_q[k], Pk.q);
+            for (int m = 0; m < k; ++m) t.mac(c[m], blk->pm[k][m]);
+            u64 tr = barrett_reduce128(t.hi, t.lo, bk);
+            c[k] = shoup(sub_mod(x[k], tr, bk.q), blk->inv[k], blk->inv_q[k], bk.q);
         }
     }
+    const CK_CONST_AS u64 *wt = const_table(blk->w);
     u64 *dst = a.digits + (((u64)ct * a.dnum + j) * a.limbs) * N + n;
     for (u32 i = 0; i < a.limbs; ++i) {
         u64 v;
-        if (i >= blk.start && i < blk.start + A) {
+        if (i >= start && i < start + A) {
             // inside the block the canonical digit reduces to the residue
             v = x[0];
 #pragma unroll
             for (int m = 1; m < A; ++m)
-                if (i == blk.start + m) v = x[m];
+                if (i == start + m) v = x[m];
         } else {
-            const DevPrime &Pi = a.primes[i];
-            const u64 *w = blk.w + (u64)i * kMaxAlpha;
+            const Barrett bi = barrett_at(primes, i);
+            const CK_CONST_AS u64 *w = wt + (u64)i * kMaxAlpha;
             if (A == 1) {
-                v = reduce64(c[0], Pi);
+                v = c[0] >= bi.q ? barrett_reduce128(0, c[0], bi) : c[0];
             } else {
                 Acc128 t;
                 t.zero();
 #pragma unroll
                 for (int m = 0; m < A; ++m) t.mac(c[m], w[m]);
-                v = reduce128(t, Pi);
+                v = barrett_reduce128(t.hi, t.lo, bi);
             }
         }
         dst[(u64)i * N] = v;
@@ -178,7 +187,7 @@ __global__ __launch_bounds__(kKsThreads) void k_modup(ModUpArgs a, u32 j_first, 
     const u32 cj = b / a.chunks;
     const u32 j = j_first + cj % n_j, ct = cj / n_j;
     const u64 n = (u64)chunk * kKsThreads + threadIdx.x;
-    modup_thread<A>(a, a.blocks[j], ct, j, n);
+    modup_thread<A>(a, const_table(a.blocks) + j, ct, j, n);
 }
 
 static int launch_modup(hipStream_t s, const ModUpArgs &mu, const std::vector<size_t> &sizes, size_t n_cts) {
@@ -378,6 +387,7 @@ __global__ __launch_bounds__(kKsThreads) void k_moddown(DownArgs a) {
     const u32 comp = rest % 2, ct = rest / 2;
     const u64 N = (u64)1 << a.logn;
     const u64 n = (u64)chunk * kKsThreads + threadIdx.x;
+    const CK_CONST_AS DevPrime *primes = const_table(a.primes);
     const u64 *x = a.acc + ((u64)ct * 2 + comp) * a.limbs * N + n;
     const u64 *add = comp == 0 ? a.add0 : a.add1;
     const Gather ga = galois_gather(n, comp == 0 ? a.kinv : 0, a.logn);
@@ -387,14 +397,16 @@ __global__ __launch_bounds__(kKsThreads) void k_moddown(DownArgs a) {
         for (u32 i = 0; i < a.q_limbs; ++i) {
             u64 v = x[(u64)i * N];
             if (add) {
+                const u64 q = primes[i].q;
                 u64 ad = add[(u64)i * N];
-                v = ga.neg ? sub_mod(v, ad, a.primes[i].q) : add_mod(v, ad, a.primes[i].q);
+                v = ga.neg ? sub_mod(v, ad, q) : add_mod(v, ad, q);
             }
             o[(u64)i * N] = v;
         }
         return;
     }
-    const ModDownTable &md = *a.md;
+    // tables through the scalar unit (const_table): uniform addresses only
+    const CK_CONST_AS ModDownTable *md = const_table(a.md);
     u64 c[K > 0 ? K : 1];
     bool neg = false;
     if (K > 0) {
@@ -403,37 +415,42 @@ __global__ __launch_bounds__(kKsThreads) void k_moddown(DownArgs a) {
         for (int m = 0; m < K; ++m) xp[m] = x[(u64)(a.q_limbs + m) * N];
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            const DevPrime &Pk = a.primes[a.q_limbs + k];
             if (k == 0) {
                 c[0] = xp[0];
             } else {
+                const Barrett bk = barrett_at(primes, a.q_limbs + k);
                 Acc128 t;
                 t.zero();
 #pragma unroll
-                for (int m = 0; m < k; ++m) t.mac(c[m], md.pm[k][m]);
-                c[k] = shoup(sub_mod(xp[k], reduce128(t, Pk), Pk.q), md.inv[k], md.inv_q[k], Pk.q);
+                for (int m = 0; m < k; ++m) t.mac(c[m], md->pm[k][m]);
+                c[k] = shoup(sub_mod(xp[k], barrett_reduce128(t.hi, t.lo, bk), bk.q), md->inv[k],
+                             md->inv_q[k], bk.q);
             }
         }
         // x_P > floor(P/2)  <=>  mixed-radix digits compare greater from the top
         int cmp = 0;
 #pragma unroll
         for (int k = K - 1; k >= 0; --k)
-            if (cmp == 0 && c[k] != md.half[k]) cmp = c[k] > md.half[k] ? 1 : -1;
+            if (cmp == 0 && c[k] != md->half[k]) cmp = c[k] > md->half[k] ? 1 : -1;
         neg = cmp > 0;
     }
+    const CK_CONST_AS u64 *wt = const_table(md->w);
+    const CK_CONST_AS u64 *pmod = const_table(md->pmod);
+    const CK_CONST_AS u64 *pinv = const_table(md->pinv);
+    const CK_CONST_AS u64 *pinv_q = const_table(md->pinv_q);
     for (u32 i = 0; i < a.q_limbs; ++i) {
-        const DevPrime &Pi = a.primes[i];
-        const u64 *w = md.w + (u64)i * kMaxSpecial;
+        const Barrett bi = barrett_at(primes, i);
+        const CK_CONST_AS u64 *w = wt + (u64)i * kMaxSpecial;
         Acc128 t;
         t.zero();
 #pragma unroll
         for (int m = 0; m < K; ++m) t.mac(c[m], w[m]);
-        u64 r = reduce128(t, Pi);  // x_P mod q_i
-        if (neg) r = sub_mod(r, md.pmod[i], Pi.q);  // (x_P - P) mod q_i
-        u64 v = shoup(sub_mod(x[(u64)i * N], r, Pi.q), md.pinv[i], md.pinv_q[i], Pi.q);
+        u64 r = barrett_reduce128(t.hi, t.lo, bi);  // x_P mod q_i
+        if (neg) r = sub_mod(r, pmod[i], bi.q);  // (x_P - P) mod q_i
+        u64 v = shoup(sub_mod(x[(u64)i * N], r, bi.q), pinv[i], pinv_q[i], bi.q);
         if (add) {
             u64 ad = add[(u64)i * N];
-            v = ga.neg ? sub_mod(v, ad, Pi.q) : add_mod(v, ad, Pi.q);
+            v = ga.neg ? sub_mod(v, ad, bi.q) : add_mod(v, ad, bi.q);
         }
         o[(u64)i * N] = v;
     }

@@ -156,4 +156,16 @@ struct Acc128 {
     }
 };
 
+#if defined(__HIPCC__)
+// Read-only kernel tables through the scalar unit: a wave-uniform address in
+// the constant address space (4) compiles to s_load, so a table entry costs
+// neither a vector-memory round trip nor a VGPR per lane.  Only for memory
+// that no kernel writes (the scalar cache is not coherent with vector stores).
+#define CK_CONST_AS __attribute__((address_space(4)))
+template <class T>
+__device__ __forceinline__ const CK_CONST_AS T *const_table(const T *p) {
+    return (const CK_CONST_AS T *)p;
+}
+#endif
+
 }  // namespace ck
