@@ -27,6 +27,17 @@ struct DevPrime {
     u32 fp;  // 1: q < 2^41, the NTT runs on the exact FP64 path
 };
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  Renumbering so that consecutive logical blocks land on one XCD keeps
+// the blocks that share source data (the chunks of one ciphertext, whose
+// Galois gathers touch the same lines) behind one L2.
+constexpr u32 kXcds = 8;
+
+__device__ __forceinline__ u32 xcd_contiguous_block(u32 b, u32 nb) {
+    const u32 full = nb - nb % kXcds;
+    return b < full ? (b % kXcds) * (full / kXcds) + b / kXcds : b;
+}
+
 // Constants of the exact rescale by a dropped limb l into limb i < l
 // (ckks/rns_rescale.rs:107-167).
 struct RescaleConst {

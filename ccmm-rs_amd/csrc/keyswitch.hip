@@ -40,6 +40,16 @@ struct ModUpBlock {
     // evaluation of the mixed-radix value at every basis limb i:
     // w[i * kMaxAlpha + m] = (q_start ... q_{start+m-1}) mod q_i
     const u64 *w;
+    // FP64 targets (q_i < 2^41): wf[(i * kMaxAlpha + m) * 4 + {0,1,2,3}] =
+    // w, fl(w / q_i), w 2^30 mod q_i, fl(that / q_i) as doubles; bit m of
+    // `wide` marks a source prime >= 2^41, whose digit is split at bit 30
+    const double *wf;
+    u32 wide;
+    // FP64 Garner for block primes q_k < 2^41: pmf[k][m] = pm[k][m], its
+    // quotient, pm[k][m] 2^30 mod q_k and its quotient (as in wf);
+    // invf[k] = inv[k] and fl(inv[k] / q_k)
+    double pmf[kMaxAlpha][kMaxAlpha][4];
+    double invf[kMaxAlpha][2];
 };
 
 struct ModDownTable {
@@ -51,6 +61,10 @@ struct ModDownTable {
     // pmod[i] = P mod q_i, pinv[i] = P^-1 mod q_i (Shoup pair)
     const u64 *w;
     const u64 *pmod, *pinv, *pinv_q;
+    // FP64 targets (q_i < 2^41): wf[(i * kMaxSpecial + m) * 4 + {0..3}] as
+    // in ModUpBlock (every P digit split at bit 30); pf[i * 4 + {0,1,2}] =
+    // P mod q_i, P^-1 mod q_i, fl(P^-1 / q_i)
+    const double *wf, *pf;
 };
 
 }  // namespace ck
@@ -123,54 +137,93 @@ __device__ __forceinline__ Barrett barrett_at(const CK_CONST_AS DevPrime *p, u32
 
 // The block and prime tables are read through the scalar unit (const_table):
 // every address below except the residues' is uniform over the wave.
+//
+// Targets q_i < 2^41 take the FP64 path: sum_m fp_mulmod(c_m, w_mi) with
+// |each term| <= 0.51 q_i, so the sum of at most 2 * kMaxAlpha terms stays
+// below 2^45 and fp_residue makes it canonical.  A digit of a source prime
+// >= 2^41 (bit m of blk->wide) enters as c mod 2^30 with w and c >> 30 with
+// w 2^30 mod q_i, both below 2^45 as fp_mulmod requires.  The Garner digits
+// of FP64 block primes are computed the same way; every other target or
+// digit takes the 128-bit accumulate and Barrett reduction.
 template <int A>
 __device__ void modup_thread(const ModUpArgs &a, const CK_CONST_AS ModUpBlock *blk, u32 ct, u32 j, u64 n) {
     const u64 N = (u64)1 << a.logn;
     const CK_CONST_AS DevPrime *primes = const_table(a.primes);
     const u32 start = blk->start;
+    const u32 wide = blk->wide;
     const Gather gi = galois_gather(n, a.kinv, a.logn);
     const u64 *src = a.in + ct * a.in_stride + (u64)start * N + gi.idx;
-    u64 x[A], c[A];
+    u64 *dst = a.digits + (((u64)ct * a.dnum + j) * a.limbs) * N + n;
+    u64 x[A];
 #pragma unroll
     for (int m = 0; m < A; ++m) {
         u64 v = src[(u64)m * N];
         x[m] = gi.neg ? neg_mod(v, primes[start + m].q) : v;
+        // inside the block the canonical digit reduces to the residue
+        dst[(u64)(start + m) * N] = x[m];
     }
-    // mixed-radix digits
+    // mixed-radix digits c_m = ch[m] 2^30 + cl[m] (wide) or cl[m]
+    double cl[A], ch[A];
+    auto digit = [&](int m) -> u64 {
+        return (wide >> m) & 1 ? (f64_to_small(ch[m]) << 30) | f64_to_small(cl[m]) : f64_to_small(cl[m]);
+    };
 #pragma unroll
     for (int k = 0; k < A; ++k) {
+        u64 ck;
         if (k == 0) {
-            c[0] = x[0];
+            ck = x[0];
+        } else if (primes[start + k].fp) {
+            const double q = primes[start + k].qd;
+            double t = small_to_f64(x[k]);
+#pragma unroll
+            for (int m = 0; m < k; ++m) {
+                t -= fp_mulmod(cl[m], blk->pmf[k][m][0], blk->pmf[k][m][1], q);
+                if ((wide >> m) & 1) t -= fp_mulmod(ch[m], blk->pmf[k][m][2], blk->pmf[k][m][3], q);
+            }
+            ck = fp_residue(fp_mulmod(t, blk->invf[k][0], blk->invf[k][1], q), q, primes[start + k].qinv);
         } else {
             const Barrett bk = barrett_at(primes, start + k);
             Acc128 t;
             t.zero();
 #pragma unroll
-            for (int m = 0; m < k; ++m) t.mac(c[m], blk->pm[k][m]);
+            for (int m = 0; m < k; ++m) t.mac(digit(m), blk->pm[k][m]);
             u64 tr = barrett_reduce128(t.hi, t.lo, bk);
-            c[k] = shoup(sub_mod(x[k], tr, bk.q), blk->inv[k], blk->inv_q[k], bk.q);
+            ck = shoup(sub_mod(x[k], tr, bk.q), blk->inv[k], blk->inv_q[k], bk.q);
+        }
+        if ((wide >> k) & 1) {
+            cl[k] = small_to_f64(ck & 0x3fffffffull);
+            ch[k] = small_to_f64(ck >> 30);
+        } else {
+            cl[k] = small_to_f64(ck);
+            ch[k] = 0.0;
         }
     }
     const CK_CONST_AS u64 *wt = const_table(blk->w);
-    u64 *dst = a.digits + (((u64)ct * a.dnum + j) * a.limbs) * N + n;
+    const CK_CONST_AS double *wft = const_table(blk->wf);
     for (u32 i = 0; i < a.limbs; ++i) {
+        if (i - start < (u32)A) continue;  // block limbs are stored above
         u64 v;
-        if (i >= start && i < start + A) {
-            // inside the block the canonical digit reduces to the residue
-            v = x[0];
+        if (primes[i].fp) {
+            const double q = primes[i].qd;
+            const CK_CONST_AS double *w = wft + (u64)i * kMaxAlpha * 4;
+            double acc = fp_mulmod(cl[0], w[0], w[1], q);
 #pragma unroll
-            for (int m = 1; m < A; ++m)
-                if (i == start + m) v = x[m];
+            for (int m = 1; m < A; ++m) acc += fp_mulmod(cl[m], w[4 * m], w[4 * m + 1], q);
+#pragma unroll
+            for (int m = 0; m < A; ++m)
+                if ((wide >> m) & 1) acc += fp_mulmod(ch[m], w[4 * m + 2], w[4 * m + 3], q);
+            v = fp_residue(acc, q, primes[i].qinv);
         } else {
             const Barrett bi = barrett_at(primes, i);
             const CK_CONST_AS u64 *w = wt + (u64)i * kMaxAlpha;
             if (A == 1) {
-                v = c[0] >= bi.q ? barrett_reduce128(0, c[0], bi) : c[0];
+                const u64 c0 = digit(0);
+                v = c0 >= bi.q ? barrett_reduce128(0, c0, bi) : c0;
             } else {
                 Acc128 t;
                 t.zero();
 #pragma unroll
-                for (int m = 0; m < A; ++m) t.mac(c[m], w[m]);
+                for (int m = 0; m < A; ++m) t.mac(digit(m), w[m]);
                 v = barrett_reduce128(t.hi, t.lo, bi);
             }
         }
@@ -182,7 +235,7 @@ __device__ void modup_thread(const ModUpArgs &a, const CK_CONST_AS ModUpBlock *b
 // a launch covers the consecutive digits [j_first, j_first + n_j) of one size.
 template <int A>
 __global__ __launch_bounds__(kKsThreads) void k_modup(ModUpArgs a, u32 j_first, u32 n_j) {
-    const u32 b = blockIdx.x;
+    const u32 b = xcd_contiguous_block(blockIdx.x, gridDim.x);
     const u32 chunk = b % a.chunks;
     const u32 cj = b / a.chunks;
     const u32 j = j_first + cj % n_j, ct = cj / n_j;
@@ -381,7 +434,7 @@ struct DownArgs {
 
 template <int K>
 __global__ __launch_bounds__(kKsThreads) void k_moddown(DownArgs a) {
-    const u32 b = blockIdx.x;
+    const u32 b = xcd_contiguous_block(blockIdx.x, gridDim.x);
     const u32 chunk = b % a.chunks;
     const u32 rest = b / a.chunks;
     const u32 comp = rest % 2, ct = rest / 2;
@@ -434,11 +487,40 @@ __global__ __launch_bounds__(kKsThreads) void k_moddown(DownArgs a) {
             if (cmp == 0 && c[k] != md->half[k]) cmp = c[k] > md->half[k] ? 1 : -1;
         neg = cmp > 0;
     }
+    // FP64 targets (q_i < 2^41): every P digit enters split at bit 30, as in
+    // ModUp; the terms sum to |acc| <= 2K 0.51 q_i, so t = x_i - acc (+ P)
+    // stays below 2^45 for the final fp_mulmod by P^-1.
+    double cl[K > 0 ? K : 1], ch[K > 0 ? K : 1];
+#pragma unroll
+    for (int m = 0; m < K; ++m) {
+        cl[m] = small_to_f64(c[m] & 0x3fffffffull);
+        ch[m] = small_to_f64(c[m] >> 30);
+    }
+    const CK_CONST_AS double *wft = const_table(md->wf);
+    const CK_CONST_AS double *pft = const_table(md->pf);
     const CK_CONST_AS u64 *wt = const_table(md->w);
     const CK_CONST_AS u64 *pmod = const_table(md->pmod);
     const CK_CONST_AS u64 *pinv = const_table(md->pinv);
     const CK_CONST_AS u64 *pinv_q = const_table(md->pinv_q);
     for (u32 i = 0; i < a.q_limbs; ++i) {
+        if (primes[i].fp) {
+            const double q = primes[i].qd;
+            const CK_CONST_AS double *w = wft + (u64)i * kMaxSpecial * 4;
+            double acc = fp_mulmod(cl[0], w[0], w[1], q) + fp_mulmod(ch[0], w[2], w[3], q);
+#pragma unroll
+            for (int m = 1; m < K; ++m)
+                acc += fp_mulmod(cl[m], w[4 * m], w[4 * m + 1], q) + fp_mulmod(ch[m], w[4 * m + 2], w[4 * m + 3], q);
+            const CK_CONST_AS double *pf = pft + (u64)i * 4;
+            double t = small_to_f64(x[(u64)i * N]) - acc;  // x_i - (x_P mod q_i)
+            if (neg) t += pf[0];                            // x_P - P
+            double v = fp_mulmod(t, pf[1], pf[2], q);
+            if (add) {
+                const double ad = small_to_f64(add[(u64)i * N]);
+                v = ga.neg ? v - ad : v + ad;
+            }
+            o[(u64)i * N] = fp_residue(v, q, primes[i].qinv);
+            continue;
+        }
         const Barrett bi = barrett_at(primes, i);
         const CK_CONST_AS u64 *w = wt + (u64)i * kMaxSpecial;
         Acc128 t;
@@ -751,6 +833,26 @@ static void fill_garner(const std::vector<u64> &mods, u64 (*pm)[kMaxAlpha], u64 
     }
 }
 
+static u64 double_bits(double d) {
+    u64 v;
+    std::memcpy(&v, &d, 8);
+    return v;
+}
+
+// FP64 weights of one w (mod q) for the FP64 target path: w, fl(w / q),
+// w 2^30 mod q, fl(w 2^30 mod q / q); zeros for q >= 2^41 (integer path).
+static void push_fp_weight(std::vector<u64> &consts, u64 w, u64 q) {
+    if (q >= host::kFpModulusBound) {
+        consts.insert(consts.end(), 4, 0);
+        return;
+    }
+    const u64 w2 = host::mulmod(w, (1ull << 30) % q, q);
+    consts.push_back(double_bits((double)w));
+    consts.push_back(double_bits((double)w / (double)q));
+    consts.push_back(double_bits((double)w2));
+    consts.push_back(double_bits((double)w2 / (double)q));
+}
+
 // ModDown constants of Q u P (the last mods.size() - q_limbs limbs are P):
 // Garner over P into md; per Q limb the mixed-radix weights, P mod q_i,
 // P^-1 mod q_i and its Shoup quotient appended to consts (the table's
@@ -807,6 +909,29 @@ static void build_moddown(const std::vector<u64> &mods, size_t q_limbs, std::vec
     for (size_t i = 0; i < q_limbs; ++i)
         consts.push_back(host::shoup_quotient(consts[md_w_off + q_limbs * kMaxSpecial + q_limbs + i],
                                               mods[i]));
+    // FP64 tables: wf[i][m][4], then pf[i][4]
+    for (size_t i = 0; i < q_limbs; ++i)
+        for (int m = 0; m < kMaxSpecial; ++m) push_fp_weight(consts, consts[md_w_off + i * kMaxSpecial + m], mods[i]);
+    for (size_t i = 0; i < q_limbs; ++i) {
+        const u64 pmod = consts[md_w_off + q_limbs * kMaxSpecial + i];
+        const u64 pinv = consts[md_w_off + q_limbs * kMaxSpecial + q_limbs + i];
+        const bool fp = mods[i] < host::kFpModulusBound;
+        consts.push_back(fp ? double_bits((double)pmod) : 0);
+        consts.push_back(fp ? double_bits((double)pinv) : 0);
+        consts.push_back(fp ? double_bits((double)pinv / (double)mods[i]) : 0);
+        consts.push_back(0);
+    }
+}
+
+// Device pointers of a ModDown table whose constants start at w (the
+// layout build_moddown appends).
+static void patch_moddown(ModDownTable &md, const u64 *w, size_t q_limbs) {
+    md.w = w;
+    md.pmod = md.w + q_limbs * kMaxSpecial;
+    md.pinv = md.pmod + q_limbs;
+    md.pinv_q = md.pinv + q_limbs;
+    md.wf = (const double *)(md.pinv_q + q_limbs);
+    md.pf = md.wf + q_limbs * kMaxSpecial * 4;
 }
 
 static int create_key(const ckks_rns_plan *plan, size_t q_limbs, const size_t *block_sizes,
@@ -850,6 +975,19 @@ static int create_key(const ckks_rns_plan *plan, size_t q_limbs, const size_t *b
         mb.size = (u32)block_sizes[j];
         std::vector<u64> bm(mods.begin() + start, mods.begin() + start + block_sizes[j]);
         fill_garner(bm, mb.pm, mb.inv, mb.inv_q, 0);
+        for (size_t kk = 1; kk < bm.size(); ++kk) {
+            const u64 qk = bm[kk];
+            if (qk >= host::kFpModulusBound) continue;
+            for (size_t m = 0; m < kk; ++m) {
+                const u64 w2 = host::mulmod(mb.pm[kk][m], (1ull << 30) % qk, qk);
+                mb.pmf[kk][m][0] = (double)mb.pm[kk][m];
+                mb.pmf[kk][m][1] = (double)mb.pm[kk][m] / (double)qk;
+                mb.pmf[kk][m][2] = (double)w2;
+                mb.pmf[kk][m][3] = (double)w2 / (double)qk;
+            }
+            mb.invf[kk][0] = (double)mb.inv[kk];
+            mb.invf[kk][1] = (double)mb.inv[kk] / (double)qk;
+        }
         mb.w = nullptr;  // patched after upload
         k->h_modup.push_back(mb);
         k->block_start.push_back(start);
@@ -866,16 +1004,25 @@ static int create_key(const ckks_rns_plan *plan, size_t q_limbs, const size_t *b
     size_t md_w_off = consts.size();
     ModDownTable md{};
     if (hybrid) build_moddown(mods, q_limbs, consts, md);
+    // FP64 ModUp weights of every block: wf[i][m][4]
+    const size_t wf_off = consts.size();
+    for (size_t j = 0; j < n_blocks; ++j) {
+        for (size_t m = 0; m < block_sizes[j]; ++m)
+            if (mods[k->block_start[j] + m] >= host::kFpModulusBound) k->h_modup[j].wide |= 1u << m;
+        for (size_t i = 0; i < B; ++i)
+            for (int m = 0; m < kMaxAlpha; ++m)
+                push_fp_weight(consts, consts[(j * B + i) * kMaxAlpha + m], mods[i]);
+    }
     CK_HIP(hipMalloc(&k->d_consts, std::max<size_t>(consts.size(), 1) * sizeof(u64)));
     CK_HIP(hipMemcpy(k->d_consts, consts.data(), consts.size() * sizeof(u64), hipMemcpyHostToDevice));
-    for (size_t j = 0; j < n_blocks; ++j) k->h_modup[j].w = k->d_consts + j * B * kMaxAlpha;
+    for (size_t j = 0; j < n_blocks; ++j) {
+        k->h_modup[j].w = k->d_consts + j * B * kMaxAlpha;
+        k->h_modup[j].wf = (const double *)(k->d_consts + wf_off) + j * B * kMaxAlpha * 4;
+    }
     CK_HIP(hipMalloc(&k->d_modup, n_blocks * sizeof(ModUpBlock)));
     CK_HIP(hipMemcpy(k->d_modup, k->h_modup.data(), n_blocks * sizeof(ModUpBlock), hipMemcpyHostToDevice));
     if (hybrid) {
-        md.w = k->d_consts + md_w_off;
-        md.pmod = md.w + q_limbs * kMaxSpecial;
-        md.pinv = md.pmod + q_limbs;
-        md.pinv_q = md.pinv + q_limbs;
+        patch_moddown(md, k->d_consts + md_w_off, q_limbs);
         CK_HIP(hipMalloc(&k->d_moddown, sizeof(ModDownTable)));
         CK_HIP(hipMemcpy(k->d_moddown, &md, sizeof(ModDownTable), hipMemcpyHostToDevice));
     }
@@ -1304,10 +1451,7 @@ int ckks_mod_down(const ckks_rns_plan *qp_plan, size_t q_limbs, const uint64_t *
         CK_HIP(hipSetDevice(ctx->device));
         CK_HIP(hipMalloc(&dc, consts.size() * sizeof(u64)));
         CK_HIP(hipMemcpy(dc, consts.data(), consts.size() * sizeof(u64), hipMemcpyHostToDevice));
-        md.w = (const u64 *)dc;
-        md.pmod = md.w + q_limbs * kMaxSpecial;
-        md.pinv = md.pmod + q_limbs;
-        md.pinv_q = md.pinv + q_limbs;
+        patch_moddown(md, (const u64 *)dc, q_limbs);
         CK_HIP(hipMalloc(&dm, sizeof(ModDownTable)));
         CK_HIP(hipMemcpy(dm, &md, sizeof(ModDownTable), hipMemcpyHostToDevice));
         it = mp->moddown.emplace(q_limbs, std::make_pair(dc, dm)).first;

@@ -140,6 +140,19 @@ CK_HD u64 f64_as_u64(double d) {
     return v;
 }
 
+// Exact conversions of integers in [0, 2^52) through the 2^52 exponent bias:
+// the double 2^52 + v carries v in its mantissa bits.
+CK_HD double small_to_f64(u64 v) { return u64_as_f64(v | 0x4330000000000000ull) - 4503599627370496.0; }
+CK_HD u64 f64_to_small(double d) { return f64_as_u64(d + 4503599627370496.0) - 0x4330000000000000ull; }
+
+// fp_canonical for results that stay below 2^52 (every q < 2^41 here).
+CK_HD u64 fp_residue(double x, double q, double qinv) {
+    double r = fp_reduce(x, q, qinv);
+    r = r < 0.0 ? r + q : r;
+    r = r >= q ? r - q : r;
+    return f64_to_small(r);
+}
+
 // 128-bit accumulator for sums of products before one reduction.
 struct Acc128 {
     u64 hi, lo;

@@ -244,13 +244,6 @@ struct RowMacArgs {
     u32 xcd_order;  // 1: remap blocks so each XCD runs a contiguous range
 };
 
-constexpr u32 kXcds = 8;
-
-__device__ __forceinline__ u32 xcd_contiguous_block(u32 b, u32 nb) {
-    const u32 full = nb - nb % kXcds;
-    return b < full ? (b % kXcds) * (full / kXcds) + b / kXcds : b;
-}
-
 // Reloads the thread id as a value the compiler cannot see through, so index
 // math derived from it is recomputed where it is used.
 __device__ __forceinline__ void opaque_tid(BlockExec &ex) {
