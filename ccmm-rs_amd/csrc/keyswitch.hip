@@ -30,6 +30,8 @@ namespace ck {
 constexpr int kMaxAlpha = 12;   // limbs per gadget block handled by ModUp
 constexpr int kMaxSpecial = 8;  // special primes handled by ModDown
 constexpr int kKsThreads = 256;
+constexpr int kFpRow = 4 + 2 * kMaxAlpha;      // doubles per FP64 ModUp row
+constexpr int kMdRow = 6 + 4 * kMaxSpecial;    // doubles per FP64 ModDown row
 
 // Constants of the exact lift of one gadget block.
 struct ModUpBlock {
@@ -40,16 +42,16 @@ struct ModUpBlock {
     // evaluation of the mixed-radix value at every basis limb i:
     // w[i * kMaxAlpha + m] = (q_start ... q_{start+m-1}) mod q_i
     const u64 *w;
-    // FP64 targets (q_i < 2^41): wf[(i * kMaxAlpha + m) * 4 + {0,1,2,3}] =
-    // w, fl(w / q_i), w 2^30 mod q_i, fl(that / q_i) as doubles; bit m of
-    // `wide` marks a source prime >= 2^41, whose digit is split at bit 30
-    const double *wf;
-    u32 wide;
-    // FP64 Garner for block primes q_k < 2^41: pmf[k][m] = pm[k][m], its
-    // quotient, pm[k][m] 2^30 mod q_k and its quotient (as in wf);
-    // invf[k] = inv[k] and fl(inv[k] / q_k)
-    double pmf[kMaxAlpha][kMaxAlpha][4];
-    double invf[kMaxAlpha][2];
+    // FP64 mode (fp_mode = 1: every block prime after the first is below
+    // 2^41).  Digit 0 enters split at bit 30 (c_0 = ch 2^30 + cl, both below
+    // 2^45 as fp_mulmod requires); every other digit is below 2^41.
+    // Garner row gf[k] (k >= 1, q_k < 2^41): q_k, 1/q_k, inv[k], fl(inv[k] / q_k),
+    // pm[k][0] 2^30 mod q_k, its quotient, then (pm[k][m], fl(pm[k][m] / q_k))
+    // for m < k.  Target row tf + i * kFpRow (q_i < 2^41): q_i, 1/q_i,
+    // w_i0 2^30 mod q_i, its quotient, then (w_im, fl(w_im / q_i)) for m < size.
+    u32 fp_mode;
+    double gf[kMaxAlpha][kFpRow];
+    const double *tf;
 };
 
 struct ModDownTable {
@@ -61,10 +63,10 @@ struct ModDownTable {
     // pmod[i] = P mod q_i, pinv[i] = P^-1 mod q_i (Shoup pair)
     const u64 *w;
     const u64 *pmod, *pinv, *pinv_q;
-    // FP64 targets (q_i < 2^41): wf[(i * kMaxSpecial + m) * 4 + {0..3}] as
-    // in ModUpBlock (every P digit split at bit 30); pf[i * 4 + {0,1,2}] =
-    // P mod q_i, P^-1 mod q_i, fl(P^-1 / q_i)
-    const double *wf, *pf;
+    // FP64 target row tf + i * kMdRow (q_i < 2^41): q_i, 1/q_i, P mod q_i,
+    // P^-1 mod q_i, fl(P^-1 / q_i), 0, then per P digit m (split at bit 30):
+    // w_im, fl(w_im / q_i), w_im 2^30 mod q_i, fl(that / q_i)
+    const double *tf;
 };
 
 }  // namespace ck
@@ -75,6 +77,8 @@ struct ckks_ksk {
     int hybrid = 0;
     size_t q_limbs = 0, p_limbs = 0, limbs = 0, dnum = 0, degree = 0;
     std::vector<size_t> block_start, block_size;
+    std::vector<int> block_fp;  // ModUp of the block runs in FP64 mode
+    ck::u64 fp_targets = 0, fp_q_targets = 0;  // FP64 target limbs of B / of Q
     ck::u64 *d_kb = nullptr, *d_ka = nullptr;  // [dnum][B][N], NTT domain
     ck::ModUpBlock *d_modup = nullptr;
     std::vector<ck::ModUpBlock> h_modup;
@@ -114,6 +118,7 @@ struct ModUpArgs {
     u64 *digits;  // [ct][dnum][B][N]
     const ModUpBlock *blocks;
     const DevPrime *primes;  // key basis B
+    u64 fp_targets;  // bit i: limb i takes the FP64 target path (fp_mode blocks)
     u32 dnum, limbs, logn, chunks;
 };
 
@@ -138,19 +143,26 @@ __device__ __forceinline__ Barrett barrett_at(const CK_CONST_AS DevPrime *p, u32
 // The block and prime tables are read through the scalar unit (const_table):
 // every address below except the residues' is uniform over the wave.
 //
-// Targets q_i < 2^41 take the FP64 path: sum_m fp_mulmod(c_m, w_mi) with
-// |each term| <= 0.51 q_i, so the sum of at most 2 * kMaxAlpha terms stays
-// below 2^45 and fp_residue makes it canonical.  A digit of a source prime
-// >= 2^41 (bit m of blk->wide) enters as c mod 2^30 with w and c >> 30 with
-// w 2^30 mod q_i, both below 2^45 as fp_mulmod requires.  The Garner digits
-// of FP64 block primes are computed the same way; every other target or
-// digit takes the 128-bit accumulate and Barrett reduction.
-template <int A>
+// Canonical FP64 residue in [0, q) of an integer-valued double, |x| < 2^45.
+__device__ __forceinline__ double fp_canon(double x, double q, double qinv) {
+    double r = fp_reduce(x, q, qinv);
+    r = r < 0.0 ? r + q : r;
+    return r >= q ? r - q : r;
+}
+
+// Integer mode: Garner digits and every target by 128-bit accumulation and
+// Barrett reduction.
+//
+// FP64 mode (blk->fp_mode): Garner digits c_k (k >= 1) and the targets whose
+// bit is set in a.fp_targets (q_i < 2^41) as sums of fp_mulmod terms, each
+// |term| <= 0.51 q, at most kMaxAlpha + 1 of them, so every sum stays below
+// 2^45 and is exact; fp_canon / fp_residue make it canonical.  Digit 0 is
+// x_0 itself, split at bit 30; the other digits are below 2^41.
+template <int A, bool FP>
 __device__ void modup_thread(const ModUpArgs &a, const CK_CONST_AS ModUpBlock *blk, u32 ct, u32 j, u64 n) {
     const u64 N = (u64)1 << a.logn;
     const CK_CONST_AS DevPrime *primes = const_table(a.primes);
     const u32 start = blk->start;
-    const u32 wide = blk->wide;
     const Gather gi = galois_gather(n, a.kinv, a.logn);
     const u64 *src = a.in + ct * a.in_stride + (u64)start * N + gi.idx;
     u64 *dst = a.digits + (((u64)ct * a.dnum + j) * a.limbs) * N + n;
@@ -162,96 +174,107 @@ __device__ void modup_thread(const ModUpArgs &a, const CK_CONST_AS ModUpBlock *b
         // inside the block the canonical digit reduces to the residue
         dst[(u64)(start + m) * N] = x[m];
     }
-    // mixed-radix digits c_m = ch[m] 2^30 + cl[m] (wide) or cl[m]
-    double cl[A], ch[A];
-    auto digit = [&](int m) -> u64 {
-        return (wide >> m) & 1 ? (f64_to_small(ch[m]) << 30) | f64_to_small(cl[m]) : f64_to_small(cl[m]);
-    };
-#pragma unroll
-    for (int k = 0; k < A; ++k) {
-        u64 ck;
-        if (k == 0) {
-            ck = x[0];
-        } else if (primes[start + k].fp) {
-            const double q = primes[start + k].qd;
-            double t = small_to_f64(x[k]);
-#pragma unroll
-            for (int m = 0; m < k; ++m) {
-                t -= fp_mulmod(cl[m], blk->pmf[k][m][0], blk->pmf[k][m][1], q);
-                if ((wide >> m) & 1) t -= fp_mulmod(ch[m], blk->pmf[k][m][2], blk->pmf[k][m][3], q);
-            }
-            ck = fp_residue(fp_mulmod(t, blk->invf[k][0], blk->invf[k][1], q), q, primes[start + k].qinv);
-        } else {
-            const Barrett bk = barrett_at(primes, start + k);
-            Acc128 t;
-            t.zero();
-#pragma unroll
-            for (int m = 0; m < k; ++m) t.mac(digit(m), blk->pm[k][m]);
-            u64 tr = barrett_reduce128(t.hi, t.lo, bk);
-            ck = shoup(sub_mod(x[k], tr, bk.q), blk->inv[k], blk->inv_q[k], bk.q);
-        }
-        if ((wide >> k) & 1) {
-            cl[k] = small_to_f64(ck & 0x3fffffffull);
-            ch[k] = small_to_f64(ck >> 30);
-        } else {
-            cl[k] = small_to_f64(ck);
-            ch[k] = 0.0;
-        }
-    }
     const CK_CONST_AS u64 *wt = const_table(blk->w);
-    const CK_CONST_AS double *wft = const_table(blk->wf);
-    for (u32 i = 0; i < a.limbs; ++i) {
-        if (i - start < (u32)A) continue;  // block limbs are stored above
-        u64 v;
-        if (primes[i].fp) {
-            const double q = primes[i].qd;
-            const CK_CONST_AS double *w = wft + (u64)i * kMaxAlpha * 4;
-            double acc = fp_mulmod(cl[0], w[0], w[1], q);
+    if (!FP) {
+        u64 c[A];
 #pragma unroll
-            for (int m = 1; m < A; ++m) acc += fp_mulmod(cl[m], w[4 * m], w[4 * m + 1], q);
+        for (int k = 0; k < A; ++k) {
+            if (k == 0) {
+                c[0] = x[0];
+            } else {
+                const Barrett bk = barrett_at(primes, start + k);
+                Acc128 t;
+                t.zero();
 #pragma unroll
-            for (int m = 0; m < A; ++m)
-                if ((wide >> m) & 1) acc += fp_mulmod(ch[m], w[4 * m + 2], w[4 * m + 3], q);
-            v = fp_residue(acc, q, primes[i].qinv);
-        } else {
+                for (int m = 0; m < k; ++m) t.mac(c[m], blk->pm[k][m]);
+                u64 tr = barrett_reduce128(t.hi, t.lo, bk);
+                c[k] = shoup(sub_mod(x[k], tr, bk.q), blk->inv[k], blk->inv_q[k], bk.q);
+            }
+        }
+        for (u32 i = 0; i < a.limbs; ++i) {
+            if (i - start < (u32)A) continue;  // block limbs are stored above
             const Barrett bi = barrett_at(primes, i);
             const CK_CONST_AS u64 *w = wt + (u64)i * kMaxAlpha;
+            u64 v;
             if (A == 1) {
-                const u64 c0 = digit(0);
-                v = c0 >= bi.q ? barrett_reduce128(0, c0, bi) : c0;
+                v = c[0] >= bi.q ? barrett_reduce128(0, c[0], bi) : c[0];
             } else {
                 Acc128 t;
                 t.zero();
 #pragma unroll
-                for (int m = 0; m < A; ++m) t.mac(digit(m), w[m]);
+                for (int m = 0; m < A; ++m) t.mac(c[m], w[m]);
                 v = barrett_reduce128(t.hi, t.lo, bi);
             }
+            dst[(u64)i * N] = v;
+        }
+        return;
+    }
+    double cl[A];  // cl[0]: low 30 bits of c_0 = x_0
+    const double ch0 = small_to_f64(x[0] >> 30);
+    cl[0] = small_to_f64(x[0] & 0x3fffffffull);
+#pragma unroll
+    for (int k = 1; k < A; ++k) {
+        const CK_CONST_AS double *g = blk->gf[k];
+        const double q = g[0];
+        double t = small_to_f64(x[k]) - (fp_mulmod(cl[0], g[6], g[7], q) + fp_mulmod(ch0, g[4], g[5], q));
+#pragma unroll
+        for (int m = 1; m < k; ++m) t -= fp_mulmod(cl[m], g[6 + 2 * m], g[7 + 2 * m], q);
+        cl[k] = fp_canon(fp_mulmod(t, g[2], g[3], q), q, g[1]);
+    }
+    const CK_CONST_AS double *tf = const_table(blk->tf);
+    const u64 fpm = a.fp_targets;
+    for (u32 i = 0; i < a.limbs; ++i) {
+        if (i - start < (u32)A) continue;  // block limbs are stored above
+        u64 v;
+        if ((fpm >> i) & 1) {
+            const CK_CONST_AS double *r = tf + (u64)i * kFpRow;
+            const double q = r[0];
+            double acc = fp_mulmod(ch0, r[2], r[3], q);
+#pragma unroll
+            for (int m = 0; m < A; ++m) acc += fp_mulmod(cl[m], r[4 + 2 * m], r[5 + 2 * m], q);
+            v = fp_residue(acc, q, r[1]);
+        } else {
+            const Barrett bi = barrett_at(primes, i);
+            const CK_CONST_AS u64 *w = wt + (u64)i * kMaxAlpha;
+            Acc128 t;
+            t.zero();
+            t.mac(x[0], w[0]);
+#pragma unroll
+            for (int m = 1; m < A; ++m) t.mac(f64_to_small(cl[m]), w[m]);
+            v = barrett_reduce128(t.hi, t.lo, bi);
         }
         dst[(u64)i * N] = v;
     }
 }
 
-// One instantiation per block size A, so each gets its own register budget;
-// a launch covers the consecutive digits [j_first, j_first + n_j) of one size.
-template <int A>
+// One instantiation per block size A (and mode), so each gets its own
+// register budget; a launch covers the consecutive digits
+// [j_first, j_first + n_j) of one size.
+template <int A, bool FP>
 __global__ __launch_bounds__(kKsThreads) void k_modup(ModUpArgs a, u32 j_first, u32 n_j) {
     const u32 b = xcd_contiguous_block(blockIdx.x, gridDim.x);
     const u32 chunk = b % a.chunks;
     const u32 cj = b / a.chunks;
     const u32 j = j_first + cj % n_j, ct = cj / n_j;
     const u64 n = (u64)chunk * kKsThreads + threadIdx.x;
-    modup_thread<A>(a, const_table(a.blocks) + j, ct, j, n);
+    modup_thread<A, FP>(a, const_table(a.blocks) + j, ct, j, n);
 }
 
-static int launch_modup(hipStream_t s, const ModUpArgs &mu, const std::vector<size_t> &sizes, size_t n_cts) {
+// sizes / fp: per digit, its block size and whether it runs in FP64 mode.
+static int launch_modup(hipStream_t s, const ModUpArgs &mu, const std::vector<size_t> &sizes,
+                        const std::vector<int> &fp, size_t n_cts) {
     for (size_t j0 = 0; j0 < sizes.size();) {
         size_t j1 = j0 + 1;
-        while (j1 < sizes.size() && sizes[j1] == sizes[j0]) ++j1;
+        while (j1 < sizes.size() && sizes[j1] == sizes[j0] && fp[j1] == fp[j0]) ++j1;
         const dim3 g((u32)(n_cts * (j1 - j0) * mu.chunks));
         const u32 jf = (u32)j0, nj = (u32)(j1 - j0);
+        const bool f = fp[j0] != 0;
         switch (sizes[j0]) {
-#define CK_MU(A) \
-    case A: hipLaunchKernelGGL(k_modup<A>, g, dim3(kKsThreads), 0, s, mu, jf, nj); break;
+#define CK_MU(A)                                                                                  \
+    case A:                                                                                       \
+        if (f) hipLaunchKernelGGL((k_modup<A, true>), g, dim3(kKsThreads), 0, s, mu, jf, nj);    \
+        else hipLaunchKernelGGL((k_modup<A, false>), g, dim3(kKsThreads), 0, s, mu, jf, nj);     \
+        break;
             CK_MU(1) CK_MU(2) CK_MU(3) CK_MU(4) CK_MU(5) CK_MU(6)
             CK_MU(7) CK_MU(8) CK_MU(9) CK_MU(10) CK_MU(11) CK_MU(12)
 #undef CK_MU
@@ -429,6 +452,7 @@ struct DownArgs {
     u64 out_stride;
     const ModDownTable *md;  // null: no special primes
     const DevPrime *primes;  // key basis
+    u64 fp_targets;  // bit i: Q limb i takes the FP64 target path
     u32 limbs, q_limbs, logn, chunks;
 };
 
@@ -487,38 +511,38 @@ __global__ __launch_bounds__(kKsThreads) void k_moddown(DownArgs a) {
             if (cmp == 0 && c[k] != md->half[k]) cmp = c[k] > md->half[k] ? 1 : -1;
         neg = cmp > 0;
     }
-    // FP64 targets (q_i < 2^41): every P digit enters split at bit 30, as in
-    // ModUp; the terms sum to |acc| <= 2K 0.51 q_i, so t = x_i - acc (+ P)
-    // stays below 2^45 for the final fp_mulmod by P^-1.
+    // FP64 targets (bit i of a.fp_targets, q_i < 2^41): every P digit enters
+    // split at bit 30; the 2K terms sum to |acc| <= 2K 0.51 q_i, so
+    // t = x_i - acc (+ P) stays below 2^45 for the final fp_mulmod by P^-1.
     double cl[K > 0 ? K : 1], ch[K > 0 ? K : 1];
 #pragma unroll
     for (int m = 0; m < K; ++m) {
         cl[m] = small_to_f64(c[m] & 0x3fffffffull);
         ch[m] = small_to_f64(c[m] >> 30);
     }
-    const CK_CONST_AS double *wft = const_table(md->wf);
-    const CK_CONST_AS double *pft = const_table(md->pf);
+    const CK_CONST_AS double *tf = const_table(md->tf);
     const CK_CONST_AS u64 *wt = const_table(md->w);
     const CK_CONST_AS u64 *pmod = const_table(md->pmod);
     const CK_CONST_AS u64 *pinv = const_table(md->pinv);
     const CK_CONST_AS u64 *pinv_q = const_table(md->pinv_q);
+    const u64 fpm = a.fp_targets;
     for (u32 i = 0; i < a.q_limbs; ++i) {
-        if (primes[i].fp) {
-            const double q = primes[i].qd;
-            const CK_CONST_AS double *w = wft + (u64)i * kMaxSpecial * 4;
-            double acc = fp_mulmod(cl[0], w[0], w[1], q) + fp_mulmod(ch[0], w[2], w[3], q);
+        if ((fpm >> i) & 1) {
+            const CK_CONST_AS double *r = tf + (u64)i * kMdRow;
+            const double q = r[0];
+            double acc = fp_mulmod(cl[0], r[6], r[7], q) + fp_mulmod(ch[0], r[8], r[9], q);
 #pragma unroll
             for (int m = 1; m < K; ++m)
-                acc += fp_mulmod(cl[m], w[4 * m], w[4 * m + 1], q) + fp_mulmod(ch[m], w[4 * m + 2], w[4 * m + 3], q);
-            const CK_CONST_AS double *pf = pft + (u64)i * 4;
+                acc += fp_mulmod(cl[m], r[6 + 4 * m], r[7 + 4 * m], q) +
+                       fp_mulmod(ch[m], r[8 + 4 * m], r[9 + 4 * m], q);
             double t = small_to_f64(x[(u64)i * N]) - acc;  // x_i - (x_P mod q_i)
-            if (neg) t += pf[0];                            // x_P - P
-            double v = fp_mulmod(t, pf[1], pf[2], q);
+            if (neg) t += r[2];                             // x_P - P
+            double v = fp_mulmod(t, r[3], r[4], q);
             if (add) {
                 const double ad = small_to_f64(add[(u64)i * N]);
                 v = ga.neg ? v - ad : v + ad;
             }
-            o[(u64)i * N] = fp_residue(v, q, primes[i].qinv);
+            o[(u64)i * N] = fp_residue(v, q, r[1]);
             continue;
         }
         const Barrett bi = barrett_at(primes, i);
@@ -839,18 +863,21 @@ static u64 double_bits(double d) {
     return v;
 }
 
-// FP64 weights of one w (mod q) for the FP64 target path: w, fl(w / q),
-// w 2^30 mod q, fl(w 2^30 mod q / q); zeros for q >= 2^41 (integer path).
-static void push_fp_weight(std::vector<u64> &consts, u64 w, u64 q) {
-    if (q >= host::kFpModulusBound) {
-        consts.insert(consts.end(), 4, 0);
-        return;
-    }
-    const u64 w2 = host::mulmod(w, (1ull << 30) % q, q);
+static bool fp_prime(u64 q) { return q < host::kFpModulusBound; }
+
+// Appends the FP64 pair of a weight w < q: w and fl(w / q).
+static void push_fp_pair(std::vector<u64> &consts, u64 w, u64 q) {
     consts.push_back(double_bits((double)w));
     consts.push_back(double_bits((double)w / (double)q));
-    consts.push_back(double_bits((double)w2));
-    consts.push_back(double_bits((double)w2 / (double)q));
+}
+
+// Bit i set when limb i (i < count <= 64) takes the FP64 target path.
+static u64 fp_target_mask(const std::vector<u64> &mods, size_t count) {
+    if (count > 64) return 0;
+    u64 mask = 0;
+    for (size_t i = 0; i < count; ++i)
+        if (fp_prime(mods[i])) mask |= 1ull << i;
+    return mask;
 }
 
 // ModDown constants of Q u P (the last mods.size() - q_limbs limbs are P):
@@ -909,17 +936,25 @@ static void build_moddown(const std::vector<u64> &mods, size_t q_limbs, std::vec
     for (size_t i = 0; i < q_limbs; ++i)
         consts.push_back(host::shoup_quotient(consts[md_w_off + q_limbs * kMaxSpecial + q_limbs + i],
                                               mods[i]));
-    // FP64 tables: wf[i][m][4], then pf[i][4]
-    for (size_t i = 0; i < q_limbs; ++i)
-        for (int m = 0; m < kMaxSpecial; ++m) push_fp_weight(consts, consts[md_w_off + i * kMaxSpecial + m], mods[i]);
+    // FP64 target rows (ModDownTable::tf), zero rows for integer targets
     for (size_t i = 0; i < q_limbs; ++i) {
-        const u64 pmod = consts[md_w_off + q_limbs * kMaxSpecial + i];
-        const u64 pinv = consts[md_w_off + q_limbs * kMaxSpecial + q_limbs + i];
-        const bool fp = mods[i] < host::kFpModulusBound;
-        consts.push_back(fp ? double_bits((double)pmod) : 0);
-        consts.push_back(fp ? double_bits((double)pinv) : 0);
-        consts.push_back(fp ? double_bits((double)pinv / (double)mods[i]) : 0);
-        consts.push_back(0);
+        const u64 q = mods[i];
+        const size_t row0 = consts.size();
+        if (fp_prime(q)) {
+            const u64 pmod = consts[md_w_off + q_limbs * kMaxSpecial + i];
+            const u64 pinv = consts[md_w_off + q_limbs * kMaxSpecial + q_limbs + i];
+            consts.push_back(double_bits((double)q));
+            consts.push_back(double_bits(1.0 / (double)q));
+            consts.push_back(double_bits((double)pmod));
+            push_fp_pair(consts, pinv, q);
+            consts.push_back(0);
+            for (size_t m = 0; m < K; ++m) {
+                const u64 w = consts[md_w_off + i * kMaxSpecial + m];
+                push_fp_pair(consts, w, q);
+                push_fp_pair(consts, host::mulmod(w, (1ull << 30) % q, q), q);
+            }
+        }
+        consts.resize(row0 + kMdRow, 0);
     }
 }
 
@@ -930,8 +965,7 @@ static void patch_moddown(ModDownTable &md, const u64 *w, size_t q_limbs) {
     md.pmod = md.w + q_limbs * kMaxSpecial;
     md.pinv = md.pmod + q_limbs;
     md.pinv_q = md.pinv + q_limbs;
-    md.wf = (const double *)(md.pinv_q + q_limbs);
-    md.pf = md.wf + q_limbs * kMaxSpecial * 4;
+    md.tf = (const double *)(md.pinv_q + q_limbs);
 }
 
 static int create_key(const ckks_rns_plan *plan, size_t q_limbs, const size_t *block_sizes,
@@ -975,23 +1009,31 @@ static int create_key(const ckks_rns_plan *plan, size_t q_limbs, const size_t *b
         mb.size = (u32)block_sizes[j];
         std::vector<u64> bm(mods.begin() + start, mods.begin() + start + block_sizes[j]);
         fill_garner(bm, mb.pm, mb.inv, mb.inv_q, 0);
-        for (size_t kk = 1; kk < bm.size(); ++kk) {
-            const u64 qk = bm[kk];
-            if (qk >= host::kFpModulusBound) continue;
-            for (size_t m = 0; m < kk; ++m) {
-                const u64 w2 = host::mulmod(mb.pm[kk][m], (1ull << 30) % qk, qk);
-                mb.pmf[kk][m][0] = (double)mb.pm[kk][m];
-                mb.pmf[kk][m][1] = (double)mb.pm[kk][m] / (double)qk;
-                mb.pmf[kk][m][2] = (double)w2;
-                mb.pmf[kk][m][3] = (double)w2 / (double)qk;
+        // FP64 mode: every block prime after the first below 2^41
+        mb.fp_mode = B <= 64 ? 1u : 0u;
+        for (size_t kk = 1; kk < bm.size(); ++kk) mb.fp_mode &= fp_prime(bm[kk]) ? 1u : 0u;
+        if (mb.fp_mode) {
+            for (size_t kk = 1; kk < bm.size(); ++kk) {
+                const u64 qk = bm[kk];
+                double *g = mb.gf[kk];
+                g[0] = (double)qk;
+                g[1] = 1.0 / (double)qk;
+                g[2] = (double)mb.inv[kk];
+                g[3] = (double)mb.inv[kk] / (double)qk;
+                const u64 w2 = host::mulmod(mb.pm[kk][0], (1ull << 30) % qk, qk);
+                g[4] = (double)w2;
+                g[5] = (double)w2 / (double)qk;
+                for (size_t m = 0; m < kk; ++m) {
+                    g[6 + 2 * m] = (double)mb.pm[kk][m];
+                    g[7 + 2 * m] = (double)mb.pm[kk][m] / (double)qk;
+                }
             }
-            mb.invf[kk][0] = (double)mb.inv[kk];
-            mb.invf[kk][1] = (double)mb.inv[kk] / (double)qk;
         }
         mb.w = nullptr;  // patched after upload
         k->h_modup.push_back(mb);
         k->block_start.push_back(start);
         k->block_size.push_back(block_sizes[j]);
+        k->block_fp.push_back((int)mb.fp_mode);
         for (size_t i = 0; i < B; ++i) {
             u64 prod = 1 % mods[i];
             for (int m = 0; m < kMaxAlpha; ++m) {
@@ -1004,20 +1046,29 @@ static int create_key(const ckks_rns_plan *plan, size_t q_limbs, const size_t *b
     size_t md_w_off = consts.size();
     ModDownTable md{};
     if (hybrid) build_moddown(mods, q_limbs, consts, md);
-    // FP64 ModUp weights of every block: wf[i][m][4]
-    const size_t wf_off = consts.size();
+    // FP64 ModUp target rows of every block (ModUpBlock::tf)
+    const size_t tf_off = consts.size();
     for (size_t j = 0; j < n_blocks; ++j) {
-        for (size_t m = 0; m < block_sizes[j]; ++m)
-            if (mods[k->block_start[j] + m] >= host::kFpModulusBound) k->h_modup[j].wide |= 1u << m;
-        for (size_t i = 0; i < B; ++i)
-            for (int m = 0; m < kMaxAlpha; ++m)
-                push_fp_weight(consts, consts[(j * B + i) * kMaxAlpha + m], mods[i]);
+        for (size_t i = 0; i < B; ++i) {
+            const size_t row0 = consts.size();
+            const u64 q = mods[i];
+            if (k->block_fp[j] && fp_prime(q)) {
+                const u64 w0 = consts[(j * B + i) * kMaxAlpha];
+                consts.push_back(double_bits((double)q));
+                consts.push_back(double_bits(1.0 / (double)q));
+                push_fp_pair(consts, host::mulmod(w0, (1ull << 30) % q, q), q);
+                for (size_t m = 0; m < block_sizes[j]; ++m) push_fp_pair(consts, consts[(j * B + i) * kMaxAlpha + m], q);
+            }
+            consts.resize(row0 + kFpRow, 0);
+        }
     }
+    k->fp_targets = fp_target_mask(mods, B);
+    k->fp_q_targets = fp_target_mask(mods, q_limbs);
     CK_HIP(hipMalloc(&k->d_consts, std::max<size_t>(consts.size(), 1) * sizeof(u64)));
     CK_HIP(hipMemcpy(k->d_consts, consts.data(), consts.size() * sizeof(u64), hipMemcpyHostToDevice));
     for (size_t j = 0; j < n_blocks; ++j) {
         k->h_modup[j].w = k->d_consts + j * B * kMaxAlpha;
-        k->h_modup[j].wf = (const double *)(k->d_consts + wf_off) + j * B * kMaxAlpha * 4;
+        k->h_modup[j].tf = (const double *)(k->d_consts + tf_off) + j * B * kFpRow;
     }
     CK_HIP(hipMalloc(&k->d_modup, n_blocks * sizeof(ModUpBlock)));
     CK_HIP(hipMemcpy(k->d_modup, k->h_modup.data(), n_blocks * sizeof(ModUpBlock), hipMemcpyHostToDevice));
@@ -1243,6 +1294,7 @@ static int key_switch_lane(const ckks_ksk *k, const u64 *x, u64 x_stride, const 
     mu.digits = digits;
     mu.blocks = k->d_modup;
     mu.primes = plan->d_primes;
+    mu.fp_targets = k->fp_targets;
     mu.dnum = (u32)D;
     mu.limbs = (u32)B;
     mu.logn = logn;
@@ -1268,7 +1320,7 @@ static int key_switch_lane(const ckks_ksk *k, const u64 *x, u64 x_stride, const 
         ModUpMfmaArgs mm{mu, k->d_bc};
         hipLaunchKernelGGL(k_modup_mfma, dim3((u32)(n_cts * D * (N / kBcRows))), dim3(256), 0, s, mm);
     } else {
-        if ((rc = launch_modup(s, mu, k->block_size, n_cts))) return rc;
+        if ((rc = launch_modup(s, mu, k->block_size, k->block_fp, n_cts))) return rc;
     }
     CK_HIP(hipGetLastError());
     // 2+3. forward NTT of every digit limb, key MAC
@@ -1315,6 +1367,7 @@ static int key_switch_lane(const ckks_ksk *k, const u64 *x, u64 x_stride, const 
     da.out_stride = out_stride;
     da.md = k->d_moddown;
     da.primes = plan->d_primes;
+    da.fp_targets = k->fp_q_targets;
     da.limbs = (u32)B;
     da.q_limbs = (u32)k->q_limbs;
     da.logn = logn;
@@ -1463,6 +1516,7 @@ int ckks_mod_down(const ckks_rns_plan *qp_plan, size_t q_limbs, const uint64_t *
     da.out_stride = 2 * q_limbs * N;  // polynomials are consecutive [p][q_limbs][N]
     da.md = (const ModDownTable *)it->second.second;
     da.primes = qp_plan->d_primes;
+    da.fp_targets = fp_target_mask(qp_plan->moduli, q_limbs);
     da.limbs = (u32)B;
     da.q_limbs = (u32)q_limbs;
     da.logn = qp_plan->logn;
