@@ -143,13 +143,6 @@ __device__ __forceinline__ Barrett barrett_at(const CK_CONST_AS DevPrime *p, u32
 // The block and prime tables are read through the scalar unit (const_table):
 // every address below except the residues' is uniform over the wave.
 //
-// Canonical FP64 residue in [0, q) of an integer-valued double, |x| < 2^45.
-__device__ __forceinline__ double fp_canon(double x, double q, double qinv) {
-    double r = fp_reduce(x, q, qinv);
-    r = r < 0.0 ? r + q : r;
-    return r >= q ? r - q : r;
-}
-
 // Integer mode: Garner digits and every target by 128-bit accumulation and
 // Barrett reduction.
 //

@@ -145,6 +145,13 @@ CK_HD u64 f64_as_u64(double d) {
 CK_HD double small_to_f64(u64 v) { return u64_as_f64(v | 0x4330000000000000ull) - 4503599627370496.0; }
 CK_HD u64 f64_to_small(double d) { return f64_as_u64(d + 4503599627370496.0) - 0x4330000000000000ull; }
 
+// Canonical FP64 residue in [0, q) of an integer-valued double, |x| < 2^45.
+CK_HD double fp_canon(double x, double q, double qinv) {
+    double r = fp_reduce(x, q, qinv);
+    r = r < 0.0 ? r + q : r;
+    return r >= q ? r - q : r;
+}
+
 // fp_canonical for results that stay below 2^52 (every q < 2^41 here).
 CK_HD u64 fp_residue(double x, double q, double qinv) {
     double r = fp_reduce(x, q, qinv);

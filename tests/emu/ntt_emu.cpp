@@ -86,3 +86,52 @@ extern "C" int emu_ntt_variant(const uint64_t *in, uint64_t *out, uint64_t q, ui
 extern "C" int emu_ntt(const uint64_t *in, uint64_t *out, uint64_t q, uint32_t logn, int forward) {
     return emu_ntt_variant(in, out, q, logn, forward, 12, 0);
 }
+
+// ---- FP64 base-conversion arithmetic ---------------------------------------
+// The FP64 target / Garner evaluations of the ModUp and ModDown kernels,
+// restated over the same modarith.hpp primitives so the tests can drive
+// them with extreme operands and compare with exact integer arithmetic.
+
+static double fp_quot(u64 w, u64 q) { return (double)w / (double)q; }
+
+// ModUp FP64 target: sum_m c_m w_m mod q.  c[0] any residue < 2^62 (split at
+// bit 30), c[m > 0] < 2^41; w[m] < q < 2^41; w0s = w[0] 2^30 mod q.
+extern "C" uint64_t emu_fp_modup_target(const uint64_t *c, const uint64_t *w, uint64_t w0s, int A, uint64_t q) {
+    const double qd = (double)q;
+    double acc = fp_mulmod(small_to_f64(c[0] >> 30), (double)w0s, fp_quot(w0s, q), qd);
+    for (int m = 0; m < A; ++m) {
+        const u64 cm = m == 0 ? (c[0] & 0x3fffffffull) : c[m];
+        acc += fp_mulmod(small_to_f64(cm), (double)w[m], fp_quot(w[m], q), qd);
+    }
+    return fp_residue(acc, qd, 1.0 / qd);
+}
+
+// ModUp FP64 Garner step k: (x_k - sum_{m<k} c_m pm[m]) inv mod q, same
+// operand ranges; pm0s = pm[0] 2^30 mod q.
+extern "C" uint64_t emu_fp_garner_step(uint64_t xk, const uint64_t *c, const uint64_t *pm, uint64_t pm0s, int k,
+                                       uint64_t inv, uint64_t q) {
+    const double qd = (double)q;
+    double t = small_to_f64(xk) - (fp_mulmod(small_to_f64(c[0] & 0x3fffffffull), (double)pm[0], fp_quot(pm[0], q), qd) +
+                                   fp_mulmod(small_to_f64(c[0] >> 30), (double)pm0s, fp_quot(pm0s, q), qd));
+    for (int m = 1; m < k; ++m) t -= fp_mulmod(small_to_f64(c[m]), (double)pm[m], fp_quot(pm[m], q), qd);
+    return f64_to_small(fp_canon(fp_mulmod(t, (double)inv, fp_quot(inv, q), qd), qd, 1.0 / qd));
+}
+
+// ModDown FP64 target: ((x - (x_P mod q) + neg P) P^-1 +- add) mod q with
+// x_P = sum_m c_m w_m, every digit c_m < 2^62 split at bit 30; w2[m] =
+// w[m] 2^30 mod q.
+extern "C" uint64_t emu_fp_moddown_target(uint64_t x, const uint64_t *c, const uint64_t *w, const uint64_t *w2, int K,
+                                          int neg, uint64_t pmod, uint64_t pinv, uint64_t add, int add_neg,
+                                          uint64_t q) {
+    const double qd = (double)q;
+    double acc = 0.0;
+    for (int m = 0; m < K; ++m)
+        acc += fp_mulmod(small_to_f64(c[m] & 0x3fffffffull), (double)w[m], fp_quot(w[m], q), qd) +
+               fp_mulmod(small_to_f64(c[m] >> 30), (double)w2[m], fp_quot(w2[m], q), qd);
+    double t = small_to_f64(x) - acc;
+    if (neg) t += (double)pmod;
+    double v = fp_mulmod(t, (double)pinv, fp_quot(pinv, q), qd);
+    const double ad = small_to_f64(add);
+    v = add_neg ? v - ad : v + ad;
+    return fp_residue(v, qd, 1.0 / qd);
+}

@@ -67,3 +67,107 @@ def test_emulated_kernel_variants_match_oracle(emu, logn, tb, blk):
         back = np.zeros(n, dtype=np.uint64)
         assert emu.emu_ntt_variant(x.ctypes.data, back.ctypes.data, q, logn, 0, tb, blk) == 0
         assert (back == plan.inverse(x)).all()
+
+
+def _is_prime(n):
+    if n < 2:
+        return False
+    for p in (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37):
+        if n % p == 0:
+            return n == p
+    d, s = n - 1, 0
+    while d % 2 == 0:
+        d //= 2
+        s += 1
+    for a in (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37):
+        x = pow(a, d, n)
+        if x in (1, n - 1):
+            continue
+        for _ in range(s - 1):
+            x = x * x % n
+            if x == n - 1:
+                break
+        else:
+            return False
+    return True
+
+
+def _prime_below(bound):
+    n = bound - 1
+    while not _is_prime(n):
+        n -= 2 if n % 2 else 1
+    return n
+
+
+# the largest FP64-path prime (< 2^41) and a chain prime; 60-bit source primes
+_FP_TARGETS = [_prime_below(1 << 41), Q40[0], Q40[-1]]
+_WIDE = [Q0[0], _prime_below(1 << 62)] + list(P60[:2])
+
+
+def _u64(vals):
+    return np.array([int(v) for v in vals], dtype=np.uint64)
+
+
+def _operand_sets(rng, q, n, lo_bound, hi_bound, trials):
+    """Random operands plus the extremes (0, 1, bound - 1) in every slot."""
+    for extreme in (0, 1, None):
+        if extreme is None:
+            yield [hi_bound - 1] * n, q - 1
+            yield [lo_bound - 1] + [hi_bound - 1] * (n - 1), q - 1
+        else:
+            yield [extreme] * n, extreme % q
+    for _ in range(trials):
+        yield [int(rng.integers(0, hi_bound)) for _ in range(n)], int(rng.integers(0, q))
+
+
+@pytest.mark.parametrize("q", _FP_TARGETS)
+def test_fp64_modup_target_and_garner_exact(emu, q):
+    """The FP64 target sum and Garner step of the ModUp kernel equal exact
+    integer arithmetic at the operand extremes the kernel admits: digit 0 up
+    to 2^62 (split at bit 30), other digits and every weight up to q - 1."""
+    emu.emu_fp_modup_target.restype = C.c_uint64
+    emu.emu_fp_modup_target.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_int, C.c_uint64]
+    emu.emu_fp_garner_step.restype = C.c_uint64
+    emu.emu_fp_garner_step.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_int, C.c_uint64,
+                                       C.c_uint64]
+    rng = np.random.default_rng(q % 1000)
+    for A in (1, 8, 9, 12):
+        for wide0 in _WIDE:
+            for digits, x in _operand_sets(rng, q, A, wide0, min(q, 1 << 41), 60):
+                digits[0] = digits[0] % wide0
+                for w in ([q - 1] * A, [int(v) for v in rng.integers(0, q, A)]):
+                    w0s = w[0] * (1 << 30) % q
+                    cd, wd = _u64(digits), _u64(w)  # kept alive across the calls
+                    got = emu.emu_fp_modup_target(cd.ctypes.data, wd.ctypes.data, w0s, A, q)
+                    assert got == sum(c * ww for c, ww in zip(digits, w)) % q
+                    if A > 1:
+                        k = A - 1
+                        inv = int(rng.integers(1, q))
+                        got = emu.emu_fp_garner_step(x, cd.ctypes.data, wd.ctypes.data, w0s, k, inv, q)
+                        want = (x - sum(c * ww for c, ww in zip(digits[:k], w[:k]))) * inv % q
+                        assert got == want
+
+
+@pytest.mark.parametrize("q", _FP_TARGETS)
+def test_fp64_moddown_target_exact(emu, q):
+    """The FP64 target evaluation of the ModDown kernel equals exact integer
+    arithmetic: 7 and 8 special-prime digits up to 2^62, x and the added b
+    component up to q - 1, both centering branches and both signs of b."""
+    f = emu.emu_fp_moddown_target
+    f.restype = C.c_uint64
+    f.argtypes = [C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_uint64, C.c_uint64,
+                  C.c_uint64, C.c_int, C.c_uint64]
+    rng = np.random.default_rng(q % 997)
+    for K in (1, 7, 8):
+        for digits, x in _operand_sets(rng, q, K, 1 << 62, 1 << 62, 60):
+            for w in ([q - 1] * K, [int(v) for v in rng.integers(0, q, K)]):
+                w2 = [ww * (1 << 30) % q for ww in w]
+                pmod, pinv = q - 1, int(rng.integers(1, q))
+                cd, wd, w2d = _u64(digits), _u64(w), _u64(w2)  # kept alive across the calls
+                for neg in (0, 1):
+                    for add, add_neg in ((0, 0), (q - 1, 0), (q - 1, 1), (int(rng.integers(0, q)), 1)):
+                        got = f(x, cd.ctypes.data, wd.ctypes.data, w2d.ctypes.data, K, neg, pmod, pinv, add,
+                                add_neg, q)
+                        xp = sum(c * ww for c, ww in zip(digits, w))
+                        want = ((x - xp + neg * pmod) * pinv + (-add if add_neg else add)) % q
+                        assert got == want

@@ -200,3 +200,35 @@ def test_rotation_decrypts_with_noisy_hybrid_key(pkg, ctx):
         diff = (dec[i].astype(object) - want[i]) % q
         worst = max(min(int(v), q - int(v)) for v in diff)
         assert worst < 2 ** 20, worst
+
+
+@pytest.mark.parametrize("case", ["wide_in_block", "basis_over_64"])
+def test_hybrid_key_switch_integer_mode_bit_exact(pkg, ctx, case):
+    """ModUp / ModDown keep their integer path where the FP64 one does not
+    apply: a gadget block with a 60-bit prime after its first limb, and a
+    key basis of more than 64 limbs (no FP64 target mask)."""
+    if case == "wide_in_block":
+        logn, blocks, K = 12, [2, 2, 1], 3
+        q_moduli = [Q40[0], P60[3], Q40[1], P60[4], Q0[0]]
+        p_moduli = P60[:3]
+    else:
+        logn, blocks, K = 12, [12] * 5 + [6], 4
+        q_moduli = [Q0[0]] + O.ntt_primes(40, 1 << logn, 65)[:65]
+        p_moduli = O.ntt_primes(61, 1 << logn, K)
+    n = 1 << logn
+    qp = q_moduli + p_moduli
+    L = len(q_moduli)
+    assert sum(blocks) == L and len(set(qp)) == len(qp)
+    rng = np.random.default_rng(len(qp))
+    kb, ka = _keys(rng, qp, n, len(blocks))
+    plan = pkg.PreparedRnsNttPlan(ctx, qp, n)
+    key = pkg.HybridKeySwitchKey(plan, L, blocks, kb, ka)
+    ct = _ct(rng, q_moduli, n, 1)
+    ct[0, 1, :, :8] = np.array(q_moduli, dtype=np.uint64)[:, None] - 1
+    ctx.set_base_conversion(ctx.BASECONV_VALU)
+    try:
+        out = pkg.to_host(key.switch(pkg.to_device(ct)))
+    finally:
+        ctx.set_base_conversion(ctx.BASECONV_AUTO)
+    bo, ao = O.hybrid_key_switch(O.RnsPlan(qp, n), L, blocks, ct[0, 0], ct[0, 1], kb, ka)
+    assert (out[0, 0] == bo).all() and (out[0, 1] == ao).all()
