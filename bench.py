@@ -266,7 +266,7 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=6, help="plain-path rotations timed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--fusion", choices=["auto", "off", "xcd", "linear"], default="auto",
+    ap.add_argument("--fusion", choices=["auto", "off", "xcd", "linear", "inverse"], default="auto",
                     help="digits' forward row pass fused with the key MAC: autotuned per key, "
                          "off, or fused with XCD-contiguous / linear block order")
     ap.add_argument("--only", choices=["all", "ntt", "serve"], default="all",
@@ -297,7 +297,8 @@ def main():
     ctx.set_key_switch_split({"auto": ctx.KS_SPLIT_AUTO, "off": ctx.KS_SPLIT_OFF,
                               "on": ctx.KS_SPLIT_ON}[args.split])
     ctx.set_key_switch_fusion({"auto": ctx.KS_FUSION_AUTO, "off": ctx.KS_FUSION_OFF,
-                               "xcd": ctx.KS_FUSION_XCD, "linear": ctx.KS_FUSION_LINEAR}[args.fusion])
+                               "xcd": ctx.KS_FUSION_XCD, "linear": ctx.KS_FUSION_LINEAR,
+                               "inverse": ctx.KS_FUSION_INVERSE}[args.fusion])
     ctx.set_base_conversion({"auto": ctx.BASECONV_AUTO, "valu": ctx.BASECONV_VALU,
                              "mfma": ctx.BASECONV_MFMA}[args.base_conversion])
 
@@ -352,7 +353,7 @@ def main():
                 for _, key in kl:
                     eng, fus, times = key.autotune(n)
                     name = ("mfma" if eng == ctx.BASECONV_MFMA else "valu") + "/" + \
-                        ["off", "xcd", "linear"][fus]
+                        ["off", "xcd", "linear", "inverse"][fus]
                     picks.setdefault(deg, {}).setdefault(name, 0)
                     picks[deg][name] += 1
                 if kl:

@@ -244,7 +244,7 @@ class Context:
         NTT_VARIANT_BLOCKED / NTT_VARIANT_TILE8K); results unchanged."""
         _check(lib().ckks_set_ntt_variant(self._h, int(logn), int(variant)))
 
-    KS_FUSION_AUTO, KS_FUSION_OFF, KS_FUSION_XCD, KS_FUSION_LINEAR = -1, 0, 1, 2
+    KS_FUSION_AUTO, KS_FUSION_OFF, KS_FUSION_XCD, KS_FUSION_LINEAR, KS_FUSION_INVERSE = -1, 0, 1, 2, 3
 
     def set_key_switch_fusion(self, mode: int):
         """Key switch: digits' forward row pass fused with the key MAC
@@ -534,9 +534,9 @@ class _KeySwitchKey:
         engine, fusion mode) pair and keeps the fastest for the AUTO modes;
         returns (engine, fusion, {(engine, fusion): ms}) over the pairs that apply."""
         be, bf = C.c_int(), C.c_int()
-        ms = (C.c_float * 6)()
+        ms = (C.c_float * 8)()
         _check(lib().ckks_autotune_key_switch(self._h, int(n_cts), C.byref(be), C.byref(bf), ms))
-        times = {(e, f): ms[(e - 1) * 3 + f] for e in (1, 2) for f in range(3) if ms[(e - 1) * 3 + f] >= 0}
+        times = {(e, f): ms[(e - 1) * 4 + f] for e in (1, 2) for f in range(4) if ms[(e - 1) * 4 + f] >= 0}
         return be.value, bf.value, times
 
     def close(self):

@@ -179,7 +179,7 @@ int ckks_set_ntt_variant(ckks_context *ctx, uint32_t logn, int variant) {
 
 int ckks_set_key_switch_fusion(ckks_context *ctx, int mode) {
     if (!ctx) return fail(CKKS_ERR_INVALID, "null context");
-    if (mode < CKKS_KS_FUSION_AUTO || mode > CKKS_KS_FUSION_LINEAR)
+    if (mode < CKKS_KS_FUSION_AUTO || mode > CKKS_KS_FUSION_INVERSE)
         return fail(CKKS_ERR_INVALID, "unknown key-switch fusion mode");
     ctx->ks_fusion = mode;
     return CKKS_OK;

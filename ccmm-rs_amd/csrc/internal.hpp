@@ -123,6 +123,15 @@ struct ckks_rns_plan {
 
 namespace ck {
 
+// Key inner product read by a fused inverse NTT in place of its input:
+// polynomial p = 2 ct + comp of the call is sum_j digits[ct][j] * key_comp[j]
+// (NTT domain, limb by limb; comp 0 takes kb, comp 1 ka).
+struct NttMac {
+    const u64 *digits;   // [ct][dnum][limbs][N]
+    const u64 *kb, *ka;  // [dnum][limbs][N]
+    u32 dnum;            // 1..8
+};
+
 // Launch helpers (ntt.hip).
 struct NttCall {
     const ckks_rns_plan *plan;
@@ -130,6 +139,7 @@ struct NttCall {
     u64 *out;
     size_t n_polys, n_limbs;
     size_t in_stride, out_stride;  // elements between consecutive polynomials
+    const NttMac *mac = nullptr;   // inverse two-pass degrees only: see NttMac
 };
 int launch_ntt(ckks_context *ctx, const NttCall *calls, size_t count, bool forward);
 int ntt_variant_for(const ckks_context *ctx, u32 logn);

@@ -1318,13 +1318,18 @@ static int key_switch_lane(const ckks_ksk *k, const u64 *x, u64 x_stride, const 
     CK_HIP(hipGetLastError());
     // 2+3. forward NTT of every digit limb, key MAC
     const int fusion = ctx->ks_fusion == CKKS_KS_FUSION_AUTO ? k->tuned_fusion : ctx->ks_fusion;
-    if (fusion != CKKS_KS_FUSION_OFF && logn > 12 && logn <= 17) {
+    const bool two_pass = logn > 12 && logn <= 17;
+    // CKKS_KS_FUSION_INVERSE: the MAC is computed inside the first inverse
+    // NTT pass (step 4) and the accumulators never reach HBM in the NTT domain
+    const bool mac_in_inverse = fusion == CKKS_KS_FUSION_INVERSE && two_pass && D <= 8;
+    if ((fusion == CKKS_KS_FUSION_XCD || fusion == CKKS_KS_FUSION_LINEAR) && two_pass) {
         if ((rc = launch_ntt_mac(ctx, plan, digits, n_cts, D, B, k->d_kb, k->d_ka, acc,
                                  fusion == CKKS_KS_FUSION_XCD)))
             return rc;
     } else {
     NttCall f{plan, digits, digits, n_cts * D, B, B * N, B * N};
     if ((rc = launch_ntt(ctx, &f, 1, true))) return rc;
+    if (!mac_in_inverse) {
     MacArgs ma{};
     ma.digits = digits;
     ma.kb = k->d_kb;
@@ -1346,8 +1351,10 @@ static int key_switch_lane(const ckks_ksk *k, const u64 *x, u64 x_stride, const 
     }
     CK_HIP(hipGetLastError());
     }
+    }
     // 4. inverse NTT of both accumulators
-    NttCall iv{plan, acc, acc, n_cts * 2, B, B * N, B * N};
+    const NttMac mac{digits, k->d_kb, k->d_ka, (u32)D};
+    NttCall iv{plan, acc, acc, n_cts * 2, B, B * N, B * N, mac_in_inverse ? &mac : nullptr};
     if ((rc = launch_ntt(ctx, &iv, 1, false))) return rc;
     // 5. ModDown (or plain finalize) with pass-through adds
     DownArgs da{};
@@ -1549,7 +1556,7 @@ int ckks_autotune_key_switch(ckks_ksk *k, size_t n_cts, int *best_engine, int *b
     if (ctx->capturing) return fail(CKKS_ERR_INVALID, "autotuning is not allowed during graph capture");
     if (n_cts == 0) return fail(CKKS_ERR_INVALID, "autotune needs at least one ciphertext");
     if (ms)
-        for (int i = 0; i < 6; ++i) ms[i] = -1.f;
+        for (int i = 0; i < 8; ++i) ms[i] = -1.f;
     const size_t poly = k->q_limbs * k->degree, per = 2 * poly;
     CK_HIP(hipSetDevice(ctx->device));
     u64 *buf = nullptr;
@@ -1566,8 +1573,9 @@ int ckks_autotune_key_switch(ckks_ksk *k, size_t n_cts, int *best_engine, int *b
         u64 *in = buf, *out = buf + n_cts * per;
         for (int engine = CKKS_BASECONV_VALU; engine <= CKKS_BASECONV_MFMA; ++engine) {
             if (engine == CKKS_BASECONV_MFMA && !k->mfma) continue;
-            for (int fusion = CKKS_KS_FUSION_OFF; fusion <= CKKS_KS_FUSION_LINEAR; ++fusion) {
+            for (int fusion = CKKS_KS_FUSION_OFF; fusion <= CKKS_KS_FUSION_INVERSE; ++fusion) {
                 if (fusion != CKKS_KS_FUSION_OFF && !can_fuse) continue;
+                if (fusion == CKKS_KS_FUSION_INVERSE && k->dnum > 8) continue;
                 ctx->base_conversion = engine;
                 ctx->ks_fusion = fusion;
                 int r = key_switch_core(k, in + poly, per, in, nullptr, per, out, per, n_cts);
@@ -1583,7 +1591,7 @@ int ckks_autotune_key_switch(ckks_ksk *k, size_t n_cts, int *best_engine, int *b
                 float t = 0.f;
                 CK_HIP(hipEventElapsedTime(&t, e0, e1));
                 t /= reps;
-                if (ms) ms[(engine - 1) * 3 + fusion] = t;
+                if (ms) ms[(engine - 1) * 4 + fusion] = t;
                 if (best_ms == 0.f || t < best_ms) {
                     best_ms = t;
                     best_e = engine;

@@ -25,6 +25,7 @@ struct DevNttDesc {
     u64 in_stride, out_stride;
     u32 n_polys, n_limbs, logn, block_start;
     u32 lt_begin;  // first limb transform (limb-major index) covered
+    NttMac mac;    // fused key inner product (k_ntt<.., MAC = true> only)
 };
 
 constexpr int kMaxNttDesc = 16;
@@ -34,9 +35,39 @@ struct NttLaunch {
     u32 n;
 };
 
+// First load of the fused inverse NTT: the key inner product of one
+// (ciphertext, component, limb) at limb offset off.  FP64 limbs sum D terms
+// fp_mulmod(digit, key) (|sum| <= 0.51 D q < 2^44 for D <= 8, inside the
+// range the first inverse FP64 butterflies accept); integer limbs accumulate
+// 128-bit products and reduce once (D q^2 < 2^64 q for q < 2^61).  Digits
+// and key entries are canonical residues.
+struct MacSrc {
+    const u64 *d;  // digit 0 of this (ct, limb); digit j at d + j * d_stride
+    const u64 *k;  // key entry 0 of this (comp, limb); entry j at k + j * k_stride
+    u64 d_stride, k_stride;
+    u32 dnum;
+    Barrett br;
+    double qd, qinv;
+    __device__ __forceinline__ u64 operator()(const TileJob &, u32 off, bool fp) const {
+        if (fp) {
+            double acc = 0.0;
+            for (u32 j = 0; j < dnum; ++j) {
+                const double w = small_to_f64(k[j * k_stride + off]);
+                acc += fp_mulmod(small_to_f64(d[j * d_stride + off]), w, w * qinv, qd);
+            }
+            return f64_as_u64(acc);
+        }
+        Acc128 t;
+        t.zero();
+        for (u32 j = 0; j < dnum; ++j) t.mac(d[j * d_stride + off], k[j * k_stride + off]);
+        return barrett_reduce128(t.hi, t.lo, br);
+    }
+};
+
 // TB = log2 of the two-pass tile (12: 256 threads, 13: 512 threads);
 // BLK = blocked inter-pass layout.  Single-pass launches use <.., 12, false>.
-template <bool FWD, bool FIRST, int TB, bool BLK>
+// MAC: first inverse pass whose input is the descriptor's key inner product.
+template <bool FWD, bool FIRST, int TB, bool BLK, bool MAC = false>
 __global__ __launch_bounds__(1 << (TB - 4)) void k_ntt(NttLaunch launch) {
     __shared__ u64 lds[1 << TB];
     const u32 b = blockIdx.x;
@@ -77,6 +108,24 @@ __global__ __launch_bounds__(1 << (TB - 4)) void k_ntt(NttLaunch launch) {
     u64 regs[16];
     BlockExec ex{1u << (TB - 4)};
     auto regs_of = [&](u32) -> u64(&)[16] { return regs; };
+    if (MAC) {
+        static_assert(!MAC || (!FWD && FIRST), "the key inner product feeds the first inverse pass");
+        const u32 ct = poly >> 1, comp = poly & 1;
+        MacSrc ms;
+        ms.d_stride = (u64)D.n_limbs * N;
+        ms.k_stride = (u64)D.n_limbs * N;
+        ms.d = D.mac.digits + (u64)ct * D.mac.dnum * ms.d_stride + (u64)limb * N;
+        ms.k = (comp ? D.mac.ka : D.mac.kb) + (u64)limb * N;
+        ms.dnum = D.mac.dnum;
+        ms.br = Barrett{P.q, P.br_hi, P.br_lo};
+        ms.qd = P.qd;
+        ms.qinv = P.qinv;
+        if (P.fp)
+            ntt_dispatch<FWD, FIRST, true, TB, BLK>(ex, regs_of, job, lds, ms);
+        else
+            ntt_dispatch<FWD, FIRST, false, TB, BLK>(ex, regs_of, job, lds, ms);
+        return;
+    }
     if (P.fp)
         ntt_dispatch<FWD, FIRST, true, TB, BLK>(ex, regs_of, job, lds);
     else
@@ -84,27 +133,28 @@ __global__ __launch_bounds__(1 << (TB - 4)) void k_ntt(NttLaunch launch) {
 }
 
 template <int TB, bool BLK>
-static void launch_variant(hipStream_t s, bool forward, bool first, const NttLaunch &l, u32 blocks) {
+static void launch_variant(hipStream_t s, bool forward, bool first, const NttLaunch &l, u32 blocks, bool mac) {
     const dim3 g(blocks), t(1u << (TB - 4));
     if (forward) {
         if (first) hipLaunchKernelGGL((k_ntt<true, true, TB, BLK>), g, t, 0, s, l);
         else hipLaunchKernelGGL((k_ntt<true, false, TB, BLK>), g, t, 0, s, l);
     } else {
-        if (first) hipLaunchKernelGGL((k_ntt<false, true, TB, BLK>), g, t, 0, s, l);
+        if (first && mac) hipLaunchKernelGGL((k_ntt<false, true, TB, BLK, true>), g, t, 0, s, l);
+        else if (first) hipLaunchKernelGGL((k_ntt<false, true, TB, BLK>), g, t, 0, s, l);
         else hipLaunchKernelGGL((k_ntt<false, false, TB, BLK>), g, t, 0, s, l);
     }
 }
 
 static void launch_pass(hipStream_t s, bool forward, bool first, const NttLaunch &l, u32 blocks,
-                        int variant = 0) {
+                        int variant = 0, bool mac = false) {
     if (!blocks) return;
     switch (variant) {
-    case CKKS_NTT_VARIANT_BLOCKED: launch_variant<12, true>(s, forward, first, l, blocks); break;
-    case CKKS_NTT_VARIANT_TILE8K: launch_variant<13, false>(s, forward, first, l, blocks); break;
+    case CKKS_NTT_VARIANT_BLOCKED: launch_variant<12, true>(s, forward, first, l, blocks, mac); break;
+    case CKKS_NTT_VARIANT_TILE8K: launch_variant<13, false>(s, forward, first, l, blocks, mac); break;
     case CKKS_NTT_VARIANT_TILE8K | CKKS_NTT_VARIANT_BLOCKED:
-        launch_variant<13, true>(s, forward, first, l, blocks);
+        launch_variant<13, true>(s, forward, first, l, blocks, mac);
         break;
-    default: launch_variant<12, false>(s, forward, first, l, blocks); break;
+    default: launch_variant<12, false>(s, forward, first, l, blocks, mac); break;
     }
 }
 
@@ -130,6 +180,9 @@ int launch_ntt(ckks_context *ctx, const NttCall *calls, size_t count, bool forwa
         if (c.n_polys > 1 && (c.in_stride < c.n_limbs * c.plan->degree ||
                               c.out_stride < c.n_limbs * c.plan->degree))
             return fail(CKKS_ERR_INVALID, "polynomial stride smaller than its limbs");
+        if (c.mac && (forward || count != 1 || !ntt_two_pass(c.plan->logn) || c.n_polys % 2 ||
+                      c.mac->dnum == 0 || c.mac->dnum > 8))
+            return fail(CKKS_ERR_INVALID, "fused key inner product: one inverse two-pass call, 1..8 digits");
         if (ntt_two_pass(c.plan->logn)) {
             total_two += c.n_polys * c.n_limbs * c.plan->degree;
             max_n = std::max(max_n, c.plan->degree);
@@ -146,8 +199,10 @@ int launch_ntt(ckks_context *ctx, const NttCall *calls, size_t count, bool forwa
         d.n_polys = (u32)c.n_polys;
         d.n_limbs = (u32)c.n_limbs;
         d.logn = c.plan->logn;
+        if (c.mac) d.mac = *c.mac;
         return d;
     };
+    const bool mac = calls[0].mac != nullptr;
     auto two_pass = [&](size_t i) { return calls[i].n_polys != 0 && ntt_two_pass(calls[i].plan->logn); };
     // single-pass entries (N <= 4096): one launch
     {
@@ -212,7 +267,7 @@ int launch_ntt(ckks_context *ctx, const NttCall *calls, size_t count, bool forwa
                     lt = 0;
                 }
             }
-            launch_pass(s, forward, true, l, blocks, var);
+            launch_pass(s, forward, true, l, blocks, var, mac);
             launch_pass(s, forward, false, l, blocks, var);
         }
         }

@@ -34,6 +34,18 @@ struct BlockExec {
 };
 #endif
 
+// Input of a pass's first load: the residue at job.src[off], converted for
+// the FP64 path when the pass reads canonical residues.  A fused caller
+// passes its own functor with the same signature (for instance one that
+// computes the value from other operands); it returns the register bits the
+// pass works on.
+struct PlainSrc {
+    CK_HD u64 operator()(const TileJob &job, u32 off, bool fp_initial) const {
+        const u64 x = job.src[off];
+        return fp_initial ? f64_as_u64((double)x) : x;
+    }
+};
+
 // BLK: the inter-pass buffer uses the blocked layout (internal_offset).
 template <int L, int CL, PassKind KIND, bool FWD, bool FP, bool BLK = false>
 struct NttPass {
@@ -93,11 +105,6 @@ struct NttPass {
     // between passes the FP64 path keeps its lazy doubles as raw bits).
     static constexpr bool INITIAL = (KIND == PASS_FULL) || (FWD ? KIND == PASS_COL : KIND == PASS_ROW);
 
-    CK_HD static u64 load_conv(u64 x) {
-        if (FP && INITIAL) return f64_as_u64((double)x);
-        return x;
-    }
-
     // lds_index is linear: thread part (computed once) ^ register part
     // (a compile-time constant per unrolled i).
     template <int I>
@@ -140,9 +147,9 @@ struct NttPass {
         return pa;
     }
 
-    template <class Exec, class Regs>
-    CK_HD static void run(Exec &ex, Regs &&regs_of, const TileJob &job, u64 *lds) {
-        run_impl<false>(ex, regs_of, job, lds);
+    template <class Exec, class Regs, class Src = PlainSrc>
+    CK_HD static void run(Exec &ex, Regs &&regs_of, const TileJob &job, u64 *lds, const Src &src = Src{}) {
+        run_impl<false>(ex, regs_of, job, lds, src);
     }
 
     // Forward row pass (and forward full pass) that stops with the tile's
@@ -154,11 +161,11 @@ struct NttPass {
     template <class Exec, class Regs>
     CK_HD static void run_to_lds(Exec &ex, Regs &&regs_of, const TileJob &job, u64 *lds) {
         static_assert(FWD && KIND != PASS_COL, "only passes that stage their output");
-        run_impl<true>(ex, regs_of, job, lds);
+        run_impl<true>(ex, regs_of, job, lds, PlainSrc{});
     }
 
-    template <bool TO_LDS, class Exec, class Regs>
-    CK_HD static void run_impl(Exec &ex, Regs &&regs_of, const TileJob &job, u64 *lds) {
+    template <bool TO_LDS, class Exec, class Regs, class Src>
+    CK_HD static void run_impl(Exec &ex, Regs &&regs_of, const TileJob &job, u64 *lds, const Src &src) {
         const PassAddr pa = pass_addr(job);
         const u32 T = S::THREADS;
         const u32 n = job.logn;
@@ -183,7 +190,7 @@ struct NttPass {
                         off = row_natural_offset<L, CL>(pa, e);
                     }
                     CK_BOUNDS(off, 1u << n);
-                    lds[lds_index<L, CL>(e)] = load_conv(job.src[off]);
+                    lds[lds_index<L, CL>(e)] = src(job, off, FP && INITIAL);
                 }
             });
             ex.barrier();
@@ -200,7 +207,7 @@ struct NttPass {
                 for (int i = 0; i < 16; ++i) {
                     u32 off = read_off(pa, coord(g, tid, i));
                     CK_BOUNDS(off, 1u << n);
-                    v[i] = load_conv(job.src[off]);
+                    v[i] = src(job, off, FP && INITIAL);
                 }
             });
         }
@@ -290,12 +297,13 @@ CK_HD u32 ntt_tiles(u32 logn, u32 tb = 12) { return logn > 12 ? (1u << (logn - t
 // transform (forward: column/full, inverse: row/full).  TB = log2 of the
 // two-pass tile size (12: 256 threads, 13: 512 threads; the full pass for
 // N <= 4096 always uses 256 threads); BLK = blocked inter-pass layout.
-template <bool FWD, bool FIRST, bool FP, int TB = 12, bool BLK = false, class Exec, class Regs>
-CK_HD void ntt_dispatch(Exec &ex, Regs &&regs_of, const TileJob &job, u64 *lds) {
+template <bool FWD, bool FIRST, bool FP, int TB = 12, bool BLK = false, class Exec, class Regs,
+          class Src = PlainSrc>
+CK_HD void ntt_dispatch(Exec &ex, Regs &&regs_of, const TileJob &job, u64 *lds, const Src &src = Src{}) {
     switch (job.logn) {
 #define CK_FULL(LG)                                                   \
     case LG:                                                          \
-        if (FIRST) NttPass<LG, 0, PASS_FULL, FWD, FP>::run(ex, regs_of, job, lds); \
+        if (FIRST) NttPass<LG, 0, PASS_FULL, FWD, FP>::run(ex, regs_of, job, lds, src); \
         break;
         CK_FULL(10)
         CK_FULL(11)
@@ -305,9 +313,9 @@ CK_HD void ntt_dispatch(Exec &ex, Regs &&regs_of, const TileJob &job, u64 *lds) 
     case LG: {                                                                              \
         constexpr int N1 = LG - (LG + 1) / 2, N2 = (LG + 1) / 2;                            \
         if (FWD == FIRST)                                                                   \
-            NttPass<N1, TB - N1, PASS_COL, FWD, FP, BLK>::run(ex, regs_of, job, lds);       \
+            NttPass<N1, TB - N1, PASS_COL, FWD, FP, BLK>::run(ex, regs_of, job, lds, src);  \
         else                                                                                \
-            NttPass<N2, TB - N2, PASS_ROW, FWD, FP, BLK>::run(ex, regs_of, job, lds);       \
+            NttPass<N2, TB - N2, PASS_ROW, FWD, FP, BLK>::run(ex, regs_of, job, lds, src);  \
         break;                                                                              \
     }
         CK_TWO(13)

@@ -88,6 +88,8 @@ int ckks_get_ntt_variant(const ckks_context *ctx, uint32_t logn, int *variant);
  * CKKS_KS_FUSION_XCD / CKKS_KS_FUSION_LINEAR: the digits' row pass is
  * fused with the MAC (the NTT-domain digits never reach HBM), with the
  * blocks sharing a key tile placed on one XCD or in plain order.
+ * CKKS_KS_FUSION_INVERSE: the MAC is computed inside the first inverse NTT
+ * pass (the NTT-domain accumulators never reach HBM; dnum <= 8).
  * CKKS_KS_FUSION_AUTO (default): the choice ckks_autotune_key_switch made
  * for the key, OFF for an untuned key.  Identical residues in every mode.
  * No reference counterpart. */
@@ -95,7 +97,8 @@ enum {
     CKKS_KS_FUSION_AUTO = -1,
     CKKS_KS_FUSION_OFF = 0,
     CKKS_KS_FUSION_XCD = 1,
-    CKKS_KS_FUSION_LINEAR = 2
+    CKKS_KS_FUSION_LINEAR = 2,
+    CKKS_KS_FUSION_INVERSE = 3
 };
 int ckks_set_key_switch_fusion(ckks_context *ctx, int mode);
 
@@ -350,7 +353,7 @@ int ckks_multiply_relinearize_rescale(const ckks_ksk *relin_key, const uint64_t 
  * ciphertexts allocated and freed here; synchronises the context stream)
  * and records the fastest pair in the key for CKKS_BASECONV_AUTO and
  * CKKS_KS_FUSION_AUTO; then times that pair with the batch split over two
- * streams and records the faster for CKKS_KS_SPLIT_AUTO.  ms (optional, 2 x 3 floats) receives the times as
+ * streams and records the faster for CKKS_KS_SPLIT_AUTO.  ms (optional, 2 x 4 floats) receives the times as
  * ms[engine - 1][fusion mode], engine VALU = 1 / MFMA = 2, -1 where a pair
  * does not apply.  Not callable during graph capture.  No reference
  * counterpart. */

@@ -89,8 +89,8 @@ def test_autotune_key_switch_engine(pkg, ctx):
     ctx.set_base_conversion(ctx.BASECONV_AUTO)
     engine, fusion, times = key.autotune(2)
     assert engine in (ctx.BASECONV_VALU, ctx.BASECONV_MFMA)
-    assert fusion in (ctx.KS_FUSION_OFF, ctx.KS_FUSION_XCD, ctx.KS_FUSION_LINEAR)
-    assert len(times) == 6 and all(t > 0 for t in times.values())
+    assert fusion in (ctx.KS_FUSION_OFF, ctx.KS_FUSION_XCD, ctx.KS_FUSION_LINEAR, ctx.KS_FUSION_INVERSE)
+    assert len(times) == 8 and all(t > 0 for t in times.values())
     assert times[(engine, fusion)] == min(times.values())
     got = pkg.to_host(key.switch(pkg.to_device(ct)))
     assert (got == want).all()
@@ -98,8 +98,9 @@ def test_autotune_key_switch_engine(pkg, ctx):
 
 @pytest.mark.parametrize("logn,variant", [(13, 0), (14, 1), (15, 2), (16, 3)])
 def test_fused_ntt_mac_matches_unfused(pkg, ctx, logn, variant):
-    """The key switch with the digits' row pass fused into the key MAC gives
-    the residues of the separate NTT + MAC kernels, for every NTT variant."""
+    """The key switch with the key MAC fused into the digits' forward row pass,
+    or into the accumulators' first inverse pass, gives the residues of the
+    separate NTT + MAC kernels, for every NTT variant."""
     n, L, K, blocks = 1 << logn, 5, 2, [3, 2]
     q_moduli = [Q0[0]] + Q40[:L - 1]
     qp = q_moduli + P60[:K]
@@ -113,7 +114,7 @@ def test_fused_ntt_mac_matches_unfused(pkg, ctx, logn, variant):
         ctx.set_key_switch_fusion(ctx.KS_FUSION_OFF)
         want = pkg.to_host(key.apply_galois(ct, k))
         got = {}
-        for mode in (ctx.KS_FUSION_XCD, ctx.KS_FUSION_LINEAR):
+        for mode in (ctx.KS_FUSION_XCD, ctx.KS_FUSION_LINEAR, ctx.KS_FUSION_INVERSE):
             ctx.set_key_switch_fusion(mode)
             got[mode] = pkg.to_host(key.apply_galois(ct, k))
     finally:
