@@ -219,7 +219,7 @@ __device__ void modup_thread(const ModUpArgs &a, const CK_CONST_AS ModUpBlock *b
     for (u32 i = 0; i < a.limbs; ++i) {
         if (i - start < (u32)A) continue;  // block limbs are stored above
         u64 v;
-        if ((fpm >> i) & 1) {
+        if (i < 64 && ((fpm >> i) & 1)) {  // the mask is empty for bases over 64 limbs
             const CK_CONST_AS double *r = tf + (u64)i * kFpRow;
             const double q = r[0];
             double acc = fp_mulmod(ch0, r[2], r[3], q);
@@ -520,7 +520,7 @@ __global__ __launch_bounds__(kKsThreads) void k_moddown(DownArgs a) {
     const CK_CONST_AS u64 *pinv_q = const_table(md->pinv_q);
     const u64 fpm = a.fp_targets;
     for (u32 i = 0; i < a.q_limbs; ++i) {
-        if ((fpm >> i) & 1) {
+        if (i < 64 && ((fpm >> i) & 1)) {  // the mask is empty for bases over 64 limbs
             const CK_CONST_AS double *r = tf + (u64)i * kMdRow;
             const double q = r[0];
             double acc = fp_mulmod(cl[0], r[6], r[7], q) + fp_mulmod(ch[0], r[8], r[9], q);
