@@ -42,9 +42,10 @@ struct ModUpBlock {
     // evaluation of the mixed-radix value at every basis limb i:
     // w[i * kMaxAlpha + m] = (q_start ... q_{start+m-1}) mod q_i
     const u64 *w;
-    // FP64 mode (fp_mode = 1: every block prime after the first is below
-    // 2^41).  Digit 0 enters split at bit 30 (c_0 = ch 2^30 + cl, both below
-    // 2^45 as fp_mulmod requires); every other digit is below 2^41.
+    // FP64 modes (fp_mode 1 / 2: every block prime after the first is below
+    // 2^41; mode 2 when the first is not, and digit 0 enters split at bit 30,
+    // c_0 = ch 2^30 + cl, both below 2^45 as fp_mulmod requires); every
+    // other digit is below 2^41.
     // Garner row gf[k] (k >= 1, q_k < 2^41): q_k, 1/q_k, inv[k], fl(inv[k] / q_k),
     // pm[k][0] 2^30 mod q_k, its quotient, then (pm[k][m], fl(pm[k][m] / q_k))
     // for m < k.  Target row tf + i * kFpRow (q_i < 2^41): q_i, 1/q_i,
@@ -146,12 +147,13 @@ __device__ __forceinline__ Barrett barrett_at(const CK_CONST_AS DevPrime *p, u32
 // Integer mode: Garner digits and every target by 128-bit accumulation and
 // Barrett reduction.
 //
-// FP64 mode (blk->fp_mode): Garner digits c_k (k >= 1) and the targets whose
-// bit is set in a.fp_targets (q_i < 2^41) as sums of fp_mulmod terms, each
-// |term| <= 0.51 q, at most kMaxAlpha + 1 of them, so every sum stays below
-// 2^45 and is exact; fp_canon / fp_residue make it canonical.  Digit 0 is
-// x_0 itself, split at bit 30; the other digits are below 2^41.
-template <int A, bool FP>
+// FP64 modes (MODE = blk->fp_mode = 1 or 2): Garner digits c_k (k >= 1) and
+// the targets whose bit is set in a.fp_targets (q_i < 2^41) as sums of
+// fp_mulmod terms, each |term| <= 0.51 q, at most kMaxAlpha + 1 of them, so
+// every sum stays below 2^45 and is exact; fp_canon / fp_residue make it
+// canonical.  Digit 0 is x_0 itself: whole in mode 1 (first block prime below
+// 2^41), split at bit 30 in mode 2; the other digits are below 2^41.
+template <int A, int MODE>
 __device__ void modup_thread(const ModUpArgs &a, const CK_CONST_AS ModUpBlock *blk, u32 ct, u32 j, u64 n) {
     const u64 N = (u64)1 << a.logn;
     const CK_CONST_AS DevPrime *primes = const_table(a.primes);
@@ -168,7 +170,7 @@ __device__ void modup_thread(const ModUpArgs &a, const CK_CONST_AS ModUpBlock *b
         dst[(u64)(start + m) * N] = x[m];
     }
     const CK_CONST_AS u64 *wt = const_table(blk->w);
-    if (!FP) {
+    if (MODE == 0) {
         u64 c[A];
 #pragma unroll
         for (int k = 0; k < A; ++k) {
@@ -202,14 +204,16 @@ __device__ void modup_thread(const ModUpArgs &a, const CK_CONST_AS ModUpBlock *b
         }
         return;
     }
-    double cl[A];  // cl[0]: low 30 bits of c_0 = x_0
-    const double ch0 = small_to_f64(x[0] >> 30);
-    cl[0] = small_to_f64(x[0] & 0x3fffffffull);
+    constexpr bool SPLIT0 = MODE == 2;
+    double cl[A];  // cl[0]: c_0 = x_0, or its low 30 bits when split
+    const double ch0 = SPLIT0 ? small_to_f64(x[0] >> 30) : 0.0;
+    cl[0] = small_to_f64(SPLIT0 ? x[0] & 0x3fffffffull : x[0]);
 #pragma unroll
     for (int k = 1; k < A; ++k) {
         const CK_CONST_AS double *g = blk->gf[k];
         const double q = g[0];
-        double t = small_to_f64(x[k]) - (fp_mulmod(cl[0], g[6], g[7], q) + fp_mulmod(ch0, g[4], g[5], q));
+        double t = small_to_f64(x[k]) - fp_mulmod(cl[0], g[6], g[7], q);
+        if (SPLIT0) t -= fp_mulmod(ch0, g[4], g[5], q);
 #pragma unroll
         for (int m = 1; m < k; ++m) t -= fp_mulmod(cl[m], g[6 + 2 * m], g[7 + 2 * m], q);
         cl[k] = fp_canon(fp_mulmod(t, g[2], g[3], q), q, g[1]);
@@ -222,9 +226,10 @@ __device__ void modup_thread(const ModUpArgs &a, const CK_CONST_AS ModUpBlock *b
         if (i < 64 && ((fpm >> i) & 1)) {  // the mask is empty for bases over 64 limbs
             const CK_CONST_AS double *r = tf + (u64)i * kFpRow;
             const double q = r[0];
-            double acc = fp_mulmod(ch0, r[2], r[3], q);
+            double acc = fp_mulmod(cl[0], r[4], r[5], q);
+            if (SPLIT0) acc += fp_mulmod(ch0, r[2], r[3], q);
 #pragma unroll
-            for (int m = 0; m < A; ++m) acc += fp_mulmod(cl[m], r[4 + 2 * m], r[5 + 2 * m], q);
+            for (int m = 1; m < A; ++m) acc += fp_mulmod(cl[m], r[4 + 2 * m], r[5 + 2 * m], q);
             v = fp_residue(acc, q, r[1]);
         } else {
             const Barrett bi = barrett_at(primes, i);
@@ -243,17 +248,17 @@ __device__ void modup_thread(const ModUpArgs &a, const CK_CONST_AS ModUpBlock *b
 // One instantiation per block size A (and mode), so each gets its own
 // register budget; a launch covers the consecutive digits
 // [j_first, j_first + n_j) of one size.
-template <int A, bool FP>
+template <int A, int MODE>
 __global__ __launch_bounds__(kKsThreads) void k_modup(ModUpArgs a, u32 j_first, u32 n_j) {
     const u32 b = xcd_contiguous_block(blockIdx.x, gridDim.x);
     const u32 chunk = b % a.chunks;
     const u32 cj = b / a.chunks;
     const u32 j = j_first + cj % n_j, ct = cj / n_j;
     const u64 n = (u64)chunk * kKsThreads + threadIdx.x;
-    modup_thread<A, FP>(a, const_table(a.blocks) + j, ct, j, n);
+    modup_thread<A, MODE>(a, const_table(a.blocks) + j, ct, j, n);
 }
 
-// sizes / fp: per digit, its block size and whether it runs in FP64 mode.
+// sizes / fp: per digit, its block size and its mode (0 integer, 1 / 2 FP64).
 static int launch_modup(hipStream_t s, const ModUpArgs &mu, const std::vector<size_t> &sizes,
                         const std::vector<int> &fp, size_t n_cts) {
     for (size_t j0 = 0; j0 < sizes.size();) {
@@ -261,12 +266,13 @@ static int launch_modup(hipStream_t s, const ModUpArgs &mu, const std::vector<si
         while (j1 < sizes.size() && sizes[j1] == sizes[j0] && fp[j1] == fp[j0]) ++j1;
         const dim3 g((u32)(n_cts * (j1 - j0) * mu.chunks));
         const u32 jf = (u32)j0, nj = (u32)(j1 - j0);
-        const bool f = fp[j0] != 0;
+        const int mode = fp[j0];
         switch (sizes[j0]) {
 #define CK_MU(A)                                                                                  \
     case A:                                                                                       \
-        if (f) hipLaunchKernelGGL((k_modup<A, true>), g, dim3(kKsThreads), 0, s, mu, jf, nj);    \
-        else hipLaunchKernelGGL((k_modup<A, false>), g, dim3(kKsThreads), 0, s, mu, jf, nj);     \
+        if (mode == 1) hipLaunchKernelGGL((k_modup<A, 1>), g, dim3(kKsThreads), 0, s, mu, jf, nj); \
+        else if (mode == 2) hipLaunchKernelGGL((k_modup<A, 2>), g, dim3(kKsThreads), 0, s, mu, jf, nj); \
+        else hipLaunchKernelGGL((k_modup<A, 0>), g, dim3(kKsThreads), 0, s, mu, jf, nj);         \
         break;
             CK_MU(1) CK_MU(2) CK_MU(3) CK_MU(4) CK_MU(5) CK_MU(6)
             CK_MU(7) CK_MU(8) CK_MU(9) CK_MU(10) CK_MU(11) CK_MU(12)
@@ -1002,9 +1008,11 @@ static int create_key(const ckks_rns_plan *plan, size_t q_limbs, const size_t *b
         mb.size = (u32)block_sizes[j];
         std::vector<u64> bm(mods.begin() + start, mods.begin() + start + block_sizes[j]);
         fill_garner(bm, mb.pm, mb.inv, mb.inv_q, 0);
-        // FP64 mode: every block prime after the first below 2^41
-        mb.fp_mode = B <= 64 ? 1u : 0u;
-        for (size_t kk = 1; kk < bm.size(); ++kk) mb.fp_mode &= fp_prime(bm[kk]) ? 1u : 0u;
+        // FP64 modes: every block prime after the first below 2^41; mode 2
+        // when the first one is not (its digit is split at bit 30)
+        bool fp_ok = B <= 64;
+        for (size_t kk = 1; kk < bm.size(); ++kk) fp_ok = fp_ok && fp_prime(bm[kk]);
+        mb.fp_mode = fp_ok ? (fp_prime(bm[0]) ? 1u : 2u) : 0u;
         if (mb.fp_mode) {
             for (size_t kk = 1; kk < bm.size(); ++kk) {
                 const u64 qk = bm[kk];

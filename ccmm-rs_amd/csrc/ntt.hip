@@ -37,10 +37,10 @@ struct NttLaunch {
 
 // First load of the fused inverse NTT: the key inner product of one
 // (ciphertext, component, limb) at limb offset off.  FP64 limbs sum D terms
-// fp_mulmod(digit, key) (|sum| <= 0.51 D q < 2^44 for D <= 8, inside the
-// range the first inverse FP64 butterflies accept); integer limbs accumulate
-// 128-bit products and reduce once (D q^2 < 2^64 q for q < 2^61).  Digits
-// and key entries are canonical residues.
+// fp_mulmod(digit, key) (|sum| <= 0.51 D q < 2^44 for D <= 8) and reduce the
+// sum into 0.51 q, the input range of the inverse FP64 butterflies; integer
+// limbs accumulate 128-bit products and reduce once (D q^2 < 2^64 q for
+// q < 2^61).  Digits and key entries are canonical residues.
 struct MacSrc {
     const u64 *d;  // digit 0 of this (ct, limb); digit j at d + j * d_stride
     const u64 *k;  // key entry 0 of this (comp, limb); entry j at k + j * k_stride
@@ -55,7 +55,7 @@ struct MacSrc {
                 const double w = small_to_f64(k[j * k_stride + off]);
                 acc += fp_mulmod(small_to_f64(d[j * d_stride + off]), w, w * qinv, qd);
             }
-            return f64_as_u64(acc);
+            return f64_as_u64(fp_reduce(acc, qd, qinv));
         }
         Acc128 t;
         t.zero();

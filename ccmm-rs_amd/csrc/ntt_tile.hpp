@@ -192,8 +192,11 @@ CK_HD void gs_bfly(u64 &x, u64 &y, const Tw &w, u64 q) {
 
 // FP64 butterflies on bit-cast doubles (see fp_mulmod).  Forward: no
 // reduction; |x| grows by at most 0.51 q per stage, so after 17 stages it is
-// below 10 q < 2^46 (q < 2^42).  Inverse: the sum path is reduced each stage
-// so both outputs stay within 0.51 q.
+// below 10 q < 2^46 (q < 2^42).  Inverse: the product output is within
+// 0.51 q; the sum doubles per stage and is reduced (into 0.51 q) only at the
+// last stage of each round (REDUCE), i.e. after at most 3 lazy stages: from
+// inputs below q, the reducing stage sees |x|, |y| < 8 q, so x - y and x + y
+// stay below 16 q < 2^45 (q < 2^41), inside fp_mulmod's and fp_reduce's range.
 CK_HD void ct_bfly_fp(u64 &x, u64 &y, const Tw &w, double q) {
     double a = u64_as_f64(x);
     double t = fp_mulmod(u64_as_f64(y), u64_as_f64(w.w), u64_as_f64(w.wq), q);
@@ -201,9 +204,10 @@ CK_HD void ct_bfly_fp(u64 &x, u64 &y, const Tw &w, double q) {
     y = f64_as_u64(a - t);
 }
 
+template <bool REDUCE>
 CK_HD void gs_bfly_fp(u64 &x, u64 &y, const Tw &w, double q, double qinv) {
     double a = u64_as_f64(x), b = u64_as_f64(y);
-    x = f64_as_u64(fp_reduce(a + b, q, qinv));
+    x = f64_as_u64(REDUCE ? fp_reduce(a + b, q, qinv) : a + b);
     y = f64_as_u64(fp_mulmod(a - b, u64_as_f64(w.w), u64_as_f64(w.wq), q));
 }
 
@@ -317,8 +321,10 @@ CK_HD void inv_round(u64 (&v)[16], const Geom &g, u32 tid, int u0, bool row_base
             u32 idx = (base << (L - 1 - u)) + (t >> (u + 1));
             CK_BOUNDS(idx, 1u << pa.logn);
             Tw w = job.tw[idx];
-            if (FP)
-                gs_bfly_fp(v[i], v[i | (1 << rb)], w, job.qd, job.qinv);
+            if (FP && j == K - 1)
+                gs_bfly_fp<true>(v[i], v[i | (1 << rb)], w, job.qd, job.qinv);
+            else if (FP)
+                gs_bfly_fp<false>(v[i], v[i | (1 << rb)], w, job.qd, job.qinv);
             else
                 gs_bfly(v[i], v[i | (1 << rb)], w, job.q);
         }

@@ -171,3 +171,26 @@ def test_fp64_moddown_target_exact(emu, q):
                         xp = sum(c * ww for c, ww in zip(digits, w))
                         want = ((x - xp + neg * pmod) * pinv + (-add if add_neg else add)) % q
                         assert got == want
+
+
+@pytest.mark.parametrize("logn", [12, 16, 17])
+def test_emulated_fp64_inverse_extremes(emu, logn):
+    """The inverse FP64 butterflies reduce the sum only once per round: pinned
+    with the largest NTT prime below 2^41 (the FP64 bound) and inputs at q - 1
+    and 0, whose sums grow fastest, against the oracle."""
+    emu.emu_ntt_variant.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32, C.c_int,
+                                    C.c_int, C.c_int]
+    n = 1 << logn
+    q = O.ntt_primes(41, n, 1)[0]
+    assert (1 << 40) < q < (1 << 41)
+    plan = O.Plan(q, n)
+    rng = np.random.default_rng(logn + 7)
+    cases = [np.full(n, q - 1, dtype=np.uint64),
+             np.where(np.arange(n) % 2 == 0, q - 1, 0).astype(np.uint64),
+             rng.integers(q - 64, q, n, dtype=np.uint64)]
+    for y in cases:
+        back = np.zeros(n, dtype=np.uint64)
+        assert emu.emu_ntt(y.ctypes.data, back.ctypes.data, q, logn, 0) == 0
+        assert (back == plan.inverse(y)).all()
+        assert emu.emu_ntt_variant(y.ctypes.data, back.ctypes.data, q, logn, 0, 13 if logn >= 14 else 12, 1) == 0
+        assert (back == plan.inverse(y)).all()
