@@ -106,10 +106,6 @@ CK_HD u64 reduce128(const Acc128 &a, const DevPrime &P) {
     return barrett_reduce128(a.hi, a.lo, Barrett{P.q, P.br_hi, P.br_lo});
 }
 
-CK_HD u64 reduce64(u64 x, const DevPrime &P) {
-    return x >= P.q ? barrett_reduce128(0, x, Barrett{P.q, P.br_hi, P.br_lo}) : x;
-}
-
 // ---- 1. ModUp -------------------------------------------------------------------
 
 struct ModUpArgs {
