@@ -108,6 +108,9 @@ _SIGNATURES = {
     "ckks_decrypt_raw": (C.c_int, [_p, _p, _p, _p, _p, _sz, _sz, _sz, _sz]),
     "ckks_bounded_digit_count": (C.c_int, [_p, _sz, C.c_uint, C.POINTER(_sz)]),
     "ckks_bounded_decompose": (C.c_int, [_p, _sz, C.c_uint, _p, _p, _sz, _sz]),
+    "ckks_bounded_digits": (C.c_int, [_p, _sz, C.c_uint, _p, _p, _sz, _sz]),
+    "ckks_bounded_digits_automorphism": (C.c_int, [_p, _sz, C.c_uint, _p, _p, _sz, _u64]),
+    "ckks_bounded_apply_galois_hoisted": (C.c_int, [_p, _u64, _p, _p, _p, _sz, _sz, _sz]),
     "ckks_ksk_destroy": (C.c_int, [_p]),
     "ckks_key_switch": (C.c_int, [_p, _p, _p, _sz, _sz, _sz]),
     "ckks_apply_galois": (C.c_int, [_p, _u64, _p, _p, _sz, _sz, _sz]),
@@ -608,6 +611,29 @@ class BoundedKeySwitchKey(_KeySwitchKey):
         self.digit_count = count
         self._finish_init(plan, h)
 
+    def digits(self, ct):
+        """Balanced digits [ct][digit_count][N] (int64) of the a components of
+        a ciphertext batch [ct][2][q_limbs][N], for apply_galois_hoisted."""
+        torch = _torch()
+        n = self._n_cts(ct, 2)
+        poly = self.q_limbs * self.plan.degree
+        assert ct.is_contiguous()
+        out = torch.empty((n, self.digit_count, self.plan.degree), dtype=torch.int64, device=ct.device)
+        # a component of ciphertext c at element (2 c + 1) * poly
+        _check(lib().ckks_bounded_digits(self.plan.handle, self.q_limbs, self.base_log, ct.data_ptr() + 8 * poly,
+                                         _ptr(out), n, 2 * poly))
+        return out
+
+    def apply_galois_hoisted(self, ct, exponent: int, a_digits, out=None):
+        """Rotation from hoisted digits (grafting/bounded_gadget.rs:176-312):
+        same residues as apply_galois; a_digits = self.digits(ct)."""
+        n = self._n_cts(ct, 2)
+        assert a_digits.dtype == _torch().int64 and a_digits.numel() == n * self.digit_count * self.plan.degree
+        out = empty((n, 2, self.q_limbs, self.plan.degree)) if out is None else out
+        _check(lib().ckks_bounded_apply_galois_hoisted(self._h, int(exponent), _ptr(ct), _ptr(a_digits), _ptr(out),
+                                                       n, 0, 0))
+        return out
+
 
 def bounded_digit_count(plan: PreparedRnsNttPlan, q_limbs: int, base_log: int) -> int:
     """BoundedGadgetLayout::new digit count over the first q_limbs limbs."""
@@ -625,6 +651,29 @@ def bounded_decompose(plan: PreparedRnsNttPlan, q_limbs: int, base_log: int, x, 
     out = empty((polys, count, len(plan.moduli), n)) if out is None else out
     _check(lib().ckks_bounded_decompose(plan.handle, int(q_limbs), int(base_log), _ptr(x), _ptr(out),
                                         polys, 0))
+    return out
+
+
+def bounded_digits(plan: PreparedRnsNttPlan, q_limbs: int, base_log: int, x):
+    """BoundedGadgetLayout::decompose on the device: x [polys][q_limbs][N]
+    -> int64 digits [polys][digit_count][N]."""
+    torch = _torch()
+    n = plan.degree
+    polys = x.numel() // (q_limbs * n)
+    count = bounded_digit_count(plan, q_limbs, base_log)
+    out = torch.empty((polys, count, n), dtype=torch.int64, device=x.device)
+    _check(lib().ckks_bounded_digits(plan.handle, int(q_limbs), int(base_log), _ptr(x), _ptr(out), polys, 0))
+    return out
+
+
+def bounded_digits_automorphism(plan: PreparedRnsNttPlan, q_limbs: int, base_log: int, digits, exponent: int):
+    """BoundedGadgetDecomposition::automorphism_hoisted on the device."""
+    n = plan.degree
+    count = bounded_digit_count(plan, q_limbs, base_log)
+    polys = digits.numel() // (count * n)
+    out = _torch().empty_like(digits)
+    _check(lib().ckks_bounded_digits_automorphism(plan.handle, int(q_limbs), int(base_log), _ptr(digits),
+                                                  _ptr(out), polys, int(exponent)))
     return out
 
 

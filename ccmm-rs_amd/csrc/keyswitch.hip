@@ -303,7 +303,8 @@ struct BoundedArgs {
     const u64 *in;  // component: limb i of ct c at in + c*in_stride + i*N
     u64 in_stride;
     u64 kinv;       // nonzero: decompose sigma_k(in) (rotation fused in)
-    u64 *digits;    // [ct][D][B][N]
+    u64 *digits;    // [ct][D][B][N] lifted digits, or
+    int64_t *raw;   // [ct][D][N] the balanced digits themselves (when non-null)
     BoundedTable t;
     const DevPrime *primes;  // key basis B
     u32 q_limbs, limbs, logn, digit_count, base_log;
@@ -355,11 +356,69 @@ __global__ __launch_bounds__(kKsThreads) void k_bounded_decompose(BoundedArgs a)
                 d = (int64_t)t;
                 carry = 0;
             }
-            u64 *o = out + (u64)j * a.limbs * N;
-            for (u32 t2 = 0; t2 < a.limbs; ++t2) o[(u64)t2 * N] = lift_signed(d, a.primes[t2]);
+            if (a.raw) {
+                a.raw[(ct * a.digit_count + j) * N + n] = d;
+            } else {
+                u64 *o = out + (u64)j * a.limbs * N;
+                for (u32 t2 = 0; t2 < a.limbs; ++t2) o[(u64)t2 * N] = lift_signed(d, a.primes[t2]);
+            }
             ++j;
         }
     }
+}
+
+// Hoisted digit automorphism (BoundedGadgetDecomposition::automorphism_hoisted,
+// grafting/bounded_gadget.rs:273-312) as a gather: output coefficient n takes
+// the digits of source g = n k^-1 mod 2N (g - N, negated, when g >= N).  A
+// negated digit vector is renormalised into [-2^(w-1), 2^(w-1)) from the
+// lowest digit up with a carry, which is what makes the result equal to the
+// balanced decomposition of sigma_k(x); the digit count leaves one bit of
+// headroom above the centered value, so the final carry is zero.  Writes the
+// digits themselves (raw) or lifts them to every limb (lifted).
+struct HoistArgs {
+    const int64_t *digits;  // [ct][D][N] balanced digits of the unrotated component
+    int64_t *raw;           // [ct][D][N] output digits, or
+    u64 *lifted;            // [ct][D][B][N] output digits lifted to every limb
+    const DevPrime *primes;
+    u64 kinv;  // k^-1 mod 2N
+    u32 limbs, logn, digit_count, base_log;
+    u64 total;  // n_cts * N
+};
+
+__global__ __launch_bounds__(kKsThreads) void k_bounded_hoist(HoistArgs a) {
+    const u64 gid = (u64)blockIdx.x * kKsThreads + threadIdx.x;
+    if (gid >= a.total) return;
+    const u64 N = (u64)1 << a.logn;
+    const u64 ct = gid >> a.logn, n = gid & (N - 1);
+    const Gather gi = galois_gather(n, a.kinv, a.logn);
+    const int64_t *src = a.digits + ct * a.digit_count * N + gi.idx;
+    const u32 w = a.base_log;
+    const __int128 half = (__int128)1 << (w - 1);
+    const __int128 mask = ((__int128)1 << w) - 1;
+    int64_t carry = 0;
+    for (u32 j = 0; j < a.digit_count; ++j) {
+        int64_t d = src[(u64)j * N];
+        if (gi.neg) {
+            // value in [-2^(w-1) - 1, 2^(w-1) + 1]; balanced remainder and carry
+            const __int128 x = (__int128)carry - d + half;
+            d = (int64_t)((x & mask) - half);
+            carry = (int64_t)(x >> w);
+        }
+        if (a.raw) {
+            a.raw[(ct * a.digit_count + j) * N + n] = d;
+        } else {
+            u64 *o = a.lifted + ((ct * a.digit_count + j) * a.limbs) * N + n;
+            for (u32 t = 0; t < a.limbs; ++t) o[(u64)t * N] = lift_signed(d, a.primes[t]);
+        }
+    }
+}
+
+static int launch_hoist(hipStream_t s, const HoistArgs &a) {
+    if (a.total == 0) return CKKS_OK;
+    hipLaunchKernelGGL(k_bounded_hoist, dim3((u32)((a.total + kKsThreads - 1) / kKsThreads)), dim3(kKsThreads), 0,
+                       s, a);
+    CK_HIP(hipGetLastError());
+    return CKKS_OK;
 }
 
 struct MacArgs {
@@ -1233,19 +1292,22 @@ static int create_bounded_key(const ckks_rns_plan *plan, size_t q_limbs, unsigne
 // (add0/add1 may be null) over the key's Q limbs.
 static int key_switch_lane(const ckks_ksk *k, const u64 *x, u64 x_stride, const u64 *add0,
                            const u64 *add1, u64 add_stride, u64 *out, u64 out_stride, size_t n_cts,
-                           u64 galois_kinv);
+                           u64 galois_kinv, const int64_t *hoisted);
 
 // Runs the key switch on one lane, or splits the batch over the context
 // stream (first half) and the auxiliary stream (second half): fork with an
 // event recorded on the context stream, join by making the context stream
 // wait for the auxiliary one.  Each lane has its own workspaces.
+// hoisted: bounded keys only; the balanced digits [ct][D][N] of the unrotated
+// component x (ckks_bounded_digits), which replace its decomposition.
 static int key_switch_core(const ckks_ksk *k, const u64 *x, u64 x_stride, const u64 *add0,
                            const u64 *add1, u64 add_stride, u64 *out, u64 out_stride, size_t n_cts,
-                           u64 galois_kinv = 0) {
+                           u64 galois_kinv = 0, const int64_t *hoisted = nullptr) {
     ckks_context *ctx = k->ctx;
     const int split = ctx->ks_split == CKKS_KS_SPLIT_AUTO ? k->tuned_split : ctx->ks_split;
     if (split != CKKS_KS_SPLIT_ON || n_cts < 8 || ctx->capturing || ctx->lane != 0)
-        return key_switch_lane(k, x, x_stride, add0, add1, add_stride, out, out_stride, n_cts, galois_kinv);
+        return key_switch_lane(k, x, x_stride, add0, add1, add_stride, out, out_stride, n_cts, galois_kinv,
+                               hoisted);
     if (!ctx->aux) {
         CK_HIP(hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking));
         CK_HIP(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
@@ -1259,11 +1321,11 @@ static int key_switch_core(const ckks_ksk *k, const u64 *x, u64 x_stride, const 
     ctx->lane = 1;
     int rc = key_switch_lane(k, x + n1 * x_stride, x_stride, add0 ? add0 + n1 * add_stride : nullptr,
                              add1 ? add1 + n1 * add_stride : nullptr, add_stride, out + n1 * out_stride,
-                             out_stride, n2, galois_kinv);
+                             out_stride, n2, galois_kinv, hoisted ? hoisted + n1 * k->dnum * k->degree : nullptr);
     ctx->stream = main;
     ctx->lane = 0;
     if (rc) return rc;
-    rc = key_switch_lane(k, x, x_stride, add0, add1, add_stride, out, out_stride, n1, galois_kinv);
+    rc = key_switch_lane(k, x, x_stride, add0, add1, add_stride, out, out_stride, n1, galois_kinv, hoisted);
     CK_HIP(hipEventRecord(ctx->ev_join, ctx->aux));
     CK_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
     return rc;
@@ -1271,7 +1333,7 @@ static int key_switch_core(const ckks_ksk *k, const u64 *x, u64 x_stride, const 
 
 static int key_switch_lane(const ckks_ksk *k, const u64 *x, u64 x_stride, const u64 *add0,
                            const u64 *add1, u64 add_stride, u64 *out, u64 out_stride, size_t n_cts,
-                           u64 galois_kinv) {
+                           u64 galois_kinv, const int64_t *hoisted) {
     ckks_context *ctx = k->ctx;
     const ckks_rns_plan *plan = k->plan;
     const u64 N = k->degree, B = k->limbs, D = k->dnum;
@@ -1312,7 +1374,21 @@ static int key_switch_lane(const ckks_ksk *k, const u64 *x, u64 x_stride, const 
         ba.digit_count = (u32)D;
         ba.base_log = k->base_log;
         ba.total = (u64)n_cts * N;
-        if ((rc = launch_bounded(s, ba, k->bounded_words))) return rc;
+        if (hoisted) {
+            HoistArgs ha{};
+            ha.digits = hoisted;
+            ha.lifted = digits;
+            ha.primes = plan->d_primes;
+            ha.kinv = galois_kinv;
+            ha.limbs = (u32)B;
+            ha.logn = logn;
+            ha.digit_count = (u32)D;
+            ha.base_log = k->base_log;
+            ha.total = (u64)n_cts * N;
+            if ((rc = launch_hoist(s, ha))) return rc;
+        } else if ((rc = launch_bounded(s, ba, k->bounded_words))) {
+            return rc;
+        }
     } else if (mfma) {
         ModUpMfmaArgs mm{mu, k->d_bc};
         hipLaunchKernelGGL(k_modup_mfma, dim3((u32)(n_cts * D * (N / kBcRows))), dim3(256), 0, s, mm);
@@ -1454,9 +1530,12 @@ int ckks_bounded_digit_count(const ckks_rns_plan *plan, size_t q_limbs, unsigned
     return CKKS_OK;
 }
 
-int ckks_bounded_decompose(const ckks_rns_plan *plan, size_t q_limbs, unsigned base_log,
-                           const uint64_t *in, uint64_t *out, size_t n_polys, size_t in_stride) {
-    if (!plan || (n_polys && (!in || !out))) return fail(CKKS_ERR_INVALID, "null argument");
+// Balanced decomposition on the device: lifted digits (lifted_out) or the
+// signed digits themselves (raw_out).
+static int bounded_decompose_impl(const ckks_rns_plan *plan, size_t q_limbs, unsigned base_log, const uint64_t *in,
+                                  uint64_t *lifted_out, int64_t *raw_out, size_t n_polys, size_t in_stride) {
+    uint64_t *out = lifted_out;
+    if (!plan || (n_polys && (!in || (!lifted_out && !raw_out)))) return fail(CKKS_ERR_INVALID, "null argument");
     if (q_limbs == 0 || q_limbs > plan->moduli.size())
         return fail(CKKS_ERR_INVALID, "Q limb count must be within the plan basis");
     ckks_context *ctx = plan->ctx;
@@ -1477,6 +1556,7 @@ int ckks_bounded_decompose(const ckks_rns_plan *plan, size_t q_limbs, unsigned b
     ba.in_stride = in_stride;
     ba.kinv = 0;
     ba.digits = out;
+    ba.raw = raw_out;
     ba.t = bounded_table((const u64 *)ws, q_limbs, h.words);
     ba.primes = plan->d_primes;
     ba.q_limbs = (u32)q_limbs;
@@ -1486,6 +1566,50 @@ int ckks_bounded_decompose(const ckks_rns_plan *plan, size_t q_limbs, unsigned b
     ba.base_log = base_log;
     ba.total = (u64)n_polys * N;
     return launch_bounded(ctx->stream, ba, h.words);
+}
+
+int ckks_bounded_decompose(const ckks_rns_plan *plan, size_t q_limbs, unsigned base_log,
+                           const uint64_t *in, uint64_t *out, size_t n_polys, size_t in_stride) {
+    if (n_polys && !out) return fail(CKKS_ERR_INVALID, "null argument");
+    return bounded_decompose_impl(plan, q_limbs, base_log, in, out, nullptr, n_polys, in_stride);
+}
+
+int ckks_bounded_digits(const ckks_rns_plan *plan, size_t q_limbs, unsigned base_log, const uint64_t *in,
+                        int64_t *out, size_t n_polys, size_t in_stride) {
+    if (n_polys && !out) return fail(CKKS_ERR_INVALID, "null argument");
+    return bounded_decompose_impl(plan, q_limbs, base_log, in, nullptr, out, n_polys, in_stride);
+}
+
+// k^-1 mod 2N for an odd Galois exponent (Newton iteration on a power of two).
+static u64 galois_inverse(u64 k, u64 two_n) {
+    u64 inv = k;
+    for (int it = 0; it < 6; ++it) inv = inv * (2 - k * inv);
+    return inv & (two_n - 1);
+}
+
+int ckks_bounded_digits_automorphism(const ckks_rns_plan *plan, size_t q_limbs, unsigned base_log,
+                                     const int64_t *in, int64_t *out, size_t n_polys, uint64_t exponent) {
+    if (!plan || (n_polys && (!in || !out))) return fail(CKKS_ERR_INVALID, "null argument");
+    if (q_limbs == 0 || q_limbs > plan->moduli.size())
+        return fail(CKKS_ERR_INVALID, "Q limb count must be within the plan basis");
+    if (in == out && n_polys) return fail(CKKS_ERR_INVALID, "automorphism output must not alias its input");
+    const u64 two_n = 2 * (u64)plan->degree;
+    const u64 k = exponent % two_n;
+    if ((k & 1) == 0) return fail(CKKS_ERR_INVALID, "CKKS automorphism exponent must be odd modulo 2N");
+    BoundedHost h;
+    int rc = build_bounded(plan->moduli, q_limbs, base_log, h);
+    if (rc || n_polys == 0) return rc;
+    HoistArgs ha{};
+    ha.digits = in;
+    ha.raw = out;
+    ha.primes = plan->d_primes;
+    ha.kinv = galois_inverse(k, two_n);
+    ha.limbs = (u32)plan->moduli.size();
+    ha.logn = plan->logn;
+    ha.digit_count = h.digit_count;
+    ha.base_log = base_log;
+    ha.total = (u64)n_polys * plan->degree;
+    return launch_hoist(plan->ctx->stream, ha);
 }
 
 int ckks_mod_down(const ckks_rns_plan *qp_plan, size_t q_limbs, const uint64_t *in, uint64_t *out,
@@ -1668,11 +1792,25 @@ int ckks_apply_galois(const ckks_ksk *key, uint64_t exponent, const uint64_t *in
     if ((k & 1) == 0) return fail(CKKS_ERR_INVALID, "CKKS automorphism exponent must be odd modulo 2N");
     if (in == out) return fail(CKKS_ERR_INVALID, "rotation output must not alias its input");
     // sigma_k is fused into the key switch as gathers with k^-1 mod 2N
-    u64 kinv = k;
-    for (int it = 0; it < 6; ++it) kinv = kinv * (2 - k * kinv);
-    kinv &= two_n - 1;
     const size_t poly = key->q_limbs * key->degree;
-    return key_switch_core(key, in + poly, in_stride, in, nullptr, in_stride, out, out_stride, n_cts, kinv);
+    return key_switch_core(key, in + poly, in_stride, in, nullptr, in_stride, out, out_stride, n_cts,
+                           galois_inverse(k, two_n));
+}
+
+int ckks_bounded_apply_galois_hoisted(const ckks_ksk *key, uint64_t exponent, const uint64_t *in,
+                                      const int64_t *a_digits, uint64_t *out, size_t n_cts, size_t in_stride,
+                                      size_t out_stride) {
+    int rc = check_ct_args(key, in, out, n_cts, in_stride, out_stride, 2);
+    if (rc || n_cts == 0) return rc;
+    if (!key->bounded) return fail(CKKS_ERR_INVALID, "hoisted rotations need a bounded gadget key");
+    if (!a_digits) return fail(CKKS_ERR_INVALID, "null argument");
+    const u64 two_n = 2 * (u64)key->degree;
+    const u64 k = exponent % two_n;
+    if ((k & 1) == 0) return fail(CKKS_ERR_INVALID, "CKKS automorphism exponent must be odd modulo 2N");
+    if (in == out) return fail(CKKS_ERR_INVALID, "rotation output must not alias its input");
+    const size_t poly = key->q_limbs * key->degree;
+    return key_switch_core(key, in + poly, in_stride, in, nullptr, in_stride, out, out_stride, n_cts,
+                           galois_inverse(k, two_n), a_digits);
 }
 
 int ckks_relinearize(const ckks_ksk *key, const uint64_t *in3, uint64_t *out, size_t n_cts,

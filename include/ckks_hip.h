@@ -321,6 +321,32 @@ int ckks_bounded_digit_count(const ckks_rns_plan *plan, size_t q_limbs, unsigned
 int ckks_bounded_decompose(const ckks_rns_plan *plan, size_t q_limbs, unsigned base_log,
                            const uint64_t *in, uint64_t *out, size_t n_polys, size_t in_stride);
 
+/* BoundedGadgetLayout::decompose (grafting/bounded_gadget.rs:59-118): the
+ * balanced digits themselves, out[p][digit][N] signed, in
+ * [-2^(base_log-1), 2^(base_log-1)).  Arguments as ckks_bounded_decompose;
+ * synchronises the context stream to upload its tables. */
+int ckks_bounded_digits(const ckks_rns_plan *plan, size_t q_limbs, unsigned base_log, const uint64_t *in,
+                        int64_t *out, size_t n_polys, size_t in_stride);
+
+/* BoundedGadgetDecomposition::automorphism_hoisted (grafting/
+ * bounded_gadget.rs:273-312): sigma_exponent applied to decomposed digits
+ * [p][digit][N] (from ckks_bounded_digits with the same q_limbs and
+ * base_log), negated digit vectors renormalised with a carry, which equals
+ * the decomposition of the rotated polynomial.  out must not alias in. */
+int ckks_bounded_digits_automorphism(const ckks_rns_plan *plan, size_t q_limbs, unsigned base_log,
+                                     const int64_t *in, int64_t *out, size_t n_polys, uint64_t exponent);
+
+/* Hoisted rotation with a bounded key: ckks_apply_galois with the
+ * decomposition of the a components taken from a_digits [ct][digit][N]
+ * (ckks_bounded_digits of the unrotated a, computed once and reused for
+ * every exponent) through the hoisted digit automorphism, as
+ * PreparedCanonicalGaloisSource (grafting/bounded_gadget.rs:176-249) and
+ * automorphism_hoisted do for the reference.  Same residues as
+ * ckks_apply_galois; the a limbs of in are not read. */
+int ckks_bounded_apply_galois_hoisted(const ckks_ksk *key, uint64_t exponent, const uint64_t *in,
+                                      const int64_t *a_digits, uint64_t *out, size_t n_cts, size_t in_stride,
+                                      size_t out_stride);
+
 int ckks_ksk_destroy(ckks_ksk *key);
 
 /* rns_key_switch_with_prepared_dynamic_ntt (grafting/rns_key_switch.rs:

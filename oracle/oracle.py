@@ -345,6 +345,34 @@ def bounded_decompose(x, moduli, base_log):
     return out
 
 
+def bounded_automorphism_hoisted(digits, exponent, base_log):
+    """BoundedGadgetDecomposition::automorphism_hoisted
+    (grafting/bounded_gadget.rs:273-312): sigma_exponent on decomposed
+    balanced digits [D][N]; a negated source's digit vector is renormalised
+    from the lowest digit up with a carry.  Python integers, small N only."""
+    d = np.asarray(digits, dtype=np.int64)
+    count, n = d.shape
+    two_n, base = 2 * n, 1 << base_log
+    half = base // 2
+    e = exponent % two_n
+    assert e % 2 == 1
+    out = np.zeros_like(d)
+    for src in range(n):
+        mapped = src * e % two_n
+        target, negate = mapped % n, mapped >= n
+        if not negate:
+            out[:, target] = d[:, src]
+            continue
+        carry = 0
+        for j in range(count):
+            value = -int(d[j, src]) + carry
+            normalized = (value + half) % base - half  # Python % is rem_euclid for base > 0
+            out[j, target] = normalized
+            carry = (value - normalized) // base
+        assert carry == 0, "hoisted digit negation exceeded gadget capacity"
+    return out
+
+
 def lift_signed(d, q):
     """lift_digit (grafting/bounded_gadget.rs:382-402): rem_euclid q."""
     d = np.ascontiguousarray(d, dtype=np.int64)

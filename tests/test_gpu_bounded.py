@@ -104,3 +104,59 @@ def test_reference_rotation_benchmark_shape(pkg, ctx):
     sa = np.stack([O.apply_automorphism(ct[0, 1, i], q, 3) for i, q in enumerate(q_moduli)])
     bo, ao = O.bounded_key_switch(O.RnsPlan(qp, n), 13, 32, sb, sa, kb, ka)
     assert (got[0, 0] == bo).all() and (got[0, 1] == ao).all()
+
+
+def _with_ties(x, moduli, w):
+    """Puts coefficients whose balanced digits sit at -2^(w-1) (where a
+    negated digit vector carries) at odd positions 1, 3, 5, 7."""
+    half = 1 << (w - 1)
+    for c, v in enumerate((-half, -half - (half << w), -(half << w), half - 1 + (-half << w))):
+        x[..., :, 2 * c + 1] = [v % q for q in moduli]
+    return x
+
+
+@pytest.mark.parametrize("logn,L,w", [(12, 4, 20), (13, 9, 61), (14, 6, 63)])
+def test_bounded_digits_and_hoisted_automorphism(pkg, ctx, logn, L, w):
+    """ckks_bounded_digits equals the oracle's balanced digits, and
+    ckks_bounded_digits_automorphism equals both the oracle's
+    automorphism_hoisted and the digits of the rotated polynomial."""
+    n = 1 << logn
+    moduli = [Q0[0]] + Q40[:L - 1]
+    rng = np.random.default_rng(3 * logn + w)
+    x = _with_ties(rand_residues(rng, moduli, n, polys=2), moduli, w)
+    plan = pkg.PreparedRnsNttPlan(ctx, moduli, n)
+    dx = pkg.to_device(x)
+    digits = pkg.bounded_digits(plan, L, w, dx)
+    got = digits.cpu().numpy()
+    for p in range(2):
+        assert (got[p] == O.bounded_decompose(x[p], moduli, w)).all(), f"poly {p}"
+    for e in (2 * n - 1, O.rotation_exponent_left(n, 3)):
+        hd = pkg.bounded_digits_automorphism(plan, L, w, digits, e).cpu().numpy()
+        rot = pkg.bounded_digits(plan, L, w, plan.apply_automorphism(dx, e, n_limbs=L)).cpu().numpy()
+        assert (hd == rot).all(), f"exponent {e}: hoisted != decomposition of the rotation"
+        if logn == 12:
+            for p in range(2):
+                assert (hd[p] == O.bounded_automorphism_hoisted(got[p], e, w)).all()
+
+
+@pytest.mark.parametrize("logn,L,K,w", [(12, 4, 1, 61), (13, 9, 2, 32), (13, 6, 0, 20)])
+def test_hoisted_rotation_matches_apply_galois(pkg, ctx, logn, L, K, w):
+    """Rotations from digits decomposed once give the residues of
+    ckks_apply_galois for every exponent, with and without special primes."""
+    n = 1 << logn
+    q_moduli = [Q0[0]] + Q40[:L - 1]
+    qp = q_moduli + P60[:K]
+    rng = np.random.default_rng(logn * 7 + w + K)
+    plan = pkg.PreparedRnsNttPlan(ctx, qp, n)
+    count = O.bounded_digit_count(q_moduli, w)
+    kb, ka = _key(rng, qp, n, count)
+    key = pkg.BoundedKeySwitchKey(plan, L, w, kb, ka)
+    ct = rand_residues(rng, q_moduli, n, polys=6).reshape(3, 2, L, n)
+    ct[:, 1] = _with_ties(ct[:, 1], q_moduli, w)
+    dct = pkg.to_device(ct)
+    digits = key.digits(dct)
+    for steps in (1, 5, -2):
+        k = O.rotation_exponent_left(n, steps) if steps > 0 else O.rotation_exponent_right(n, -steps)
+        want = pkg.to_host(key.apply_galois(dct, k))
+        got = pkg.to_host(key.apply_galois_hoisted(dct, k, digits))
+        assert (got == want).all(), f"steps {steps}"

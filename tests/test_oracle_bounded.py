@@ -245,3 +245,25 @@ def test_hybrid_qp_key_switch_matches_big_integer_composition():
         got = bo if comp == 0 else ao
         for i, q in enumerate(q_moduli):
             assert got[i].tolist() == [r % q for r in rounded], f"component {comp} limb {i}"
+
+
+@pytest.mark.parametrize("w", [7, 20, 61, 63])
+def test_hoisted_digit_automorphism_equals_canonical(w):
+    """automorphism_hoisted(decompose(x)) == decompose(sigma(x)), the
+    reference's own check (grafting/bounded_gadget.rs:428-488), including
+    ties: coefficients whose digits sit at -2^(w-1), where negation carries."""
+    n = 64
+    moduli = [Q0[0]] + Q40[:3]
+    Q = math.prod(moduli)
+    rng = np.random.default_rng(w)
+    x = np.stack([rng.integers(0, q, n, dtype=np.uint64) for q in moduli])
+    half = 1 << (w - 1)
+    ties = [-half, -half - (half << w), -(half << w), half - 1 + (-half << w)]
+    for c, v in enumerate(ties):
+        x[:, 2 * c + 1] = [v % q for q in moduli]
+    d = O.bounded_decompose(x, moduli, w)
+    assert (d == -half).any()
+    for e in (3, 5, 2 * n - 1, 5 ** 7 % (2 * n), 13):
+        sx = np.stack([O.apply_automorphism(x[i], q, e) for i, q in enumerate(moduli)])
+        want = O.bounded_decompose(sx, moduli, w)
+        assert (O.bounded_automorphism_hoisted(d, e, w) == want).all(), f"exponent {e}"
