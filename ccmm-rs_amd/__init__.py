@@ -111,6 +111,9 @@ _SIGNATURES = {
     "ckks_bounded_digits": (C.c_int, [_p, _sz, C.c_uint, _p, _p, _sz, _sz]),
     "ckks_bounded_digits_automorphism": (C.c_int, [_p, _sz, C.c_uint, _p, _p, _sz, _u64]),
     "ckks_bounded_apply_galois_hoisted": (C.c_int, [_p, _u64, _p, _p, _p, _sz, _sz, _sz]),
+    "ckks_bounded_hoist": (C.c_int, [_p, _p, _sz, _sz, C.POINTER(_p)]),
+    "ckks_bounded_rotate_hoisted": (C.c_int, [_p, _p, _u64, _p, _p, _sz, _sz]),
+    "ckks_bounded_hoist_destroy": (C.c_int, [_p]),
     "ckks_ksk_destroy": (C.c_int, [_p]),
     "ckks_key_switch": (C.c_int, [_p, _p, _p, _sz, _sz, _sz]),
     "ckks_apply_galois": (C.c_int, [_p, _u64, _p, _p, _sz, _sz, _sz]),
@@ -624,6 +627,16 @@ class BoundedKeySwitchKey(_KeySwitchKey):
                                          _ptr(out), n, 2 * poly))
         return out
 
+    def hoist(self, ct) -> HoistedDigits:
+        """Decompose, lift and transform the a components of ct once."""
+        return HoistedDigits(self, ct)
+
+    def rotate_hoisted(self, hoisted: HoistedDigits, ct, exponent: int, out=None):
+        """ckks_bounded_rotate_hoisted: same residues as apply_galois(ct, exponent)."""
+        out = empty((hoisted.n_cts, 2, self.q_limbs, self.plan.degree)) if out is None else out
+        _check(lib().ckks_bounded_rotate_hoisted(hoisted.handle, self._h, int(exponent), _ptr(ct), _ptr(out), 0, 0))
+        return out
+
     def apply_galois_hoisted(self, ct, exponent: int, a_digits, out=None):
         """Rotation from hoisted digits (grafting/bounded_gadget.rs:176-312):
         same residues as apply_galois; a_digits = self.digits(ct)."""
@@ -633,6 +646,34 @@ class BoundedKeySwitchKey(_KeySwitchKey):
         _check(lib().ckks_bounded_apply_galois_hoisted(self._h, int(exponent), _ptr(ct), _ptr(a_digits), _ptr(out),
                                                        n, 0, 0))
         return out
+
+
+class HoistedDigits:
+    """ckks_bounded_hoist: the NTT-domain digits of a ciphertext batch's a
+    components, decomposed once for many rotations (rotate_hoisted)."""
+
+    def __init__(self, key: "BoundedKeySwitchKey", ct):
+        self.ctx = key.ctx
+        self.n_cts = key._n_cts(ct, 2)
+        h = _p()
+        _check(lib().ckks_bounded_hoist(key.handle, _ptr(ct), self.n_cts, 0, C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        # after the context is gone (interpreter exit) the handle is left alone
+        if getattr(self, "_h", None) and self.ctx.handle:
+            lib().ckks_bounded_hoist_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def bounded_digit_count(plan: PreparedRnsNttPlan, q_limbs: int, base_log: int) -> int:

@@ -98,6 +98,22 @@ struct ckks_ksk {
     signed char *d_bt = nullptr;
 };
 
+// Digits of a ciphertext batch's a components for hoisted bounded rotations:
+// decomposed, lifted to the key basis and transformed once.
+struct ckks_hoisted {
+    ckks_context *ctx = nullptr;
+    const ckks_rns_plan *plan = nullptr;
+    size_t q_limbs = 0, digit_count = 0, n_cts = 0;
+    unsigned base_log = 0;
+    ck::u64 *d_digits = nullptr;  // [ct][digit][B][N], NTT domain
+    // (ct, coefficient) whose digit vector holds a -2^(w-1): negating it
+    // carries, so a rotation that negates one takes the canonical path
+    std::vector<std::pair<ck::u32, ck::u32>> ties;
+    ~ckks_hoisted() {
+        if (d_digits) hipFree(d_digits);
+    }
+};
+
 namespace ck {
 
 // ---- device helpers --------------------------------------------------------
@@ -305,6 +321,7 @@ struct BoundedArgs {
     u64 kinv;       // nonzero: decompose sigma_k(in) (rotation fused in)
     u64 *digits;    // [ct][D][B][N] lifted digits, or
     int64_t *raw;   // [ct][D][N] the balanced digits themselves (when non-null)
+    unsigned char *ties;  // [ct][N] (when non-null): 1 where a digit is -2^(w-1)
     BoundedTable t;
     const DevPrime *primes;  // key basis B
     u32 q_limbs, limbs, logn, digit_count, base_log;
@@ -335,6 +352,7 @@ __global__ __launch_bounds__(kKsThreads) void k_bounded_decompose(BoundedArgs a)
     u64 *out = a.digits + ct * (u64)a.digit_count * a.limbs * N + n;
     u64 lo = 0, hi = 0, carry = 0;
     u32 nb = 0, j = 0;
+    bool tie = false;
 #pragma unroll
     for (int k = 0; k < W + 2; ++k) {
         const u64 word = k < W ? X[k] : fill;
@@ -356,6 +374,7 @@ __global__ __launch_bounds__(kKsThreads) void k_bounded_decompose(BoundedArgs a)
                 d = (int64_t)t;
                 carry = 0;
             }
+            tie |= d == -(int64_t)half_base;
             if (a.raw) {
                 a.raw[(ct * a.digit_count + j) * N + n] = d;
             } else {
@@ -365,6 +384,7 @@ __global__ __launch_bounds__(kKsThreads) void k_bounded_decompose(BoundedArgs a)
             ++j;
         }
     }
+    if (a.ties) a.ties[ct * N + n] = tie ? 1 : 0;
 }
 
 // Hoisted digit automorphism (BoundedGadgetDecomposition::automorphism_hoisted,
@@ -426,8 +446,15 @@ struct MacArgs {
     const u64 *kb, *ka;  // [dnum][B][N]
     u64 *acc;            // [ct][2][B][N]
     const DevPrime *primes;
+    u64 perm_k;  // nonzero: digit n is read at the NTT-domain Galois position of k
     u32 n_cts, dnum, limbs, logn, chunks;
 };
+
+// NTT-domain Galois permutation (ring/ntt_galois_permutation.rs:5-19): the
+// digit of sigma_k(p) at n is p's at ((k (2n + 1) mod 2N) - 1) / 2.
+CK_HD u64 mac_digit_index(u64 n, u64 k, u32 logn) {
+    return k ? ((k * (2 * n + 1)) & ((2ull << logn) - 1)) >> 1 : n;
+}
 
 // Each thread owns one residue position (i, n) and keeps the key entries of
 // every digit in registers while it walks the batch, so a key is read from
@@ -446,8 +473,9 @@ __global__ __launch_bounds__(kKsThreads) void k_ks_mac(MacArgs a) {
         kb[j] = a.kb[((u64)j * a.limbs + i) * N + n];
         ka[j] = a.ka[((u64)j * a.limbs + i) * N + n];
     }
+    const u64 nd = mac_digit_index(n, a.perm_k, a.logn);
     for (u32 ct = 0; ct < a.n_cts; ++ct) {
-        const u64 *d = a.digits + (((u64)ct * D) * a.limbs + i) * N + n;
+        const u64 *d = a.digits + (((u64)ct * D) * a.limbs + i) * N + nd;
         Acc128 sb, sa;
         sb.zero();
         sa.zero();
@@ -474,7 +502,7 @@ __global__ __launch_bounds__(kKsThreads) void k_ks_mac_any(MacArgs a) {
     const u64 N = (u64)1 << a.logn;
     const u64 n = (u64)chunk * kKsThreads + threadIdx.x;
     const DevPrime P = a.primes[i];
-    const u64 *d = a.digits + (((u64)ct * a.dnum) * a.limbs + i) * N + n;
+    const u64 *d = a.digits + (((u64)ct * a.dnum) * a.limbs + i) * N + mac_digit_index(n, a.perm_k, a.logn);
     u64 rb = 0, ra = 0;
     for (u32 j0 = 0; j0 < a.dnum; j0 += 8) {
         Acc128 sb, sa;
@@ -1292,7 +1320,7 @@ static int create_bounded_key(const ckks_rns_plan *plan, size_t q_limbs, unsigne
 // (add0/add1 may be null) over the key's Q limbs.
 static int key_switch_lane(const ckks_ksk *k, const u64 *x, u64 x_stride, const u64 *add0,
                            const u64 *add1, u64 add_stride, u64 *out, u64 out_stride, size_t n_cts,
-                           u64 galois_kinv, const int64_t *hoisted);
+                           u64 galois_kinv, const int64_t *hoisted, const u64 *ntt_digits, u64 perm_k);
 
 // Runs the key switch on one lane, or splits the batch over the context
 // stream (first half) and the auxiliary stream (second half): fork with an
@@ -1300,14 +1328,19 @@ static int key_switch_lane(const ckks_ksk *k, const u64 *x, u64 x_stride, const 
 // wait for the auxiliary one.  Each lane has its own workspaces.
 // hoisted: bounded keys only; the balanced digits [ct][D][N] of the unrotated
 // component x (ckks_bounded_digits), which replace its decomposition.
+// ntt_digits: bounded keys only; the lifted NTT-domain digits [ct][D][B][N]
+// of the unrotated component (ckks_bounded_hoist), read through the
+// NTT-domain Galois permutation of perm_k; decomposition and forward NTT
+// are skipped.
 static int key_switch_core(const ckks_ksk *k, const u64 *x, u64 x_stride, const u64 *add0,
                            const u64 *add1, u64 add_stride, u64 *out, u64 out_stride, size_t n_cts,
-                           u64 galois_kinv = 0, const int64_t *hoisted = nullptr) {
+                           u64 galois_kinv = 0, const int64_t *hoisted = nullptr,
+                           const u64 *ntt_digits = nullptr, u64 perm_k = 0) {
     ckks_context *ctx = k->ctx;
     const int split = ctx->ks_split == CKKS_KS_SPLIT_AUTO ? k->tuned_split : ctx->ks_split;
     if (split != CKKS_KS_SPLIT_ON || n_cts < 8 || ctx->capturing || ctx->lane != 0)
         return key_switch_lane(k, x, x_stride, add0, add1, add_stride, out, out_stride, n_cts, galois_kinv,
-                               hoisted);
+                               hoisted, ntt_digits, perm_k);
     if (!ctx->aux) {
         CK_HIP(hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking));
         CK_HIP(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
@@ -1321,11 +1354,13 @@ static int key_switch_core(const ckks_ksk *k, const u64 *x, u64 x_stride, const 
     ctx->lane = 1;
     int rc = key_switch_lane(k, x + n1 * x_stride, x_stride, add0 ? add0 + n1 * add_stride : nullptr,
                              add1 ? add1 + n1 * add_stride : nullptr, add_stride, out + n1 * out_stride,
-                             out_stride, n2, galois_kinv, hoisted ? hoisted + n1 * k->dnum * k->degree : nullptr);
+                             out_stride, n2, galois_kinv, hoisted ? hoisted + n1 * k->dnum * k->degree : nullptr,
+                             ntt_digits ? ntt_digits + n1 * k->dnum * k->limbs * k->degree : nullptr, perm_k);
     ctx->stream = main;
     ctx->lane = 0;
     if (rc) return rc;
-    rc = key_switch_lane(k, x, x_stride, add0, add1, add_stride, out, out_stride, n1, galois_kinv, hoisted);
+    rc = key_switch_lane(k, x, x_stride, add0, add1, add_stride, out, out_stride, n1, galois_kinv, hoisted,
+                         ntt_digits, perm_k);
     CK_HIP(hipEventRecord(ctx->ev_join, ctx->aux));
     CK_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
     return rc;
@@ -1333,17 +1368,17 @@ static int key_switch_core(const ckks_ksk *k, const u64 *x, u64 x_stride, const 
 
 static int key_switch_lane(const ckks_ksk *k, const u64 *x, u64 x_stride, const u64 *add0,
                            const u64 *add1, u64 add_stride, u64 *out, u64 out_stride, size_t n_cts,
-                           u64 galois_kinv, const int64_t *hoisted) {
+                           u64 galois_kinv, const int64_t *hoisted, const u64 *ntt_digits, u64 perm_k) {
     ckks_context *ctx = k->ctx;
     const ckks_rns_plan *plan = k->plan;
     const u64 N = k->degree, B = k->limbs, D = k->dnum;
     const u32 logn = plan->logn;
     hipStream_t s = ctx->stream;
     int rc;
-    void *wd, *wa;
-    if ((rc = ctx->workspace(4, n_cts * D * B * N * sizeof(u64), &wd))) return rc;
+    void *wd = nullptr, *wa;
+    if (!ntt_digits && (rc = ctx->workspace(4, n_cts * D * B * N * sizeof(u64), &wd))) return rc;
     if ((rc = ctx->workspace(5, n_cts * 2 * B * N * sizeof(u64), &wa))) return rc;
-    u64 *digits = (u64 *)wd, *acc = (u64 *)wa;
+    u64 *digits = ntt_digits ? const_cast<u64 *>(ntt_digits) : (u64 *)wd, *acc = (u64 *)wa;
     const u32 chunks = (u32)(N / kKsThreads);
     // 1. ModUp
     ModUpArgs mu{};
@@ -1360,7 +1395,9 @@ static int key_switch_lane(const ckks_ksk *k, const u64 *x, u64 x_stride, const 
     mu.chunks = chunks;
     const int engine = ctx->base_conversion == CKKS_BASECONV_AUTO ? k->tuned_engine : ctx->base_conversion;
     const bool mfma = k->mfma && engine == CKKS_BASECONV_MFMA;
-    if (k->bounded) {
+    if (ntt_digits) {
+        // decomposed, lifted and transformed once (ckks_bounded_hoist)
+    } else if (k->bounded) {
         BoundedArgs ba{};
         ba.in = x;
         ba.in_stride = x_stride;
@@ -1401,14 +1438,14 @@ static int key_switch_lane(const ckks_ksk *k, const u64 *x, u64 x_stride, const 
     const bool two_pass = logn > 12 && logn <= 17;
     // CKKS_KS_FUSION_INVERSE: the MAC is computed inside the first inverse
     // NTT pass (step 4) and the accumulators never reach HBM in the NTT domain
-    const bool mac_in_inverse = fusion == CKKS_KS_FUSION_INVERSE && two_pass && D <= 8;
-    if ((fusion == CKKS_KS_FUSION_XCD || fusion == CKKS_KS_FUSION_LINEAR) && two_pass) {
+    const bool mac_in_inverse = fusion == CKKS_KS_FUSION_INVERSE && two_pass && D <= 8 && !ntt_digits;
+    if ((fusion == CKKS_KS_FUSION_XCD || fusion == CKKS_KS_FUSION_LINEAR) && two_pass && !ntt_digits) {
         if ((rc = launch_ntt_mac(ctx, plan, digits, n_cts, D, B, k->d_kb, k->d_ka, acc,
                                  fusion == CKKS_KS_FUSION_XCD)))
             return rc;
     } else {
     NttCall f{plan, digits, digits, n_cts * D, B, B * N, B * N};
-    if ((rc = launch_ntt(ctx, &f, 1, true))) return rc;
+    if (!ntt_digits && (rc = launch_ntt(ctx, &f, 1, true))) return rc;
     if (!mac_in_inverse) {
     MacArgs ma{};
     ma.digits = digits;
@@ -1416,6 +1453,7 @@ static int key_switch_lane(const ckks_ksk *k, const u64 *x, u64 x_stride, const 
     ma.ka = k->d_ka;
     ma.acc = acc;
     ma.primes = plan->d_primes;
+    ma.perm_k = ntt_digits ? perm_k : 0;
     ma.n_cts = (u32)n_cts;
     ma.dnum = (u32)D;
     ma.limbs = (u32)B;
@@ -1811,6 +1849,89 @@ int ckks_bounded_apply_galois_hoisted(const ckks_ksk *key, uint64_t exponent, co
     const size_t poly = key->q_limbs * key->degree;
     return key_switch_core(key, in + poly, in_stride, in, nullptr, in_stride, out, out_stride, n_cts,
                            galois_inverse(k, two_n), a_digits);
+}
+
+int ckks_bounded_hoist(const ckks_ksk *key, const uint64_t *in, size_t n_cts, size_t in_stride,
+                       ckks_hoisted **out) {
+    if (!key || !out || (n_cts && !in)) return fail(CKKS_ERR_INVALID, "null argument");
+    if (!key->bounded) return fail(CKKS_ERR_INVALID, "hoisted rotations need a bounded gadget key");
+    ckks_context *ctx = key->ctx;
+    if (ctx->capturing) return fail(CKKS_ERR_INVALID, "hoisting allocates and synchronises: not capturable");
+    const size_t N = key->degree, B = key->limbs, D = key->dnum, poly = key->q_limbs * N;
+    if (in_stride == 0) in_stride = 2 * poly;
+    if (in_stride < 2 * poly) return fail(CKKS_ERR_INVALID, "ciphertext stride smaller than its limbs");
+    std::unique_ptr<ckks_hoisted> h(new ckks_hoisted());
+    h->ctx = ctx;
+    h->plan = key->plan;
+    h->q_limbs = key->q_limbs;
+    h->digit_count = D;
+    h->n_cts = n_cts;
+    h->base_log = key->base_log;
+    if (n_cts) {
+        CK_HIP(hipSetDevice(ctx->device));
+        CK_HIP(hipMalloc(&h->d_digits, n_cts * D * B * N * sizeof(u64)));
+        struct Flags {
+            unsigned char *p = nullptr;
+            ~Flags() {
+                if (p) hipFree(p);
+            }
+        } flags;
+        CK_HIP(hipMalloc(&flags.p, n_cts * N));
+        BoundedArgs ba{};
+        ba.in = in + poly;
+        ba.in_stride = in_stride;
+        ba.kinv = 0;
+        ba.digits = h->d_digits;
+        ba.ties = flags.p;
+        ba.t = bounded_table(key->d_bounded, key->q_limbs, key->bounded_words);
+        ba.primes = key->plan->d_primes;
+        ba.q_limbs = (u32)key->q_limbs;
+        ba.limbs = (u32)B;
+        ba.logn = key->plan->logn;
+        ba.digit_count = (u32)D;
+        ba.base_log = key->base_log;
+        ba.total = (u64)n_cts * N;
+        int rc = launch_bounded(ctx->stream, ba, key->bounded_words);
+        if (rc) return rc;
+        NttCall f{key->plan, h->d_digits, h->d_digits, n_cts * D, B, B * N, B * N};
+        if ((rc = launch_ntt(ctx, &f, 1, true))) return rc;
+        std::vector<unsigned char> host(n_cts * N);
+        CK_HIP(hipStreamSynchronize(ctx->stream));
+        CK_HIP(hipMemcpy(host.data(), flags.p, host.size(), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < host.size(); ++i)
+            if (host[i]) h->ties.emplace_back((u32)(i / N), (u32)(i % N));
+    }
+    *out = h.release();
+    return CKKS_OK;
+}
+
+int ckks_bounded_rotate_hoisted(const ckks_hoisted *h, const ckks_ksk *key, uint64_t exponent, const uint64_t *in,
+                                uint64_t *out, size_t in_stride, size_t out_stride) {
+    if (!h) return fail(CKKS_ERR_INVALID, "null argument");
+    const size_t n_cts = h->n_cts;
+    int rc = check_ct_args(key, in, out, n_cts, in_stride, out_stride, 2);
+    if (rc || n_cts == 0) return rc;
+    if (!key->bounded || key->plan != h->plan || key->q_limbs != h->q_limbs || key->base_log != h->base_log ||
+        key->dnum != h->digit_count)
+        return fail(CKKS_ERR_INVALID, "key does not match the hoisted digits (plan, Q limbs, base_log)");
+    const u64 two_n = 2 * (u64)key->degree;
+    const u64 k = exponent % two_n;
+    if ((k & 1) == 0) return fail(CKKS_ERR_INVALID, "CKKS automorphism exponent must be odd modulo 2N");
+    if (in == out) return fail(CKKS_ERR_INVALID, "rotation output must not alias its input");
+    // sigma_k maps coefficient s to s k mod 2N, negated when that is >= N; a
+    // negated tie carries, and the hoisted digits would differ from the
+    // decomposition of the rotated a
+    for (const auto &t : h->ties)
+        if (((u64)t.second * k & (two_n - 1)) >= key->degree)
+            return ckks_apply_galois(key, exponent, in, out, n_cts, in_stride, out_stride);
+    const size_t poly = key->q_limbs * key->degree;
+    return key_switch_core(key, in + poly, in_stride, in, nullptr, in_stride, out, out_stride, n_cts,
+                           galois_inverse(k, two_n), nullptr, h->d_digits, k);
+}
+
+int ckks_bounded_hoist_destroy(ckks_hoisted *h) {
+    delete h;
+    return CKKS_OK;
 }
 
 int ckks_relinearize(const ckks_ksk *key, const uint64_t *in3, uint64_t *out, size_t n_cts,

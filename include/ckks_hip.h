@@ -347,6 +347,25 @@ int ckks_bounded_apply_galois_hoisted(const ckks_ksk *key, uint64_t exponent, co
                                       const int64_t *a_digits, uint64_t *out, size_t n_cts, size_t in_stride,
                                       size_t out_stride);
 
+/* Hoisted rotations in the NTT domain (the use PreparedCanonicalGaloisSource,
+ * grafting/bounded_gadget.rs:176-249, serves: many rotations of one
+ * ciphertext).  ckks_bounded_hoist decomposes the a components of n_cts
+ * ciphertexts once, lifts the digits to the key basis and transforms them
+ * (device memory owned by the handle; synchronises the context stream).
+ * ckks_bounded_rotate_hoisted then rotates the same ciphertexts (in: their
+ * b components are read) with any bounded key of the same plan, Q limbs and
+ * base_log: sigma_k of a digit is a permutation in the NTT domain, read by
+ * the key MAC, so no decomposition or forward NTT runs.  A digit vector
+ * holding -2^(base_log-1) carries when negated; when sigma_k negates one,
+ * the call takes the ckks_apply_galois path.  Same residues as
+ * ckks_apply_galois either way.  Destroy the handle before the context. */
+typedef struct ckks_hoisted ckks_hoisted;
+int ckks_bounded_hoist(const ckks_ksk *key, const uint64_t *in, size_t n_cts, size_t in_stride,
+                       ckks_hoisted **out);
+int ckks_bounded_rotate_hoisted(const ckks_hoisted *hoisted, const ckks_ksk *key, uint64_t exponent,
+                                const uint64_t *in, uint64_t *out, size_t in_stride, size_t out_stride);
+int ckks_bounded_hoist_destroy(ckks_hoisted *hoisted);
+
 int ckks_ksk_destroy(ckks_ksk *key);
 
 /* rns_key_switch_with_prepared_dynamic_ntt (grafting/rns_key_switch.rs:

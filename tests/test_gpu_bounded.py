@@ -160,3 +160,31 @@ def test_hoisted_rotation_matches_apply_galois(pkg, ctx, logn, L, K, w):
         want = pkg.to_host(key.apply_galois(dct, k))
         got = pkg.to_host(key.apply_galois_hoisted(dct, k, digits))
         assert (got == want).all(), f"steps {steps}"
+
+
+@pytest.mark.parametrize("logn,L,K,w", [(12, 4, 1, 61), (13, 9, 2, 32), (14, 6, 0, 20)])
+def test_ntt_domain_hoisted_rotations(pkg, ctx, logn, L, K, w):
+    """Digits decomposed and transformed once, rotated by permutation in the
+    NTT domain: the residues of apply_galois for every exponent; exponents
+    that negate a tie coefficient (2N - 1 here) take the canonical path."""
+    n = 1 << logn
+    q_moduli = [Q0[0]] + Q40[:L - 1]
+    qp = q_moduli + P60[:K]
+    rng = np.random.default_rng(logn * 11 + w + K)
+    plan = pkg.PreparedRnsNttPlan(ctx, qp, n)
+    count = O.bounded_digit_count(q_moduli, w)
+    kb, ka = _key(rng, qp, n, count)
+    key = pkg.BoundedKeySwitchKey(plan, L, w, kb, ka)
+    other = pkg.BoundedKeySwitchKey(plan, L, w, *_key(rng, qp, n, count))
+    ct = rand_residues(rng, q_moduli, n, polys=6).reshape(3, 2, L, n)
+    ct[1, 1] = _with_ties(ct[1, 1], q_moduli, w)  # ties at coefficients 1, 3, 5, 7 of ct 1
+    dct = pkg.to_device(ct)
+    hoisted = key.hoist(dct)
+    try:
+        for k in (O.rotation_exponent_left(n, 1), O.rotation_exponent_left(n, 7), 2 * n - 1, 5):
+            for kk in (key, other):
+                want = pkg.to_host(kk.apply_galois(dct, k))
+                got = pkg.to_host(kk.rotate_hoisted(hoisted, dct, k))
+                assert (got == want).all(), f"exponent {k}"
+    finally:
+        hoisted.close()
