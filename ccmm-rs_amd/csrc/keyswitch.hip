@@ -94,6 +94,8 @@ struct ckks_ksk {
     int tuned_engine = CKKS_BASECONV_VALU;  // engine of CKKS_BASECONV_AUTO for this key
     int tuned_fusion = CKKS_KS_FUSION_OFF;  // mode of CKKS_KS_FUSION_AUTO for this key
     int tuned_split = CKKS_KS_SPLIT_OFF;    // mode of CKKS_KS_SPLIT_AUTO for this key
+    // hoisted rotations: key entries [2][D][B][N] permuted by k^-1, per k
+    mutable std::map<ck::u64, ck::u64 *> hoist_keys;
     ck::BcPlan *d_bc = nullptr;
     signed char *d_bt = nullptr;
 };
@@ -446,14 +448,34 @@ struct MacArgs {
     const u64 *kb, *ka;  // [dnum][B][N]
     u64 *acc;            // [ct][2][B][N]
     const DevPrime *primes;
-    u64 perm_k;  // nonzero: digit n is read at the NTT-domain Galois position of k
     u32 n_cts, dnum, limbs, logn, chunks;
 };
 
-// NTT-domain Galois permutation (ring/ntt_galois_permutation.rs:5-19): the
-// digit of sigma_k(p) at n is p's at ((k (2n + 1) mod 2N) - 1) / 2.
-CK_HD u64 mac_digit_index(u64 n, u64 k, u32 logn) {
-    return k ? ((k * (2 * n + 1)) & ((2ull << logn) - 1)) >> 1 : n;
+// NTT-domain Galois permutation (ring/ntt_galois_permutation.rs:5-19) over
+// whole limbs: out[n] = in[((k (2n + 1) mod 2N) - 1) / 2], so that
+// NTT(sigma_k(p)) = permute(NTT(p), k); permuting by k^-1 inverts it.
+struct NttPermArgs {
+    const u64 *in;
+    u64 *out;
+    u64 k;
+    u64 total;  // limbs * N
+    u32 logn;
+};
+
+__global__ __launch_bounds__(kKsThreads) void k_ntt_permute(NttPermArgs a) {
+    const u64 gid = (u64)blockIdx.x * kKsThreads + threadIdx.x;
+    if (gid >= a.total) return;
+    const u64 N = (u64)1 << a.logn;
+    const u64 n = gid & (N - 1);
+    a.out[gid] = a.in[(gid - n) + (((a.k * (2 * n + 1)) & (2 * N - 1)) >> 1)];
+}
+
+static int launch_ntt_permute(hipStream_t s, const u64 *in, u64 *out, u64 k, u64 total, u32 logn) {
+    if (total == 0) return CKKS_OK;
+    NttPermArgs a{in, out, k, total, logn};
+    hipLaunchKernelGGL(k_ntt_permute, dim3((u32)((total + kKsThreads - 1) / kKsThreads)), dim3(kKsThreads), 0, s, a);
+    CK_HIP(hipGetLastError());
+    return CKKS_OK;
 }
 
 // Each thread owns one residue position (i, n) and keeps the key entries of
@@ -473,9 +495,8 @@ __global__ __launch_bounds__(kKsThreads) void k_ks_mac(MacArgs a) {
         kb[j] = a.kb[((u64)j * a.limbs + i) * N + n];
         ka[j] = a.ka[((u64)j * a.limbs + i) * N + n];
     }
-    const u64 nd = mac_digit_index(n, a.perm_k, a.logn);
     for (u32 ct = 0; ct < a.n_cts; ++ct) {
-        const u64 *d = a.digits + (((u64)ct * D) * a.limbs + i) * N + nd;
+        const u64 *d = a.digits + (((u64)ct * D) * a.limbs + i) * N + n;
         Acc128 sb, sa;
         sb.zero();
         sa.zero();
@@ -502,7 +523,7 @@ __global__ __launch_bounds__(kKsThreads) void k_ks_mac_any(MacArgs a) {
     const u64 N = (u64)1 << a.logn;
     const u64 n = (u64)chunk * kKsThreads + threadIdx.x;
     const DevPrime P = a.primes[i];
-    const u64 *d = a.digits + (((u64)ct * a.dnum) * a.limbs + i) * N + mac_digit_index(n, a.perm_k, a.logn);
+    const u64 *d = a.digits + (((u64)ct * a.dnum) * a.limbs + i) * N + n;
     u64 rb = 0, ra = 0;
     for (u32 j0 = 0; j0 < a.dnum; j0 += 8) {
         Acc128 sb, sa;
@@ -1318,9 +1339,17 @@ static int create_bounded_key(const ckks_rns_plan *plan, size_t q_limbs, unsigne
 // ct at x + c*x_stride) and writes
 //   out[c][0] = add0 + KS_b(x),  out[c][1] = add1 + KS_a(x)
 // (add0/add1 may be null) over the key's Q limbs.
+// Inputs of a hoisted rotation by k: NTT-domain digits of the unrotated
+// component and the key entries permuted by k^-1.
+struct HoistedMac {
+    const u64 *digits;   // [ct][D][B][N]
+    const u64 *kb, *ka;  // [D][B][N]
+    u64 k;
+};
+
 static int key_switch_lane(const ckks_ksk *k, const u64 *x, u64 x_stride, const u64 *add0,
                            const u64 *add1, u64 add_stride, u64 *out, u64 out_stride, size_t n_cts,
-                           u64 galois_kinv, const int64_t *hoisted, const u64 *ntt_digits, u64 perm_k);
+                           u64 galois_kinv, const int64_t *hoisted, const HoistedMac *hm);
 
 // Runs the key switch on one lane, or splits the batch over the context
 // stream (first half) and the auxiliary stream (second half): fork with an
@@ -1328,19 +1357,18 @@ static int key_switch_lane(const ckks_ksk *k, const u64 *x, u64 x_stride, const 
 // wait for the auxiliary one.  Each lane has its own workspaces.
 // hoisted: bounded keys only; the balanced digits [ct][D][N] of the unrotated
 // component x (ckks_bounded_digits), which replace its decomposition.
-// ntt_digits: bounded keys only; the lifted NTT-domain digits [ct][D][B][N]
-// of the unrotated component (ckks_bounded_hoist), read through the
-// NTT-domain Galois permutation of perm_k; decomposition and forward NTT
-// are skipped.
+// hm: bounded keys only; NTT-domain digits of the unrotated component
+// (ckks_bounded_hoist) with the key permuted by k^-1: the decomposition and
+// forward NTT are skipped and the MAC output is permuted by k.
 static int key_switch_core(const ckks_ksk *k, const u64 *x, u64 x_stride, const u64 *add0,
                            const u64 *add1, u64 add_stride, u64 *out, u64 out_stride, size_t n_cts,
                            u64 galois_kinv = 0, const int64_t *hoisted = nullptr,
-                           const u64 *ntt_digits = nullptr, u64 perm_k = 0) {
+                           const HoistedMac *hm = nullptr) {
     ckks_context *ctx = k->ctx;
     const int split = ctx->ks_split == CKKS_KS_SPLIT_AUTO ? k->tuned_split : ctx->ks_split;
     if (split != CKKS_KS_SPLIT_ON || n_cts < 8 || ctx->capturing || ctx->lane != 0)
         return key_switch_lane(k, x, x_stride, add0, add1, add_stride, out, out_stride, n_cts, galois_kinv,
-                               hoisted, ntt_digits, perm_k);
+                               hoisted, hm);
     if (!ctx->aux) {
         CK_HIP(hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking));
         CK_HIP(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
@@ -1352,15 +1380,19 @@ static int key_switch_core(const ckks_ksk *k, const u64 *x, u64 x_stride, const 
     hipStream_t main = ctx->stream;
     ctx->stream = ctx->aux;
     ctx->lane = 1;
+    HoistedMac hm2{};
+    if (hm) {
+        hm2 = *hm;
+        hm2.digits += n1 * k->dnum * k->limbs * k->degree;
+    }
     int rc = key_switch_lane(k, x + n1 * x_stride, x_stride, add0 ? add0 + n1 * add_stride : nullptr,
                              add1 ? add1 + n1 * add_stride : nullptr, add_stride, out + n1 * out_stride,
                              out_stride, n2, galois_kinv, hoisted ? hoisted + n1 * k->dnum * k->degree : nullptr,
-                             ntt_digits ? ntt_digits + n1 * k->dnum * k->limbs * k->degree : nullptr, perm_k);
+                             hm ? &hm2 : nullptr);
     ctx->stream = main;
     ctx->lane = 0;
     if (rc) return rc;
-    rc = key_switch_lane(k, x, x_stride, add0, add1, add_stride, out, out_stride, n1, galois_kinv, hoisted,
-                         ntt_digits, perm_k);
+    rc = key_switch_lane(k, x, x_stride, add0, add1, add_stride, out, out_stride, n1, galois_kinv, hoisted, hm);
     CK_HIP(hipEventRecord(ctx->ev_join, ctx->aux));
     CK_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
     return rc;
@@ -1368,7 +1400,7 @@ static int key_switch_core(const ckks_ksk *k, const u64 *x, u64 x_stride, const 
 
 static int key_switch_lane(const ckks_ksk *k, const u64 *x, u64 x_stride, const u64 *add0,
                            const u64 *add1, u64 add_stride, u64 *out, u64 out_stride, size_t n_cts,
-                           u64 galois_kinv, const int64_t *hoisted, const u64 *ntt_digits, u64 perm_k) {
+                           u64 galois_kinv, const int64_t *hoisted, const HoistedMac *hm) {
     ckks_context *ctx = k->ctx;
     const ckks_rns_plan *plan = k->plan;
     const u64 N = k->degree, B = k->limbs, D = k->dnum;
@@ -1376,9 +1408,12 @@ static int key_switch_lane(const ckks_ksk *k, const u64 *x, u64 x_stride, const 
     hipStream_t s = ctx->stream;
     int rc;
     void *wd = nullptr, *wa;
-    if (!ntt_digits && (rc = ctx->workspace(4, n_cts * D * B * N * sizeof(u64), &wd))) return rc;
+    // hoisted: slot 4 holds the unpermuted MAC output instead of the digits
+    const size_t slot4 = hm ? n_cts * 2 * B * N : n_cts * D * B * N;
+    if ((rc = ctx->workspace(4, slot4 * sizeof(u64), &wd))) return rc;
     if ((rc = ctx->workspace(5, n_cts * 2 * B * N * sizeof(u64), &wa))) return rc;
-    u64 *digits = ntt_digits ? const_cast<u64 *>(ntt_digits) : (u64 *)wd, *acc = (u64 *)wa;
+    u64 *digits = hm ? const_cast<u64 *>(hm->digits) : (u64 *)wd, *acc = (u64 *)wa;
+    u64 *mac_out = hm ? (u64 *)wd : acc;
     const u32 chunks = (u32)(N / kKsThreads);
     // 1. ModUp
     ModUpArgs mu{};
@@ -1395,7 +1430,7 @@ static int key_switch_lane(const ckks_ksk *k, const u64 *x, u64 x_stride, const 
     mu.chunks = chunks;
     const int engine = ctx->base_conversion == CKKS_BASECONV_AUTO ? k->tuned_engine : ctx->base_conversion;
     const bool mfma = k->mfma && engine == CKKS_BASECONV_MFMA;
-    if (ntt_digits) {
+    if (hm) {
         // decomposed, lifted and transformed once (ckks_bounded_hoist)
     } else if (k->bounded) {
         BoundedArgs ba{};
@@ -1438,22 +1473,21 @@ static int key_switch_lane(const ckks_ksk *k, const u64 *x, u64 x_stride, const 
     const bool two_pass = logn > 12 && logn <= 17;
     // CKKS_KS_FUSION_INVERSE: the MAC is computed inside the first inverse
     // NTT pass (step 4) and the accumulators never reach HBM in the NTT domain
-    const bool mac_in_inverse = fusion == CKKS_KS_FUSION_INVERSE && two_pass && D <= 8 && !ntt_digits;
-    if ((fusion == CKKS_KS_FUSION_XCD || fusion == CKKS_KS_FUSION_LINEAR) && two_pass && !ntt_digits) {
+    const bool mac_in_inverse = fusion == CKKS_KS_FUSION_INVERSE && two_pass && D <= 8 && !hm;
+    if ((fusion == CKKS_KS_FUSION_XCD || fusion == CKKS_KS_FUSION_LINEAR) && two_pass && !hm) {
         if ((rc = launch_ntt_mac(ctx, plan, digits, n_cts, D, B, k->d_kb, k->d_ka, acc,
                                  fusion == CKKS_KS_FUSION_XCD)))
             return rc;
     } else {
     NttCall f{plan, digits, digits, n_cts * D, B, B * N, B * N};
-    if (!ntt_digits && (rc = launch_ntt(ctx, &f, 1, true))) return rc;
+    if (!hm && (rc = launch_ntt(ctx, &f, 1, true))) return rc;
     if (!mac_in_inverse) {
     MacArgs ma{};
     ma.digits = digits;
-    ma.kb = k->d_kb;
-    ma.ka = k->d_ka;
-    ma.acc = acc;
+    ma.kb = hm ? hm->kb : k->d_kb;
+    ma.ka = hm ? hm->ka : k->d_ka;
+    ma.acc = mac_out;
     ma.primes = plan->d_primes;
-    ma.perm_k = ntt_digits ? perm_k : 0;
     ma.n_cts = (u32)n_cts;
     ma.dnum = (u32)D;
     ma.limbs = (u32)B;
@@ -1468,6 +1502,9 @@ static int key_switch_lane(const ckks_ksk *k, const u64 *x, u64 x_stride, const 
             hipLaunchKernelGGL(k_ks_mac_any, dim3((u32)(n_cts * B * chunks)), dim3(kKsThreads), 0, s, ma);
     }
     CK_HIP(hipGetLastError());
+    // hoisted: the MAC ran over sigma_k^-1-permuted keys; permuting its
+    // output by k gives sum_j NTT(sigma_k(d_j)) * key_j
+    if (hm && (rc = launch_ntt_permute(s, mac_out, acc, hm->k, (u64)n_cts * 2 * B * N, logn))) return rc;
     }
     }
     // 4. inverse NTT of both accumulators
@@ -1703,6 +1740,7 @@ int ckks_ksk_destroy(ckks_ksk *k) {
     hipSetDevice(k->ctx->device);
     hipStreamSynchronize(k->ctx->stream);
     k->ctx->keys.erase(k);
+    for (auto &e : k->hoist_keys) hipFree(e.second);
     hipFree(k->d_kb);
     hipFree(k->d_ka);
     hipFree(k->d_modup);
@@ -1924,9 +1962,26 @@ int ckks_bounded_rotate_hoisted(const ckks_hoisted *h, const ckks_ksk *key, uint
     for (const auto &t : h->ties)
         if (((u64)t.second * k & (two_n - 1)) >= key->degree)
             return ckks_apply_galois(key, exponent, in, out, n_cts, in_stride, out_stride);
+    const u64 kinv = galois_inverse(k, two_n);
+    const size_t DBN = key->dnum * key->limbs * key->degree;
+    auto it = key->hoist_keys.find(k);
+    if (it == key->hoist_keys.end()) {
+        ckks_context *ctx = key->ctx;
+        if (ctx->capturing) return fail(CKKS_ERR_INVALID, "hoisted key tables cannot be built during graph capture");
+        u64 *pk = nullptr;
+        CK_HIP(hipSetDevice(ctx->device));
+        CK_HIP(hipMalloc(&pk, 2 * DBN * sizeof(u64)));
+        if ((rc = launch_ntt_permute(ctx->stream, key->d_kb, pk, kinv, DBN, key->plan->logn)) ||
+            (rc = launch_ntt_permute(ctx->stream, key->d_ka, pk + DBN, kinv, DBN, key->plan->logn))) {
+            hipFree(pk);
+            return rc;
+        }
+        it = key->hoist_keys.emplace(k, pk).first;
+    }
+    const HoistedMac hm{h->d_digits, it->second, it->second + DBN, k};
     const size_t poly = key->q_limbs * key->degree;
-    return key_switch_core(key, in + poly, in_stride, in, nullptr, in_stride, out, out_stride, n_cts,
-                           galois_inverse(k, two_n), nullptr, h->d_digits, k);
+    return key_switch_core(key, in + poly, in_stride, in, nullptr, in_stride, out, out_stride, n_cts, kinv,
+                           nullptr, &hm);
 }
 
 int ckks_bounded_hoist_destroy(ckks_hoisted *h) {
