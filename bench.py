@@ -136,6 +136,52 @@ def measure_ntt(torch, plan, polys, limbs, n, reps=5):
     return limb_transforms, ntt_ms, ntt_gbps, intt_gbps
 
 
+# The 55-bit NTT prime pool of the reference's N=2^16 rotation benchmark
+# (ckks/hybrid_rns_galois_key.rs:1152-1200): Q = the first 13, P = the next 2.
+UHD_POOL = [0x7fffffffba0001, 0x7fffffffaa0001, 0x7fffffff7e0001, 0x7fffffff380001, 0x7ffffffef00001,
+            0x7ffffffeba0001, 0x7ffffffeac0001, 0x7ffffffe700001, 0x7ffffffe600001, 0x7ffffffe4c0001,
+            0x7ffffffe220001, 0x7ffffffe160001, 0x7ffffffdec0001, 0x7ffffffdd00001, 0x7ffffffdce0001]
+
+
+def bounded_rotation_benchmark(torch, pkg, ctx, batch=8, rotations=8):
+    """The reference's own rotation type: hybrid-QP Galois keys with bounded
+    base-2^32 digits (23 digits) at N=2^16, Q = 13 x 55-bit, P = 2 x 55-bit;
+    `batch` ciphertexts rotated by 2^0..2^(rotations-1) slots, each key
+    switch decomposing the rotated a (canonical) or hoisting the digits once
+    (ckks_bounded_hoist + ckks_bounded_rotate_hoisted, same residues)."""
+    N = 1 << 16
+    plan = pkg.PreparedRnsNttPlan(ctx, UHD_POOL, N)
+    D = pkg.bounded_digit_count(plan, 13, 32)
+    exps = [rotation_exponent_left(N, 1 << i) for i in range(rotations)]
+    keys = []
+    for i in range(rotations):
+        kb, ka = synthetic_key(torch, D, 15, N)
+        keys.append(pkg.BoundedKeySwitchKey(plan, 13, 32, kb, ka))
+    cts = rand_residues(torch, (batch, 2, 13, N), "cuda")
+    out = torch.empty_like(cts)
+    # warm: workspaces and the permuted key tables of every exponent
+    h = keys[0].hoist(cts)
+    for e, key in zip(exps, keys):
+        key.apply_galois(cts, e, out)
+        key.rotate_hoisted(h, cts, e, out)
+    h.close()
+    torch.cuda.synchronize()
+    with Timer(torch) as tc:
+        for e, key in zip(exps, keys):
+            key.apply_galois(cts, e, out)
+    with Timer(torch) as th:
+        h = keys[0].hoist(cts)
+        for e, key in zip(exps, keys):
+            key.rotate_hoisted(h, cts, e, out)
+    h.close()
+    n_rot = batch * rotations
+    return {"workload": f"{batch} ct x {rotations} rotations at N=2^16, Q 13 x 55-bit, P 2 x 55-bit, "
+                        f"bounded digits base 2^32 ({D} digits)",
+            "canonical_rotations_per_s": round(n_rot / (tc.ms / 1e3), 1),
+            "hoisted_rotations_per_s": round(n_rot / (th.ms / 1e3), 1),
+            "hoist_speedup": round(tc.ms / th.ms, 2)}
+
+
 def serving_benchmark(torch, pkg, ctx, n_requests=40, rate=4.0, seed=0):
     """Continuous-batching serving of a Poisson stream (seeded) of encrypted
     jobs on one GPU: 40% public 1024x1024 matrix x encrypted vector (N=2^14,
@@ -273,6 +319,8 @@ def main():
                     help="ntt: only the NTT measurement and a copy of known size (for counter runs); "
                          "serve: only the serving loop")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop measurement")
+    ap.add_argument("--no-bounded", action="store_true",
+                    help="skip the bounded-digit (canonical vs hoisted) rotation measurement")
     ap.add_argument("--serve-requests", type=int, default=40)
     ap.add_argument("--serve-rate", type=float, default=4.0)
     ap.add_argument("--split", choices=["auto", "off", "on"], default="auto",
@@ -471,6 +519,10 @@ def main():
             "traffic": traffic,
         },
     }
+    if not args.no_bounded:
+        print("# bounded-digit rotations", file=sys.stderr, flush=True)
+        result["bounded_hybrid_qp"] = bounded_rotation_benchmark(torch, pkg, ctx)
+        print(f"# bounded: {result['bounded_hybrid_qp']}", file=sys.stderr, flush=True)
     if not args.no_serving:
         print("# serving loop", file=sys.stderr, flush=True)
         result["serving"] = serving_benchmark(torch, pkg, ctx, args.serve_requests, args.serve_rate)
