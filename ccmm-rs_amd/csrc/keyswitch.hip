@@ -512,6 +512,51 @@ __global__ __launch_bounds__(kKsThreads) void k_ks_mac(MacArgs a) {
     }
 }
 
+// More than 8 digits over a batch (the bounded gadget: 23 digits at the
+// reference's N=2^16 shape): chunks of C digits, one launch each, with the
+// chunk's key entries in registers while the thread walks the batch, so a
+// key is read once per launch; the first chunk writes the accumulators and
+// later ones add to them.
+template <int C>
+__global__ __launch_bounds__(kKsThreads) void k_ks_mac_chunk(MacArgs a, u32 j0, u32 first) {
+    const u32 b = blockIdx.x;
+    const u32 chunk = b % a.chunks;
+    const u32 i = b / a.chunks;
+    const u64 N = (u64)1 << a.logn;
+    const u64 n = (u64)chunk * kKsThreads + threadIdx.x;
+    const DevPrime P = a.primes[i];
+    const u32 cnt = a.dnum - j0 < (u32)C ? a.dnum - j0 : (u32)C;
+    u64 kb[C], ka[C];
+#pragma unroll
+    for (int j = 0; j < C; ++j) {
+        const bool on = (u32)j < cnt;
+        kb[j] = on ? a.kb[((u64)(j0 + j) * a.limbs + i) * N + n] : 0;
+        ka[j] = on ? a.ka[((u64)(j0 + j) * a.limbs + i) * N + n] : 0;
+    }
+    for (u32 ct = 0; ct < a.n_cts; ++ct) {
+        const u64 *d = a.digits + (((u64)ct * a.dnum + j0) * a.limbs + i) * N + n;
+        Acc128 sb, sa;
+        sb.zero();
+        sa.zero();
+#pragma unroll
+        for (int j = 0; j < C; ++j) {
+            if ((u32)j < cnt) {
+                u64 dv = d[(u64)j * a.limbs * N];
+                sb.mac(dv, kb[j]);
+                sa.mac(dv, ka[j]);
+            }
+        }
+        u64 *o = a.acc + ((u64)ct * 2 * a.limbs + i) * N + n;
+        u64 rb = reduce128(sb, P), ra = reduce128(sa, P);
+        if (!first) {
+            rb = add_mod(rb, o[0], P.q);
+            ra = add_mod(ra, o[(u64)a.limbs * N], P.q);
+        }
+        o[0] = rb;
+        o[(u64)a.limbs * N] = ra;
+    }
+}
+
 // Generic digit count (the RNS gadget with singleton blocks has dnum = L):
 // key entries are streamed per ciphertext; partial sums are reduced every 8
 // products so the 128-bit accumulator cannot overflow.
@@ -1499,7 +1544,13 @@ static int key_switch_lane(const ckks_ksk *k, const u64 *x, u64 x_stride, const 
         CK_MAC(1) CK_MAC(2) CK_MAC(3) CK_MAC(4) CK_MAC(5) CK_MAC(6) CK_MAC(7) CK_MAC(8)
 #undef CK_MAC
         default:
-            hipLaunchKernelGGL(k_ks_mac_any, dim3((u32)(n_cts * B * chunks)), dim3(kKsThreads), 0, s, ma);
+            if (n_cts >= 2) {
+                for (u32 j0 = 0; j0 < (u32)D; j0 += 8)
+                    hipLaunchKernelGGL(k_ks_mac_chunk<8>, dim3((u32)(B * chunks)), dim3(kKsThreads), 0, s, ma, j0,
+                                       j0 == 0 ? 1u : 0u);
+            } else {
+                hipLaunchKernelGGL(k_ks_mac_any, dim3((u32)(n_cts * B * chunks)), dim3(kKsThreads), 0, s, ma);
+            }
     }
     CK_HIP(hipGetLastError());
     // hoisted: the MAC ran over sigma_k^-1-permuted keys; permuting its
