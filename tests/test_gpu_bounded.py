@@ -188,3 +188,29 @@ def test_ntt_domain_hoisted_rotations(pkg, ctx, logn, L, K, w):
                 assert (got == want).all(), f"exponent {k}"
     finally:
         hoisted.close()
+
+
+def test_hoisted_rotations_split_over_two_streams(pkg, ctx):
+    """Hoisted rotations of a batch split over two streams (lane offsets of
+    the hoisted digits, per-lane MAC workspaces) give the single-stream
+    residues of apply_galois."""
+    n, L, K, w = 1 << 13, 6, 2, 32
+    q_moduli = [Q0[0]] + Q40[:L - 1]
+    qp = q_moduli + P60[:K]
+    rng = np.random.default_rng(99)
+    plan = pkg.PreparedRnsNttPlan(ctx, qp, n)
+    count = O.bounded_digit_count(q_moduli, w)
+    key = pkg.BoundedKeySwitchKey(plan, L, w, *_key(rng, qp, n, count))
+    dct = pkg.to_device(rand_residues(rng, q_moduli, n, polys=18).reshape(9, 2, L, n))
+    k = O.rotation_exponent_left(n, 3)
+    ctx.set_key_switch_split(ctx.KS_SPLIT_OFF)
+    want = pkg.to_host(key.apply_galois(dct, k))
+    hoisted = key.hoist(dct)
+    try:
+        ctx.set_key_switch_split(ctx.KS_SPLIT_ON)
+        got = pkg.to_host(key.rotate_hoisted(hoisted, dct, k))
+        got_canonical = pkg.to_host(key.apply_galois(dct, k))
+    finally:
+        ctx.set_key_switch_split(ctx.KS_SPLIT_AUTO)
+        hoisted.close()
+    assert (got == want).all() and (got_canonical == want).all()
