@@ -39,7 +39,7 @@ def test_rns_gadget_key_switch_bit_exact(pkg, ctx, logn, L, blocks):
 @pytest.mark.parametrize("engine", ["mfma", "valu"])
 @pytest.mark.parametrize("logn,L,K,blocks", [(12, 4, 1, [2, 2]), (12, 4, 2, [1, 3]), (13, 9, 4, [5, 4]),
                                              (15, 13, 4, [5, 4, 4]), (16, 25, 7, [9, 8, 8]),
-                                             (12, 12, 3, [3, 9])])
+                                             (16, 17, 4, [5, 4, 4, 4]), (12, 12, 3, [3, 9])])
 def test_hybrid_key_switch_bit_exact(pkg, ctx, logn, L, K, blocks, engine):
     """Both base-conversion engines (int8 MFMA and VALU) against the oracle."""
     n = 1 << logn
