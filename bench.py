@@ -84,7 +84,7 @@ class Timer:
 def cpu_baseline(q_moduli, p_moduli, n, blocks, seconds=10.0):
     """The oracle's rotation (automorphism + hybrid key switch) on one core."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O  # the checker, timed here as the CPU baseline
+    import oracle as O  # the oracle, timed here as the CPU baseline
 
     O.build()
     rng = np.random.default_rng(0)
@@ -332,8 +332,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: every GPU box of this build has one; "
-              "sharding across GPUs is by ciphertext and is not run here", file=sys.stderr)
+        print("bench.py measures one GPU; sharding across GPUs is by ciphertext and is not "
+              "run by this benchmark", file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -3,7 +3,7 @@
  *
  * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference
  * Rust algorithms (Confidential-Intelligence-Lab/ccmm-rs, src/ring, src/grafting,
- * src/ckks).  It is the checker the HIP product path is compared against; it
+ * src/ckks).  It is the oracle the HIP product path is compared against; it
  * is never linked into, loaded by, or called from the product library.  Only
  * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it.
  *

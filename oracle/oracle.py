@@ -1,7 +1,7 @@
 """ctypes wrapper around the CPU oracle (libckks_oracle.so).
 
 TEST INFRASTRUCTURE ONLY: used by tests/, ``__graft_entry__.smoke()`` and the
-``cpu_baseline`` leg of bench.py as the checker.  The HIP product never
+``cpu_baseline`` leg of bench.py, as the oracle.  The HIP product never
 imports this module.
 
 All arrays are numpy ``uint64`` (residues), limb-major ``[limb][N]``.

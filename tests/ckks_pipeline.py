@@ -2,7 +2,7 @@
 config (N=4096, one 60-bit base prime + 3 40-bit primes, one 60-bit special
 prime, scale 2^40, ternary secret, Gaussian sigma 3.2, seed 0), key
 generation from host-sampled randomness, encryption and decryption.  Built
-on the oracle; used as the checker of the GPU pipeline."""
+on the oracle; the GPU pipeline is compared with it."""
 from __future__ import annotations
 
 import math

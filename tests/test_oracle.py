@@ -1,5 +1,5 @@
 """Pins the CPU oracle (a C restatement of the reference algorithms) before
-it is trusted as the checker of the GPU path.
+it is trusted as the oracle of the GPU path.
 
 Every oracle function is checked against something independent of its own
 code: the reference's known-answer vectors and fixtures, closed forms
