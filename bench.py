@@ -309,7 +309,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--mixed", type=int, default=16, help="ciphertexts at N=2^15 in each step")
-    ap.add_argument("--plain-sample", type=int, default=6, help="plain-path rotations timed")
+    ap.add_argument("--plain-sample", type=int, default=24, help="plain-path rotations timed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--fusion", choices=["auto", "off", "xcd", "linear", "inverse"], default="auto",
