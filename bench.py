@@ -12,10 +12,19 @@ Workload (batched core primitives, N = 2^16):
 
 Reported (one JSON line):
   value           batched hybrid-key-switched rotations per second
+  batched_n65536_rotations_per_s
+                  the N = 2^16 calls of the step alone (HIP events around each)
   plain_*         the plain per-op GPU path: the reference's own rotation
                   algorithm (RNS-gadget key switch with singleton blocks,
                   rotate_left_rns_ckks_with_prepared_ntt) one ciphertext per
-                  call, timed on a sample of rotations
+                  call, timed on a sample of rotations; speedup_vs_plain is the
+                  N = 2^16 batched rate over it
+  hybrid_one_ct_per_call_*
+                  the dnum = 3 hybrid key one ciphertext per call (batching
+                  alone, same algorithm)
+  bounded_hybrid_qp
+                  the reference's bounded-digit rotation shape, canonical
+                  and hoisted
   ntt_*           forward NTT over the batch's 96 digit limbs per ciphertext
   p50/p99_ms      latency of one batched rotation call (64 ciphertexts)
   roofline        the forward NTT (dominant kernel pair) against HBM peak
@@ -471,6 +480,15 @@ def main():
     del plain_key
     print(f"# plain: {tp.ms / args.plain_sample:.2f} ms/rotation, {plain_value:.0f} rot/s", file=sys.stderr,
           flush=True)
+    # the same dnum=3 hybrid key, one ciphertext per call (no batching)
+    k0, key0 = keys[0]
+    key0.apply_galois(one, k0, one_out)
+    torch.cuda.synchronize()
+    with Timer(torch) as th1:
+        for i in range(args.plain_sample):
+            key0.apply_galois(cts[i % B:i % B + 1], k0, one_out)
+    single_hybrid_value = args.plain_sample / (th1.ms / 1e3)
+    print(f"# hybrid, one ciphertext per call: {single_hybrid_value:.0f} rot/s", file=sys.stderr, flush=True)
 
     limb_transforms, ntt_ms, ntt_gbps, intt_gbps = measure_ntt(torch, qp_plan, B * len(blocks), L + K, N)
 
@@ -504,6 +522,8 @@ def main():
         "batched_n65536_rotations_per_s": round(n16_value, 1),
         "plain_rotations_per_s": round(plain_value, 1),
         "speedup_vs_plain": round(n16_value / plain_value, 2),
+        "hybrid_one_ct_per_call_rotations_per_s": round(single_hybrid_value, 1),
+        "speedup_vs_hybrid_one_ct_per_call": round(n16_value / single_hybrid_value, 2),
         "ntt_forward_gbps": round(ntt_gbps, 1),
         "ntt_inverse_gbps": round(intt_gbps, 1),
         "autotune": tuned,
