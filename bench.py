@@ -324,9 +324,9 @@ def main():
     ap.add_argument("--fusion", choices=["auto", "off", "xcd", "linear", "inverse"], default="auto",
                     help="digits' forward row pass fused with the key MAC: autotuned per key, "
                          "off, or fused with XCD-contiguous / linear block order")
-    ap.add_argument("--only", choices=["all", "ntt", "serve"], default="all",
+    ap.add_argument("--only", choices=["all", "ntt", "serve", "bounded"], default="all",
                     help="ntt: only the NTT measurement and a copy of known size (for counter runs); "
-                         "serve: only the serving loop")
+                         "serve: only the serving loop; bounded: only the bounded-digit rotations")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop measurement")
     ap.add_argument("--no-bounded", action="store_true",
                     help="skip the bounded-digit (canonical vs hoisted) rotation measurement")
@@ -370,6 +370,10 @@ def main():
     qp_plan = pkg.PreparedRnsNttPlan(ctx, q_moduli + p_moduli, N)
     qp2 = q_moduli[:L2] + p_moduli[:K2]
     qp_plan2 = pkg.PreparedRnsNttPlan(ctx, qp2, N2)
+
+    if args.only == "bounded":
+        print(json.dumps(bounded_rotation_benchmark(torch, pkg, ctx)), flush=True)
+        return
 
     if args.only == "serve":
         print(json.dumps(serving_benchmark(torch, pkg, ctx, args.serve_requests, args.serve_rate)), flush=True)

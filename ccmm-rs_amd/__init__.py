@@ -632,7 +632,14 @@ class BoundedKeySwitchKey(_KeySwitchKey):
         return HoistedDigits(self, ct)
 
     def rotate_hoisted(self, hoisted: HoistedDigits, ct, exponent: int, out=None):
-        """ckks_bounded_rotate_hoisted: same residues as apply_galois(ct, exponent)."""
+        """ckks_bounded_rotate_hoisted: same residues as apply_galois(ct, exponent).
+        ct must hold the ciphertexts the digits were hoisted from (their b
+        components are read); the first call per exponent builds a permuted
+        copy of this key, kept with the key."""
+        if self._n_cts(ct, 2) != hoisted.n_cts:
+            raise ValueError("ct must hold the ciphertexts the digits were hoisted from")
+        if out is not None and self._n_cts(out, 2) != hoisted.n_cts:
+            raise ValueError("out must hold as many ciphertexts as were hoisted")
         out = empty((hoisted.n_cts, 2, self.q_limbs, self.plan.degree)) if out is None else out
         _check(lib().ckks_bounded_rotate_hoisted(hoisted.handle, self._h, int(exponent), _ptr(ct), _ptr(out), 0, 0))
         return out

@@ -533,19 +533,29 @@ __global__ __launch_bounds__(kKsThreads) void k_ks_mac_chunk(MacArgs a, u32 j0, 
         kb[j] = on ? a.kb[((u64)(j0 + j) * a.limbs + i) * N + n] : 0;
         ka[j] = on ? a.ka[((u64)(j0 + j) * a.limbs + i) * N + n] : 0;
     }
+    // the next ciphertext's digits are loaded before this one's products,
+    // so a thread keeps 2C loads in flight
+    const u64 d_ct = (u64)a.dnum * a.limbs * N, d_j = (u64)a.limbs * N;
+    const u64 *d = a.digits + ((u64)j0 * a.limbs + i) * N + n;
+    u64 dv[C];
+#pragma unroll
+    for (int j = 0; j < C; ++j) dv[j] = (u32)j < cnt ? d[j * d_j] : 0;
     for (u32 ct = 0; ct < a.n_cts; ++ct) {
-        const u64 *d = a.digits + (((u64)ct * a.dnum + j0) * a.limbs + i) * N + n;
+        u64 nx[C];
+        const bool more = ct + 1 < a.n_cts;
+        d += d_ct;
+#pragma unroll
+        for (int j = 0; j < C; ++j) nx[j] = more && (u32)j < cnt ? d[j * d_j] : 0;
         Acc128 sb, sa;
         sb.zero();
         sa.zero();
 #pragma unroll
         for (int j = 0; j < C; ++j) {
-            if ((u32)j < cnt) {
-                u64 dv = d[(u64)j * a.limbs * N];
-                sb.mac(dv, kb[j]);
-                sa.mac(dv, ka[j]);
-            }
+            sb.mac(dv[j], kb[j]);
+            sa.mac(dv[j], ka[j]);
         }
+#pragma unroll
+        for (int j = 0; j < C; ++j) dv[j] = nx[j];
         u64 *o = a.acc + ((u64)ct * 2 * a.limbs + i) * N + n;
         u64 rb = reduce128(sb, P), ra = reduce128(sa, P);
         if (!first) {

@@ -358,7 +358,11 @@ int ckks_bounded_apply_galois_hoisted(const ckks_ksk *key, uint64_t exponent, co
  * the key MAC, so no decomposition or forward NTT runs.  A digit vector
  * holding -2^(base_log-1) carries when negated; when sigma_k negates one,
  * the call takes the ckks_apply_galois path.  Same residues as
- * ckks_apply_galois either way.  Destroy the handle before the context. */
+ * ckks_apply_galois either way.  The first call with a given key and
+ * exponent builds a copy of that key permuted by exponent^-1 (as much device
+ * memory as the key; kept until ckks_ksk_destroy; refused under graph
+ * capture), so calls with one key from several host threads must be
+ * serialised by the caller.  Destroy the handle before the context. */
 typedef struct ckks_hoisted ckks_hoisted;
 int ckks_bounded_hoist(const ckks_ksk *key, const uint64_t *in, size_t n_cts, size_t in_stride,
                        ckks_hoisted **out);

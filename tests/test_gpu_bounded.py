@@ -46,7 +46,9 @@ def _key(rng, plan_moduli, n, count):
     return rand_residues(rng, plan_moduli, n, count), rand_residues(rng, plan_moduli, n, count)
 
 
-@pytest.mark.parametrize("logn,L,w", [(12, 4, 20), (13, 6, 61), (16, 9, 40)])
+# digit counts 10, 5, 10, 21 and 31 over two ciphertexts: one key MAC launch
+# of up to 8, 16 or 24 digits, and two launches of 24 + 7
+@pytest.mark.parametrize("logn,L,w", [(12, 4, 20), (13, 6, 61), (16, 9, 40), (12, 6, 13), (12, 8, 11)])
 def test_bounded_relinearize_bit_exact(pkg, ctx, logn, L, w):
     n = 1 << logn
     moduli = [Q0[0]] + Q40[:L - 1]
@@ -64,7 +66,8 @@ def test_bounded_relinearize_bit_exact(pkg, ctx, logn, L, w):
         assert (got[c, 0] == bo).all() and (got[c, 1] == ao).all(), f"ct {c}"
 
 
-@pytest.mark.parametrize("logn,L,K,w,steps", [(12, 4, 1, 61, 1), (13, 9, 2, 32, 5), (15, 13, 4, 45, -3)])
+@pytest.mark.parametrize("logn,L,K,w,steps", [(12, 4, 1, 61, 1), (13, 9, 2, 32, 5), (15, 13, 4, 45, -3),
+                                               (12, 6, 2, 13, 3)])
 def test_hybrid_qp_rotation_bit_exact(pkg, ctx, logn, L, K, w, steps):
     n = 1 << logn
     q_moduli = [Q0[0]] + Q40[:L - 1]
