@@ -140,6 +140,9 @@ struct NttCall {
     size_t n_polys, n_limbs;
     size_t in_stride, out_stride;  // elements between consecutive polynomials
     const NttMac *mac = nullptr;   // inverse two-pass degrees only: see NttMac
+    // inverse only, 0 = off: the transform reads its input through the
+    // NTT-domain Galois permutation by this odd exponent (in != out)
+    u64 galois_k = 0;
 };
 int launch_ntt(ckks_context *ctx, const NttCall *calls, size_t count, bool forward);
 int ntt_variant_for(const ckks_context *ctx, u32 logn);

@@ -327,14 +327,22 @@ struct BoundedArgs {
     BoundedTable t;
     const DevPrime *primes;  // key basis B
     u32 q_limbs, limbs, logn, digit_count, base_log;
+    u32 small;  // 1: 2^(base_log-1) < every key prime (see lift_digit)
     u64 total;  // n_cts * N
 };
 
-CK_HD u64 lift_signed(int64_t d, const DevPrime &P) {
-    const Barrett br{P.q, P.br_hi, P.br_lo};
+// A balanced digit (|d| <= 2^(w-1)) lifted to [0, q) of key prime t: one
+// conditional add when 2^(w-1) < q (small, wave-uniform), a Barrett reduction
+// otherwise.  The prime table is read through the scalar unit.
+__device__ __forceinline__ u64 lift_digit(int64_t d, const CK_CONST_AS DevPrime *pr, u32 t, u32 small) {
+    if (small) {
+        const u64 q = pr[t].q;
+        return d >= 0 ? (u64)d : q - (u64)(-d);
+    }
+    const Barrett br = barrett_at(pr, t);
     if (d >= 0) return barrett_reduce128(0, (u64)d, br);
     const u64 m = barrett_reduce128(0, (u64)(-(d + 1)) + 1, br);
-    return m ? P.q - m : 0;
+    return m ? br.q - m : 0;
 }
 
 template <int W>
@@ -381,7 +389,8 @@ __global__ __launch_bounds__(kKsThreads) void k_bounded_decompose(BoundedArgs a)
                 a.raw[(ct * a.digit_count + j) * N + n] = d;
             } else {
                 u64 *o = out + (u64)j * a.limbs * N;
-                for (u32 t2 = 0; t2 < a.limbs; ++t2) o[(u64)t2 * N] = lift_signed(d, a.primes[t2]);
+                const CK_CONST_AS DevPrime *pr = const_table(a.primes);
+                for (u32 t2 = 0; t2 < a.limbs; ++t2) o[(u64)t2 * N] = lift_digit(d, pr, t2, a.small);
             }
             ++j;
         }
@@ -404,6 +413,7 @@ struct HoistArgs {
     const DevPrime *primes;
     u64 kinv;  // k^-1 mod 2N
     u32 limbs, logn, digit_count, base_log;
+    u32 small;  // as BoundedArgs::small
     u64 total;  // n_cts * N
 };
 
@@ -430,7 +440,8 @@ __global__ __launch_bounds__(kKsThreads) void k_bounded_hoist(HoistArgs a) {
             a.raw[(ct * a.digit_count + j) * N + n] = d;
         } else {
             u64 *o = a.lifted + ((ct * a.digit_count + j) * a.limbs) * N + n;
-            for (u32 t = 0; t < a.limbs; ++t) o[(u64)t * N] = lift_signed(d, a.primes[t]);
+            const CK_CONST_AS DevPrime *pr = const_table(a.primes);
+            for (u32 t = 0; t < a.limbs; ++t) o[(u64)t * N] = lift_digit(d, pr, t, a.small);
         }
     }
 }
@@ -1312,6 +1323,14 @@ static BoundedTable bounded_table(const u64 *d_blob, size_t q_limbs, unsigned W)
     return crt_table(d_blob, q_limbs, W);
 }
 
+// 1 when every one of the first `limbs` primes of the plan exceeds 2^(w-1)
+static u32 digits_below_primes(const ckks_rns_plan *plan, size_t limbs, unsigned w) {
+    if (w < 1 || w > 63) return 0;
+    for (size_t t = 0; t < limbs && t < plan->moduli.size(); ++t)
+        if (plan->moduli[t] <= ((u64)1 << (w - 1))) return 0;
+    return 1;
+}
+
 static int launch_bounded(hipStream_t s, const BoundedArgs &a, unsigned W) {
     const dim3 g((u32)((a.total + kKsThreads - 1) / kKsThreads));
     switch (W) {
@@ -1500,6 +1519,7 @@ static int key_switch_lane(const ckks_ksk *k, const u64 *x, u64 x_stride, const 
         ba.logn = logn;
         ba.digit_count = (u32)D;
         ba.base_log = k->base_log;
+        ba.small = digits_below_primes(plan, B, k->base_log);
         ba.total = (u64)n_cts * N;
         if (hoisted) {
             HoistArgs ha{};
@@ -1511,6 +1531,7 @@ static int key_switch_lane(const ckks_ksk *k, const u64 *x, u64 x_stride, const 
             ha.logn = logn;
             ha.digit_count = (u32)D;
             ha.base_log = k->base_log;
+            ha.small = ba.small;
             ha.total = (u64)n_cts * N;
             if ((rc = launch_hoist(s, ha))) return rc;
         } else if ((rc = launch_bounded(s, ba, k->bounded_words))) {
@@ -1563,14 +1584,14 @@ static int key_switch_lane(const ckks_ksk *k, const u64 *x, u64 x_stride, const 
             }
     }
     CK_HIP(hipGetLastError());
-    // hoisted: the MAC ran over sigma_k^-1-permuted keys; permuting its
-    // output by k gives sum_j NTT(sigma_k(d_j)) * key_j
-    if (hm && (rc = launch_ntt_permute(s, mac_out, acc, hm->k, (u64)n_cts * 2 * B * N, logn))) return rc;
     }
     }
-    // 4. inverse NTT of both accumulators
+    // 4. inverse NTT of both accumulators.  Hoisted: the MAC ran over
+    // sigma_k^-1-permuted keys; the inverse NTT reads its output permuted by
+    // k, which is sum_j NTT(sigma_k(d_j)) * key_j
     const NttMac mac{digits, k->d_kb, k->d_ka, (u32)D};
-    NttCall iv{plan, acc, acc, n_cts * 2, B, B * N, B * N, mac_in_inverse ? &mac : nullptr};
+    NttCall iv{plan, hm ? mac_out : acc, acc, n_cts * 2, B, B * N, B * N, mac_in_inverse ? &mac : nullptr,
+               hm ? hm->k : 0};
     if ((rc = launch_ntt(ctx, &iv, 1, false))) return rc;
     // 5. ModDown (or plain finalize) with pass-through adds
     DownArgs da{};
@@ -1700,6 +1721,7 @@ static int bounded_decompose_impl(const ckks_rns_plan *plan, size_t q_limbs, uns
     ba.logn = plan->logn;
     ba.digit_count = h.digit_count;
     ba.base_log = base_log;
+    ba.small = digits_below_primes(plan, plan->moduli.size(), base_log);
     ba.total = (u64)n_polys * N;
     return launch_bounded(ctx->stream, ba, h.words);
 }

@@ -25,7 +25,8 @@ struct DevNttDesc {
     u64 in_stride, out_stride;
     u32 n_polys, n_limbs, logn, block_start;
     u32 lt_begin;  // first limb transform (limb-major index) covered
-    NttMac mac;    // fused key inner product (k_ntt<.., MAC = true> only)
+    NttMac mac;    // fused key inner product (k_ntt<.., kSrcMac> only)
+    u32 galois_k;  // NTT-domain Galois permutation on the load (k_ntt<.., kSrcGalois> only)
 };
 
 constexpr int kMaxNttDesc = 16;
@@ -64,10 +65,26 @@ struct MacSrc {
     }
 };
 
+// Input of the first inverse pass read through the NTT-domain Galois
+// permutation (ring/ntt_galois_permutation.rs:5-19): element n of the limb is
+// in[((k (2n + 1)) mod 2N - 1) / 2], k odd; 2N divides 2^32, so the 32-bit
+// product may wrap.
+struct GaloisSrc {
+    u32 k;
+    __device__ __forceinline__ u64 operator()(const TileJob &job, u32 off, bool fp_initial) const {
+        const u32 m = (((2u * off + 1u) * k) & ((2u << job.logn) - 1u)) >> 1;
+        const u64 x = job.src[m];
+        return fp_initial ? f64_as_u64((double)x) : x;
+    }
+};
+
+enum { kSrcPlain = 0, kSrcMac = 1, kSrcGalois = 2 };
+
 // TB = log2 of the two-pass tile (12: 256 threads, 13: 512 threads);
 // BLK = blocked inter-pass layout.  Single-pass launches use <.., 12, false>.
-// MAC: first inverse pass whose input is the descriptor's key inner product.
-template <bool FWD, bool FIRST, int TB, bool BLK, bool MAC = false>
+// SRC: the first inverse pass may read the descriptor's key inner product
+// (kSrcMac) or its input permuted by a Galois exponent (kSrcGalois).
+template <bool FWD, bool FIRST, int TB, bool BLK, int SRC = kSrcPlain>
 __global__ __launch_bounds__(1 << (TB - 4)) void k_ntt(NttLaunch launch) {
     __shared__ u64 lds[1 << TB];
     const u32 b = blockIdx.x;
@@ -108,8 +125,16 @@ __global__ __launch_bounds__(1 << (TB - 4)) void k_ntt(NttLaunch launch) {
     u64 regs[16];
     BlockExec ex{1u << (TB - 4)};
     auto regs_of = [&](u32) -> u64(&)[16] { return regs; };
-    if (MAC) {
-        static_assert(!MAC || (!FWD && FIRST), "the key inner product feeds the first inverse pass");
+    static_assert(SRC == kSrcPlain || (!FWD && FIRST), "load variants feed the first inverse pass");
+    if (SRC == kSrcGalois) {
+        const GaloisSrc gs{D.galois_k};
+        if (P.fp)
+            ntt_dispatch<FWD, FIRST, true, TB, BLK>(ex, regs_of, job, lds, gs);
+        else
+            ntt_dispatch<FWD, FIRST, false, TB, BLK>(ex, regs_of, job, lds, gs);
+        return;
+    }
+    if (SRC == kSrcMac) {
         const u32 ct = poly >> 1, comp = poly & 1;
         MacSrc ms;
         ms.d_stride = (u64)D.n_limbs * N;
@@ -133,28 +158,30 @@ __global__ __launch_bounds__(1 << (TB - 4)) void k_ntt(NttLaunch launch) {
 }
 
 template <int TB, bool BLK>
-static void launch_variant(hipStream_t s, bool forward, bool first, const NttLaunch &l, u32 blocks, bool mac) {
+static void launch_variant(hipStream_t s, bool forward, bool first, const NttLaunch &l, u32 blocks, int src) {
     const dim3 g(blocks), t(1u << (TB - 4));
     if (forward) {
         if (first) hipLaunchKernelGGL((k_ntt<true, true, TB, BLK>), g, t, 0, s, l);
         else hipLaunchKernelGGL((k_ntt<true, false, TB, BLK>), g, t, 0, s, l);
     } else {
-        if (first && mac) hipLaunchKernelGGL((k_ntt<false, true, TB, BLK, true>), g, t, 0, s, l);
+        if (first && src == kSrcMac) hipLaunchKernelGGL((k_ntt<false, true, TB, BLK, kSrcMac>), g, t, 0, s, l);
+        else if (first && src == kSrcGalois)
+            hipLaunchKernelGGL((k_ntt<false, true, TB, BLK, kSrcGalois>), g, t, 0, s, l);
         else if (first) hipLaunchKernelGGL((k_ntt<false, true, TB, BLK>), g, t, 0, s, l);
         else hipLaunchKernelGGL((k_ntt<false, false, TB, BLK>), g, t, 0, s, l);
     }
 }
 
 static void launch_pass(hipStream_t s, bool forward, bool first, const NttLaunch &l, u32 blocks,
-                        int variant = 0, bool mac = false) {
+                        int variant = 0, int src = kSrcPlain) {
     if (!blocks) return;
     switch (variant) {
-    case CKKS_NTT_VARIANT_BLOCKED: launch_variant<12, true>(s, forward, first, l, blocks, mac); break;
-    case CKKS_NTT_VARIANT_TILE8K: launch_variant<13, false>(s, forward, first, l, blocks, mac); break;
+    case CKKS_NTT_VARIANT_BLOCKED: launch_variant<12, true>(s, forward, first, l, blocks, src); break;
+    case CKKS_NTT_VARIANT_TILE8K: launch_variant<13, false>(s, forward, first, l, blocks, src); break;
     case CKKS_NTT_VARIANT_TILE8K | CKKS_NTT_VARIANT_BLOCKED:
-        launch_variant<13, true>(s, forward, first, l, blocks, mac);
+        launch_variant<13, true>(s, forward, first, l, blocks, src);
         break;
-    default: launch_variant<12, false>(s, forward, first, l, blocks, mac); break;
+    default: launch_variant<12, false>(s, forward, first, l, blocks, src); break;
     }
 }
 
@@ -183,6 +210,9 @@ int launch_ntt(ckks_context *ctx, const NttCall *calls, size_t count, bool forwa
         if (c.mac && (forward || count != 1 || !ntt_two_pass(c.plan->logn) || c.n_polys % 2 ||
                       c.mac->dnum == 0 || c.mac->dnum > 8))
             return fail(CKKS_ERR_INVALID, "fused key inner product: one inverse two-pass call, 1..8 digits");
+        if (c.galois_k && (forward || count != 1 || c.mac || (c.galois_k & 1) == 0 ||
+                           c.galois_k >= 2 * (u64)c.plan->degree || c.in == c.out))
+            return fail(CKKS_ERR_INVALID, "Galois-permuted load: one inverse call, odd exponent below 2N, in != out");
         if (ntt_two_pass(c.plan->logn)) {
             total_two += c.n_polys * c.n_limbs * c.plan->degree;
             max_n = std::max(max_n, c.plan->degree);
@@ -200,9 +230,10 @@ int launch_ntt(ckks_context *ctx, const NttCall *calls, size_t count, bool forwa
         d.n_limbs = (u32)c.n_limbs;
         d.logn = c.plan->logn;
         if (c.mac) d.mac = *c.mac;
+        d.galois_k = (u32)c.galois_k;
         return d;
     };
-    const bool mac = calls[0].mac != nullptr;
+    const int src = calls[0].mac ? kSrcMac : calls[0].galois_k ? kSrcGalois : kSrcPlain;
     auto two_pass = [&](size_t i) { return calls[i].n_polys != 0 && ntt_two_pass(calls[i].plan->logn); };
     // single-pass entries (N <= 4096): one launch
     {
@@ -216,7 +247,7 @@ int launch_ntt(ckks_context *ctx, const NttCall *calls, size_t count, bool forwa
             l.d[l.n++] = d;
             blocks += d.n_polys * d.n_limbs;
         }
-        launch_pass(s, forward, true, l, blocks);
+        launch_pass(s, forward, true, l, blocks, 0, src);
     }
     if (total_two) {
         size_t cap = ctx->ntt_chunk_bytes / sizeof(u64);
@@ -267,7 +298,7 @@ int launch_ntt(ckks_context *ctx, const NttCall *calls, size_t count, bool forwa
                     lt = 0;
                 }
             }
-            launch_pass(s, forward, true, l, blocks, var, mac);
+            launch_pass(s, forward, true, l, blocks, var, src);
             launch_pass(s, forward, false, l, blocks, var);
         }
         }
