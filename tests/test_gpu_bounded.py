@@ -217,3 +217,37 @@ def test_hoisted_rotations_split_over_two_streams(pkg, ctx):
         ctx.set_key_switch_split(ctx.KS_SPLIT_AUTO)
         hoisted.close()
     assert (got == want).all() and (got_canonical == want).all()
+
+
+@pytest.mark.parametrize("logn,variant", [(13, 0), (13, 1), (14, 2), (14, 3)])
+def test_hoisted_rotations_in_every_ntt_variant(pkg, ctx, logn, variant):
+    """The hoisted rotation's inverse NTT reads the key MAC output through the
+    Galois permutation in its first pass; every two-pass kernel variant
+    (4096/8192-residue tiles, plain/blocked inter-pass layout) gives the
+    residues of the oracle's rotation."""
+    n, L, K, w = 1 << logn, 4, 1, 32
+    q_moduli = [Q0[0]] + Q40[:L - 1]
+    qp = q_moduli + P60[:K]
+    rng = np.random.default_rng(logn * 13 + variant)
+    plan = pkg.PreparedRnsNttPlan(ctx, qp, n)
+    count = O.bounded_digit_count(q_moduli, w)
+    kb, ka = _key(rng, qp, n, count)
+    key = pkg.BoundedKeySwitchKey(plan, L, w, kb, ka)
+    ct = rand_residues(rng, q_moduli, n, polys=4).reshape(2, 2, L, n)
+    k = O.rotation_exponent_left(n, 3)
+    saved = ctx.ntt_variant(logn)
+    ctx.set_ntt_variant(logn, variant)
+    try:
+        hoisted = key.hoist(pkg.to_device(ct))
+        try:
+            got = pkg.to_host(key.rotate_hoisted(hoisted, pkg.to_device(ct), k))
+        finally:
+            hoisted.close()
+    finally:
+        ctx.set_ntt_variant(logn, saved)
+    orc = O.RnsPlan(qp, n)
+    for c in range(2):
+        sb = np.stack([O.apply_automorphism(ct[c, 0, i], q, k) for i, q in enumerate(q_moduli)])
+        sa = np.stack([O.apply_automorphism(ct[c, 1, i], q, k) for i, q in enumerate(q_moduli)])
+        bo, ao = O.bounded_key_switch(orc, L, w, sb, sa, kb, ka)
+        assert (got[c, 0] == bo).all() and (got[c, 1] == ao).all(), f"ct {c}"
