@@ -27,22 +27,25 @@ struct CrtTable {
 
 // x: limb i of the coefficient at x[i * limb_stride]; neg: the gathered
 // coefficient is negated (Galois gather).  Returns true for a negative value.
+// The prime and CRT tables are wave-uniform and read through the scalar unit
+// (no kernel writes them).
 template <int W>
 __device__ __forceinline__ bool crt_centered(const u64 *x, u64 limb_stride, bool neg, u32 q_limbs,
                                              const DevPrime *primes, const CrtTable &t, u64 (&X)[W]) {
 #pragma unroll
     for (int k = 0; k < W; ++k) X[k] = 0;
+    const CK_CONST_AS DevPrime *pr = const_table(primes);
     for (u32 i = 0; i < q_limbs; ++i) {
-        const DevPrime &P = primes[i];
-        const Barrett br{P.q, P.br_hi, P.br_lo};
+        const Barrett br{pr[i].q, pr[i].br_hi, pr[i].br_lo};
+        const u64 q = br.q;
         u64 xi = x[(u64)i * limb_stride];
-        if (neg) xi = neg_mod(xi, P.q);
-        const u64 *pw = t.pow64 + (u64)i * W;
+        if (neg) xi = neg_mod(xi, q);
+        const CK_CONST_AS u64 *pw = const_table(t.pow64) + (u64)i * W;
         u64 r = 0;
 #pragma unroll
-        for (int k = 0; k < W; ++k) r = add_mod(r, mul_mod(X[k], pw[k], br), P.q);
-        const u64 v = mul_mod(sub_mod(xi, r, P.q), t.inv[i], br);
-        const u64 *pre = t.prefix + (u64)i * W;
+        for (int k = 0; k < W; ++k) r = add_mod(r, mul_mod(X[k], pw[k], br), q);
+        const u64 v = mul_mod(sub_mod(xi, r, q), const_table(t.inv)[i], br);
+        const CK_CONST_AS u64 *pre = const_table(t.prefix) + (u64)i * W;
         u64 carry = 0;
 #pragma unroll
         for (int k = 0; k < W; ++k) {
@@ -56,16 +59,18 @@ __device__ __forceinline__ bool crt_centered(const u64 *x, u64 limb_stride, bool
             carry = hi;
         }
     }
+    const CK_CONST_AS u64 *half = const_table(t.half);
+    const CK_CONST_AS u64 *qw = const_table(t.qw);
     int cmp = 0;
 #pragma unroll
     for (int k = W - 1; k >= 0; --k)
-        if (cmp == 0 && X[k] != t.half[k]) cmp = X[k] > t.half[k] ? 1 : -1;
+        if (cmp == 0 && X[k] != half[k]) cmp = X[k] > half[k] ? 1 : -1;
     const bool negative = cmp > 0;
     if (negative) {
         u64 borrow = 0;
 #pragma unroll
         for (int k = 0; k < W; ++k) {
-            const u64 q = t.qw[k];
+            const u64 q = qw[k];
             const u64 d1 = X[k] - q;
             const u64 b1 = X[k] < q;
             const u64 d2 = d1 - borrow;
