@@ -76,10 +76,11 @@ struct NttPass {
         constexpr int K = round_k(L, I);
         constexpr int s0 = round_start(L, I);
         constexpr bool row = (KIND == PASS_ROW);
+        constexpr u32 UM = uniform_twiddle_mask(g, L, CL, K, s0, FWD, row);
         if (FWD)
-            fwd_round<L, CL, K, FP>(v, g, tid, s0, row, pa, job);
+            fwd_round<L, CL, K, FP, UM>(v, g, tid, s0, row, pa, job);
         else
-            inv_round<L, CL, K, FP>(v, g, tid, s0, row, pa, job);
+            inv_round<L, CL, K, FP, UM>(v, g, tid, s0, row, pa, job);
     }
 
     // global offset (within the limb) of the element at tile coordinate e on

@@ -279,8 +279,47 @@ CK_HD u32 row_twiddle_base(const PassAddr &pa, u32 e) {
 
 // ---- one round of butterflies on the 16 registers -------------------------------
 
+// True when bits [lo, hi) of the tile coordinate are the same in every lane
+// of a wave64 for each register: register bits, or thread bits mapped to
+// thread-id bits >= 6.
+constexpr bool wave_uniform_bits(const Geom &g, int lo, int hi) {
+    for (int b = lo; b < hi; ++b) {
+        bool ok = false;
+        for (int r = 0; r < 4; ++r) ok = ok || g.reg_pos[r] == b;
+        for (int i = 6; i < g.n_thr; ++i) ok = ok || g.thr_pos[i] == b;
+        if (!ok) return false;
+    }
+    return true;
+}
+
+// Bit j set: the twiddle index of stage j of the round is wave-uniform (it
+// reads t bits [L - s, L) forward, [u + 1, L) inverse, and the row bits in a
+// row pass), so the twiddle is read through the scalar unit.
+constexpr u32 uniform_twiddle_mask(const Geom &g, int L, int CL, int K, int s0, bool fwd, bool row) {
+    u32 m = 0;
+    for (int j = 0; j < K; ++j) {
+        const int s = s0 + j;
+        const bool t_ok = fwd ? wave_uniform_bits(g, L - s, L) : wave_uniform_bits(g, s + 1, L);
+        if (t_ok && (!row || wave_uniform_bits(g, L, L + CL))) m |= 1u << j;
+    }
+    return m;
+}
+
+// Twiddle idx of the table; UNIFORM: idx is the same in every lane.
+template <bool UNIFORM>
+CK_HD Tw load_twiddle(const Tw *tw, u32 idx) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (UNIFORM) {
+        const u32 ui = __builtin_amdgcn_readfirstlane(idx);
+        const CK_CONST_AS Tw *ct = const_table(tw);
+        return Tw{ct[ui].w, ct[ui].wq};
+    }
+#endif
+    return tw[idx];
+}
+
 // Forward round: stages s0..s0+k-1 (butterfly bit of stage s is L-1-s).
-template <int L, int CL, int K, bool FP>
+template <int L, int CL, int K, bool FP, u32 UM = 0>
 CK_HD void fwd_round(u64 (&v)[16], const Geom &g, u32 tid, int s0, bool row_base,
                      const PassAddr &pa, const TileJob &job) {
 #pragma unroll
@@ -295,7 +334,7 @@ CK_HD void fwd_round(u64 (&v)[16], const Geom &g, u32 tid, int s0, bool row_base
             u32 base = row_base ? row_twiddle_base<L, CL>(pa, e) : 1u;
             u32 idx = (base << s) + (t >> (L - s));
             CK_BOUNDS(idx, 1u << pa.logn);
-            Tw w = job.tw[idx];
+            const Tw w = (UM >> j) & 1u ? load_twiddle<true>(job.tw, idx) : load_twiddle<false>(job.tw, idx);
             if (FP)
                 ct_bfly_fp(v[i], v[i | (1 << rb)], w, job.qd);
             else
@@ -305,7 +344,7 @@ CK_HD void fwd_round(u64 (&v)[16], const Geom &g, u32 tid, int s0, bool row_base
 }
 
 // Inverse round: GS stages u0..u0+k-1 (butterfly bit of stage u is u).
-template <int L, int CL, int K, bool FP>
+template <int L, int CL, int K, bool FP, u32 UM = 0>
 CK_HD void inv_round(u64 (&v)[16], const Geom &g, u32 tid, int u0, bool row_base,
                      const PassAddr &pa, const TileJob &job) {
 #pragma unroll
@@ -320,7 +359,7 @@ CK_HD void inv_round(u64 (&v)[16], const Geom &g, u32 tid, int u0, bool row_base
             u32 base = row_base ? row_twiddle_base<L, CL>(pa, e) : 1u;
             u32 idx = (base << (L - 1 - u)) + (t >> (u + 1));
             CK_BOUNDS(idx, 1u << pa.logn);
-            Tw w = job.tw[idx];
+            const Tw w = (UM >> j) & 1u ? load_twiddle<true>(job.tw, idx) : load_twiddle<false>(job.tw, idx);
             if (FP && j == K - 1)
                 gs_bfly_fp<true>(v[i], v[i | (1 << rb)], w, job.qd, job.qinv);
             else if (FP)

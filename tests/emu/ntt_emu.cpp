@@ -135,3 +135,83 @@ extern "C" uint64_t emu_fp_moddown_target(uint64_t x, const uint64_t *c, const u
     v = add_neg ? v - ad : v + ad;
     return fp_residue(v, qd, 1.0 / qd);
 }
+
+// ---- wave-uniform twiddle stages ------------------------------------------
+// Stages that uniform_twiddle_mask flags read their twiddle through the
+// scalar unit (readfirstlane of the index).  For every pass geometry the
+// kernels instantiate, walk each flagged stage and count waves whose 64
+// lanes disagree on the twiddle index.
+template <int L, int CL, PassKind KIND, bool FWD>
+static void uniform_pass(u32 logn, int &flagged, int &bad) {
+    typedef NttPass<L, CL, KIND, FWD, false> P;
+    const int T = (1 << (L + CL)) / 16;
+    PassAddr pa{};
+    pa.logn = logn;
+    pa.n2 = (logn + 1) / 2;
+    pa.n1 = logn - pa.n2;
+    const u32 tiles = KIND == PASS_FULL ? 1 : (1u << logn) >> (L + CL);
+    const bool row = KIND == PASS_ROW;
+    for (int I = 0; I < n_rounds(L); ++I) {
+        const Geom g = P::geom(I);
+        const int K = round_k(L, I), s0 = round_start(L, I);
+        const u32 um = uniform_twiddle_mask(g, L, CL, K, s0, FWD, row);
+        for (int j = 0; j < K; ++j) {
+            if (!((um >> j) & 1)) continue;
+            ++flagged;
+            const int s = s0 + j, rb = FWD ? K - 1 - j : j;
+            for (u32 tile = 0; tile < tiles; ++tile) {
+                pa.tile = tile;
+                for (int i = 0; i < 16; ++i) {
+                    if (i & (1 << rb)) continue;
+                    for (int w0 = 0; w0 < T; w0 += 64) {
+                        long first = -1;
+                        for (int tid = w0; tid < w0 + 64 && tid < T; ++tid) {
+                            const u32 e = coord(g, tid, i), t = e & ((1u << L) - 1);
+                            const u32 base = row ? row_twiddle_base<L, CL>(pa, e) : 1u;
+                            const u32 idx = FWD ? (base << s) + (t >> (L - s)) : (base << (L - 1 - s)) + (t >> (s + 1));
+                            if (first < 0) first = idx;
+                            else if ((long)idx != first) ++bad;
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
+template <int L, int CL, PassKind KIND>
+static void uniform_both(u32 logn, int &flagged, int &bad) {
+    uniform_pass<L, CL, KIND, true>(logn, flagged, bad);
+    uniform_pass<L, CL, KIND, false>(logn, flagged, bad);
+}
+
+// Returns the number of non-uniform waves over every flagged stage; the
+// number of flagged stages goes to *flagged.
+extern "C" int emu_twiddle_uniform(int *flagged_out) {
+    int flagged = 0, bad = 0;
+    uniform_both<10, 0, PASS_FULL>(10, flagged, bad);
+    uniform_both<11, 0, PASS_FULL>(11, flagged, bad);
+    uniform_both<12, 0, PASS_FULL>(12, flagged, bad);
+    // two-pass, 4096-residue tiles: column L = n1, row L = n2 = ceil(logn / 2)
+    uniform_both<6, 6, PASS_COL>(13, flagged, bad);
+    uniform_both<7, 5, PASS_ROW>(13, flagged, bad);
+    uniform_both<7, 5, PASS_COL>(14, flagged, bad);
+    uniform_both<7, 5, PASS_ROW>(14, flagged, bad);
+    uniform_both<7, 5, PASS_COL>(15, flagged, bad);
+    uniform_both<8, 4, PASS_ROW>(15, flagged, bad);
+    uniform_both<8, 4, PASS_COL>(16, flagged, bad);
+    uniform_both<8, 4, PASS_ROW>(16, flagged, bad);
+    uniform_both<8, 4, PASS_COL>(17, flagged, bad);
+    uniform_both<9, 3, PASS_ROW>(17, flagged, bad);
+    // 8192-residue tiles (N >= 2^14)
+    uniform_both<7, 6, PASS_COL>(14, flagged, bad);
+    uniform_both<7, 6, PASS_ROW>(14, flagged, bad);
+    uniform_both<7, 6, PASS_COL>(15, flagged, bad);
+    uniform_both<8, 5, PASS_ROW>(15, flagged, bad);
+    uniform_both<8, 5, PASS_COL>(16, flagged, bad);
+    uniform_both<8, 5, PASS_ROW>(16, flagged, bad);
+    uniform_both<8, 5, PASS_COL>(17, flagged, bad);
+    uniform_both<9, 4, PASS_ROW>(17, flagged, bad);
+    *flagged_out = flagged;
+    return bad;
+}

@@ -194,3 +194,13 @@ def test_emulated_fp64_inverse_extremes(emu, logn):
         assert (back == plan.inverse(y)).all()
         assert emu.emu_ntt_variant(y.ctypes.data, back.ctypes.data, q, logn, 0, 13 if logn >= 14 else 12, 1) == 0
         assert (back == plan.inverse(y)).all()
+
+
+def test_scalar_twiddle_stages_are_wave_uniform(emu):
+    """The NTT stages whose twiddle the kernels read through the scalar unit
+    (readfirstlane of the index) have one twiddle index per wave64 in every
+    pass geometry the kernels instantiate."""
+    flagged = C.c_int(0)
+    bad = emu.emu_twiddle_uniform(C.byref(flagged))
+    assert flagged.value > 0
+    assert bad == 0
