@@ -172,10 +172,23 @@ def empty(shape, device="cuda"):
     return torch.empty(shape, dtype=torch.int64, device=device)
 
 
-def _ptr(t) -> int:
+def _ptr(t, numel: int | None = None, dtype: str = "int64", name: str = "tensor") -> int:
+    """Device address of a contiguous GPU tensor of the given dtype holding at
+    least `numel` elements; anything else raises ValueError before the library
+    sees the pointer (a host pointer or a short buffer would fault the GPU)."""
     if t is None:
         return None
-    assert t.is_contiguous() or t.dim() == 0
+    torch = _torch()
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f"{name} must be a torch tensor")
+    if t.dtype != getattr(torch, dtype):
+        raise ValueError(f"{name} must be {dtype}, not {str(t.dtype).replace('torch.', '')}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    if numel is not None and t.numel() < numel:
+        raise ValueError(f"{name} holds {t.numel()} elements, {numel} needed")
+    if not t.is_cuda:
+        raise ValueError(f"{name} must be a GPU tensor")
     return t.data_ptr()
 
 
@@ -333,27 +346,40 @@ class PreparedRnsNttPlan:
     def _shape(self, t, n_limbs):
         n_limbs = n_limbs or len(self.moduli)
         per = n_limbs * self.degree
-        assert t.numel() % per == 0, "tensor size must be a multiple of limbs x N"
+        if t.numel() % per != 0 or t.numel() == 0:
+            raise ValueError("tensor size must be a positive multiple of limbs x N")
         return t.numel() // per, n_limbs, per
+
+    def _count(self, t, limbs_per_item: int) -> int:
+        """Number of items of limbs_per_item x N residues in t (ValueError
+        unless t holds a positive whole number of them)."""
+        per = int(limbs_per_item) * self.degree
+        if per <= 0 or t.numel() == 0 or t.numel() % per != 0:
+            raise ValueError(f"tensor size must be a positive multiple of {limbs_per_item} x N")
+        return t.numel() // per
 
     def forward(self, x, out=None, n_limbs=None):
         """Forward negacyclic NTT of every limb of every polynomial in x
         (limb-major [poly][limb][N]); out may be x."""
         n_polys, n_limbs, per = self._shape(x, n_limbs)
         out = x if out is None else out
-        _check(lib().ckks_ntt_forward(self._h, _ptr(x), _ptr(out), n_polys, n_limbs, per))
+        _check(lib().ckks_ntt_forward(self._h, _ptr(x, n_polys * per, name="x"),
+                                      _ptr(out, n_polys * per, name="out"), n_polys, n_limbs, per))
         return out
 
     def inverse(self, x, out=None, n_limbs=None):
         n_polys, n_limbs, per = self._shape(x, n_limbs)
         out = x if out is None else out
-        _check(lib().ckks_ntt_inverse(self._h, _ptr(x), _ptr(out), n_polys, n_limbs, per))
+        _check(lib().ckks_ntt_inverse(self._h, _ptr(x, n_polys * per, name="x"),
+                                      _ptr(out, n_polys * per, name="out"), n_polys, n_limbs, per))
         return out
 
     # --- pointwise (NttPolynomial / Polynomial arithmetic) ---
     def _ew(self, fn, x, y, out, n_limbs):
         n_polys, n_limbs, per = self._shape(x, n_limbs)
-        _check(fn(self._h, _ptr(x), _ptr(y), _ptr(out), n_polys, n_limbs, per))
+        m = n_polys * per
+        _check(fn(self._h, _ptr(x, m, name="x"), _ptr(y, m, name="y"), _ptr(out, m, name="out"),
+                  n_polys, n_limbs, per))
         return out
 
     def pointwise_mul(self, x, y, out=None, n_limbs=None):
@@ -362,7 +388,9 @@ class PreparedRnsNttPlan:
 
     def pointwise_mul_add_assign(self, acc, x, y, n_limbs=None):
         n_polys, n_limbs, per = self._shape(acc, n_limbs)
-        _check(lib().ckks_poly_mac(self._h, _ptr(acc), _ptr(x), _ptr(y), n_polys, n_limbs, per))
+        m = n_polys * per
+        _check(lib().ckks_poly_mac(self._h, _ptr(acc, m, name="acc"), _ptr(x, m, name="x"),
+                                   _ptr(y, m, name="y"), n_polys, n_limbs, per))
         return acc
 
     def add(self, x, y, out=None, n_limbs=None):
@@ -376,21 +404,24 @@ class PreparedRnsNttPlan:
     def neg(self, x, out=None, n_limbs=None):
         out = empty(x.shape) if out is None else out
         n_polys, n_limbs, per = self._shape(x, n_limbs)
-        _check(lib().ckks_poly_neg(self._h, _ptr(x), _ptr(out), n_polys, n_limbs, per))
+        _check(lib().ckks_poly_neg(self._h, _ptr(x, n_polys * per, name="x"),
+                                   _ptr(out, n_polys * per, name="out"), n_polys, n_limbs, per))
         return out
 
     def apply_automorphism(self, x, exponent: int, out=None, n_limbs=None):
         """apply_automorphism (ckks/automorphism.rs:8) per limb, coefficient domain."""
         out = empty(x.shape) if out is None else out
         n_polys, n_limbs, per = self._shape(x, n_limbs)
-        _check(lib().ckks_automorphism(self._h, _ptr(x), _ptr(out), n_polys, n_limbs, per, exponent))
+        _check(lib().ckks_automorphism(self._h, _ptr(x, n_polys * per, name="x"),
+                                       _ptr(out, n_polys * per, name="out"), n_polys, n_limbs, per, exponent))
         return out
 
     def permute_negacyclic_ntt_galois(self, x, exponent: int, out=None, n_limbs=None):
         """permute_negacyclic_ntt_galois (ring/ntt_galois_permutation.rs:5)."""
         out = empty(x.shape) if out is None else out
         n_polys, n_limbs, per = self._shape(x, n_limbs)
-        _check(lib().ckks_ntt_galois_permute(self._h, _ptr(x), _ptr(out), n_polys, n_limbs, per,
+        _check(lib().ckks_ntt_galois_permute(self._h, _ptr(x, n_polys * per, name="x"),
+                                             _ptr(out, n_polys * per, name="out"), n_polys, n_limbs, per,
                                              exponent))
         return out
 
@@ -399,17 +430,19 @@ class PreparedRnsNttPlan:
         """rescale_rns_ckks_to_next (ckks/rns_rescale.rs:19) on a batch of
         ciphertexts [ct][2][n_limbs][N] -> [ct][2][n_limbs-1][N]."""
         n = self.degree
-        n_cts = ct.numel() // (2 * n_limbs * n)
+        n_cts = self._count(ct, 2 * n_limbs)
         out = empty((n_cts, 2, n_limbs - 1, n)) if out is None else out
-        _check(lib().ckks_rescale(self._h, _ptr(ct), _ptr(out), n_cts, n_limbs, 2 * n_limbs * n,
-                                  2 * (n_limbs - 1) * n))
+        _check(lib().ckks_rescale(self._h, _ptr(ct, n_cts * 2 * n_limbs * n, name="ct"),
+                                  _ptr(out, n_cts * 2 * (n_limbs - 1) * n, name="out"), n_cts, n_limbs,
+                                  2 * n_limbs * n, 2 * (n_limbs - 1) * n))
         return out
 
     def mod_switch_to_next(self, ct, n_limbs: int, out=None):
         n = self.degree
-        n_cts = ct.numel() // (2 * n_limbs * n)
+        n_cts = self._count(ct, 2 * n_limbs)
         out = empty((n_cts, 2, n_limbs - 1, n)) if out is None else out
-        _check(lib().ckks_mod_switch(self._h, _ptr(ct), _ptr(out), n_cts, n_limbs,
+        _check(lib().ckks_mod_switch(self._h, _ptr(ct, n_cts * 2 * n_limbs * n, name="ct"),
+                                     _ptr(out, n_cts * 2 * (n_limbs - 1) * n, name="out"), n_cts, n_limbs,
                                      2 * n_limbs * n, 2 * (n_limbs - 1) * n))
         return out
 
@@ -418,10 +451,12 @@ class PreparedRnsNttPlan:
         ct [n][2][n_limbs][N] (coefficient domain) times NTT-domain plaintexts
         pt_ntt ([n_limbs][N] shared, or [n][n_limbs][N] one per ciphertext)."""
         n = self.degree
-        n_cts = ct.numel() // (2 * n_limbs * n)
+        n_cts = self._count(ct, 2 * n_limbs)
         per_ct = pt_ntt.numel() == n_cts * n_limbs * n and n_cts > 1
         out = empty((n_cts, 2, n_limbs, n)) if out is None else out
-        _check(lib().ckks_multiply_plain(self._h, _ptr(ct), _ptr(pt_ntt), _ptr(out), n_cts, n_limbs,
+        _check(lib().ckks_multiply_plain(self._h, _ptr(ct, n_cts * 2 * n_limbs * n, name="ct"),
+                                         _ptr(pt_ntt, (n_cts if per_ct else 1) * n_limbs * n, name="pt_ntt"),
+                                         _ptr(out, n_cts * 2 * n_limbs * n, name="out"), n_cts, n_limbs,
                                          2 * n_limbs * n, n_limbs * n if per_ct else 0,
                                          2 * n_limbs * n))
         return out
@@ -429,9 +464,11 @@ class PreparedRnsNttPlan:
     def tensor(self, ct0, ct1, n_limbs: int, out=None):
         """rns_tensor_with_prepared_ntt (grafting/rns_key_switch.rs:1724)."""
         n = self.degree
-        n_cts = ct0.numel() // (2 * n_limbs * n)
+        n_cts = self._count(ct0, 2 * n_limbs)
         out = empty((n_cts, 3, n_limbs, n)) if out is None else out
-        _check(lib().ckks_tensor(self._h, _ptr(ct0), _ptr(ct1), _ptr(out), n_cts, n_limbs,
+        m = n_cts * 2 * n_limbs * n
+        _check(lib().ckks_tensor(self._h, _ptr(ct0, m, name="ct0"), _ptr(ct1, m, name="ct1"),
+                                 _ptr(out, n_cts * 3 * n_limbs * n, name="out"), n_cts, n_limbs,
                                  2 * n_limbs * n, 3 * n_limbs * n))
         return out
 
@@ -439,9 +476,10 @@ class PreparedRnsNttPlan:
         """auxiliary_mod_down_rns_candidate (ring/auxiliary_modulus.rs:89): x
         [p][plan limbs][N] (Q then P) -> [p][q_limbs][N]."""
         n = self.degree
-        polys = x.numel() // (len(self.moduli) * n)
+        polys = self._count(x, len(self.moduli))
         out = empty((polys, q_limbs, n)) if out is None else out
-        _check(lib().ckks_mod_down(self._h, int(q_limbs), _ptr(x), _ptr(out), polys))
+        _check(lib().ckks_mod_down(self._h, int(q_limbs), _ptr(x, polys * len(self.moduli) * n, name="x"),
+                                   _ptr(out, polys * int(q_limbs) * n, name="out"), polys))
         return out
 
     def encrypt_raw(self, message, error, secret, a, out=None, n_limbs=None):
@@ -450,7 +488,11 @@ class PreparedRnsNttPlan:
         [p][limbs][N]; returns b."""
         n_polys, n_limbs, per = self._shape(message, n_limbs)
         out = empty(message.shape) if out is None else out
-        _check(lib().ckks_encrypt_raw(self._h, _ptr(message), _ptr(error), _ptr(secret), _ptr(a), _ptr(out),
+        m, n = n_polys * per, self.degree
+        _check(lib().ckks_encrypt_raw(self._h, _ptr(message, m, name="message"),
+                                      _ptr(error, n_polys * n, name="error"),
+                                      _ptr(secret, n, dtype="int8", name="secret"), _ptr(a, m, name="a"),
+                                      _ptr(out, m, name="out"),
                                       n_polys, n_limbs, per))
         return out
 
@@ -458,7 +500,10 @@ class PreparedRnsNttPlan:
         """b + a*s (grafting/rns_key_switch.rs:906-947)."""
         n_polys, n_limbs, per = self._shape(b, n_limbs)
         out = empty(b.shape) if out is None else out
-        _check(lib().ckks_decrypt_raw(self._h, _ptr(b), _ptr(a), _ptr(secret), _ptr(out), n_polys, n_limbs,
+        m = n_polys * per
+        _check(lib().ckks_decrypt_raw(self._h, _ptr(b, m, name="b"), _ptr(a, m, name="a"),
+                                      _ptr(secret, self.degree, dtype="int8", name="secret"),
+                                      _ptr(out, m, name="out"), n_polys, n_limbs,
                                       per, per))
         return out
 
@@ -502,36 +547,46 @@ class _KeySwitchKey:
 
     def _n_cts(self, t, polys):
         per = polys * self.q_limbs * self.plan.degree
-        assert t.numel() % per == 0, "ciphertext batch size mismatch"
+        if t.numel() % per != 0 or t.numel() == 0:
+            raise ValueError("ciphertext batch size mismatch")
         return t.numel() // per
+
+    def _ct_ptrs(self, n, ins, out, out_limbs=None):
+        """Checked pointers of the input batches [(tensor, polys)] and of out
+        ([n][2][out_limbs][N])."""
+        N = self.plan.degree
+        ptrs = [_ptr(t, n * polys * self.q_limbs * N, name=nm) for nm, t, polys in ins]
+        ol = self.q_limbs if out_limbs is None else out_limbs
+        return ptrs + [_ptr(out, n * 2 * ol * N, name="out")]
 
     def switch(self, ct, out=None):
         """rns_key_switch_with_prepared_dynamic_ntt (grafting/rns_key_switch.rs:2526)
         or the dnum hybrid switch."""
         n = self._n_cts(ct, 2)
         out = empty((n, 2, self.q_limbs, self.plan.degree)) if out is None else out
-        _check(lib().ckks_key_switch(self._h, _ptr(ct), _ptr(out), n, 0, 0))
+        _check(lib().ckks_key_switch(self._h, *self._ct_ptrs(n, [("ct", ct, 2)], out), n, 0, 0))
         return out
 
     def apply_galois(self, ct, exponent: int, out=None):
         """apply_rns_galois_automorphism_with_prepared_ntt (ckks/rns_galois_key.rs:439)."""
         n = self._n_cts(ct, 2)
         out = empty((n, 2, self.q_limbs, self.plan.degree)) if out is None else out
-        _check(lib().ckks_apply_galois(self._h, exponent, _ptr(ct), _ptr(out), n, 0, 0))
+        _check(lib().ckks_apply_galois(self._h, exponent, *self._ct_ptrs(n, [("ct", ct, 2)], out), n, 0, 0))
         return out
 
     def relinearize(self, ct3, out=None):
         """rns_relinearize_with_prepared_dynamic_ntt (grafting/rns_key_switch.rs:2113)."""
         n = self._n_cts(ct3, 3)
         out = empty((n, 2, self.q_limbs, self.plan.degree)) if out is None else out
-        _check(lib().ckks_relinearize(self._h, _ptr(ct3), _ptr(out), n, 0, 0))
+        _check(lib().ckks_relinearize(self._h, *self._ct_ptrs(n, [("ct3", ct3, 3)], out), n, 0, 0))
         return out
 
     def multiply_relinearize_rescale(self, ct0, ct1, out=None):
         """multiply_relinearize_rescale_rns_ckks_with_prepared_ntt (ckks/rns_multiply.rs:161)."""
         n = self._n_cts(ct0, 2)
         out = empty((n, 2, self.q_limbs - 1, self.plan.degree)) if out is None else out
-        _check(lib().ckks_multiply_relinearize_rescale(self._h, _ptr(ct0), _ptr(ct1), _ptr(out), n,
+        ptrs = self._ct_ptrs(n, [("ct0", ct0, 2), ("ct1", ct1, 2)], out, self.q_limbs - 1)
+        _check(lib().ckks_multiply_relinearize_rescale(self._h, *ptrs, n,
                                                        0, 0))
         return out
 
@@ -620,7 +675,7 @@ class BoundedKeySwitchKey(_KeySwitchKey):
         torch = _torch()
         n = self._n_cts(ct, 2)
         poly = self.q_limbs * self.plan.degree
-        assert ct.is_contiguous()
+        _ptr(ct, n * 2 * poly, name="ct")
         out = torch.empty((n, self.digit_count, self.plan.degree), dtype=torch.int64, device=ct.device)
         # a component of ciphertext c at element (2 c + 1) * poly
         _check(lib().ckks_bounded_digits(self.plan.handle, self.q_limbs, self.base_log, ct.data_ptr() + 8 * poly,
@@ -641,17 +696,20 @@ class BoundedKeySwitchKey(_KeySwitchKey):
         if out is not None and self._n_cts(out, 2) != hoisted.n_cts:
             raise ValueError("out must hold as many ciphertexts as were hoisted")
         out = empty((hoisted.n_cts, 2, self.q_limbs, self.plan.degree)) if out is None else out
-        _check(lib().ckks_bounded_rotate_hoisted(hoisted.handle, self._h, int(exponent), _ptr(ct), _ptr(out), 0, 0))
+        _check(lib().ckks_bounded_rotate_hoisted(hoisted.handle, self._h, int(exponent),
+                                                 *self._ct_ptrs(hoisted.n_cts, [("ct", ct, 2)], out), 0, 0))
         return out
 
     def apply_galois_hoisted(self, ct, exponent: int, a_digits, out=None):
         """Rotation from hoisted digits (grafting/bounded_gadget.rs:176-312):
         same residues as apply_galois; a_digits = self.digits(ct)."""
         n = self._n_cts(ct, 2)
-        assert a_digits.dtype == _torch().int64 and a_digits.numel() == n * self.digit_count * self.plan.degree
+        if a_digits.numel() != n * self.digit_count * self.plan.degree:
+            raise ValueError("a_digits must hold digit_count x N digits per ciphertext")
         out = empty((n, 2, self.q_limbs, self.plan.degree)) if out is None else out
-        _check(lib().ckks_bounded_apply_galois_hoisted(self._h, int(exponent), _ptr(ct), _ptr(a_digits), _ptr(out),
-                                                       n, 0, 0))
+        p_ct, p_out = self._ct_ptrs(n, [("ct", ct, 2)], out)
+        _check(lib().ckks_bounded_apply_galois_hoisted(self._h, int(exponent), p_ct,
+                                                       _ptr(a_digits, name="a_digits"), p_out, n, 0, 0))
         return out
 
 
@@ -663,7 +721,8 @@ class HoistedDigits:
         self.ctx = key.ctx
         self.n_cts = key._n_cts(ct, 2)
         h = _p()
-        _check(lib().ckks_bounded_hoist(key.handle, _ptr(ct), self.n_cts, 0, C.byref(h)))
+        p_ct = _ptr(ct, self.n_cts * 2 * key.q_limbs * key.plan.degree, name="ct")
+        _check(lib().ckks_bounded_hoist(key.handle, p_ct, self.n_cts, 0, C.byref(h)))
         self._h = h
 
     @property
@@ -694,11 +753,11 @@ def bounded_decompose(plan: PreparedRnsNttPlan, q_limbs: int, base_log: int, x, 
     """BoundedGadgetLayout::decompose + lift_digit on the device: x
     [polys][q_limbs][N] -> [polys][digit_count][plan limbs][N]."""
     n = plan.degree
-    polys = x.numel() // (q_limbs * n)
+    polys = plan._count(x, q_limbs)
     count = bounded_digit_count(plan, q_limbs, base_log)
     out = empty((polys, count, len(plan.moduli), n)) if out is None else out
-    _check(lib().ckks_bounded_decompose(plan.handle, int(q_limbs), int(base_log), _ptr(x), _ptr(out),
-                                        polys, 0))
+    _check(lib().ckks_bounded_decompose(plan.handle, int(q_limbs), int(base_log), _ptr(x, polys * q_limbs * n, name="x"),
+                                        _ptr(out, polys * count * len(plan.moduli) * n, name="out"), polys, 0))
     return out
 
 
@@ -707,10 +766,11 @@ def bounded_digits(plan: PreparedRnsNttPlan, q_limbs: int, base_log: int, x):
     -> int64 digits [polys][digit_count][N]."""
     torch = _torch()
     n = plan.degree
-    polys = x.numel() // (q_limbs * n)
+    polys = plan._count(x, q_limbs)
     count = bounded_digit_count(plan, q_limbs, base_log)
     out = torch.empty((polys, count, n), dtype=torch.int64, device=x.device)
-    _check(lib().ckks_bounded_digits(plan.handle, int(q_limbs), int(base_log), _ptr(x), _ptr(out), polys, 0))
+    _check(lib().ckks_bounded_digits(plan.handle, int(q_limbs), int(base_log), _ptr(x, polys * q_limbs * n, name="x"),
+                                     _ptr(out), polys, 0))
     return out
 
 
@@ -718,9 +778,10 @@ def bounded_digits_automorphism(plan: PreparedRnsNttPlan, q_limbs: int, base_log
     """BoundedGadgetDecomposition::automorphism_hoisted on the device."""
     n = plan.degree
     count = bounded_digit_count(plan, q_limbs, base_log)
-    polys = digits.numel() // (count * n)
+    polys = plan._count(digits, count)
     out = _torch().empty_like(digits)
-    _check(lib().ckks_bounded_digits_automorphism(plan.handle, int(q_limbs), int(base_log), _ptr(digits),
+    _check(lib().ckks_bounded_digits_automorphism(plan.handle, int(q_limbs), int(base_log),
+                                                  _ptr(digits, polys * count * n, name="digits"),
                                                   _ptr(out), polys, int(exponent)))
     return out
 
@@ -736,7 +797,8 @@ def ntt_batch(ctx: Context, entries, forward: bool = True):
     arr = (NttBatchEntry * len(entries))()
     for i, (plan, x, out, n_limbs) in enumerate(entries):
         n_polys, n_limbs, per = plan._shape(x, n_limbs)
-        arr[i] = NttBatchEntry(plan.handle, _ptr(x), _ptr(out if out is not None else x), n_polys,
+        arr[i] = NttBatchEntry(plan.handle, _ptr(x, n_polys * per, name="x"),
+                               _ptr(out if out is not None else x, n_polys * per, name="out"), n_polys,
                                n_limbs, per)
     fn = lib().ckks_ntt_forward_batch if forward else lib().ckks_ntt_inverse_batch
     _check(fn(ctx.handle, C.cast(arr, _p), len(entries)))

@@ -135,6 +135,8 @@ int ckks_encode_coefficients(const ckks_rns_plan *plan, const double *coeffs, do
     if (!coeffs || !out) return fail(CKKS_ERR_INVALID, "null buffer");
     if (!(scale > 0.0) || !std::isfinite(scale)) return fail(CKKS_ERR_INVALID, "CKKS scale must be positive and finite");
     ckks_context *ctx = plan->ctx;
+    // the overflow flag is read back on the host before returning
+    if (ctx->capturing) return fail(CKKS_ERR_INVALID, "encode reads its overflow flag back: not capturable");
     void *w;
     if ((rc = ctx->workspace(2, sizeof(int), &w))) return rc;
     CK_HIP(hipMemsetAsync(w, 0, sizeof(int), ctx->stream));

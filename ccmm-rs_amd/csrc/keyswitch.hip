@@ -72,6 +72,9 @@ struct ModDownTable {
 
 }  // namespace ck
 
+// permuted key copies kept per bounded key for hoisted rotations (LRU)
+constexpr size_t kHoistKeyCacheMax = 8;
+
 struct ckks_ksk {
     ckks_context *ctx = nullptr;
     const ckks_rns_plan *plan = nullptr;  // key basis B
@@ -94,8 +97,11 @@ struct ckks_ksk {
     int tuned_engine = CKKS_BASECONV_VALU;  // engine of CKKS_BASECONV_AUTO for this key
     int tuned_fusion = CKKS_KS_FUSION_OFF;  // mode of CKKS_KS_FUSION_AUTO for this key
     int tuned_split = CKKS_KS_SPLIT_OFF;    // mode of CKKS_KS_SPLIT_AUTO for this key
-    // hoisted rotations: key entries [2][D][B][N] permuted by k^-1, per k
+    // hoisted rotations: key entries [2][D][B][N] permuted by k^-1, per k,
+    // at most kHoistKeyCacheMax of them; hoist_lru lists the exponents most
+    // recently used first, and the last one is freed when the cache is full
     mutable std::map<ck::u64, ck::u64 *> hoist_keys;
+    mutable std::vector<ck::u64> hoist_lru;
     ck::BcPlan *d_bc = nullptr;
     signed char *d_bt = nullptr;
 };
@@ -1766,6 +1772,7 @@ int ckks_bounded_digits_automorphism(const ckks_rns_plan *plan, size_t q_limbs, 
     ha.logn = plan->logn;
     ha.digit_count = h.digit_count;
     ha.base_log = base_log;
+    ha.small = digits_below_primes(plan, plan->moduli.size(), base_log);
     ha.total = (u64)n_polys * plan->degree;
     return launch_hoist(plan->ctx->stream, ha);
 }
@@ -2011,6 +2018,7 @@ int ckks_bounded_hoist(const ckks_ksk *key, const uint64_t *in, size_t n_cts, si
         ba.logn = key->plan->logn;
         ba.digit_count = (u32)D;
         ba.base_log = key->base_log;
+        ba.small = digits_below_primes(key->plan, B, key->base_log);
         ba.total = (u64)n_cts * N;
         int rc = launch_bounded(ctx->stream, ba, key->bounded_words);
         if (rc) return rc;
@@ -2048,19 +2056,34 @@ int ckks_bounded_rotate_hoisted(const ckks_hoisted *h, const ckks_ksk *key, uint
     const u64 kinv = galois_inverse(k, two_n);
     const size_t DBN = key->dnum * key->limbs * key->degree;
     auto it = key->hoist_keys.find(k);
+    auto &lru = key->hoist_lru;
     if (it == key->hoist_keys.end()) {
         ckks_context *ctx = key->ctx;
         if (ctx->capturing) return fail(CKKS_ERR_INVALID, "hoisted key tables cannot be built during graph capture");
-        u64 *pk = nullptr;
         CK_HIP(hipSetDevice(ctx->device));
-        CK_HIP(hipMalloc(&pk, 2 * DBN * sizeof(u64)));
+        if (lru.size() >= kHoistKeyCacheMax) {
+            // the evicted copy may still be read by queued work on the stream
+            CK_HIP(hipStreamSynchronize(ctx->stream));
+            hipFree(key->hoist_keys[lru.back()]);
+            key->hoist_keys.erase(lru.back());
+            lru.pop_back();
+        }
+        u64 *pk = nullptr;
+        if (hipMalloc(&pk, 2 * DBN * sizeof(u64)) != hipSuccess) {
+            // no room for a permuted copy: the canonical rotation needs none
+            (void)hipGetLastError();
+            return ckks_apply_galois(key, exponent, in, out, n_cts, in_stride, out_stride);
+        }
         if ((rc = launch_ntt_permute(ctx->stream, key->d_kb, pk, kinv, DBN, key->plan->logn)) ||
             (rc = launch_ntt_permute(ctx->stream, key->d_ka, pk + DBN, kinv, DBN, key->plan->logn))) {
             hipFree(pk);
             return rc;
         }
         it = key->hoist_keys.emplace(k, pk).first;
+    } else {
+        lru.erase(std::find(lru.begin(), lru.end(), k));
     }
+    lru.insert(lru.begin(), k);
     const HoistedMac hm{h->d_digits, it->second, it->second + DBN, k};
     const size_t poly = key->q_limbs * key->degree;
     return key_switch_core(key, in + poly, in_stride, in, nullptr, in_stride, out, out_stride, n_cts, kinv,

@@ -58,16 +58,18 @@ def encode(plan: PreparedRnsNttPlan, emb: CanonicalEmbedding, slots, scale: floa
     coeffs = emb.slots_to_coefficients(slots).contiguous()
     p = coeffs.shape[0]
     out = empty((p, n_limbs, plan.degree))
-    _check(lib().ckks_encode_coefficients(plan.handle, _ptr(coeffs), float(scale), _ptr(out), p, n_limbs))
+    _check(lib().ckks_encode_coefficients(plan.handle, _ptr(coeffs, p * plan.degree, dtype="float64", name="coeffs"),
+                                          float(scale), _ptr(out), p, n_limbs))
     return out[0] if single else out
 
 
 def decode_coefficients(plan: PreparedRnsNttPlan, residues, scale: float, n_limbs: int | None = None):
     """Exact centered CRT / scale on the device: [p][limbs][N] -> [p][N] float64."""
     n_limbs = len(plan.moduli) if n_limbs is None else n_limbs
-    p = residues.numel() // (n_limbs * plan.degree)
+    p = plan._count(residues, n_limbs)
     out = torch.empty((p, plan.degree), dtype=torch.float64, device="cuda")
-    _check(lib().ckks_decode_coefficients(plan.handle, _ptr(residues), float(scale), _ptr(out), p, n_limbs))
+    _check(lib().ckks_decode_coefficients(plan.handle, _ptr(residues, p * n_limbs * plan.degree, name="residues"),
+                                          float(scale), _ptr(out, dtype="float64"), p, n_limbs))
     return out
 
 

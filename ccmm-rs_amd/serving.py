@@ -101,6 +101,22 @@ class Program:
                 regs[dst] = regs[a].copy()
         return regs[self.out_reg]
 
+    def register_limbs(self, n_limbs: int):
+        """Per op, (limbs of its source register a, limbs of the output
+        register after the program) as levels move: rescale, mod switch and
+        multiply+relinearize+rescale drop one limb; everything else keeps a's.
+        The last entry's second field is the output's limb count (the server
+        itself rejects programs that read empty registers)."""
+        regs = [int(n_limbs) if i < self.n_inputs else 0 for i in range(self.n_regs)]
+        out = []
+        for code, dst, a, _b, _k, _e, _pt in self.ops:
+            la = regs[a] if 0 <= a < self.n_regs else 0
+            if 0 <= dst < self.n_regs:
+                regs[dst] = la - 1 if code in (OP_RESCALE, OP_MOD_SWITCH, OP_MUL_RELIN_RESCALE) else la
+            out.append((la, 0))
+        final = regs[self.out_reg] if 0 <= self.out_reg < self.n_regs else 0
+        return [(la, final) for la, _ in out] or [(0, final)]
+
     def counts(self) -> dict:
         names = {OP_ROTATE: "rotate", OP_MUL_PLAIN: "mul_plain", OP_ADD: "add", OP_RESCALE: "rescale",
                  OP_MUL_RELIN_RESCALE: "mul_relin_rescale", OP_COPY: "copy", OP_MOD_SWITCH: "mod_switch"}
@@ -122,11 +138,14 @@ class Server:
 
     def submit(self, plan: PreparedRnsNttPlan, n_limbs: int, inputs, output, program: Program) -> int:
         ops = (_Op * max(len(program.ops), 1))()
+        n = plan.degree
+        limbs = program.register_limbs(int(n_limbs))
         for i, (code, dst, a, b, key, exp, pt) in enumerate(program.ops):
             ops[i] = _Op(code, dst, a, b, key.handle if key is not None else None, exp,
-                         _ptr(pt) if pt is not None else None)
+                         _ptr(pt, limbs[i][0] * n, name="plaintext") if pt is not None else None)
         job = _Job(plan.handle, int(n_limbs), program.n_regs, program.out_reg, program.n_inputs,
-                   _ptr(inputs), _ptr(output), C.cast(ops, _p), len(program.ops))
+                   _ptr(inputs, program.n_inputs * 2 * int(n_limbs) * n, name="inputs"),
+                   _ptr(output, 2 * limbs[-1][1] * n, name="output"), C.cast(ops, _p), len(program.ops))
         t = C.c_uint64()
         _check(_fn("ckks_server_submit")(self._h, C.addressof(job), C.byref(t)))
         self._jobs[t.value] = (job, ops, program, inputs, output)

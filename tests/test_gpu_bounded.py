@@ -251,3 +251,27 @@ def test_hoisted_rotations_in_every_ntt_variant(pkg, ctx, logn, variant):
         sa = np.stack([O.apply_automorphism(ct[c, 1, i], q, k) for i, q in enumerate(q_moduli)])
         bo, ao = O.bounded_key_switch(orc, L, w, sb, sa, kb, ka)
         assert (got[c, 0] == bo).all() and (got[c, 1] == ao).all(), f"ct {c}"
+
+
+def test_hoisted_key_cache_evicts_and_rebuilds(pkg, ctx):
+    """A key keeps at most 8 permuted copies for hoisted rotations; rotating by
+    12 exponents, then by the first ones again (evicted and rebuilt), gives the
+    residues of apply_galois every time."""
+    n, L, K, w = 1 << 12, 4, 1, 61
+    q_moduli = [Q0[0]] + Q40[:L - 1]
+    qp = q_moduli + P60[:K]
+    rng = np.random.default_rng(4242)
+    plan = pkg.PreparedRnsNttPlan(ctx, qp, n)
+    count = O.bounded_digit_count(q_moduli, w)
+    key = pkg.BoundedKeySwitchKey(plan, L, w, *_key(rng, qp, n, count))
+    dct = pkg.to_device(rand_residues(rng, q_moduli, n, polys=4).reshape(2, 2, L, n))
+    hoisted = key.hoist(dct)
+    try:
+        steps = list(range(1, 13)) + [1, 2, 3, 12]
+        for s in steps:
+            k = O.rotation_exponent_left(n, s)
+            want = pkg.to_host(key.apply_galois(dct, k))
+            got = pkg.to_host(key.rotate_hoisted(hoisted, dct, k))
+            assert (got == want).all(), f"rotation by {s}"
+    finally:
+        hoisted.close()
