@@ -82,6 +82,9 @@ _SIGNATURES = {
     "ckks_poly_sub": (C.c_int, [_p, _p, _p, _p, _sz, _sz, _sz]),
     "ckks_poly_neg": (C.c_int, [_p, _p, _p, _sz, _sz, _sz]),
     "ckks_automorphism": (C.c_int, [_p, _p, _p, _sz, _sz, _sz, _u64]),
+    "ckks_ntt_mul_monomial": (C.c_int, [_p, _p, _p, _sz, _sz, _sz, C.c_int64]),
+    "ckks_mul_monomial": (C.c_int, [_p, _p, _p, _sz, _sz, _sz, _u64, C.c_int]),
+    "ckks_poly_scalar_mul": (C.c_int, [_p, _p, _p, _p, _sz, _sz, _sz]),
     "ckks_ntt_galois_permute": (C.c_int, [_p, _p, _p, _sz, _sz, _sz, _u64]),
     "ckks_rescale": (C.c_int, [_p, _p, _p, _sz, _sz, _sz, _sz]),
     "ckks_mod_switch": (C.c_int, [_p, _p, _p, _sz, _sz, _sz, _sz]),
@@ -94,6 +97,7 @@ _SIGNATURES = {
     "ckks_rns_ksk_create": (C.c_int, [_p, _p, _sz, _p, _p, C.POINTER(_p)]),
     "ckks_hybrid_ksk_create": (C.c_int, [_p, _sz, _p, _sz, _p, _p, C.POINTER(_p)]),
     "ckks_bounded_ksk_create": (C.c_int, [_p, _sz, C.c_uint, _p, _p, C.POINTER(_p)]),
+    "ckks_keygen": (C.c_int, [_p, C.c_int, _sz, _p, _sz, C.c_uint, _p, _p, _p, _p, _p, _p]),
     "ckks_encrypt_raw": (C.c_int, [_p, _p, _p, _p, _p, _p, _sz, _sz, _sz]),
     "ckks_encode_coefficients": (C.c_int, [_p, _p, C.c_double, _p, _sz, _sz]),
     "ckks_mod_down": (C.c_int, [_p, _sz, _p, _p, _sz]),
@@ -408,6 +412,42 @@ class PreparedRnsNttPlan:
                                    _ptr(out, n_polys * per, name="out"), n_polys, n_limbs, per))
         return out
 
+    def ntt_mul_monomial_signed(self, x, exponent: int, out=None, n_limbs=None):
+        """RnsNttPolynomial::mul_monomial_signed (ring/rns_ntt.rs:215): x * X^e
+        in the NTT domain (e signed, reduced rem_euclid 2N)."""
+        out = empty(x.shape) if out is None else out
+        n_polys, n_limbs, per = self._shape(x, n_limbs)
+        _check(lib().ckks_ntt_mul_monomial(self._h, _ptr(x, n_polys * per, name="x"),
+                                           _ptr(out, n_polys * per, name="out"), n_polys, n_limbs, per,
+                                           int(exponent)))
+        return out
+
+    def mul_monomial_signed(self, x, exponent: int, sign: int = 1, out=None, n_limbs=None):
+        """Polynomial::mul_monomial_signed (ring/polynomial.rs:164) per limb:
+        sign * X^exponent * x, coefficient domain."""
+        if exponent < 0:
+            raise ValueError("the coefficient-domain monomial exponent is unsigned")
+        out = empty(x.shape) if out is None else out
+        n_polys, n_limbs, per = self._shape(x, n_limbs)
+        _check(lib().ckks_mul_monomial(self._h, _ptr(x, n_polys * per, name="x"),
+                                       _ptr(out, n_polys * per, name="out"), n_polys, n_limbs, per,
+                                       int(exponent), int(sign)))
+        return out
+
+    def scalar_mul(self, x, scalars, out=None, n_limbs=None):
+        """Polynomial::scalar_mul (ring/polynomial.rs:144) per limb: limb i
+        times scalars[i] mod q_i (one int for every limb, or one per limb)."""
+        n_polys, n_limbs, per = self._shape(x, n_limbs)
+        if isinstance(scalars, int):
+            scalars = [scalars] * n_limbs
+        sc = np.ascontiguousarray([int(v) % (1 << 64) for v in scalars], dtype=np.uint64)
+        if len(sc) != n_limbs:
+            raise ValueError("one scalar per limb")
+        out = empty(x.shape) if out is None else out
+        _check(lib().ckks_poly_scalar_mul(self._h, _ptr(x, n_polys * per, name="x"), sc.ctypes.data,
+                                          _ptr(out, n_polys * per, name="out"), n_polys, n_limbs, per))
+        return out
+
     def apply_automorphism(self, x, exponent: int, out=None, n_limbs=None):
         """apply_automorphism (ckks/automorphism.rs:8) per limb, coefficient domain."""
         out = empty(x.shape) if out is None else out
@@ -616,6 +656,67 @@ def _sizes(seq):
     return np.ascontiguousarray(list(seq), dtype=np.uintp)
 
 
+def _key_material(plan, key_b, key_a, entries=None):
+    """Pointers to key entries [entries][plan limbs][N]: numpy arrays (copied
+    from host memory) or int64 GPU tensors such as keygen() returns (copied
+    on the device).  Returns (ptr_b, ptr_a, keep-alive objects, entries)."""
+    torch = _torch()
+    per = len(plan.moduli) * plan.degree
+    if isinstance(key_b, torch.Tensor) or isinstance(key_a, torch.Tensor):
+        n = key_b.numel() // per if entries is None else entries
+        if n == 0 or key_b.numel() != n * per or key_a.numel() != n * per:
+            raise ValueError("key entries must have shape [entries][plan limbs][N]")
+        return _ptr(key_b, n * per, name="key_b"), _ptr(key_a, n * per, name="key_a"), (key_b, key_a), n
+    kb = np.ascontiguousarray(key_b, dtype=np.uint64)
+    ka = np.ascontiguousarray(key_a, dtype=np.uint64)
+    if kb.size != ka.size or kb.size % per or kb.size == 0 or (entries is not None and kb.size != entries * per):
+        raise ValueError("key entries must have shape [entries][plan limbs][N]")
+    return kb.ctypes.data, ka.ctypes.data, (kb, ka), kb.size // per
+
+
+KEYGEN_RNS_GADGET, KEYGEN_HYBRID, KEYGEN_BOUNDED = 0, 1, 2
+
+
+def keygen(plan, kind: int, q_limbs: int, source, secret, a, e, block_sizes=(), base_log: int = 0):
+    """ckks_keygen: NTT-domain key entries (key_b, key_a), each
+    [entries][plan limbs][N], from host-sampled randomness a
+    [entries][limbs][N] (uniform canonical) and e [entries][N] (int64), for
+    the gadget multiple of `source` ([limbs][N] coefficient domain) under the
+    ternary `secret` ([N] int8).  Entry counts: blocks (RNS gadget, hybrid)
+    or the bounded digit count."""
+    N, B = plan.degree, len(plan.moduli)
+    if kind == KEYGEN_BOUNDED:
+        entries = bounded_digit_count(plan, q_limbs, base_log)
+    else:
+        entries = len(block_sizes)
+    bs = _sizes(block_sizes or [0])
+    kb, ka = empty((entries, B, N)), empty((entries, B, N))
+    _check(lib().ckks_keygen(plan.handle, int(kind), int(q_limbs), bs.ctypes.data, len(block_sizes),
+                             int(base_log), _ptr(source, B * N, name="source"),
+                             _ptr(secret, N, dtype="int8", name="secret"), _ptr(a, entries * B * N, name="a"),
+                             _ptr(e, entries * N, name="e"), _ptr(kb), _ptr(ka)))
+    return kb, ka
+
+
+def project_secret(plan, secret):
+    """The ternary secret [N] (int8) as residues [plan limbs][N] (project_secret)."""
+    torch = _torch()
+    s = secret.to(torch.int64)
+    q = torch.tensor([int(v) for v in plan.moduli], dtype=torch.int64, device=s.device)[:, None]
+    return torch.where(s[None, :] < 0, q + s[None, :], s[None, :].expand(len(plan.moduli), -1)).contiguous()
+
+
+def galois_source(plan, secret, exponent: int):
+    """sigma_k(s) over the plan's limbs: the source of a Galois key."""
+    return plan.apply_automorphism(project_secret(plan, secret), int(exponent))
+
+
+def square_source(plan, secret):
+    """s^2 over the plan's limbs: the source of a relinearization key."""
+    f = plan.forward(project_secret(plan, secret))
+    return plan.inverse(plan.pointwise_mul(f, f))
+
+
 class RnsKeySwitchKey(_KeySwitchKey):
     """PreparedRnsKeySwitchKey / PreparedRnsMultiplicationKey
     (grafting/rns_key_switch.rs:1930-2430): NTT-domain entries
@@ -624,12 +725,18 @@ class RnsKeySwitchKey(_KeySwitchKey):
     def __init__(self, plan: PreparedRnsNttPlan, block_sizes, key_b, key_a):
         self.q_limbs = len(plan.moduli)
         bs = _sizes(block_sizes)
-        kb = np.ascontiguousarray(key_b, dtype=np.uint64)
-        ka = np.ascontiguousarray(key_a, dtype=np.uint64)
+        pb, pa, _keep, _ = _key_material(plan, key_b, key_a, len(bs))
         h = _p()
-        _check(lib().ckks_rns_ksk_create(plan.handle, bs.ctypes.data, len(bs), kb.ctypes.data,
-                                         ka.ctypes.data, C.byref(h)))
+        _check(lib().ckks_rns_ksk_create(plan.handle, bs.ctypes.data, len(bs), pb, pa, C.byref(h)))
         self._finish_init(plan, h)
+
+    @classmethod
+    def generate(cls, plan: PreparedRnsNttPlan, block_sizes, source, secret, a, e):
+        """RnsKeySwitchKey::generate_with_distribution_ntt_rng
+        (grafting/rns_key_switch.rs:1155-1258) on the device from host-sampled
+        a, e: entry j encrypts E_j * source under secret."""
+        kb, ka = keygen(plan, KEYGEN_RNS_GADGET, len(plan.moduli), source, secret, a, e, block_sizes=block_sizes)
+        return cls(plan, block_sizes, kb, ka)
 
 
 class HybridKeySwitchKey(_KeySwitchKey):
@@ -639,12 +746,18 @@ class HybridKeySwitchKey(_KeySwitchKey):
     def __init__(self, plan: PreparedRnsNttPlan, q_limbs: int, block_sizes, key_b, key_a):
         self.q_limbs = int(q_limbs)
         bs = _sizes(block_sizes)
-        kb = np.ascontiguousarray(key_b, dtype=np.uint64)
-        ka = np.ascontiguousarray(key_a, dtype=np.uint64)
+        pb, pa, _keep, _ = _key_material(plan, key_b, key_a, len(bs))
         h = _p()
-        _check(lib().ckks_hybrid_ksk_create(plan.handle, self.q_limbs, bs.ctypes.data, len(bs),
-                                            kb.ctypes.data, ka.ctypes.data, C.byref(h)))
+        _check(lib().ckks_hybrid_ksk_create(plan.handle, self.q_limbs, bs.ctypes.data, len(bs), pb, pa,
+                                            C.byref(h)))
         self._finish_init(plan, h)
+
+    @classmethod
+    def generate(cls, plan: PreparedRnsNttPlan, q_limbs: int, block_sizes, source, secret, a, e):
+        """dnum hybrid key on the device from host-sampled a, e: entry j
+        encrypts P * E_j * source under secret over Q u P."""
+        kb, ka = keygen(plan, KEYGEN_HYBRID, q_limbs, source, secret, a, e, block_sizes=block_sizes)
+        return cls(plan, q_limbs, block_sizes, kb, ka)
 
 
 class BoundedKeySwitchKey(_KeySwitchKey):
@@ -657,17 +770,21 @@ class BoundedKeySwitchKey(_KeySwitchKey):
     def __init__(self, plan: PreparedRnsNttPlan, q_limbs: int, base_log: int, key_b, key_a):
         self.q_limbs = int(q_limbs)
         self.base_log = int(base_log)
-        kb = np.ascontiguousarray(key_b, dtype=np.uint64)
-        ka = np.ascontiguousarray(key_a, dtype=np.uint64)
         count = bounded_digit_count(plan, self.q_limbs, self.base_log)
-        want = (count, len(plan.moduli), plan.degree)
-        if kb.shape != want or ka.shape != want:
-            raise CkksError(f"bounded key entries must have shape {want}")
+        pb, pa, _keep, _ = _key_material(plan, key_b, key_a, count)
         h = _p()
-        _check(lib().ckks_bounded_ksk_create(plan.handle, self.q_limbs, self.base_log, kb.ctypes.data,
-                                             ka.ctypes.data, C.byref(h)))
+        _check(lib().ckks_bounded_ksk_create(plan.handle, self.q_limbs, self.base_log, pb, pa, C.byref(h)))
         self.digit_count = count
         self._finish_init(plan, h)
+
+    @classmethod
+    def generate(cls, plan: PreparedRnsNttPlan, q_limbs: int, base_log: int, source, secret, a, e):
+        """PreparedHybridRnsGaloisKey::generate_with_distribution_ntt_rng
+        (ckks/hybrid_rns_galois_key.rs:61-170) / BoundedRnsMultiplicationKey
+        (grafting/bounded_key_switch.rs:22-118) on the device from host-sampled
+        a, e: entry j encrypts P * 2^(w j) * source under secret."""
+        kb, ka = keygen(plan, KEYGEN_BOUNDED, q_limbs, source, secret, a, e, base_log=base_log)
+        return cls(plan, q_limbs, base_log, kb, ka)
 
     def digits(self, ct):
         """Balanced digits [ct][digit_count][N] (int64) of the a components of

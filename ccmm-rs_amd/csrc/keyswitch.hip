@@ -1143,10 +1143,73 @@ static void patch_moddown(ModDownTable &md, const u64 *w, size_t q_limbs) {
     md.tf = (const double *)(md.pinv_q + q_limbs);
 }
 
+// ---- key material --------------------------------------------------------------------
+
+// Flags a residue that is not below its limb's prime (key material handed in
+// as device memory is checked where it lies).
+__global__ __launch_bounds__(256) void k_check_canonical(const u64 *x, u64 elems, u32 logn, u32 limbs,
+                                                         const DevPrime *primes, int *bad) {
+    for (u64 i = (u64)blockIdx.x * 256 + threadIdx.x; i < elems; i += (u64)gridDim.x * 256)
+        if (x[i] >= primes[(u32)((i >> logn) % limbs)].q) *bad = 1;
+}
+
+static bool is_device_pointer(const void *p) {
+    hipPointerAttribute_t at{};
+    if (hipPointerGetAttributes(&at, p) != hipSuccess) {
+        (void)hipGetLastError();  // pageable host memory is not known to HIP
+        return false;
+    }
+    return at.type == hipMemoryTypeDevice;
+}
+
+// Copies key entries [entries][limbs][N] (b and a) into new device buffers
+// *db, *da after checking every residue is canonical.  The sources may be
+// host arrays or device buffers (ckks_keygen writes device buffers); device
+// sources are read on the context stream, after the work queued there.
+static int upload_key_material(const ckks_rns_plan *plan, size_t kelems, const u64 *key_b, const u64 *key_a,
+                               u64 **db, u64 **da) {
+    ckks_context *ctx = plan->ctx;
+    const size_t N = plan->degree, B = plan->moduli.size();
+    if (ctx->capturing) return fail(CKKS_ERR_INVALID, "keys cannot be created during graph capture");
+    const bool dev_b = is_device_pointer(key_b), dev_a = is_device_pointer(key_a);
+    if (dev_b != dev_a) return fail(CKKS_ERR_INVALID, "key_b and key_a must both be host or both device memory");
+    if (!dev_b) {
+        const auto &mods = plan->moduli;
+        for (size_t e = 0; e < kelems; ++e) {
+            u64 q = mods[(e / N) % B];
+            if (key_b[e] >= q || key_a[e] >= q) return fail(CKKS_ERR_INVALID, "key residues must be canonical");
+        }
+    }
+    CK_HIP(hipMalloc(db, kelems * sizeof(u64)));
+    CK_HIP(hipMalloc(da, kelems * sizeof(u64)));
+    if (!dev_b) {
+        CK_HIP(hipMemcpy(*db, key_b, kelems * sizeof(u64), hipMemcpyHostToDevice));
+        CK_HIP(hipMemcpy(*da, key_a, kelems * sizeof(u64), hipMemcpyHostToDevice));
+        return CKKS_OK;
+    }
+    void *w;
+    int rc = ctx->workspace(2, sizeof(int), &w);
+    if (rc) return rc;
+    hipStream_t s = ctx->stream;
+    CK_HIP(hipMemcpyAsync(*db, key_b, kelems * sizeof(u64), hipMemcpyDeviceToDevice, s));
+    CK_HIP(hipMemcpyAsync(*da, key_a, kelems * sizeof(u64), hipMemcpyDeviceToDevice, s));
+    CK_HIP(hipMemsetAsync(w, 0, sizeof(int), s));
+    const u32 grid = (u32)std::min<u64>((kelems + 255) / 256, 4096);
+    for (const u64 *x : {(const u64 *)*db, (const u64 *)*da})
+        hipLaunchKernelGGL(k_check_canonical, dim3(grid), dim3(256), 0, s, x, (u64)kelems, plan->logn, (u32)B,
+                           plan->d_primes, (int *)w);
+    CK_HIP(hipGetLastError());
+    int bad = 0;
+    CK_HIP(hipMemcpyAsync(&bad, w, sizeof(int), hipMemcpyDeviceToHost, s));
+    CK_HIP(hipStreamSynchronize(s));
+    if (bad) return fail(CKKS_ERR_INVALID, "key residues must be canonical");
+    return CKKS_OK;
+}
+
 static int create_key(const ckks_rns_plan *plan, size_t q_limbs, const size_t *block_sizes,
                       size_t n_blocks, const u64 *key_b, const u64 *key_a, bool hybrid,
                       ckks_ksk **out) {
-    if (!plan || !block_sizes || !key_b || !key_a || !out) return fail(CKKS_ERR_INVALID, "null argument");
+    if (!plan || !block_sizes || !out) return fail(CKKS_ERR_INVALID, "null argument");
     if (n_blocks == 0) return fail(CKKS_ERR_INVALID, "RNS gadget layout must contain at least one block");
     const size_t B = plan->moduli.size();
     if (q_limbs == 0 || q_limbs > B) return fail(CKKS_ERR_INVALID, "Q limb count must be within the key basis");
@@ -1293,18 +1356,15 @@ static int create_key(const ckks_rns_plan *plan, size_t q_limbs, const size_t *b
             k->mfma = 1;
         }
     }
-    // key material
-    const size_t kelems = n_blocks * B * N;
-    for (size_t e = 0; e < kelems; ++e) {
-        u64 q = mods[(e / N) % B];
-        if (key_b[e] >= q || key_a[e] >= q)
-            return fail(CKKS_ERR_INVALID, "key residues must be canonical");
-    }
-    CK_HIP(hipMalloc(&k->d_kb, kelems * sizeof(u64)));
-    CK_HIP(hipMalloc(&k->d_ka, kelems * sizeof(u64)));
-    CK_HIP(hipMemcpy(k->d_kb, key_b, kelems * sizeof(u64), hipMemcpyHostToDevice));
-    CK_HIP(hipMemcpy(k->d_ka, key_a, kelems * sizeof(u64), hipMemcpyHostToDevice));
+    // key material (host or device)
     ctx->keys.insert(k.get());
+    if (key_b && key_a) {
+        const int rc = upload_key_material(plan, n_blocks * B * N, key_b, key_a, &k->d_kb, &k->d_ka);
+        if (rc) {
+            ckks_ksk_destroy(k.release());
+            return rc;
+        }
+    }
     *out = k.release();
     return CKKS_OK;
 }
@@ -1374,20 +1434,14 @@ static int create_bounded_key(const ckks_rns_plan *plan, size_t q_limbs, unsigne
             blocks.push_back(b);
             left -= b;
         }
-        // key material of the digit count is uploaded below; pass a dummy of
-        // matching size for validation
-        std::vector<u64> zeros(blocks.size() * B * plan->degree, 0);
-        rc = create_key(plan, q_limbs, blocks.data(), blocks.size(), zeros.data(), zeros.data(), true, &k);
+        // the key material of the digit count is uploaded below
+        rc = create_key(plan, q_limbs, blocks.data(), blocks.size(), nullptr, nullptr, true, &k);
     } else {
         std::vector<size_t> blocks(q_limbs, 1);
-        std::vector<u64> zeros(q_limbs * B * plan->degree, 0);
-        rc = create_key(plan, q_limbs, blocks.data(), blocks.size(), zeros.data(), zeros.data(), false, &k);
+        rc = create_key(plan, q_limbs, blocks.data(), blocks.size(), nullptr, nullptr, false, &k);
     }
     if (rc) return rc;
     std::unique_ptr<ckks_ksk, int (*)(ckks_ksk *)> guard(k, ckks_ksk_destroy);
-    hipFree(k->d_kb);
-    hipFree(k->d_ka);
-    k->d_kb = k->d_ka = nullptr;
     hipFree(k->d_bc);
     hipFree(k->d_bt);
     k->d_bc = nullptr;
@@ -1398,17 +1452,9 @@ static int create_bounded_key(const ckks_rns_plan *plan, size_t q_limbs, unsigne
     k->bounded_words = h.words;
     k->dnum = h.digit_count;
     const size_t N = plan->degree, kelems = (size_t)h.digit_count * B * N;
-    const auto &mods = plan->moduli;
-    for (size_t e = 0; e < kelems; ++e) {
-        u64 q = mods[(e / N) % B];
-        if (key_b[e] >= q || key_a[e] >= q) return fail(CKKS_ERR_INVALID, "key residues must be canonical");
-    }
+    if ((rc = upload_key_material(plan, kelems, key_b, key_a, &k->d_kb, &k->d_ka))) return rc;
     CK_HIP(hipMalloc(&k->d_bounded, h.blob.size() * sizeof(u64)));
     CK_HIP(hipMemcpy(k->d_bounded, h.blob.data(), h.blob.size() * sizeof(u64), hipMemcpyHostToDevice));
-    CK_HIP(hipMalloc(&k->d_kb, kelems * sizeof(u64)));
-    CK_HIP(hipMalloc(&k->d_ka, kelems * sizeof(u64)));
-    CK_HIP(hipMemcpy(k->d_kb, key_b, kelems * sizeof(u64), hipMemcpyHostToDevice));
-    CK_HIP(hipMemcpy(k->d_ka, key_a, kelems * sizeof(u64), hipMemcpyHostToDevice));
     *out = guard.release();
     return CKKS_OK;
 }
@@ -1668,11 +1714,13 @@ extern "C" {
 int ckks_rns_ksk_create(const ckks_rns_plan *plan, const size_t *block_sizes, size_t n_blocks,
                         const uint64_t *key_b, const uint64_t *key_a, ckks_ksk **out) {
     if (!plan) return fail(CKKS_ERR_INVALID, "null plan");
+    if (!key_b || !key_a) return fail(CKKS_ERR_INVALID, "null argument");
     return create_key(plan, plan->moduli.size(), block_sizes, n_blocks, key_b, key_a, false, out);
 }
 
 int ckks_hybrid_ksk_create(const ckks_rns_plan *qp_plan, size_t q_limbs, const size_t *block_sizes,
                            size_t n_blocks, const uint64_t *key_b, const uint64_t *key_a, ckks_ksk **out) {
+    if (!key_b || !key_a) return fail(CKKS_ERR_INVALID, "null argument");
     return create_key(qp_plan, q_limbs, block_sizes, n_blocks, key_b, key_a, true, out);
 }
 

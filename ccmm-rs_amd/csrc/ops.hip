@@ -2,6 +2,8 @@
 // automorphisms in both domains, exact rescale, mod switch and the tensor.
 // All are HBM-bound streams; each thread owns 2 consecutive residues of one
 // limb so loads/stores are 16-byte and a block never straddles limbs.
+#include <algorithm>
+
 #include "host_math.hpp"
 #include "internal.hpp"
 
@@ -161,6 +163,151 @@ static int permute(bool ntt_domain, const ckks_rns_plan *plan, const u64 *in, u6
         hipLaunchKernelGGL(k_ntt_galois, grid, dim3(kEwThreads), 0, plan->ctx->stream, a);
     else
         hipLaunchKernelGGL(k_automorphism, grid, dim3(kEwThreads), 0, plan->ctx->stream, a);
+    CK_HIP(hipGetLastError());
+    return CKKS_OK;
+}
+
+// ---- monomials and scalars -----------------------------------------------------
+
+struct MonoArgs {
+    const u64 *in;
+    u64 *out;
+    const DevPrime *primes;
+    u64 poly_stride;
+    u64 e;       // exponent mod 2N
+    u32 negate;  // coefficient domain: the external sign is -1
+    u32 n_limbs, logn, chunks;
+};
+
+// NTT domain (ring/ntt_polynomial.rs:133-175): evaluation k times
+// r_k^e = psi^((2k+1) e mod 2N), read from the forward twiddle table
+// T[i] = psi^bitrev(i) (psi^t = T[bitrev(t)] for t < N, psi^(N+t) = -psi^t).
+// The reference walks the same powers as a geometric progression.
+__global__ __launch_bounds__(kEwThreads) void k_ntt_monomial(MonoArgs a) {
+    const u32 b = blockIdx.x;
+    const u32 chunk = b % a.chunks;
+    const u32 pl = b / a.chunks;
+    const u32 limb = pl % a.n_limbs, poly = pl / a.n_limbs;
+    const DevPrime P = a.primes[limb];
+    const Barrett br{P.q, P.br_hi, P.br_lo};
+    const u64 n = (u64)1 << a.logn, mask2 = 2 * n - 1;
+    const size_t base = poly * a.poly_stride + ((size_t)limb << a.logn);
+#pragma unroll
+    for (int i = 0; i < kEwPerThread; ++i) {
+        const u64 k = (u64)chunk * kEwChunk + threadIdx.x * kEwPerThread + i;
+        const u64 t = ((2 * k + 1) * a.e) & mask2;
+        const u64 idx = __brev((u32)(t & (n - 1))) >> (32 - a.logn);
+        const u64 raw = P.fwd[idx].w;
+        const u64 w = P.fp ? (u64)__longlong_as_double((long long)raw) : raw;
+        const u64 r = mul_mod(a.in[base + k], w, br);
+        a.out[base + k] = t >= n ? neg_mod(r, P.q) : r;
+    }
+}
+
+// Coefficient domain (ring/polynomial.rs:164-200) as a gather: output d
+// takes source (d - shift) mod N, negated by the exponent's sign (e >= N),
+// the external sign and the wrap (d < shift).
+__global__ __launch_bounds__(kEwThreads) void k_monomial(MonoArgs a) {
+    const u32 b = blockIdx.x;
+    const u32 chunk = b % a.chunks;
+    const u32 pl = b / a.chunks;
+    const u32 limb = pl % a.n_limbs, poly = pl / a.n_limbs;
+    const u64 q = a.primes[limb].q;
+    const u64 n = (u64)1 << a.logn;
+    const u64 shift = a.e & (n - 1);
+    const u32 base_neg = (a.e >= n ? 1u : 0u) ^ a.negate;
+    const size_t base = poly * a.poly_stride + ((size_t)limb << a.logn);
+#pragma unroll
+    for (int i = 0; i < kEwPerThread; ++i) {
+        const u64 d = (u64)chunk * kEwChunk + threadIdx.x * kEwPerThread + i;
+        const u64 src = (d - shift) & (n - 1);
+        const u64 v = a.in[base + src];
+        a.out[base + d] = (base_neg ^ (d < shift ? 1u : 0u)) ? neg_mod(v, q) : v;
+    }
+}
+
+constexpr u32 kMaxScalarLimbs = 128;
+
+struct ScalarArgs {
+    const u64 *in;
+    u64 *out;
+    const DevPrime *primes;
+    u64 poly_stride;
+    u32 n_limbs, logn, chunks;
+    u64 w[kMaxScalarLimbs];  // scalar mod q_i, Shoup quotient
+    u64 wq[kMaxScalarLimbs];
+};
+
+__global__ __launch_bounds__(kEwThreads) void k_scalar_mul(ScalarArgs a) {
+    const u32 b = blockIdx.x;
+    const u32 chunk = b % a.chunks;
+    const u32 pl = b / a.chunks;
+    const u32 limb = pl % a.n_limbs, poly = pl / a.n_limbs;
+    const u64 q = a.primes[limb].q;
+    const u64 w = a.w[limb], wq = a.wq[limb];
+    const size_t base = poly * a.poly_stride + ((size_t)limb << a.logn) + (size_t)chunk * kEwChunk +
+                        threadIdx.x * kEwPerThread;
+#pragma unroll
+    for (int i = 0; i < kEwPerThread; ++i) a.out[base + i] = shoup(a.in[base + i], w, wq, q);
+}
+
+static int check_poly_call(const ckks_rns_plan *plan, const u64 *in, const u64 *out, size_t n_polys,
+                           size_t n_limbs, size_t &poly_stride, const char *what) {
+    int rc = check_plan_limbs(plan, n_limbs, what);
+    if (rc) return rc;
+    if (n_polys && (!in || !out)) return fail(CKKS_ERR_INVALID, "null buffer");
+    if (n_polys > 1 && poly_stride < n_limbs * plan->degree)
+        return fail(CKKS_ERR_INVALID, "polynomial stride smaller than its limbs");
+    if (n_polys <= 1) poly_stride = n_limbs * plan->degree;
+    return CKKS_OK;
+}
+
+static int monomial(bool ntt_domain, const ckks_rns_plan *plan, const u64 *in, u64 *out, size_t n_polys,
+                    size_t n_limbs, size_t poly_stride, u64 e, int sign) {
+    int rc = check_poly_call(plan, in, out, n_polys, n_limbs, poly_stride, "monomial");
+    if (rc || n_polys == 0) return rc;
+    if (!ntt_domain && in == out) return fail(CKKS_ERR_INVALID, "monomial output must not alias its input");
+    if (sign != 1 && sign != -1) return fail(CKKS_ERR_INVALID, "monomial sign must be either 1 or -1");
+    MonoArgs a{};
+    a.in = in;
+    a.out = out;
+    a.primes = plan->d_primes;
+    a.poly_stride = poly_stride;
+    a.e = e % (2 * (u64)plan->degree);
+    a.negate = sign == -1 ? 1u : 0u;
+    a.n_limbs = (u32)n_limbs;
+    a.logn = plan->logn;
+    a.chunks = (u32)(plan->degree / kEwChunk);
+    const dim3 grid((u32)(n_polys * n_limbs * a.chunks));
+    if (ntt_domain)
+        hipLaunchKernelGGL(k_ntt_monomial, grid, dim3(kEwThreads), 0, plan->ctx->stream, a);
+    else
+        hipLaunchKernelGGL(k_monomial, grid, dim3(kEwThreads), 0, plan->ctx->stream, a);
+    CK_HIP(hipGetLastError());
+    return CKKS_OK;
+}
+
+static int scalar_mul(const ckks_rns_plan *plan, const u64 *in, const u64 *scalars, u64 *out, size_t n_polys,
+                      size_t n_limbs, size_t poly_stride) {
+    int rc = check_poly_call(plan, in, out, n_polys, n_limbs, poly_stride, "scalar multiply");
+    if (rc || n_polys == 0) return rc;
+    if (!scalars) return fail(CKKS_ERR_INVALID, "null scalars");
+    if (n_limbs > kMaxScalarLimbs) return fail(CKKS_ERR_UNSUPPORTED, "scalar multiply of more than 128 limbs");
+    ScalarArgs a{};
+    a.in = in;
+    a.out = out;
+    a.primes = plan->d_primes;
+    a.poly_stride = poly_stride;
+    a.n_limbs = (u32)n_limbs;
+    a.logn = plan->logn;
+    a.chunks = (u32)(plan->degree / kEwChunk);
+    for (size_t i = 0; i < n_limbs; ++i) {
+        const u64 q = plan->moduli[i];
+        a.w[i] = scalars[i] % q;
+        a.wq[i] = host::shoup_quotient(a.w[i], q);
+    }
+    hipLaunchKernelGGL(k_scalar_mul, dim3((u32)(n_polys * n_limbs * a.chunks)), dim3(kEwThreads), 0,
+                       plan->ctx->stream, a);
     CK_HIP(hipGetLastError());
     return CKKS_OK;
 }
@@ -505,6 +652,161 @@ static int rlwe_raw(const ckks_rns_plan *plan, const u64 *m, size_t m_stride, co
     return CKKS_OK;
 }
 
+// Key generation from host-sampled randomness (see ckks_keygen): entry j of
+// the key is the encryption under s of g_j * source, g_j the gadget factor of
+// entry j per limb, computed in the NTT domain by linearity,
+//   NTT(b_j) = NTT(g_j * source + e_j) - NTT(a_j) * NTT(s),
+// which equals the forward NTT of encrypt_rns_raw's b = m + e - a*s.
+struct KeygenArgs {
+    const u64 *factors;   // [entries][limbs] gadget factors, canonical
+    const u64 *source;    // [limbs][N], coefficient domain
+    const int64_t *e;     // [entries][N]
+    const u64 *s_ntt;     // [limbs][N]
+    const u64 *ka;        // [entries][limbs][N], NTT(a)
+    u64 *kb;              // [entries][limbs][N]
+    const DevPrime *primes;
+    u32 limbs, logn;
+    u64 total;            // entries * limbs * N
+};
+
+__global__ __launch_bounds__(kEwThreads) void k_keygen_message(KeygenArgs a) {
+    const u64 gid = (u64)blockIdx.x * kEwThreads + threadIdx.x;
+    if (gid >= a.total) return;
+    const u64 N = (u64)1 << a.logn;
+    const u64 row = gid >> a.logn;  // entry * limbs + limb
+    const u32 limb = (u32)(row % a.limbs);
+    const u64 j = row / a.limbs;
+    const DevPrime P = a.primes[limb];
+    const Barrett br{P.q, P.br_hi, P.br_lo};
+    const u64 q = P.q;
+    const u64 x = gid & (N - 1);
+    const u64 m = mul_mod(a.factors[row], a.source[((u64)limb << a.logn) + x], br);
+    const int64_t ev = a.e[j * N + x];
+    u64 ep;
+    if (ev >= 0) {
+        ep = (u64)ev % q;
+    } else {
+        const u64 mag = ((u64)(-(ev + 1)) + 1) % q;
+        ep = mag ? q - mag : 0;
+    }
+    a.kb[gid] = add_mod(m, ep, q);
+}
+
+__global__ __launch_bounds__(kEwThreads) void k_keygen_finish(KeygenArgs a) {
+    const u64 gid = (u64)blockIdx.x * kEwThreads + threadIdx.x;
+    if (gid >= a.total) return;
+    const u64 N = (u64)1 << a.logn;
+    const u32 limb = (u32)((gid >> a.logn) % a.limbs);
+    const DevPrime P = a.primes[limb];
+    const Barrett br{P.q, P.br_hi, P.br_lo};
+    const u64 as = mul_mod(a.ka[gid], a.s_ntt[((u64)limb << a.logn) + (gid & (N - 1))], br);
+    a.kb[gid] = sub_mod(a.kb[gid], as, P.q);
+}
+
+// Flags (bit 0) a secret coefficient outside {-1, 0, 1}, (bit 1) a nonzero
+// one, (bit 2) a residue of `a` that is not canonical.
+__global__ __launch_bounds__(kEwThreads) void k_keygen_check(const int8_t *s, const u64 *a, u64 a_total,
+                                                             const DevPrime *primes, u32 limbs, u32 logn,
+                                                             int *flags) {
+    const u64 N = (u64)1 << logn;
+    for (u64 i = (u64)blockIdx.x * kEwThreads + threadIdx.x; i < a_total; i += (u64)gridDim.x * kEwThreads) {
+        if (i < N) {
+            const int v = s[i];
+            if (v < -1 || v > 1) flags[0] = 1;
+            if (v != 0) flags[1] = 1;
+        }
+        if (a[i] >= primes[(u32)((i >> logn) % limbs)].q) flags[2] = 1;
+    }
+}
+
+static int keygen(const ckks_rns_plan *plan, int kind, size_t q_limbs, const size_t *block_sizes, size_t n_blocks,
+                  unsigned base_log, const u64 *source, const int8_t *secret, const u64 *a, const int64_t *e,
+                  u64 *key_b, u64 *key_a) {
+    if (!plan || !source || !secret || !a || !e || !key_b || !key_a) return fail(CKKS_ERR_INVALID, "null argument");
+    const auto &mods = plan->moduli;
+    const size_t B = mods.size(), N = plan->degree;
+    if (q_limbs == 0 || q_limbs > B) return fail(CKKS_ERR_INVALID, "Q limb count must be within the key basis");
+    ckks_context *ctx = plan->ctx;
+    if (ctx->capturing) return fail(CKKS_ERR_INVALID, "key generation reads its checks back: not capturable");
+    // P mod every limb (1 without special primes)
+    std::vector<u64> p_mod(B);
+    for (size_t i = 0; i < B; ++i) {
+        u64 v = 1 % mods[i];
+        for (size_t k = q_limbs; k < B; ++k) v = host::mulmod(v, mods[k] % mods[i], mods[i]);
+        p_mod[i] = v;
+    }
+    std::vector<u64> factors;
+    size_t entries = 0;
+    if (kind == CKKS_KEYGEN_RNS_GADGET || kind == CKKS_KEYGEN_HYBRID) {
+        if (kind == CKKS_KEYGEN_RNS_GADGET && q_limbs != B)
+            return fail(CKKS_ERR_INVALID, "RNS gadget keys have no special primes");
+        if (kind == CKKS_KEYGEN_HYBRID && q_limbs == B)
+            return fail(CKKS_ERR_INVALID, "hybrid keys need special primes");
+        if (!block_sizes || n_blocks == 0)
+            return fail(CKKS_ERR_INVALID, "RNS gadget layout must contain at least one block");
+        size_t start = 0;
+        for (size_t j = 0; j < n_blocks; ++j) {
+            if (block_sizes[j] == 0) return fail(CKKS_ERR_INVALID, "RNS gadget blocks must be nonempty");
+            for (size_t i = 0; i < B; ++i)
+                factors.push_back(i >= start && i < start + block_sizes[j] ? p_mod[i] : 0);
+            start += block_sizes[j];
+        }
+        if (start != q_limbs) return fail(CKKS_ERR_INVALID, "RNS gadget blocks must cover the full basis exactly");
+        entries = n_blocks;
+    } else if (kind == CKKS_KEYGEN_BOUNDED) {
+        size_t count = 0;
+        int rc = ckks_bounded_digit_count(plan, q_limbs, base_log, &count);
+        if (rc) return rc;
+        std::vector<u64> g(B);
+        for (size_t i = 0; i < B; ++i) g[i] = p_mod[i];
+        const u64 two_w = base_log >= 64 ? 0 : (1ull << base_log);
+        for (size_t j = 0; j < count; ++j) {
+            for (size_t i = 0; i < B; ++i) {
+                factors.push_back(g[i]);
+                g[i] = host::mulmod(g[i], two_w % mods[i], mods[i]);
+            }
+        }
+        entries = count;
+    } else {
+        return fail(CKKS_ERR_INVALID, "unknown key kind");
+    }
+    const size_t poly = B * N, total = entries * poly;
+    void *w;
+    const size_t f_off = poly, flag_off = poly + factors.size();
+    int rc = ctx->workspace(1, (flag_off + 2) * sizeof(u64), &w);
+    if (rc) return rc;
+    u64 *s_ntt = (u64 *)w, *d_factors = s_ntt + f_off;
+    int *flags = (int *)(s_ntt + flag_off);
+    hipStream_t st = ctx->stream;
+    CK_HIP(hipMemcpyAsync(d_factors, factors.data(), factors.size() * sizeof(u64), hipMemcpyHostToDevice, st));
+    CK_HIP(hipMemsetAsync(flags, 0, 3 * sizeof(int), st));
+    const u32 check_grid = (u32)std::min<u64>((total + kEwThreads - 1) / kEwThreads, 4096);
+    hipLaunchKernelGGL(k_keygen_check, dim3(check_grid), dim3(kEwThreads), 0, st, secret, a, (u64)total,
+                       plan->d_primes, (u32)B, plan->logn, flags);
+    CK_HIP(hipGetLastError());
+    int hflags[3] = {0, 0, 0};
+    CK_HIP(hipMemcpyAsync(hflags, flags, 3 * sizeof(int), hipMemcpyDeviceToHost, st));
+    CK_HIP(hipStreamSynchronize(st));
+    if (hflags[0]) return fail(CKKS_ERR_INVALID, "RNS secret coefficients must be ternary");
+    if (!hflags[1]) return fail(CKKS_ERR_INVALID, "RNS target secret must be nonzero");
+    if (hflags[2]) return fail(CKKS_ERR_INVALID, "uniform residues must be canonical");
+    ProjectArgs pa{secret, s_ntt, plan->d_primes, (u32)B, plan->logn};
+    hipLaunchKernelGGL(k_project_secret, dim3((u32)((poly + kEwThreads - 1) / kEwThreads)), dim3(kEwThreads), 0,
+                       st, pa);
+    CK_HIP(hipGetLastError());
+    KeygenArgs ka{d_factors, source, e, s_ntt, key_a, key_b, plan->d_primes, (u32)B, plan->logn, (u64)total};
+    const dim3 grid((u32)((total + kEwThreads - 1) / kEwThreads));
+    hipLaunchKernelGGL(k_keygen_message, grid, dim3(kEwThreads), 0, st, ka);
+    CK_HIP(hipGetLastError());
+    NttCall f[3] = {{plan, s_ntt, s_ntt, 1, B, poly, poly},
+                    {plan, a, key_a, entries, B, poly, poly},
+                    {plan, key_b, key_b, entries, B, poly, poly}};
+    if ((rc = launch_ntt(ctx, f, 3, true))) return rc;
+    hipLaunchKernelGGL(k_keygen_finish, grid, dim3(kEwThreads), 0, st, ka);
+    CK_HIP(hipGetLastError());
+    return CKKS_OK;
+}
+
 }  // namespace ck
 
 using namespace ck;
@@ -517,6 +819,12 @@ int ckks_encrypt_raw(const ckks_rns_plan *plan, const uint64_t *message, const i
     if (!plan) return fail(CKKS_ERR_INVALID, "null plan");
     return rlwe_raw(plan, message, poly_stride, error, secret, a, poly_stride, b, poly_stride, n_polys,
                     n_limbs, true);
+}
+
+int ckks_keygen(const ckks_rns_plan *plan, int kind, size_t q_limbs, const size_t *block_sizes, size_t n_blocks,
+                unsigned base_log, const uint64_t *source, const int8_t *secret, const uint64_t *a,
+                const int64_t *e, uint64_t *key_b, uint64_t *key_a) {
+    return keygen(plan, kind, q_limbs, block_sizes, n_blocks, base_log, source, secret, a, e, key_b, key_a);
 }
 
 int ckks_decrypt_raw(const ckks_rns_plan *plan, const uint64_t *b, const uint64_t *a, const int8_t *secret,
@@ -550,6 +858,24 @@ int ckks_poly_sub(const ckks_rns_plan *plan, const uint64_t *x, const uint64_t *
 int ckks_poly_neg(const ckks_rns_plan *plan, const uint64_t *x, uint64_t *out, size_t n_polys,
                   size_t n_limbs, size_t poly_stride) {
     return elementwise(EW_NEG, plan, x, nullptr, out, n_polys, n_limbs, poly_stride);
+}
+
+int ckks_ntt_mul_monomial(const ckks_rns_plan *plan, const uint64_t *in, uint64_t *out, size_t n_polys,
+                          size_t n_limbs, size_t poly_stride, int64_t exponent) {
+    if (!plan) return fail(CKKS_ERR_INVALID, "null plan");
+    const int64_t period = 2 * (int64_t)plan->degree;
+    const int64_t e = ((exponent % period) + period) % period;  // rem_euclid
+    return monomial(true, plan, in, out, n_polys, n_limbs, poly_stride, (u64)e, 1);
+}
+
+int ckks_mul_monomial(const ckks_rns_plan *plan, const uint64_t *in, uint64_t *out, size_t n_polys,
+                      size_t n_limbs, size_t poly_stride, uint64_t exponent, int sign) {
+    return monomial(false, plan, in, out, n_polys, n_limbs, poly_stride, exponent, sign);
+}
+
+int ckks_poly_scalar_mul(const ckks_rns_plan *plan, const uint64_t *in, const uint64_t *scalars, uint64_t *out,
+                         size_t n_polys, size_t n_limbs, size_t poly_stride) {
+    return scalar_mul(plan, in, scalars, out, n_polys, n_limbs, poly_stride);
 }
 
 int ckks_automorphism(const ckks_rns_plan *plan, const uint64_t *in, uint64_t *out, size_t n_polys,

@@ -195,6 +195,22 @@ int ckks_poly_sub(const ckks_rns_plan *plan, const uint64_t *x, const uint64_t *
 int ckks_poly_neg(const ckks_rns_plan *plan, const uint64_t *x, uint64_t *out,
                   size_t n_polys, size_t n_limbs, size_t poly_stride);
 
+/* NttPolynomial::mul_monomial_signed (ring/ntt_polynomial.rs:133-175) and
+ * RnsNttPolynomial::mul_monomial_signed (ring/rns_ntt.rs:215-235): NTT-domain
+ * product by X^exponent, evaluation k scaled by psi^((2k+1) e mod 2N) with
+ * e = exponent rem_euclid 2N.  out may alias in. */
+int ckks_ntt_mul_monomial(const ckks_rns_plan *plan, const uint64_t *in, uint64_t *out, size_t n_polys,
+                          size_t n_limbs, size_t poly_stride, int64_t exponent);
+/* Polynomial::mul_monomial_signed (ring/polynomial.rs:164-200) per limb:
+ * coefficient-domain sign * X^exponent * x (exponent reduced mod 2N, sign 1
+ * or -1).  out must not alias in. */
+int ckks_mul_monomial(const ckks_rns_plan *plan, const uint64_t *in, uint64_t *out, size_t n_polys,
+                      size_t n_limbs, size_t poly_stride, uint64_t exponent, int sign);
+/* Polynomial::scalar_mul (ring/polynomial.rs:144-155) per limb: limb i times
+ * scalars[i] mod q_i (scalars: host array of n_limbs values, any u64; at
+ * most 128 limbs).  Either domain.  out may alias in. */
+int ckks_poly_scalar_mul(const ckks_rns_plan *plan, const uint64_t *in, const uint64_t *scalars, uint64_t *out,
+                         size_t n_polys, size_t n_limbs, size_t poly_stride);
 /* apply_automorphism (ckks/automorphism.rs:8-46) per limb, coefficient
  * domain; exponent odd modulo 2N.  out must not alias in. */
 int ckks_automorphism(const ckks_rns_plan *plan, const uint64_t *in, uint64_t *out,
@@ -283,8 +299,9 @@ typedef struct ckks_ksk ckks_ksk;
 
 /* PreparedRnsKeySwitchKey::new / PreparedRnsMultiplicationKey
  * (grafting/rns_key_switch.rs:1930-2013, 2347-2430): NTT-domain key
- * entries, block-major [block][limb] pairs (b, a), uploaded from host
- * arrays key_b, key_a of n_blocks * n_limbs * N residues each.
+ * entries, block-major [block][limb] pairs (b, a), copied from key_b,
+ * key_a of n_blocks * n_limbs * N residues each (host arrays or device
+ * buffers; device buffers are read on the context stream and checked there).
  * RnsGadgetLayout::new (grafting/rns_gadget.rs:37-67) block sizes. */
 int ckks_rns_ksk_create(const ckks_rns_plan *plan, const size_t *block_sizes, size_t n_blocks,
                         const uint64_t *key_b, const uint64_t *key_a, ckks_ksk **out);
@@ -310,6 +327,38 @@ int ckks_hybrid_ksk_create(const ckks_rns_plan *qp_plan, size_t q_limbs, const s
  * which gives the same residues. */
 int ckks_bounded_ksk_create(const ckks_rns_plan *plan, size_t q_limbs, unsigned base_log,
                             const uint64_t *key_b, const uint64_t *key_a, ckks_ksk **out);
+
+/* Key generation on the device from host-sampled randomness, for the three
+ * key kinds above:
+ *   RnsKeySwitchKey::generate_with_distribution_ntt_rng and
+ *   RnsMultiplicationKey::generate_with_distribution_ntt_rng
+ *     (grafting/rns_key_switch.rs:1155-1258, 1430-1512): CKKS_KEYGEN_RNS_GADGET;
+ *   the dnum-block hybrid key of ckks_hybrid_ksk_create: CKKS_KEYGEN_HYBRID;
+ *   PreparedHybridRnsGaloisKey::generate_with_distribution_ntt_rng
+ *     (ckks/hybrid_rns_galois_key.rs:61-170) and BoundedRnsMultiplicationKey
+ *     (grafting/bounded_key_switch.rs:22-118): CKKS_KEYGEN_BOUNDED.
+ * Each entry is encrypt_rns_raw_with_distribution_ntt_rng
+ * (grafting/rns_key_switch.rs:245-383) of a gadget multiple of `source`
+ * under `secret`, in the NTT domain:
+ *   key_b[j] = NTT(g_j * source + e_j - a_j * secret),  key_a[j] = NTT(a_j)
+ * limb by limb over the plan's basis, where g_j is
+ *   RNS gadget: 1 on the limbs of block j, 0 elsewhere (q_limbs = plan limbs);
+ *   hybrid:     P on the limbs of block j, 0 elsewhere (P = product of the
+ *               plan limbs after the first q_limbs);
+ *   bounded:    P * 2^(base_log * j) on every limb (P = 1 without special
+ *               primes), j < ckks_bounded_digit_count.
+ * The reference samples a_j (uniform per limb) and e_j (one logical error
+ * per entry) from its RNG inside; here the caller samples them.  All
+ * buffers are device memory: source [limb][N] coefficient domain (sigma_k(s)
+ * for a Galois key, s^2 for a relinearization key), secret [N] ternary,
+ * a [entries][limb][N] canonical, e [entries][N], outputs key_b and key_a
+ * [entries][limb][N], which ckks_*_ksk_create accept as device pointers.
+ * Checks the secret (ternary, nonzero) and a (canonical) and synchronises
+ * the stream: not capturable. */
+enum ckks_keygen_kind { CKKS_KEYGEN_RNS_GADGET = 0, CKKS_KEYGEN_HYBRID = 1, CKKS_KEYGEN_BOUNDED = 2 };
+int ckks_keygen(const ckks_rns_plan *plan, int kind, size_t q_limbs, const size_t *block_sizes, size_t n_blocks,
+                unsigned base_log, const uint64_t *source, const int8_t *secret, const uint64_t *a,
+                const int64_t *e, uint64_t *key_b, uint64_t *key_a);
 int ckks_bounded_digit_count(const ckks_rns_plan *plan, size_t q_limbs, unsigned base_log,
                              size_t *count);
 

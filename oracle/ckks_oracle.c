@@ -420,6 +420,26 @@ void orc_ntt_mul_monomial(const orc_plan *p, const uint64_t *in, uint64_t *out, 
     }
 }
 
+/* Polynomial::mul_monomial_signed (ring/polynomial.rs:164-200): sign * X^e * x
+ * in Z_q[X]/(X^N + 1) as a signed coefficient permutation; e reduced mod 2N. */
+void orc_mul_monomial(const uint64_t *in, uint64_t *out, size_t n, uint64_t q, uint64_t exponent, int sign) {
+    uint64_t e = exponent % (2 * (uint64_t)n);
+    size_t shift = (size_t)(e % n);
+    int exp_neg = e >= n, ext_neg = sign == -1;
+    for (size_t src = 0; src < n; ++src) {
+        size_t sum = src + shift;
+        int wrap = sum >= n;
+        size_t dst = wrap ? sum - n : sum;
+        out[dst] = (exp_neg ^ ext_neg ^ wrap) ? orc_neg(in[src], q) : in[src];
+    }
+}
+
+/* Polynomial::scalar_mul (ring/polynomial.rs:144-155): coefficient * scalar
+ * mod q (Modulus::mul reduces the full product; the scalar may exceed q). */
+void orc_scalar_mul(const uint64_t *in, uint64_t *out, size_t n, uint64_t q, uint64_t scalar) {
+    for (size_t i = 0; i < n; ++i) out[i] = (uint64_t)(((unsigned __int128)in[i] * scalar) % q);
+}
+
 /* ======================================================================
  * R8 / C1  Galois automorphisms
  * ==================================================================== */

@@ -70,6 +70,8 @@ void orc_poly_add(const uint64_t *x, const uint64_t *y, uint64_t *out, size_t n,
 void orc_poly_sub(const uint64_t *x, const uint64_t *y, uint64_t *out, size_t n, uint64_t q);
 void orc_negacyclic_mul_naive(const uint64_t *x, const uint64_t *y, uint64_t *out, size_t n, uint64_t q);
 void orc_ntt_mul_monomial(const orc_plan *plan, const uint64_t *in, uint64_t *out, int64_t exponent);
+void orc_mul_monomial(const uint64_t *in, uint64_t *out, size_t n, uint64_t q, uint64_t exponent, int sign);
+void orc_scalar_mul(const uint64_t *in, uint64_t *out, size_t n, uint64_t q, uint64_t scalar);
 
 /* ---- R8 / C1: Galois automorphisms ------------------------------------- */
 void orc_permute_ntt_galois(const uint64_t *in, uint64_t *out, size_t n, uint64_t exponent);

@@ -63,6 +63,8 @@ def lib():
             "orc_poly_sub": (None, [P, P, P, sz, u64]),
             "orc_negacyclic_mul_naive": (None, [P, P, P, sz, u64]),
             "orc_ntt_mul_monomial": (None, [P, P, P, C.c_int64]),
+            "orc_mul_monomial": (None, [P, P, C.c_size_t, C.c_uint64, C.c_uint64, C.c_int]),
+            "orc_scalar_mul": (None, [P, P, C.c_size_t, C.c_uint64, C.c_uint64]),
             "orc_permute_ntt_galois": (None, [P, P, sz, u64]),
             "orc_apply_automorphism": (None, [P, P, sz, u64, u64]),
             "orc_rotation_exponent_left": (u64, [sz, sz]),
@@ -250,6 +252,22 @@ def permute_ntt_galois(x, k):
 def apply_automorphism(x, q, k):
     x = _u64(x); out = np.empty_like(x)
     lib().orc_apply_automorphism(_p(x), _p(out), x.size, q, k)
+    return out
+
+
+def mul_monomial(x, q, exponent: int, sign: int = 1):
+    """Polynomial::mul_monomial_signed (ring/polynomial.rs:164-200)."""
+    x = _u64(x)
+    out = np.empty_like(x)
+    lib().orc_mul_monomial(_p(x), _p(out), len(x), q, exponent, sign)
+    return out
+
+
+def scalar_mul(x, q, scalar: int):
+    """Polynomial::scalar_mul (ring/polynomial.rs:144-155)."""
+    x = _u64(x)
+    out = np.empty_like(x)
+    lib().orc_scalar_mul(_p(x), _p(out), len(x), q, scalar)
     return out
 
 

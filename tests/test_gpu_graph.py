@@ -82,3 +82,35 @@ def test_graph_captured_rotation_schedule(pkg):
             assert not (want == eager).all()
         graph.close()
         ctx.close()
+
+
+def test_encode_refuses_capture_and_context_stays_usable(pkg):
+    """Encoding reads an overflow flag back to the host, which a captured
+    graph cannot do: inside graph_begin/graph_end it fails with a status
+    code, the capture still closes, and the context encodes afterwards."""
+    import torch
+
+    n, L = 1 << 12, 3
+    moduli = [Q0[0]] + Q40[:L - 1]
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        ctx = pkg.Context(0, stream=stream)
+        plan = pkg.PreparedRnsNttPlan(ctx, moduli, n)
+        emb = pkg.encoding.CanonicalEmbedding(n)
+        slots = torch.zeros(n // 2, dtype=torch.complex128)
+        slots[:4] = torch.tensor([0.5, -0.25, 1.0, 0.0], dtype=torch.complex128)
+        want = pkg.to_host(pkg.encoding.encode(plan, emb, slots, 2.0 ** 30))
+        # the embedding's FFT runs before the capture opens
+        coeffs = emb.slots_to_coefficients(slots.cuda()[None]).contiguous()
+        out = pkg.empty((1, L, n))
+        ctx.graph_begin()
+        with pytest.raises(pkg.CkksError, match="capturable"):
+            pkg._check(pkg.lib().ckks_encode_coefficients(plan.handle, pkg._ptr(coeffs, dtype="float64"),
+                                                          2.0 ** 30, pkg._ptr(out), 1, L))
+        g = ctx.graph_end()
+        g.close()
+        again = pkg.to_host(pkg.encoding.encode(plan, emb, slots, 2.0 ** 30))
+        stream.synchronize()
+        assert (again == want).all()
+        plan.close()
+        ctx.close()

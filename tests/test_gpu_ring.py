@@ -201,3 +201,43 @@ def test_tensor(pkg, ctx, logn, limbs):
     for c in range(2):
         c0, c1, c2 = O.tensor(orc, ct0[c, 0], ct0[c, 1], ct1[c, 0], ct1[c, 1])
         assert (out[c, 0] == c0).all() and (out[c, 1] == c1).all() and (out[c, 2] == c2).all()
+
+
+@pytest.mark.parametrize("logn", [10, 12, 16])
+def test_monomials_and_scalars_bit_exact(pkg, ctx, logn):
+    """RnsNttPolynomial::mul_monomial_signed (ring/rns_ntt.rs:215), the
+    coefficient-domain Polynomial::mul_monomial_signed and scalar_mul
+    (ring/polynomial.rs:144, 164) against the oracle, on 40-bit (FP64 table)
+    and 60-bit primes."""
+    n = 1 << logn
+    moduli = MODULI[:3] + MODULI[-2:]
+    rng = np.random.default_rng(300 + logn)
+    x = rand_residues(rng, moduli, n, polys=2)
+    x[0, :, :4] = np.array(moduli, dtype=np.uint64)[:, None] - 1
+    plan = _plan(pkg, ctx, n, moduli)
+    orc = O.RnsPlan(moduli, n)
+    d = pkg.to_device(x)
+    for e in [0, 1, -1, n - 1, n, n + 1, 2 * n - 1, 2 * n + 3, 12345, -54321]:
+        got = pkg.to_host(plan.ntt_mul_monomial_signed(d, e))
+        for p in range(2):
+            want = np.stack([O.ntt_mul_monomial(orc.plans[i], x[p, i], e) for i in range(len(moduli))])
+            assert (got[p] == want).all(), f"NTT monomial e={e}"
+        if e >= 0:
+            for sign in (1, -1):
+                got = pkg.to_host(plan.mul_monomial_signed(d, e, sign))
+                for p in range(2):
+                    want = np.stack([O.mul_monomial(x[p, i], q, e, sign) for i, q in enumerate(moduli)])
+                    assert (got[p] == want).all(), f"monomial e={e} sign={sign}"
+    # in place in the NTT domain
+    dd = d.clone()
+    plan.ntt_mul_monomial_signed(dd, 7, out=dd)
+    assert (pkg.to_host(dd) == pkg.to_host(plan.ntt_mul_monomial_signed(d, 7))).all()
+    scalars = [0, moduli[1] + 5, (1 << 64) - 1, 0x9E3779B97F4A7C15, moduli[4] - 1]
+    got = pkg.to_host(plan.scalar_mul(d, scalars))
+    for p in range(2):
+        want = np.stack([O.scalar_mul(x[p, i], q, scalars[i]) for i, q in enumerate(moduli)])
+        assert (got[p] == want).all()
+    with pytest.raises(pkg.CkksError, match="sign"):
+        plan.mul_monomial_signed(d, 3, 2)
+    with pytest.raises(pkg.CkksError, match="alias"):
+        plan.mul_monomial_signed(d, 3, 1, out=d)

@@ -25,7 +25,10 @@ SCRIPT = textwrap.dedent("""
     q = [Q0[0]] + Q40[:L - 1]
     qp = q + P60[:K]
     rng = np.random.default_rng(5)
-    ctx = pkg.Context(0)
+    import torch
+    stream = torch.cuda.Stream()  # graph capture needs a non-null stream
+    torch.cuda.set_stream(stream)
+    ctx = pkg.Context(0, stream=stream)
     plan = pkg.PreparedRnsNttPlan(ctx, qp, n)
     qplan = pkg.PreparedRnsNttPlan(ctx, q, n)
     cnt = pkg.bounded_digit_count(plan, L, w)

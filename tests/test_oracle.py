@@ -211,6 +211,39 @@ def test_monomial_multiplication_in_ntt_domain():
         assert (got == O.negacyclic_mul_naive(a, mono, q)).all()
 
 
+def test_coefficient_monomial_is_a_negacyclic_product():
+    # ring/polynomial.rs:164-200 (and its tests): sign * X^e * a equals the
+    # naive negacyclic product with the monomial; X^(2N) = 1, X^N = -1
+    n, q = 64, 12289
+    rng = np.random.default_rng(5)
+    a = rng.integers(0, q, n, dtype=np.uint64)
+    for e in [0, 1, 5, 63, 64, 65, 127, 128, 129, 1000]:
+        for sign in (1, -1):
+            mono = np.zeros(n, dtype=np.uint64)
+            ee = e % (2 * n)
+            neg = (ee >= n) ^ (sign == -1)
+            mono[ee % n] = q - 1 if neg else 1
+            assert (O.mul_monomial(a, q, e, sign) == O.negacyclic_mul_naive(a, mono, q)).all(), (e, sign)
+    # the NTT-domain form of the same product (ring/ntt_polynomial.rs:133)
+    plan = O.Plan(q, n)
+    for e in [3, 64, 100]:
+        assert (plan.inverse(O.ntt_mul_monomial(plan, plan.forward(a), e)) == O.mul_monomial(a, q, e)).all()
+
+
+def test_scalar_mul_known_values():
+    # Modulus::mul reduces the full 128-bit product: scalars above q, 0, q-1
+    q = 1152921504606584833
+    x = np.array([0, 1, q - 1, 123456789123456789, q // 2], dtype=np.uint64)
+    for scalar in [0, 1, q - 1, q, q + 5, (1 << 64) - 1, 0x9E3779B97F4A7C15]:
+        want = [(int(v) * scalar) % q for v in x]
+        assert O.scalar_mul(x, q, scalar).tolist() == want
+    # linearity through the NTT: NTT(c a) = c NTT(a)
+    n, q2 = 64, 12289
+    plan = O.Plan(q2, n)
+    a = np.random.default_rng(6).integers(0, q2, n, dtype=np.uint64)
+    assert (plan.forward(O.scalar_mul(a, q2, 777)) == O.scalar_mul(plan.forward(a), q2, 777)).all()
+
+
 # ------------------------------------------------------------------ R8 / C1
 def test_automorphism_known_answer():
     # ckks/automorphism.rs automorphism_matches_manual_negacyclic_mapping
