@@ -107,6 +107,9 @@ _SIGNATURES = {
     "ckks_server_step": (C.c_int, [_p, C.POINTER(_sz)]),
     "ckks_server_poll": (C.c_int, [_p, C.c_uint64, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "ckks_server_drain": (C.c_int, [_p]),
+    "ckks_profile_get": (C.c_int, [_p, _p]),
+    "ckks_profile_reset": (C.c_int, [_p]),
+    "ckks_server_cancel": (C.c_int, [_p, C.c_uint64]),
     "ckks_server_stats": (C.c_int, [_p, _p]),
     "ckks_server_destroy": (C.c_int, [_p]),
     "ckks_decrypt_raw": (C.c_int, [_p, _p, _p, _p, _p, _sz, _sz, _sz, _sz]),
@@ -281,6 +284,18 @@ class Context:
         """Key switches of >= 8 ciphertexts split over two streams (ON), not
         (OFF), or the key's tuned choice (AUTO, the default)."""
         _check(lib().ckks_set_key_switch_split(self._h, int(mode)))
+
+    def execution_profile(self):
+        """ckks_profile_get: the operations run on this context so far, as a
+        ckks.ExecutionProfile (execution/mod.rs:162-230)."""
+        from .ckks import ExecutionProfile
+
+        raw = (C.c_uint64 * 12)()
+        _check(lib().ckks_profile_get(self._h, C.cast(raw, _p)))
+        return ExecutionProfile(*[int(v) for v in raw])
+
+    def reset_execution_profile(self):
+        _check(lib().ckks_profile_reset(self._h))
 
     def ntt_variant(self, logn: int) -> int:
         v = C.c_int()

@@ -98,6 +98,57 @@ class ExecutionTrace:
         return sum(1 for e in self._events if e.kind == kind)
 
 
+_PROFILE_FIELDS = {
+    ExecutionEventKind.CIPHERTEXT_PLAINTEXT_MULTIPLY: "ciphertext_plaintext_multiplies",
+    ExecutionEventKind.CIPHERTEXT_CIPHERTEXT_MULTIPLY: "ciphertext_ciphertext_multiplies",
+    ExecutionEventKind.ADD: "additions",
+    ExecutionEventKind.RELINEARIZE: "relinearizations",
+    ExecutionEventKind.RESCALE: "rescales",
+    ExecutionEventKind.MOD_SWITCH: "modulus_switches",
+    ExecutionEventKind.AUTOMORPHISM: "automorphisms",
+    ExecutionEventKind.ROTATE: "rotations",
+    ExecutionEventKind.NTT_FORWARD: "ntt_forwards",
+    ExecutionEventKind.NTT_INVERSE: "ntt_inverses",
+}
+
+
+@dataclass
+class ExecutionProfile:
+    """ExecutionProfile (execution/mod.rs:162-230): O(1) operation counts.
+    The device fills one per context (Context.execution_profile); the two
+    last fields have no reference counterpart: every key switch, and the
+    single-limb transforms behind the NTT counts."""
+    ciphertext_plaintext_multiplies: int = 0
+    ciphertext_ciphertext_multiplies: int = 0
+    additions: int = 0
+    relinearizations: int = 0
+    rescales: int = 0
+    modulus_switches: int = 0
+    automorphisms: int = 0
+    rotations: int = 0
+    ntt_forwards: int = 0
+    ntt_inverses: int = 0
+    key_switches: int = 0
+    ntt_limb_transforms: int = 0
+
+    def count(self, kind: ExecutionEventKind) -> int:
+        """The count of an event kind; the eBLAS / CCMM kinds have none (0)."""
+        f = _PROFILE_FIELDS.get(kind)
+        return getattr(self, f) if f else 0
+
+    def total_operations(self) -> int:
+        """Sum of the reference's ten counted kinds."""
+        return sum(getattr(self, f) for f in _PROFILE_FIELDS.values())
+
+    def __sub__(self, other: "ExecutionProfile") -> "ExecutionProfile":
+        return ExecutionProfile(**{k: getattr(self, k) - getattr(other, k) for k in self.__dataclass_fields__})
+
+    @classmethod
+    def from_trace(cls, trace: ExecutionTrace) -> "ExecutionProfile":
+        """The counts of a trace's events."""
+        return cls(**{f: trace.count(k) for k, f in _PROFILE_FIELDS.items()})
+
+
 # ---------------------------------------------------------------- chain
 
 class ModulusChain:
@@ -285,9 +336,13 @@ class RnsCkksEvaluator:
     a prefix of it)."""
 
     def __init__(self, chain: ModulusChain, keys: RnsCkksEvaluationKeys, plan: PreparedRnsNttPlan,
-                 trace: Optional[ExecutionTrace] = None):
+                 trace: Optional[ExecutionTrace] = None, device_events: bool = False):
         _require(tuple(plan.moduli) == chain.top(), "prepared RNS NTT plan basis must match the chain top basis")
         self.chain, self.keys, self.plan, self.trace = chain, keys, plan, trace
+        # device_events: after each operation's event, the NTT transforms the
+        # device ran for it (from the context's execution profile)
+        self.device_events = device_events
+        self._seen = plan.ctx.execution_profile() if (trace is not None and device_events) else None
 
     def _record(self, kind, ct: RnsCkksCiphertext, after: Optional[RnsCkksCiphertext] = None):
         if self.trace is None:
@@ -296,6 +351,14 @@ class RnsCkksEvaluator:
         self.trace.record(ExecutionEvent(kind, ring_degree=ct.degree, level=ct.level,
                                          level_after=after.level, rns_limbs=len(ct.basis),
                                          rns_limbs_after=len(after.basis)))
+        if self._seen is not None:
+            now = self.plan.ctx.execution_profile()
+            d = now - self._seen
+            self._seen = now
+            for ek, n in ((ExecutionEventKind.NTT_FORWARD, d.ntt_forwards),
+                          (ExecutionEventKind.NTT_INVERSE, d.ntt_inverses)):
+                for _ in range(n):
+                    self.trace.record(ExecutionEvent(ek, ring_degree=ct.degree, rns_limbs=len(ct.basis)))
 
     def _wrap(self, data, state) -> RnsCkksCiphertext:
         return RnsCkksCiphertext(data, state, self.chain, self.plan.degree)

@@ -119,7 +119,15 @@ struct ckks_graph {
     ckks_context *ctx = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    ckks_execution_profile counts = {};  // operations recorded in the capture
 };
+
+static void add_profile(ckks_execution_profile &to, const ckks_execution_profile &d) {
+    const size_t n = sizeof(ckks_execution_profile) / sizeof(uint64_t);
+    auto *a = reinterpret_cast<uint64_t *>(&to);
+    auto *b = reinterpret_cast<const uint64_t *>(&d);
+    for (size_t i = 0; i < n; ++i) a[i] += b[i];
+}
 
 extern "C" {
 
@@ -129,6 +137,7 @@ int ckks_graph_begin(ckks_context *ctx) {
     if (ctx->capturing) return fail(CKKS_ERR_INVALID, "a graph capture is already open");
     CK_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
     ctx->capturing = true;
+    ctx->captured = ckks_execution_profile{};
     return CKKS_OK;
 }
 
@@ -138,6 +147,7 @@ int ckks_graph_end(ckks_context *ctx, ckks_graph **out) {
     ctx->capturing = false;
     std::unique_ptr<ckks_graph> g(new ckks_graph());
     g->ctx = ctx;
+    g->counts = ctx->captured;
     CK_HIP(hipStreamEndCapture(ctx->stream, &g->graph));
     CK_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
     ctx->graphs.insert(g.get());
@@ -148,6 +158,19 @@ int ckks_graph_end(ckks_context *ctx, ckks_graph **out) {
 int ckks_graph_launch(ckks_graph *g) {
     if (!g) return fail(CKKS_ERR_INVALID, "null graph");
     CK_HIP(hipGraphLaunch(g->exec, g->ctx->stream));
+    add_profile(g->ctx->counts(), g->counts);
+    return CKKS_OK;
+}
+
+int ckks_profile_get(const ckks_context *ctx, ckks_execution_profile *out) {
+    if (!ctx || !out) return fail(CKKS_ERR_INVALID, "null argument");
+    *out = ctx->profile;
+    return CKKS_OK;
+}
+
+int ckks_profile_reset(ckks_context *ctx) {
+    if (!ctx) return fail(CKKS_ERR_INVALID, "null context");
+    ctx->profile = ckks_execution_profile{};
     return CKKS_OK;
 }
 

@@ -101,9 +101,21 @@ struct ckks_context {
     uint8_t ntt_variant[32] = {};  // two-pass NTT kernel variant per log2 N
     int ks_fusion = CKKS_KS_FUSION_AUTO;  // digits' row pass fused with the key MAC
     bool capturing = false;  // a hipGraph capture of `stream` is open
+    // operations run so far (ckks_profile_get); during a capture the counts
+    // go to `captured` and are added on each launch of the graph
+    ckks_execution_profile profile = {}, captured = {};
+    ckks_execution_profile &counts() { return capturing ? captured : profile; }
     std::vector<void *> owned;  // misc device allocations freed at destroy
 
     int workspace(int slot, size_t bytes, void **out);
+};
+
+// Keeps trial runs (autotuning) out of the execution profile.
+struct ProfileGuard {
+    ckks_context *ctx;
+    ckks_execution_profile saved;
+    explicit ProfileGuard(ckks_context *c) : ctx(c), saved(c->profile) {}
+    ~ProfileGuard() { ctx->profile = saved; }
 };
 
 struct ckks_rns_plan {

@@ -1491,6 +1491,13 @@ static int key_switch_core(const ckks_ksk *k, const u64 *x, u64 x_stride, const 
                            u64 galois_kinv = 0, const int64_t *hoisted = nullptr,
                            const HoistedMac *hm = nullptr) {
     ckks_context *ctx = k->ctx;
+    if (ctx->lane == 0) {
+        // one key switch per ciphertext; a Galois one is a rotation, or an
+        // automorphism when it conjugates (k = 2N - 1 is its own inverse)
+        ckks_execution_profile &pc = ctx->counts();
+        pc.key_switches += n_cts;
+        if (galois_kinv) (galois_kinv == 2 * (u64)k->degree - 1 ? pc.automorphisms : pc.rotations) += n_cts;
+    }
     const int split = ctx->ks_split == CKKS_KS_SPLIT_AUTO ? k->tuned_split : ctx->ks_split;
     if (split != CKKS_KS_SPLIT_ON || n_cts < 8 || ctx->capturing || ctx->lane != 0)
         return key_switch_lane(k, x, x_stride, add0, add1, add_stride, out, out_stride, n_cts, galois_kinv,
@@ -1684,6 +1691,13 @@ static int key_switch_lane(const ckks_ksk *k, const u64 *x, u64 x_stride, const 
     }
     CK_HIP(hipGetLastError());
     return CKKS_OK;
+}
+
+// Device bytes a key holds: its entries and the permuted copies kept for
+// hoisted rotations (tables are a few KB and not counted).
+size_t ksk_bytes(const ckks_ksk *k) {
+    const size_t entries = 2 * k->dnum * k->limbs * k->degree * sizeof(u64);
+    return entries * (1 + k->hoist_keys.size());
 }
 
 int ksk_info(const ckks_ksk *k, size_t *q_limbs, size_t *degree) {
@@ -1896,6 +1910,7 @@ int ckks_autotune_key_switch(ckks_ksk *k, size_t n_cts, int *best_engine, int *b
     if (!k || !best_engine || !best_fusion) return fail(CKKS_ERR_INVALID, "null argument");
     ckks_context *ctx = k->ctx;
     if (ctx->capturing) return fail(CKKS_ERR_INVALID, "autotuning is not allowed during graph capture");
+    ProfileGuard profile_guard(ctx);
     if (n_cts == 0) return fail(CKKS_ERR_INVALID, "autotune needs at least one ciphertext");
     if (ms)
         for (int i = 0; i < 8; ++i) ms[i] = -1.f;
@@ -2148,7 +2163,9 @@ int ckks_relinearize(const ckks_ksk *key, const uint64_t *in3, uint64_t *out, si
     int rc = check_ct_args(key, in3, out, n_cts, in_stride, out_stride, 3);
     if (rc || n_cts == 0) return rc;
     const size_t poly = key->q_limbs * key->degree;
-    return key_switch_core(key, in3 + 2 * poly, in_stride, in3, in3 + poly, in_stride, out, out_stride, n_cts);
+    rc = key_switch_core(key, in3 + 2 * poly, in_stride, in3, in3 + poly, in_stride, out, out_stride, n_cts);
+    if (rc == CKKS_OK) key->ctx->counts().relinearizations += n_cts;
+    return rc;
 }
 
 int ckks_multiply_relinearize_rescale(const ckks_ksk *key, const uint64_t *ct0, const uint64_t *ct1,
@@ -2168,6 +2185,7 @@ int ckks_multiply_relinearize_rescale(const ckks_ksk *key, const uint64_t *ct0, 
     if ((rc = ckks_tensor(key->plan, ct0, ct1, t3, n_cts, L, in_stride, 3 * poly))) return rc;
     if ((rc = key_switch_core(key, t3 + 2 * poly, 3 * poly, t3, t3 + poly, 3 * poly, t2, 2 * poly, n_cts)))
         return rc;
+    ctx->counts().relinearizations += n_cts;
     return ckks_rescale(key->plan, t2, out, n_cts, L, 2 * poly, out_stride);
 }
 

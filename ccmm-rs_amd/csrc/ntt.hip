@@ -218,6 +218,13 @@ int launch_ntt(ckks_context *ctx, const NttCall *calls, size_t count, bool forwa
             max_n = std::max(max_n, c.plan->degree);
         }
     }
+    {
+        ckks_execution_profile &pc = ctx->counts();
+        for (size_t i = 0; i < count; ++i) {
+            (forward ? pc.ntt_forwards : pc.ntt_inverses) += calls[i].n_polys;
+            pc.ntt_limb_transforms += calls[i].n_polys * calls[i].n_limbs;
+        }
+    }
     hipStream_t s = ctx->stream;
     auto base_desc = [&](const NttCall &c) {
         DevNttDesc d{};
@@ -443,6 +450,8 @@ int launch_ntt_mac(ckks_context *ctx, const ckks_rns_plan *plan, const u64 *digi
     void *ws;
     int rc = ctx->workspace(0, lts * N * sizeof(u64), &ws);
     if (rc) return rc;
+    ctx->counts().ntt_forwards += polys;
+    ctx->counts().ntt_limb_transforms += polys * limbs;
     // column pass of every digit limb into the inter-pass buffer
     NttLaunch l{};
     DevNttDesc &d = l.d[0];
@@ -513,6 +522,7 @@ extern "C" int ckks_autotune_ntt(const ckks_rns_plan *plan, size_t n_polys, size
     if (!plan || !best_variant) return fail(CKKS_ERR_INVALID, "null argument");
     ckks_context *ctx = plan->ctx;
     if (ctx->capturing) return fail(CKKS_ERR_INVALID, "autotuning is not allowed during graph capture");
+    ProfileGuard profile_guard(ctx);
     int rc = check_plan_limbs(plan, n_limbs, "NTT autotune");
     if (rc) return rc;
     if (n_polys == 0) return fail(CKKS_ERR_INVALID, "autotune needs at least one polynomial");

@@ -834,9 +834,16 @@ int ckks_decrypt_raw(const ckks_rns_plan *plan, const uint64_t *b, const uint64_
                     n_limbs, false);
 }
 
+// ExecutionProfile count of a successful call (the plan is valid then).
+static int counted(int rc, const ckks_rns_plan *plan, uint64_t ckks_execution_profile::*field, size_t n) {
+    if (rc == CKKS_OK && n) plan->ctx->counts().*field += n;
+    return rc;
+}
+
 int ckks_multiply_plain(const ckks_rns_plan *plan, const uint64_t *in, const uint64_t *pt, uint64_t *out,
                         size_t n_cts, size_t n_limbs, size_t in_stride, size_t pt_stride, size_t out_stride) {
-    return multiply_plain(plan, in, pt, out, n_cts, n_limbs, in_stride, pt_stride, out_stride);
+    return counted(multiply_plain(plan, in, pt, out, n_cts, n_limbs, in_stride, pt_stride, out_stride), plan,
+                   &ckks_execution_profile::ciphertext_plaintext_multiplies, n_cts);
 }
 
 int ckks_poly_mul(const ckks_rns_plan *plan, const uint64_t *x, const uint64_t *y, uint64_t *out,
@@ -849,11 +856,13 @@ int ckks_poly_mac(const ckks_rns_plan *plan, uint64_t *acc, const uint64_t *x, c
 }
 int ckks_poly_add(const ckks_rns_plan *plan, const uint64_t *x, const uint64_t *y, uint64_t *out,
                   size_t n_polys, size_t n_limbs, size_t poly_stride) {
-    return elementwise(EW_ADD, plan, x, y, out, n_polys, n_limbs, poly_stride);
+    return counted(elementwise(EW_ADD, plan, x, y, out, n_polys, n_limbs, poly_stride), plan,
+                   &ckks_execution_profile::additions, n_polys);
 }
 int ckks_poly_sub(const ckks_rns_plan *plan, const uint64_t *x, const uint64_t *y, uint64_t *out,
                   size_t n_polys, size_t n_limbs, size_t poly_stride) {
-    return elementwise(EW_SUB, plan, x, y, out, n_polys, n_limbs, poly_stride);
+    return counted(elementwise(EW_SUB, plan, x, y, out, n_polys, n_limbs, poly_stride), plan,
+                   &ckks_execution_profile::additions, n_polys);
 }
 int ckks_poly_neg(const ckks_rns_plan *plan, const uint64_t *x, uint64_t *out, size_t n_polys,
                   size_t n_limbs, size_t poly_stride) {
@@ -880,27 +889,32 @@ int ckks_poly_scalar_mul(const ckks_rns_plan *plan, const uint64_t *in, const ui
 
 int ckks_automorphism(const ckks_rns_plan *plan, const uint64_t *in, uint64_t *out, size_t n_polys,
                       size_t n_limbs, size_t poly_stride, uint64_t exponent) {
-    return permute(false, plan, in, out, n_polys, n_limbs, poly_stride, exponent);
+    return counted(permute(false, plan, in, out, n_polys, n_limbs, poly_stride, exponent), plan,
+                   &ckks_execution_profile::automorphisms, n_polys);
 }
 
 int ckks_ntt_galois_permute(const ckks_rns_plan *plan, const uint64_t *in, uint64_t *out,
                             size_t n_polys, size_t n_limbs, size_t poly_stride, uint64_t exponent) {
-    return permute(true, plan, in, out, n_polys, n_limbs, poly_stride, exponent);
+    return counted(permute(true, plan, in, out, n_polys, n_limbs, poly_stride, exponent), plan,
+                   &ckks_execution_profile::automorphisms, n_polys);
 }
 
 int ckks_rescale(const ckks_rns_plan *plan, const uint64_t *in, uint64_t *out, size_t n_cts,
                  size_t n_limbs, size_t in_stride, size_t out_stride) {
-    return rescale_like(true, plan, in, out, n_cts, n_limbs, in_stride, out_stride);
+    return counted(rescale_like(true, plan, in, out, n_cts, n_limbs, in_stride, out_stride), plan,
+                   &ckks_execution_profile::rescales, n_cts);
 }
 
 int ckks_mod_switch(const ckks_rns_plan *plan, const uint64_t *in, uint64_t *out, size_t n_cts,
                     size_t n_limbs, size_t in_stride, size_t out_stride) {
-    return rescale_like(false, plan, in, out, n_cts, n_limbs, in_stride, out_stride);
+    return counted(rescale_like(false, plan, in, out, n_cts, n_limbs, in_stride, out_stride), plan,
+                   &ckks_execution_profile::modulus_switches, n_cts);
 }
 
 int ckks_tensor(const ckks_rns_plan *plan, const uint64_t *ct0, const uint64_t *ct1, uint64_t *out,
                 size_t n_cts, size_t n_limbs, size_t in_stride, size_t out_stride) {
-    return tensor(plan, ct0, ct1, out, n_cts, n_limbs, in_stride, out_stride);
+    return counted(tensor(plan, ct0, ct1, out, n_cts, n_limbs, in_stride, out_stride), plan,
+                   &ckks_execution_profile::ciphertext_ciphertext_multiplies, n_cts);
 }
 
 }  // extern "C"

@@ -43,6 +43,7 @@ __global__ __launch_bounds__(256) void k_copy_list(CopyList l) {
 }
 
 int ksk_info(const ckks_ksk *k, size_t *q_limbs, size_t *degree);
+size_t ksk_bytes(const ckks_ksk *k);
 
 class HipExec {
   public:
@@ -170,9 +171,10 @@ class HipExec {
                 break;
             }
             case serve::OP_ADD:
+            case serve::OP_SUB:
                 if ((rc = batch(1, n * per, &in2))) return rc;
                 if ((rc = gather(g, in2, per, [](const serve::Op &o) { return o.b; }))) return rc;
-                rc = ckks_poly_add(plan, in, in2, out, 2 * n, L, L * N);
+                rc = (g.code == serve::OP_ADD ? ckks_poly_add : ckks_poly_sub)(plan, in, in2, out, 2 * n, L, L * N);
                 break;
             case serve::OP_RESCALE:
                 rc = ckks_rescale(plan, in, out, n, L, per, per_out);
@@ -308,6 +310,7 @@ int ckks_server_submit(ckks_server *s, const ckks_job *job, uint64_t *ticket) {
         if (it != s->keys.end()) return &it->second;
         serve::KeyInfo ki{};
         if (ksk_info((const ckks_ksk *)k, &ki.q_limbs, &ki.degree)) return nullptr;
+        ki.bytes = ksk_bytes((const ckks_ksk *)k);
         return &(s->keys[k] = ki);
     };
     const std::string why = serve::validate(j, lookup);
@@ -315,6 +318,7 @@ int ckks_server_submit(ckks_server *s, const ckks_job *job, uint64_t *ticket) {
         ++s->sched->stats().rejected;
         return fail(CKKS_ERR_INVALID, "serving job rejected: " + why);
     }
+    j.resident = serve::resident_set(j, lookup);
     if (!s->sched->fits_ever(j)) {
         ++s->sched->stats().rejected;
         return fail(CKKS_ERR_OOM, "serving job footprint exceeds the HBM budget");
@@ -336,8 +340,21 @@ int ckks_server_poll(ckks_server *s, uint64_t ticket, int *state, double *latenc
     s->sched->retire();
     const serve::Request *r = s->sched->find(ticket);
     if (!r) return fail(CKKS_ERR_INVALID, "unknown ticket");
-    *state = r->state == 3 ? 2 : (r->state == 0 ? 0 : 1);
-    if (latency_ms) *latency_ms = r->state == 3 ? r->t_done - r->t_submit : -1.0;
+    switch (r->state) {
+        case serve::RS_WAITING: *state = 0; break;
+        case serve::RS_DONE: *state = 2; break;
+        case serve::RS_CANCELLED: *state = 3; break;
+        case serve::RS_FAILED: *state = 4; break;
+        default: *state = 1;
+    }
+    if (latency_ms) *latency_ms = r->state == serve::RS_DONE ? r->t_done - r->t_submit : -1.0;
+    return CKKS_OK;
+}
+
+int ckks_server_cancel(ckks_server *s, uint64_t ticket) {
+    if (!s) return fail(CKKS_ERR_INVALID, "null server");
+    if (!s->sched->find(ticket)) return fail(CKKS_ERR_INVALID, "unknown ticket");
+    if (!s->sched->cancel(ticket)) return fail(CKKS_ERR_INVALID, "request already finished");
     return CKKS_OK;
 }
 
@@ -374,6 +391,9 @@ int ckks_server_stats(ckks_server *s, ckks_server_statistics *out) {
     }
     out->mean_ms = st.latencies_ms.empty() ? 0 : sum / (double)st.latencies_ms.size();
     out->max_ms = mx;
+    out->cancelled = st.cancelled;
+    out->failed = st.failed;
+    out->resident_bytes = s->sched->resident_bytes();
     return CKKS_OK;
 }
 

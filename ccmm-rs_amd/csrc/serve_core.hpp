@@ -33,6 +33,7 @@ enum OpCode : uint32_t {
     OP_MUL_RELIN_RESCALE = 5,  // dst = rescale(relin(a x b)): one limb fewer
     OP_COPY = 6,               // dst = a
     OP_MOD_SWITCH = 7,         // dst = a without its last limb (scale kept)
+    OP_SUB = 8,                // dst = a - b (same level)
 };
 
 struct Op {
@@ -46,6 +47,16 @@ struct Op {
 struct KeyInfo {
     size_t q_limbs;  // limbs of the ciphertexts the key applies to
     size_t degree;
+    size_t bytes = 0;  // device bytes the key holds resident
+};
+
+// Device memory a job reads without owning it: a key or a plaintext.  It is
+// charged against the budget once while any admitted job reads it
+// (evaluation keys stay resident across requests; the accounting model of
+// evaluation_key_storage, ckks/key_accounting.rs:276-344, per resident key).
+struct Resource {
+    const void *ptr;
+    size_t bytes;
 };
 
 struct Job {
@@ -54,6 +65,7 @@ struct Job {
     const uint64_t *input;
     uint64_t *output;
     std::vector<Op> ops;
+    std::vector<Resource> resident;  // distinct keys and plaintexts the ops read
 };
 
 struct Request {
@@ -64,9 +76,14 @@ struct Request {
     void *regs = nullptr;           // executor-owned register block
     size_t bytes = 0;               // register footprint
     double t_submit = 0, t_admit = 0, t_done = 0;
-    int state = 0;                  // 0 waiting, 1 active, 2 finishing (launched), 3 done
+    // 0 waiting, 1 active, 2 finishing (launched), 3 done, 4 cancelled,
+    // 5 failed (its registers could not be allocated with nothing in flight)
+    int state = 0;
+    bool cancel = false;            // active: drop at the next step
     void *done_event = nullptr;     // executor-owned completion marker
 };
+
+enum RequestState { RS_WAITING = 0, RS_ACTIVE = 1, RS_FINISHING = 2, RS_DONE = 3, RS_CANCELLED = 4, RS_FAILED = 5 };
 
 inline size_t footprint(const Job &j) { return j.n_regs * 2 * j.n_limbs * j.degree * sizeof(uint64_t); }
 
@@ -100,6 +117,7 @@ std::string validate(const Job &j, KeyLookup &&key_info) {
                 limbs[o.dst] = limbs[o.a];
                 break;
             case OP_ADD:
+            case OP_SUB:
                 if (o.b >= j.n_regs || limbs[o.b] != limbs[o.a]) return "addition needs two ciphertexts at one level" + at;
                 limbs[o.dst] = limbs[o.a];
                 break;
@@ -126,6 +144,30 @@ std::string validate(const Job &j, KeyLookup &&key_info) {
     return "";
 }
 
+// The keys and plaintexts a validated job reads, each once, with their
+// device bytes (a plaintext holds its op's limb count of residues).
+template <class KeyLookup>
+std::vector<Resource> resident_set(const Job &j, KeyLookup &&key_info) {
+    std::vector<Resource> out;
+    auto add = [&](const void *p, size_t bytes) {
+        for (const Resource &r : out)
+            if (r.ptr == p) return;
+        out.push_back(Resource{p, bytes});
+    };
+    std::vector<size_t> limbs(j.n_regs, 0);
+    for (size_t i = 0; i < j.n_inputs; ++i) limbs[i] = j.n_limbs;
+    for (const Op &o : j.ops) {
+        const size_t la = limbs[o.a];
+        if ((o.code == OP_ROTATE || o.code == OP_MUL_RELIN_RESCALE) && o.key) {
+            const KeyInfo *ki = key_info(o.key);
+            add(o.key, ki ? ki->bytes : 0);
+        }
+        if (o.code == OP_MUL_PLAIN && o.plaintext) add(o.plaintext, la * j.degree * sizeof(uint64_t));
+        limbs[o.dst] = (o.code == OP_RESCALE || o.code == OP_MOD_SWITCH || o.code == OP_MUL_RELIN_RESCALE) ? la - 1 : la;
+    }
+    return out;
+}
+
 // Ops of several requests that one batched launch serves.
 struct Group {
     uint32_t code;
@@ -137,7 +179,7 @@ struct Group {
 };
 
 struct Stats {
-    size_t submitted = 0, completed = 0, rejected = 0;
+    size_t submitted = 0, completed = 0, rejected = 0, cancelled = 0, failed = 0;
     size_t peak_active = 0, peak_bytes = 0, launches = 0, steps = 0;
     std::vector<double> latencies_ms;
 };
@@ -173,23 +215,65 @@ class Scheduler {
         return r->ticket;
     }
 
-    bool fits_ever(const Job &j) const { return footprint(j) <= budget_; }
+    bool fits_ever(const Job &j) const {
+        size_t bytes = footprint(j);
+        for (const Resource &res : j.resident) bytes += res.bytes;
+        return bytes <= budget_;
+    }
+
+    // Cancels a waiting or active request: a waiting one leaves the queue, an
+    // active one issues no further op and gives its registers back at the
+    // next step (its launched work is ordered before any reuse on the one
+    // stream).  Returns false once the request has finished.
+    bool cancel(uint64_t ticket) {
+        auto it = all_.find(ticket);
+        if (it == all_.end()) return false;
+        Request *r = it->second;
+        if (r->state == RS_WAITING) {
+            waiting_.erase(std::find(waiting_.begin(), waiting_.end(), r));
+            r->state = RS_CANCELLED;
+            ++stats_.cancelled;
+            return true;
+        }
+        if (r->state == RS_ACTIVE) {
+            r->cancel = true;
+            return true;
+        }
+        return false;
+    }
 
     // One scheduler iteration; returns an executor error code or 0.
     int step(size_t *launched = nullptr) {
         ++stats_.steps;
         retire();
+        for (Request *r : active_)
+            if (r->state == RS_ACTIVE && r->cancel) {
+                drop(*r);
+                r->state = RS_CANCELLED;
+                ++stats_.cancelled;
+            }
+        active_.erase(std::remove_if(active_.begin(), active_.end(), [](Request *r) { return r->state != RS_ACTIVE; }),
+                      active_.end());
         // admission in arrival order; a request that does not fit blocks the
         // ones behind it (no overtaking, so large jobs cannot starve)
         while (!waiting_.empty() && active_.size() < max_active_) {
             Request *r = waiting_.front();
-            if (used_ + r->bytes > budget_) break;
-            if (!ex_.alloc(*r)) break;
-            used_ += r->bytes;
+            const size_t cost = r->bytes + unaccounted(*r);
+            if (used_ + cost > budget_) break;
+            if (!ex_.alloc(*r)) {
+                // memory frees up only when active requests finish: with none
+                // in flight the request cannot ever be admitted
+                if (!active_.empty()) break;
+                waiting_.pop_front();
+                r->state = RS_FAILED;
+                ++stats_.failed;
+                continue;
+            }
+            charge(*r);
             stats_.peak_bytes = std::max(stats_.peak_bytes, used_);
             r->limbs.assign(r->job.n_regs, 0);
             for (size_t i = 0; i < r->job.n_inputs; ++i) r->limbs[i] = r->job.n_limbs;
-            r->state = 1;
+            r->state = RS_ACTIVE;
             r->t_admit = ex_.now_ms();
             waiting_.pop_front();
             active_.push_back(r);
@@ -244,8 +328,7 @@ class Scheduler {
             r->state = 2;
             // registers go back now: later launches are ordered after this
             // request's last one on the same stream
-            ex_.release(*r);
-            used_ -= r->bytes;
+            drop(*r);
         }
         active_.erase(std::remove_if(active_.begin(), active_.end(), [](Request *r) { return r->state != 1; }),
                       active_.end());
@@ -283,9 +366,47 @@ class Scheduler {
     size_t waiting() const { return waiting_.size(); }
     size_t active() const { return active_.size(); }
     size_t in_use_bytes() const { return used_; }
+    size_t resident_bytes() const {
+        size_t b = 0;
+        for (const auto &kv : resident_) b += kv.second.first;
+        return b;
+    }
     Stats &stats() { return stats_; }
 
   private:
+    // bytes of r's keys and plaintexts that no admitted request holds yet
+    size_t unaccounted(const Request &r) const {
+        size_t b = 0;
+        for (const Resource &res : r.job.resident)
+            if (!resident_.count(res.ptr)) b += res.bytes;
+        return b;
+    }
+
+    void charge(Request &r) {
+        used_ += r.bytes;
+        for (const Resource &res : r.job.resident) {
+            auto &e = resident_[res.ptr];
+            if (e.second++ == 0) {
+                e.first = res.bytes;
+                used_ += res.bytes;
+            }
+        }
+    }
+
+    // registers back to the executor, resident bytes released with their
+    // last reader
+    void drop(Request &r) {
+        ex_.release(r);
+        used_ -= r.bytes;
+        for (const Resource &res : r.job.resident) {
+            auto it = resident_.find(res.ptr);
+            if (it != resident_.end() && --it->second.second == 0) {
+                used_ -= it->second.first;
+                resident_.erase(it);
+            }
+        }
+    }
+
     Exec &ex_;
     size_t budget_, max_active_, used_ = 0;
     uint64_t next_ticket_ = 0;
@@ -293,6 +414,7 @@ class Scheduler {
     std::vector<Request *> active_;
     std::vector<Request *> finishing_;
     std::map<uint64_t, Request *> all_;
+    std::map<const void *, std::pair<size_t, size_t>> resident_;  // ptr -> (bytes, readers)
     Stats stats_;
 };
 

@@ -106,6 +106,29 @@ class Builder:
                 prog.add(y, y, tmp)
         prog.rescale(y, y)
 
+    def transform_signed(self, prog: Program, x, y, tmp, sdiags: dict, limbs: int):
+        """y = M x for an operator given by signed-offset diagonals over the
+        whole slot vector, y[s] = sum_o D_o[s] x[s + o], where D_o[s] is zero
+        unless s + o stays inside s's own row / block: no cyclic rotation
+        couples independent rows (the masks of the reference's packed FFT2
+        stages, eblas/fft.rs:2700-2800).  One rescale at the end."""
+        first = True
+        for o, vals in sorted(sdiags.items()):
+            vals = np.asarray(vals, dtype=np.complex128)
+            assert vals.shape == (self.S,)
+            if o == 0:
+                src = x
+            else:
+                self._rot(prog, tmp, x, o, limbs)
+                src = tmp
+            if first:
+                prog.mul_plain(y, src, self._pt(vals, limbs), values=vals)
+                first = False
+            else:
+                prog.mul_plain(tmp, src, self._pt(vals, limbs), values=vals)
+                prog.add(y, y, tmp)
+        prog.rescale(y, y)
+
     def transform_bsgs(self, prog: Program, x, y, regs: list, diags: dict, period: int, limbs: int,
                        unit: int = 1, g: Optional[int] = None):
         """y = M x by baby-step / giant-step over diagonal offsets that are
@@ -209,6 +232,121 @@ def fft_program(b: Builder, d: int, limbs: int) -> Program:
         b.transform_direct(prog, 0, 1, 2, st, d, limbs - t)
         prog.copy(0, 1)
     return prog
+
+
+def dif_stage_matrix(d: int, m: int, sign: float = -1.0) -> np.ndarray:
+    """Radix-2 decimation-in-frequency stage of half-span m on length d
+    (PackedFft2DifStageDiagonals, eblas/fft.rs:2728-2800): in each block of
+    2m, out[p] = in[p] + in[p+m], out[p+m] = (in[p] - in[p+m]) w^p,
+    w = exp(sign 2 pi i / 2m)."""
+    M = np.zeros((d, d), dtype=np.complex128)
+    for j in range(d):
+        p = j % (2 * m)
+        if p < m:
+            M[j, j] = 1.0
+            M[j, j + m] = 1.0
+        else:
+            w = np.exp(sign * 2j * np.pi * (p - m) / (2 * m))
+            M[j, j - m] = w
+            M[j, j] = -w
+    return M
+
+
+def dit_stage_matrix(d: int, m: int) -> np.ndarray:
+    """Radix-2 decimation-in-time stage of span m on length d (dense form of
+    dft_stage_diagonals, forward sign)."""
+    M = np.zeros((d, d), dtype=np.complex128)
+    for j in range(d):
+        p = j % (2 * m)
+        if p < m:
+            w = np.exp(-2j * np.pi * p / (2 * m))
+            M[j, j] = 1.0
+            M[j, j + m] = w
+        else:
+            w = np.exp(-2j * np.pi * (p - m) / (2 * m))
+            M[j, j - m] = 1.0
+            M[j, j] = -w
+    return M
+
+
+def radix4_matrices(stages: list) -> list:
+    """Pairs of consecutive radix-2 stage matrices multiplied into radix-4
+    stages (a final radix-2 stage when the count is odd)."""
+    out = []
+    for i in range(0, len(stages), 2):
+        out.append(stages[i + 1] @ stages[i] if i + 1 < len(stages) else stages[i])
+    return out
+
+
+def signed_diagonals(M: np.ndarray) -> dict:
+    """{o: D_o} with D_o[j] = M[j][j + o] (zero where j + o leaves [0, d))."""
+    d = M.shape[0]
+    out = {}
+    for o in range(-(d - 1), d):
+        j = np.arange(max(0, -o), min(d, d - o))
+        v = M[j, j + o]
+        if np.any(v != 0):
+            full = np.zeros(d, dtype=np.complex128)
+            full[j] = v
+            out[o] = full
+    return out
+
+
+def fft2_programs(b: Builder, rows: int, cols: int, limbs: int):
+    """Encrypted 2-D DFT of a rows x cols real or complex image held as two
+    ciphertexts of rows/2 x cols slots each (top and bottom halves,
+    row-major, columns in bit-reversed order; rows/2 * cols = the slot count).
+    The column DFT starts with the decimation-in-frequency butterfly across
+    the halves (u = top + bottom, v = (top - bottom) w^r, w = exp(-2 pi i /
+    rows)), which needs no rotation; the twiddle w^r of v is folded into the
+    first row stage's diagonals (it is constant along a row and the row
+    stages never mix rows).  Then each half runs the row DFT (radix-4
+    decimation-in-time stages, natural-order output) and the column DFT of
+    length rows/2 (radix-4 decimation-in-frequency stages, bit-reversed
+    output), one level per stage.  Returns (even, odd) programs with inputs
+    (top, bottom): their outputs hold the spectrum rows 2 bitrev(j) and
+    2 bitrev(j) + 1 at slot row j."""
+    half = rows // 2
+    S = b.S
+    assert half * cols == S and rows & (rows - 1) == 0 and cols & (cols - 1) == 0
+    row_stages = radix4_matrices([dit_stage_matrix(cols, 1 << t) for t in range(cols.bit_length() - 1)])
+    col_stages = radix4_matrices([dif_stage_matrix(half, half >> (t + 1)) for t in range(half.bit_length() - 1)])
+    slot = np.arange(S)
+    r_of, c_of = slot // cols, slot % cols
+
+    def row_sdiags(M, twist=None):
+        out = {}
+        for o, v in signed_diagonals(M).items():
+            full = v[c_of] * (twist if twist is not None else 1.0)
+            if np.any(full != 0):
+                out[o] = full
+        return out
+
+    def col_sdiags(M):
+        return {o * cols: v[r_of] for o, v in signed_diagonals(M).items()}
+
+    twist = np.exp(-2j * np.pi * r_of / rows)
+    progs = []
+    for odd in (False, True):
+        prog = Program(n_regs=5, out_reg=2, n_inputs=2)
+        (prog.sub if odd else prog.add)(2, 0, 1)
+        lv = limbs
+        for t, M in enumerate(row_stages):
+            b.transform_signed(prog, 2, 3, 4, row_sdiags(M, twist if (odd and t == 0) else None), lv)
+            prog.copy(2, 3)
+            lv -= 1
+        for M in col_stages:
+            b.transform_signed(prog, 2, 3, 4, col_sdiags(M), lv)
+            prog.copy(2, 3)
+            lv -= 1
+        progs.append(prog)
+    return progs[0], progs[1]
+
+
+def fft2_levels(rows: int, cols: int) -> int:
+    """Levels fft2_programs uses: one per radix-4 (or final radix-2) stage."""
+    lr, lc = cols.bit_length() - 1, (rows // 2).bit_length() - 1
+    return (lr + 1) // 2 + (lc + 1) // 2
 
 
 def ccmm_permutations(d: int):

@@ -15,7 +15,7 @@ from typing import Optional
 
 from . import PreparedRnsNttPlan, _check, _p, _ptr, _sz, lib
 
-OP_ROTATE, OP_MUL_PLAIN, OP_ADD, OP_RESCALE, OP_MUL_RELIN_RESCALE, OP_COPY, OP_MOD_SWITCH = 1, 2, 3, 4, 5, 6, 7
+OP_ROTATE, OP_MUL_PLAIN, OP_ADD, OP_RESCALE, OP_MUL_RELIN_RESCALE, OP_COPY, OP_MOD_SWITCH, OP_SUB = 1, 2, 3, 4, 5, 6, 7, 8
 
 
 class _Op(C.Structure):
@@ -31,7 +31,8 @@ class _Job(C.Structure):
 class _Stats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("submitted", "completed", "rejected", "waiting", "active",
                                          "peak_active", "launches", "steps", "peak_bytes")] + \
-               [(n, C.c_double) for n in ("p50_ms", "p99_ms", "mean_ms", "max_ms")]
+               [(n, C.c_double) for n in ("p50_ms", "p99_ms", "mean_ms", "max_ms")] + \
+               [(n, C.c_uint64) for n in ("cancelled", "failed", "resident_bytes")]
 
 
 def _fn(name):
@@ -61,6 +62,10 @@ class Program:
 
     def add(self, dst, a, b):
         self.ops.append((OP_ADD, dst, a, b, None, 0, None))
+        self.meta.append(None)
+
+    def sub(self, dst, a, b):
+        self.ops.append((OP_SUB, dst, a, b, None, 0, None))
         self.meta.append(None)
 
     def rescale(self, dst, a):
@@ -95,6 +100,8 @@ class Program:
                 regs[dst] = regs[a] * meta
             elif code == OP_ADD:
                 regs[dst] = regs[a] + regs[b]
+            elif code == OP_SUB:
+                regs[dst] = regs[a] - regs[b]
             elif code == OP_MUL_RELIN_RESCALE:
                 regs[dst] = regs[a] * regs[b]
             else:
@@ -118,7 +125,7 @@ class Program:
         return [(la, final) for la, _ in out] or [(0, final)]
 
     def counts(self) -> dict:
-        names = {OP_ROTATE: "rotate", OP_MUL_PLAIN: "mul_plain", OP_ADD: "add", OP_RESCALE: "rescale",
+        names = {OP_ROTATE: "rotate", OP_MUL_PLAIN: "mul_plain", OP_ADD: "add", OP_SUB: "sub", OP_RESCALE: "rescale",
                  OP_MUL_RELIN_RESCALE: "mul_relin_rescale", OP_COPY: "copy", OP_MOD_SWITCH: "mod_switch"}
         out: dict = {}
         for op in self.ops:
@@ -127,7 +134,7 @@ class Program:
 
 
 class Server:
-    WAITING, ADMITTED, DONE = 0, 1, 2
+    WAITING, ADMITTED, DONE, CANCELLED, FAILED = 0, 1, 2, 3, 4
 
     def __init__(self, ctx, hbm_budget_bytes: int, max_active: int = 0):
         self.ctx = ctx
@@ -159,9 +166,14 @@ class Server:
     def poll(self, ticket: int):
         st, lat = C.c_int(), C.c_double()
         _check(_fn("ckks_server_poll")(self._h, int(ticket), C.byref(st), C.byref(lat)))
-        if st.value == self.DONE:
+        if st.value in (self.DONE, self.CANCELLED, self.FAILED):
             self._jobs.pop(ticket, None)
         return st.value, (lat.value if st.value == self.DONE else None)
+
+    def cancel(self, ticket: int):
+        """Drop a waiting or admitted request (ckks_server_cancel)."""
+        _check(_fn("ckks_server_cancel")(self._h, int(ticket)))
+        self._jobs.pop(ticket, None) if self.poll(ticket)[0] == self.CANCELLED else None
 
     def drain(self):
         _check(_fn("ckks_server_drain")(self._h))

@@ -465,8 +465,11 @@ int ckks_autotune_key_switch(ckks_ksk *key, size_t n_cts, int *best_engine, int 
  * 0.. n_inputs-1 start as copies of the inputs; `output` receives register
  * `out_reg` at the end), validated at submission (register use, levels, key bases).  Each
  * ckks_server_step retires finished requests, admits waiting ones in
- * arrival order while their register footprint (n_regs * 2 * n_limbs * N *
- * 8 bytes) fits the HBM budget and the active cap, then runs the next op of
+ * arrival order while their footprint fits the HBM budget and the active
+ * cap -- the registers (n_regs * 2 * n_limbs * N * 8 bytes) plus every key and
+ * plaintext the program reads that no admitted request holds yet (a key or
+ * plaintext is charged once while any admitted request reads it, and
+ * released with its last reader) -- then runs the next op of
  * every active request, batching ops that share (op, key, exponent, plan,
  * level) into one launch.  Evaluation keys and plans stay resident and are
  * owned by the caller; they must outlive the jobs that name them.  All work
@@ -478,7 +481,8 @@ enum {
     CKKS_OP_RESCALE = 4,            /* dst = rescale(a) */
     CKKS_OP_MUL_RELIN_RESCALE = 5,  /* dst = rescale(relin(a x b)), relinearization key */
     CKKS_OP_COPY = 6,               /* dst = a */
-    CKKS_OP_MOD_SWITCH = 7          /* dst = a without its last limb */
+    CKKS_OP_MOD_SWITCH = 7,         /* dst = a without its last limb */
+    CKKS_OP_SUB = 8                 /* dst = a - b */
 };
 typedef struct ckks_op {
     uint32_t code, dst, a, b;
@@ -499,6 +503,8 @@ typedef struct ckks_server_stats {
     uint64_t submitted, completed, rejected, waiting, active, peak_active, launches, steps;
     uint64_t peak_bytes;
     double p50_ms, p99_ms, mean_ms, max_ms; /* submit -> completion observed */
+    uint64_t cancelled, failed;             /* see ckks_server_cancel / poll state 4 */
+    uint64_t resident_bytes;                /* keys and plaintexts charged now */
 } ckks_server_statistics;
 typedef struct ckks_server ckks_server;
 int ckks_server_create(ckks_context *ctx, size_t hbm_budget_bytes, size_t max_active, ckks_server **out);
@@ -506,11 +512,47 @@ int ckks_server_create(ckks_context *ctx, size_t hbm_budget_bytes, size_t max_ac
  * footprint exceeds the whole budget; the op list is copied. */
 int ckks_server_submit(ckks_server *server, const ckks_job *job, uint64_t *ticket);
 int ckks_server_step(ckks_server *server, size_t *launched);
-/* state: 0 waiting, 1 admitted, 2 done (latency_ms set, else -1). */
+/* state: 0 waiting, 1 admitted, 2 done (latency_ms set, else -1),
+ * 3 cancelled, 4 failed: its registers could not be allocated while no other
+ * request was in flight (device memory exhausted outside the budget). */
 int ckks_server_poll(ckks_server *server, uint64_t ticket, int *state, double *latency_ms);
+/* Cancels a waiting request (it leaves the queue) or an admitted one (no
+ * further op is issued; its registers and resident charges are released at
+ * the next step; the output is not written).  CKKS_ERR_INVALID for an unknown
+ * ticket or a request that already finished. */
+int ckks_server_cancel(ckks_server *server, uint64_t ticket);
 int ckks_server_drain(ckks_server *server);
 int ckks_server_stats(ckks_server *server, ckks_server_statistics *out);
 int ckks_server_destroy(ckks_server *server);
+
+/* ----------------------------------------------------- execution profile */
+/* ExecutionProfile (execution/mod.rs:162-230) observed on the device: the
+ * context counts the operations its entry points run, per ciphertext (per
+ * polynomial for NTTs, additions and Galois permutations), in the
+ * reference's ExecutionEventKind vocabulary:
+ *   ciphertext_plaintext_multiplies  ckks_multiply_plain (per ciphertext)
+ *   ciphertext_ciphertext_multiplies ckks_tensor (also inside
+ *                                    ckks_multiply_relinearize_rescale)
+ *   additions                        ckks_poly_add / ckks_poly_sub (per polynomial)
+ *   relinearizations                 ckks_relinearize, ckks_multiply_relinearize_rescale
+ *   rescales / modulus_switches      ckks_rescale / ckks_mod_switch
+ *   automorphisms                    ckks_automorphism, ckks_ntt_galois_permute
+ *                                    (per polynomial); key-switched conjugations
+ *   rotations                        key-switched Galois automorphisms other
+ *                                    than conjugation (ckks_apply_galois, hoisted)
+ *   ntt_forwards / ntt_inverses      RNS polynomial transforms, including those
+ *                                    inside key switching, encryption and keygen
+ * and two counts the reference has no field for: key_switches (every key
+ * switch, per ciphertext) and ntt_limb_transforms (single-limb transforms
+ * behind the NTT counts).  A captured graph adds the counts of the calls it
+ * recorded each time it is launched (the capture itself counts nothing). */
+typedef struct ckks_execution_profile {
+    uint64_t ciphertext_plaintext_multiplies, ciphertext_ciphertext_multiplies, additions, relinearizations;
+    uint64_t rescales, modulus_switches, automorphisms, rotations, ntt_forwards, ntt_inverses;
+    uint64_t key_switches, ntt_limb_transforms;
+} ckks_execution_profile;
+int ckks_profile_get(const ckks_context *ctx, ckks_execution_profile *out);
+int ckks_profile_reset(ckks_context *ctx);
 
 /* ------------------------------------------------------------ graphs */
 /* Rotation schedules (BSGS of CCMM / FFT) captured once as a hipGraph and

@@ -12,12 +12,15 @@ using namespace ck::serve;
 
 struct FakeExec {
     int lag = 1, fail_code = 0;
+    int allocs = 0, fail_allocs_from = -1, fail_allocs_to = -1;  // allocation indices that fail
     double clock = 0;
     long launches = 0, group_members = 0;
     std::vector<uint64_t> done_order;
     std::vector<std::pair<Request *, int>> pending;  // request, steps left
     char pool[1 << 16];
     bool alloc(Request &r) {
+        const int i = allocs++;
+        if (i >= fail_allocs_from && i < fail_allocs_to) return false;
         r.regs = pool;
         return true;
     }
@@ -48,7 +51,7 @@ struct FakeExec {
     double now_ms() const { return clock; }
 };
 
-static KeyInfo kKey{3, 8};
+static KeyInfo kKey{3, 8, 1000};
 static const KeyInfo *lookup(const void *k) { return k == (const void *)&kKey ? &kKey : nullptr; }
 static uint64_t dummy_in[1], dummy_out[1];
 
@@ -111,6 +114,96 @@ int serve_emu_run(int n_jobs, const int *kinds, const int *arrive, int budget_jo
     out[6] = rc;
     for (size_t i = 0; i < ex.done_order.size(); ++i) order[i] = ex.done_order[i];
     return 0;
+}
+
+static int run_until_idle(Scheduler<FakeExec> &s, FakeExec &ex, int max_steps = 1000) {
+    for (int i = 0; i < max_steps && !s.idle(); ++i) {
+        int rc = s.step();
+        if (rc) return rc;
+        ex.tick();
+    }
+    return s.idle() ? 0 : -2;
+}
+
+static Job resident_job(int kind) {
+    Job j = make_job(kind, 2);
+    j.resident = resident_set(j, lookup);
+    return j;
+}
+
+// Scenarios of cancellation, allocation failure and resident accounting;
+// out[] receives the scenario's observations (see each case).
+int serve_emu_scenario(int which, long *out) {
+    FakeExec ex;
+    const size_t foot = footprint(make_job(0, 2));
+    const size_t shared = 1000 + 3 * 8 * 8;  // the key and the plaintext of a kind-0 job
+    switch (which) {
+        case 0: {  // cancel a waiting and an active request
+            Scheduler<FakeExec> s(ex, foot + shared, 0);  // room for one job, its key and plaintext
+            uint64_t t[3];
+            for (int i = 0; i < 3; ++i) t[i] = s.submit(resident_job(0));
+            s.step();  // admits t0
+            out[0] = s.find(t[0])->state;
+            out[1] = s.cancel(t[2]);  // waiting -> cancelled at once
+            out[2] = s.find(t[2])->state;
+            out[3] = s.cancel(t[0]);  // active -> dropped at the next step
+            s.step();                  // drops t0, admits t1
+            out[4] = s.find(t[0])->state;
+            out[5] = s.find(t[1])->state;
+            out[6] = run_until_idle(s, ex);
+            out[7] = s.find(t[1])->state;
+            out[8] = (long)s.stats().completed;
+            out[9] = (long)s.stats().cancelled;
+            out[10] = (long)s.in_use_bytes();
+            out[11] = s.cancel(t[1]);  // finished: refused
+            return 0;
+        }
+        case 1: {  // allocation fails with nothing in flight: the request fails
+            ex.fail_allocs_from = 0;
+            ex.fail_allocs_to = 1;
+            Scheduler<FakeExec> s(ex, 4 * foot + 1000, 0);
+            uint64_t a = s.submit(resident_job(0)), b = s.submit(resident_job(1));
+            out[0] = run_until_idle(s, ex);
+            out[1] = s.find(a)->state;
+            out[2] = s.find(b)->state;
+            out[3] = (long)s.stats().failed;
+            out[4] = (long)s.stats().completed;
+            out[5] = (long)s.in_use_bytes();
+            return 0;
+        }
+        case 2: {  // allocation fails while another request runs: it waits
+            ex.fail_allocs_from = 1;
+            ex.fail_allocs_to = 3;
+            Scheduler<FakeExec> s(ex, 4 * foot + 1000, 0);
+            uint64_t a = s.submit(resident_job(0)), b = s.submit(resident_job(0));
+            s.step();
+            out[0] = s.find(a)->state;
+            out[1] = s.find(b)->state;  // still waiting
+            out[2] = run_until_idle(s, ex);
+            out[3] = s.find(b)->state;
+            out[4] = (long)s.stats().failed;
+            out[5] = (long)s.stats().completed;
+            return 0;
+        }
+        case 3: {  // a shared key is charged once; released with its last reader
+            Scheduler<FakeExec> s(ex, 2 * foot + shared, 0);  // two jobs fit only if key and plaintext count once
+            s.submit(resident_job(0));
+            s.submit(resident_job(0));
+            s.step();
+            out[0] = (long)s.active();
+            out[1] = (long)s.resident_bytes();  // key 1000 + one shared plaintext (3 limbs x 8 x 8 B)
+            out[2] = (long)s.in_use_bytes();
+            out[3] = run_until_idle(s, ex);
+            out[4] = (long)s.resident_bytes();
+            out[5] = (long)s.in_use_bytes();
+            out[6] = (long)s.fits_ever(resident_job(0));
+            Scheduler<FakeExec> tight(ex, foot + shared - 1, 0);  // registers fit, key and plaintext do not
+            out[7] = (long)tight.fits_ever(resident_job(0));
+            out[8] = (long)foot;
+            return 0;
+        }
+        default: return -1;
+    }
 }
 
 // Validation message of a malformed program, case 0..6.

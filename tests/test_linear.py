@@ -70,3 +70,40 @@ def test_diagonal_algebra():
     pd = LIN.permutation_diagonals(cols)
     md = LIN.matrix_diagonals(P)
     assert sorted(pd) == sorted(md) and all(np.allclose(pd[i], md[i]) for i in md)
+
+
+@pytest.mark.parametrize("rows,cols", [(16, 16), (8, 32), (32, 8), (256, 256)])
+def test_fft2_schedule(rows, cols):
+    """2-D DFT of a rows x cols image held as two half-image ciphertexts:
+    the even / odd programs' FP64 execution equals numpy's fft2 at the
+    spectrum rows 2 bitrev(j) and 2 bitrev(j) + 1, with every rotation
+    confined to its row / column by the signed-offset diagonals."""
+    slots = rows // 2 * cols
+    rng = np.random.default_rng(rows * cols)
+    X = rng.uniform(-1, 1, (rows, cols))
+    cp = LIN.bitrev_permutation(cols)
+    top, bot = X[: rows // 2][:, cp].ravel(), X[rows // 2:][:, cp].ravel()
+    pe, po = LIN.fft2_programs(_builder(slots), rows, cols, limbs=20)
+    Y = np.fft.fft2(X)
+    rp = LIN.bitrev_permutation(rows // 2)
+    assert np.allclose(pe.simulate([top, bot]).reshape(rows // 2, cols), Y[2 * rp], atol=1e-9)
+    assert np.allclose(po.simulate([top, bot]).reshape(rows // 2, cols), Y[2 * rp + 1], atol=1e-9)
+    levels = LIN.fft2_levels(rows, cols)
+    assert pe.counts()["rescale"] == po.counts()["rescale"] == levels
+    # rotations per stage: at most 6 signed offsets of a radix-4 stage
+    assert pe.counts()["rotate"] <= 6 * levels
+
+
+def test_signed_diagonals_do_not_wrap():
+    """A signed-offset diagonal never pairs an element with one of another
+    row: D_o[j] = 0 wherever j + o leaves [0, d)."""
+    d = 16
+    M = LIN.dit_stage_matrix(d, 4) @ LIN.dit_stage_matrix(d, 2)
+    sd = LIN.signed_diagonals(M)
+    rebuilt = np.zeros_like(M)
+    for o, v in sd.items():
+        for j in range(d):
+            if v[j] != 0:
+                assert 0 <= j + o < d
+                rebuilt[j, j + o] = v[j]
+    assert np.allclose(rebuilt, M)

@@ -21,7 +21,52 @@ def emu():
     lib.serve_emu_run.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int,
                                   C.c_void_p, C.c_void_p]
     lib.serve_emu_validate.argtypes = [C.c_int, C.c_char_p, C.c_int]
+    lib.serve_emu_scenario.argtypes = [C.c_int, C.c_void_p]
     return lib
+
+
+def scenario(emu, which):
+    out = np.zeros(16, dtype=np.int64)
+    assert emu.serve_emu_scenario(which, out.ctypes.data) == 0
+    return out.tolist()
+
+
+WAITING, ACTIVE, FINISHING, DONE, CANCELLED, FAILED = range(6)
+
+
+def test_cancel_waiting_and_active_requests(emu):
+    o = scenario(emu, 0)
+    assert o[0] == ACTIVE
+    assert o[1] == 1 and o[2] == CANCELLED      # waiting: out of the queue at once
+    assert o[3] == 1 and o[4] == CANCELLED      # active: dropped at the next step
+    assert o[5] == ACTIVE                       # its room admits the next request
+    assert o[6] == 0 and o[7] == DONE
+    assert o[8] == 1 and o[9] == 2 and o[10] == 0  # everything released
+    assert o[11] == 0                           # a finished request cannot be cancelled
+
+
+def test_allocation_failure_with_nothing_in_flight_fails_the_request(emu):
+    o = scenario(emu, 1)
+    assert o[0] == 0                            # the loop ends (no endless retry)
+    assert o[1] == FAILED and o[2] == DONE
+    assert o[3] == 1 and o[4] == 1 and o[5] == 0
+
+
+def test_allocation_failure_while_others_run_waits(emu):
+    o = scenario(emu, 2)
+    assert o[0] == ACTIVE and o[1] == WAITING
+    assert o[2] == 0 and o[3] == DONE and o[4] == 0 and o[5] == 2
+
+
+def test_shared_key_is_charged_once(emu):
+    o = scenario(emu, 3)
+    foot = o[8]
+    plaintext = 3 * 8 * 8
+    assert o[0] == 2                            # both admitted under 2 x regs + one key
+    assert o[1] == 1000 + plaintext
+    assert o[2] == 2 * foot + 1000 + plaintext
+    assert o[3] == 0 and o[4] == 0 and o[5] == 0
+    assert o[6] == 1 and o[7] == 0              # a budget without room for the key rejects
 
 
 def run(emu, kinds, arrive, budget, cap=0, lag=1, fail=0):
