@@ -1034,16 +1034,37 @@ static void rng_refill(orc_rng *rng) {
     rng->idx = 0;
 }
 
+/* PCG32's XSH-RR output permutation of a 64-bit LCG state (O'Neill, "PCG:
+ * A Family of Simple Fast Space-Efficient Statistically Good Algorithms",
+ * pcg32_random_r). */
+uint32_t orc_pcg32_xsh_rr(uint64_t state) {
+    uint32_t xs = (uint32_t)(((state >> 18) ^ state) >> 27);
+    uint32_t rot = (uint32_t)(state >> 59);
+    return (xs >> rot) | (xs << ((32 - rot) & 31));
+}
+
+/* The LCG step state * 6364136223846793005 + inc (PCG32's multiplier). */
+uint64_t orc_pcg32_step(uint64_t state, uint64_t inc) { return state * 6364136223846793005ull + inc; }
+
+/* rand_core 0.6.4 SeedableRng::seed_from_u64: PCG32 with the increment
+ * 11634580027462260723, output taken from the advanced state, filling the
+ * 32-byte ChaCha key word by word (little endian). */
 void orc_rng_seed(orc_rng *rng, uint64_t state) {
-    const uint64_t mul = 6364136223846793005ull, inc = 11634580027462260723ull;
+    const uint64_t inc = 11634580027462260723ull;
     for (int i = 0; i < 8; ++i) {
-        state = state * mul + inc;
-        uint32_t xs = (uint32_t)(((state >> 18) ^ state) >> 27);
-        uint32_t rot = (uint32_t)(state >> 59);
-        rng->key[i] = (xs >> rot) | (xs << ((32 - rot) & 31));
+        state = orc_pcg32_step(state, inc);
+        rng->key[i] = orc_pcg32_xsh_rr(state);
     }
     rng->counter = 0;
     rng->idx = 64; /* empty: refill on first use */
+}
+
+/* A stream with the given key at the given block counter (test access to
+ * the ChaCha stream layout). */
+void orc_rng_set_key(orc_rng *rng, const uint32_t key[8], uint64_t counter) {
+    memcpy(rng->key, key, sizeof rng->key);
+    rng->counter = counter;
+    rng->idx = 64;
 }
 
 uint32_t orc_rng_next_u32(orc_rng *rng) {
@@ -1079,20 +1100,53 @@ void orc_sample_uniform(orc_rng *rng, uint64_t q, size_t n, uint64_t *out) {
     }
 }
 
-/* DiscreteGaussianTable (rlwe/error.rs:58-108) with uniform_open_unit
- * (error.rs:131-140). */
+/* DiscreteGaussianTable::new (rlwe/error.rs:73-88): weights
+ * exp(-x^2 / (2 sigma^2)) for x in [-tail, tail], tail = ceil(16 sigma),
+ * summed left to right.  Returns the entry count (cap: room in weights). */
+size_t orc_gaussian_table(double sigma, double *weights, size_t cap, double *total) {
+    int64_t tail = (int64_t)ceil(16.0 * sigma);
+    size_t cnt = (size_t)(2 * tail + 1);
+    double t = 0.0;
+    for (int64_t x = -tail; x <= tail; ++x) {
+        double xf = (double)x;
+        double w = exp(-(xf * xf) / (2.0 * sigma * sigma));
+        if ((size_t)(x + tail) < cap) weights[x + tail] = w;
+        t += w;
+    }
+    *total = t;
+    return cnt;
+}
+
+/* uniform_open_unit (rlwe/error.rs:131-140): the top 53 bits of next_u64,
+ * centred in their interval, in (0, 1). */
+double orc_uniform_open_unit(orc_rng *rng) {
+    uint64_t v = orc_rng_next_u64(rng) >> 11;
+    return ((double)v + 0.5) * (1.0 / (double)((uint64_t)1 << 53));
+}
+
+/* Census of rand 0.8's single-sample u8 range 0..3 over every u32 draw
+ * (widening multiply by 3, accept while the low half is within the zone):
+ * counts[0..2] the accepted draws giving 0, 1, 2; counts[3] the rejected. */
+void orc_ternary_census(uint64_t counts[4]) {
+    const uint32_t range = 3;
+    const uint32_t zone = 0xffffffffu - ((0xffffffffu - range + 1) % range);
+    counts[0] = counts[1] = counts[2] = counts[3] = 0;
+    uint32_t v = 0;
+    do {
+        uint64_t m = (uint64_t)v * range;
+        if ((uint32_t)m <= zone) ++counts[m >> 32];
+        else ++counts[3];
+    } while (++v != 0);
+}
+
+/* DiscreteGaussianTable::sample (rlwe/error.rs:90-108). */
 void orc_sample_gaussian(orc_rng *rng, double sigma, size_t n, int64_t *out) {
     int64_t tail = (int64_t)ceil(16.0 * sigma);
     size_t cnt = (size_t)(2 * tail + 1);
     double *wt = malloc(cnt * sizeof(double)), total = 0.0;
-    for (int64_t x = -tail; x <= tail; ++x) {
-        double xf = (double)x;
-        wt[x + tail] = exp(-(xf * xf) / (2.0 * sigma * sigma));
-        total += wt[x + tail];
-    }
+    orc_gaussian_table(sigma, wt, cnt, &total);
     for (size_t i = 0; i < n; ++i) {
-        uint64_t v = orc_rng_next_u64(rng) >> 11;
-        double target = ((double)v + 0.5) * (1.0 / (double)((uint64_t)1 << 53)) * total;
+        double target = orc_uniform_open_unit(rng) * total;
         double cum = 0.0;
         int64_t res = tail;
         for (size_t k = 0; k < cnt; ++k) {

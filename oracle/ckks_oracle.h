@@ -176,6 +176,12 @@ void orc_chacha20_block(const uint32_t key[8], uint64_t counter, uint32_t out[16
 void orc_sample_uniform(orc_rng *rng, uint64_t q, size_t n, uint64_t *out);
 void orc_sample_gaussian(orc_rng *rng, double sigma, size_t n, int64_t *out);
 void orc_sample_ternary(orc_rng *rng, size_t n, int8_t *out);
+uint32_t orc_pcg32_xsh_rr(uint64_t state);
+uint64_t orc_pcg32_step(uint64_t state, uint64_t inc);
+void orc_rng_set_key(orc_rng *rng, const uint32_t key[8], uint64_t counter);
+size_t orc_gaussian_table(double sigma, double *weights, size_t cap, double *total);
+double orc_uniform_open_unit(orc_rng *rng);
+void orc_ternary_census(uint64_t counts[4]);
 
 #ifdef __cplusplus
 }

@@ -94,6 +94,12 @@ def lib():
             "orc_sample_uniform": (None, [P, u64, sz, P]),
             "orc_sample_gaussian": (None, [P, C.c_double, sz, P]),
             "orc_sample_ternary": (None, [P, sz, P]),
+            "orc_pcg32_xsh_rr": (C.c_uint32, [u64]),
+            "orc_pcg32_step": (u64, [u64, u64]),
+            "orc_rng_set_key": (None, [P, P, u64]),
+            "orc_gaussian_table": (sz, [C.c_double, P, sz, C.POINTER(C.c_double)]),
+            "orc_uniform_open_unit": (C.c_double, [P]),
+            "orc_ternary_census": (None, [P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -459,6 +465,40 @@ class Rng:
         out = np.empty(n, dtype=np.int8)
         lib().orc_sample_ternary(self.h, n, _p(out))
         return out
+
+    def uniform_open_unit(self):
+        return lib().orc_uniform_open_unit(self.h)
+
+    @classmethod
+    def with_key(cls, key_words, counter=0):
+        """A ChaCha20 stream with the given 8-word key from block `counter`."""
+        r = cls(0)
+        k = np.ascontiguousarray(key_words, dtype=np.uint32)
+        lib().orc_rng_set_key(r.h, _p(k), counter)
+        return r
+
+
+def pcg32_xsh_rr(state: int) -> int:
+    return lib().orc_pcg32_xsh_rr(state)
+
+
+def pcg32_step(state: int, inc: int) -> int:
+    return lib().orc_pcg32_step(state, inc)
+
+
+def gaussian_table(sigma: float):
+    """(weights, total) of DiscreteGaussianTable::new (rlwe/error.rs:73-88)."""
+    total = C.c_double()
+    cnt = lib().orc_gaussian_table(sigma, None, 0, C.byref(total))
+    w = np.empty(cnt, dtype=np.float64)
+    lib().orc_gaussian_table(sigma, _p(w), cnt, C.byref(total))
+    return w, total.value
+
+
+def ternary_census():
+    out = np.zeros(4, dtype=np.uint64)
+    lib().orc_ternary_census(_p(out))
+    return [int(v) for v in out]
 
 
 def chacha20_block(key_words, counter):

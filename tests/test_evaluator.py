@@ -94,6 +94,25 @@ def test_trace_counts():
     assert len(tr) == 3 and tr.count(CK.ExecutionEventKind.ROTATE) == 2 and not tr.is_empty()
 
 
+def test_execution_profile_counts_and_total():
+    """ExecutionProfile::{count, total_operations} (execution/mod.rs:176-230):
+    eBLAS / CCMM kinds count 0, the total sums the ten counted kinds (not the
+    two device-only fields), and a trace's events give the same counts."""
+    K = CK.ExecutionEventKind
+    p = CK.ExecutionProfile(ciphertext_plaintext_multiplies=2, rotations=3, ntt_forwards=5, rescales=1,
+                            key_switches=3, ntt_limb_transforms=40)
+    assert p.count(K.ROTATE) == 3 and p.count(K.NTT_FORWARD) == 5 and p.count(K.CIPHERTEXT_PLAINTEXT_MULTIPLY) == 2
+    for kind in (K.EBLAS_DOT, K.EBLAS_GEMV, K.EBLAS_GEMM, K.CPMM, K.CCMM_SCALAR, K.CCMM_STRUCTURED):
+        assert p.count(kind) == 0
+    assert p.total_operations() == 11
+    tr = CK.ExecutionTrace()
+    for kind in (K.ROTATE, K.ROTATE, K.RESCALE, K.EBLAS_GEMM, K.NTT_INVERSE):
+        tr.record(CK.ExecutionEvent(kind))
+    q = CK.ExecutionProfile.from_trace(tr)
+    assert q.rotations == 2 and q.rescales == 1 and q.ntt_inverses == 1 and q.total_operations() == 4
+    assert (p - CK.ExecutionProfile(rotations=1)).rotations == 2
+
+
 @pytest.mark.gpu
 def test_gpu_evaluator_sequence_matches_oracle(pkg, ctx):
     """add -> multiply (relin + rescale) -> rotate_left(3) -> conjugate ->

@@ -195,3 +195,38 @@ def test_rejections(pkg, ctx, fx):
         server.submit(fx.plan, len(Q), cts, out, bad)
     assert server.stats()["rejected"] == 1
     server.close()
+
+
+def test_cancel_waiting_and_active_requests(pkg, ctx, fx):
+    """A cancelled waiting request never runs; a cancelled active one stops
+    and gives its registers and resident key charges back; the request that
+    remains produces the residues it produces alone."""
+    jobs = _jobs(fx)
+    gemv = jobs[0]
+    server = pkg.serving.Server(ctx, hbm_budget_bytes=1 << 30)
+    t_alone, cts, out_alone = _submit(fx, server, gemv)
+    server.drain()
+    ref = out_alone.clone()
+    server.close()
+    foot = gemv[1].n_regs * 2 * len(Q) * N * 8
+    server = pkg.serving.Server(ctx, hbm_budget_bytes=1 << 30, max_active=1)
+    outs = []
+    for _ in range(3):
+        out = torch.zeros_like(ref)
+        outs.append((server.submit(fx.plan, len(Q), cts, out, gemv[1]), out))
+    server.step()
+    (t0, o0), (t1, o1), (t2, o2) = outs
+    assert server.poll(t0)[0] == server.ADMITTED and server.poll(t2)[0] == server.WAITING
+    server.cancel(t2)
+    assert server.poll(t2)[0] == server.CANCELLED
+    server.cancel(t0)
+    server.drain()
+    assert server.poll(t0)[0] == server.CANCELLED and server.poll(t1)[0] == server.DONE
+    assert torch.equal(o1, ref)
+    assert not torch.any(o2)  # never ran
+    st = server.stats()
+    assert st["cancelled"] == 2 and st["completed"] == 1 and st["resident_bytes"] == 0
+    assert st["peak_bytes"] >= foot  # registers plus the keys and diagonals the job reads
+    with pytest.raises(pkg.CkksError, match="finished"):
+        server.cancel(t1)
+    server.close()
