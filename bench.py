@@ -26,9 +26,12 @@ Reported (one JSON line):
                   the reference's bounded-digit rotation shape, canonical
                   and hoisted
   ntt_*           forward NTT over the batch's 96 digit limbs per ciphertext
-  p50/p99_ms      latency of one batched rotation call (64 ciphertexts)
+  p50/p99_ms      latency of one batched rotation call (64 ciphertexts at
+                  N = 2^16 and the 16 at N = 2^15 in one launch sequence)
   roofline        the forward NTT (dominant kernel pair) against HBM peak
-  cpu_baseline    the CPU oracle's hybrid rotation on one core, bounded sample
+  cpu_baseline    the CPU oracle's hybrid rotation on all the host cores the
+                  job may use (one ciphertext per thread), bounded sample;
+                  cpu_baseline_one_core the same on one core
 """
 from __future__ import annotations
 
@@ -90,8 +93,23 @@ class Timer:
         self.ms = self.e0.elapsed_time(self.e1)
 
 
-def cpu_baseline(q_moduli, p_moduli, n, blocks, seconds=10.0):
-    """The oracle's rotation (automorphism + hybrid key switch) on one core."""
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(q_moduli, p_moduli, n, blocks, seconds=10.0, threads=1):
+    """The oracle's rotation (automorphism + hybrid key switch) on `threads`
+    host threads, each rotating its own ciphertext (independent work items;
+    the oracle's C calls release the GIL), for about `seconds`."""
+    import concurrent.futures as cf
+
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O  # the oracle, timed here as the CPU baseline
 
@@ -102,22 +120,28 @@ def cpu_baseline(q_moduli, p_moduli, n, blocks, seconds=10.0):
     L = len(q_moduli)
     kb = rng.integers(0, 1 << 39, (len(blocks), len(qp), n), dtype=np.uint64)
     ka = rng.integers(0, 1 << 39, (len(blocks), len(qp), n), dtype=np.uint64)
-    b = rng.integers(0, 1 << 39, (L, n), dtype=np.uint64)
-    a = rng.integers(0, 1 << 39, (L, n), dtype=np.uint64)
     k = rotation_exponent_left(n, 1)
-    done = 0
+
+    def worker(seed, t0):
+        r = np.random.default_rng(seed)
+        b = r.integers(0, 1 << 39, (L, n), dtype=np.uint64)
+        a = r.integers(0, 1 << 39, (L, n), dtype=np.uint64)
+        done = 0
+        while time.perf_counter() - t0 < seconds:
+            rb = np.stack([O.apply_automorphism(b[i], q, k) for i, q in enumerate(q_moduli)])
+            ra = np.stack([O.apply_automorphism(a[i], q, k) for i, q in enumerate(q_moduli)])
+            O.hybrid_key_switch(plan, L, blocks, rb, ra, kb, ka)
+            done += 1
+        return done
+
     t0 = time.perf_counter()
-    while True:
-        rb = np.stack([O.apply_automorphism(b[i], q, k) for i, q in enumerate(q_moduli)])
-        ra = np.stack([O.apply_automorphism(a[i], q, k) for i, q in enumerate(q_moduli)])
-        O.hybrid_key_switch(plan, L, blocks, rb, ra, kb, ka)
-        done += 1
-        el = time.perf_counter() - t0
-        if el >= seconds:
-            break
-    return {"value": done / el, "unit": "rotations/s", "cores": 1, "kind": "port",
-            "sample": f"{done} hybrid rotations of one ciphertext at N=2^16, L=24, dnum=3, K=7 "
-                      f"({el:.1f} s, oracle in C, one thread)"}
+    with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+        done = sum(ex.map(lambda s: worker(s, t0), range(threads)))
+    el = time.perf_counter() - t0
+    return {"value": done / el, "unit": "rotations/s", "cores": threads, "kind": "port",
+            "sample": f"{done} hybrid rotations at N=2^16, L=24, dnum=3, K=7, one ciphertext per thread "
+                      f"({el:.1f} s, oracle in C, {threads} thread{'s' if threads > 1 else ''}; "
+                      f"host: {os.cpu_count()} logical CPUs, {_cpu_model()})"}
 
 
 def measure_ntt(torch, plan, polys, limbs, n, reps=5):
@@ -318,9 +342,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--mixed", type=int, default=16, help="ciphertexts at N=2^15 in each step")
+    ap.add_argument("--mixed-launch", choices=["batch", "separate"], default="batch",
+                    help="the N=2^15 ciphertexts ride in the N=2^16 call (ckks_apply_galois_batch) "
+                         "or run as calls of their own")
     ap.add_argument("--plain-sample", type=int, default=24, help="plain-path rotations timed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-threads", type=int, default=16, help="threads of the all-core CPU baseline")
     ap.add_argument("--fusion", choices=["auto", "off", "xcd", "linear", "inverse"], default="auto",
                     help="digits' forward row pass fused with the key MAC: autotuned per key, "
                          "off, or fused with XCD-contiguous / linear block order")
@@ -428,17 +456,24 @@ def main():
     cts2 = rand_residues(torch, (args.mixed, 2, L2, N2), dev) if args.mixed else None
     out2 = torch.empty_like(cts2) if args.mixed else None
 
-    def step(lat=None):
+    batched = args.mixed and args.mixed_launch == "batch"
+
+    def step(lat=None, only16=False):
         for idx, (k, key) in enumerate(keys):
             if lat is not None:
                 e0 = torch.cuda.Event(enable_timing=True)
                 e1 = torch.cuda.Event(enable_timing=True)
                 e0.record()
-            key.apply_galois(cts, k, out)
+            if batched and not only16:
+                # both degrees in one launch sequence (one NTT launch per pass)
+                k2, key2 = keys2[idx]
+                pkg.apply_galois_batch(ctx, [(key, k, cts, out), (key2, k2, cts2, out2)])
+            else:
+                key.apply_galois(cts, k, out)
             if lat is not None:
                 e1.record()
                 lat.append((e0, e1))
-            if args.mixed:
+            if args.mixed and not batched and not only16:
                 k2, key2 = keys2[idx]
                 key2.apply_galois(cts2, k2, out2)
 
@@ -463,7 +498,11 @@ def main():
     p99 = float(np.percentile(lats, 99))
     # the N=2^16 calls alone (events bracket each one), the like-for-like
     # figure for the comparison with the plain path, which runs at N=2^16
-    n16_value = B * len(lats) / (sum(lats) / 1e3)
+    lat16 = []
+    step(lat16, only16=True)
+    torch.cuda.synchronize()
+    lat16 = [e0.elapsed_time(e1) for e0, e1 in lat16]
+    n16_value = B * len(lat16) / (sum(lat16) / 1e3)
     print(f"# batched: {ms_step:.1f} ms/step, {value:.0f} rot/s (wall {wall:.2f}s); "
           f"N=2^16 calls alone {n16_value:.0f} rot/s", file=sys.stderr, flush=True)
 
@@ -533,6 +572,8 @@ def main():
         "autotune": tuned,
         "p50_ms": round(p50, 3),
         "p99_ms": round(p99, 3),
+        "latency_call": (f"one rotation call of {B} ct at N=2^16" +
+                         (f" and {args.mixed} ct at N=2^15 (one launch sequence)" if batched else "")),
         "roofline": {
             "bound": "hbm",
             "kernel": "forward NTT (column + row pass), 2*N*8 bytes per limb transform",
@@ -554,7 +595,11 @@ def main():
               f"p99 {result['serving']['p99_ms']} ms", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         print("# cpu baseline", file=sys.stderr, flush=True)
-        result["cpu_baseline"] = cpu_baseline(q_moduli, p_moduli, N, blocks, args.cpu_seconds)
+        # all the host cores this job may use (the GPU box grants 16 per GPU),
+        # then one core as the reference's own single-threaded path runs
+        threads = max(1, min(args.cpu_threads, len(os.sched_getaffinity(0))))
+        result["cpu_baseline"] = cpu_baseline(q_moduli, p_moduli, N, blocks, args.cpu_seconds, threads)
+        result["cpu_baseline_one_core"] = cpu_baseline(q_moduli, p_moduli, N, blocks, args.cpu_seconds / 2, 1)
     print(json.dumps(result), flush=True)
 
 

@@ -122,8 +122,10 @@ _SIGNATURES = {
     "ckks_bounded_rotate_hoisted": (C.c_int, [_p, _p, _u64, _p, _p, _sz, _sz]),
     "ckks_bounded_hoist_destroy": (C.c_int, [_p]),
     "ckks_ksk_destroy": (C.c_int, [_p]),
+    "ckks_ksk_bytes": (C.c_int, [_p, C.POINTER(_sz)]),
     "ckks_key_switch": (C.c_int, [_p, _p, _p, _sz, _sz, _sz]),
     "ckks_apply_galois": (C.c_int, [_p, _u64, _p, _p, _sz, _sz, _sz]),
+    "ckks_apply_galois_batch": (C.c_int, [_p, _p, _sz]),
     "ckks_relinearize": (C.c_int, [_p, _p, _p, _sz, _sz, _sz]),
     "ckks_multiply_relinearize_rescale": (C.c_int, [_p, _p, _p, _p, _sz, _sz, _sz]),
 }
@@ -645,6 +647,13 @@ class _KeySwitchKey:
                                                        0, 0))
         return out
 
+    @property
+    def nbytes(self) -> int:
+        """Device bytes of the key (ckks_ksk_bytes; logical_payload_bytes)."""
+        b = _sz()
+        _check(lib().ckks_ksk_bytes(self._h, C.byref(b)))
+        return b.value
+
     def autotune(self, n_cts: int):
         """Times the key switch of n_cts ciphertexts for every (base-conversion
         engine, fusion mode) pair and keeps the fastest for the AUTO modes;
@@ -916,6 +925,23 @@ def bounded_digits_automorphism(plan: PreparedRnsNttPlan, q_limbs: int, base_log
                                                   _ptr(digits, polys * count * n, name="digits"),
                                                   _ptr(out), polys, int(exponent)))
     return out
+
+
+class GaloisBatchEntry(C.Structure):
+    _fields_ = [("key", _p), ("exponent", _u64), ("input", _p), ("output", _p), ("n_cts", _sz),
+                ("in_stride", _sz), ("out_stride", _sz)]
+
+
+def apply_galois_batch(ctx: Context, entries):
+    """ckks_apply_galois_batch over [(key, exponent, ct, out)]: rotations of
+    batches of uneven degree and level in one launch sequence; each out
+    equals key.apply_galois(ct, exponent)."""
+    arr = (GaloisBatchEntry * max(len(entries), 1))()
+    for i, (key, exponent, ct, out) in enumerate(entries):
+        n = key._n_cts(ct, 2)
+        p_ct, p_out = key._ct_ptrs(n, [("ct", ct, 2)], out)
+        arr[i] = GaloisBatchEntry(key.handle, int(exponent), p_ct, p_out, n, 0, 0)
+    _check(lib().ckks_apply_galois_batch(ctx.handle, C.cast(arr, _p), len(entries)))
 
 
 class NttBatchEntry(C.Structure):

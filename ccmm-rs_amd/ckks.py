@@ -313,6 +313,60 @@ class RnsCkksEvaluationKeys:
         return not self.levels
 
 
+@dataclass
+class RnsCkksLevelKeyStorage:
+    level: int = 0
+    multiplication_bytes: int = 0
+    galois_key_count: int = 0
+    galois_bytes: int = 0
+    total_resident_bytes: int = 0
+
+
+@dataclass
+class RnsCkksEvaluationKeyStorage:
+    levels: list = None
+    multiplication_bytes: int = 0
+    galois_key_count: int = 0
+    galois_bytes: int = 0
+    total_resident_bytes: int = 0
+
+    def to_report_string(self) -> str:
+        """Deterministic text report (RnsCkksEvaluationKeyStorage::
+        to_report_string, ckks/key_accounting.rs:346-395; the hybrid-policy
+        columns of the reference are zero here: its Grafting hybrid policy is
+        not on this path)."""
+        out = ["CKKS_EVALUATION_KEY_STORAGE"]
+        for lv in self.levels or []:
+            out.append(f"level={lv.level} multiplication_bytes={lv.multiplication_bytes} "
+                       f"galois_key_count={lv.galois_key_count} galois_bytes={lv.galois_bytes} "
+                       f"hybrid_base_bytes=0 prepared_cache_bytes=0 hybrid_prepared_total_bytes=0 "
+                       f"total_resident_bytes={lv.total_resident_bytes}")
+        out.append(f"TOTAL multiplication_bytes={self.multiplication_bytes} "
+                   f"galois_key_count={self.galois_key_count} galois_bytes={self.galois_bytes} "
+                   f"hybrid_base_bytes=0 prepared_cache_bytes=0 hybrid_prepared_total_bytes=0 "
+                   f"total_resident_bytes={self.total_resident_bytes}")
+        return "\n".join(out) + "\n"
+
+
+def evaluation_key_storage(keys: RnsCkksEvaluationKeys, key_bytes=None) -> RnsCkksEvaluationKeyStorage:
+    """evaluation_key_storage (ckks/key_accounting.rs:276-344): resident key
+    bytes per level and in total, in level order.  key_bytes(key) gives a
+    key's bytes (default: the device figure, key.nbytes)."""
+    key_bytes = key_bytes or (lambda k: k.nbytes)
+    rep = RnsCkksEvaluationKeyStorage(levels=[])
+    for level in sorted(keys.levels):
+        lk = keys.levels[level]
+        mb = key_bytes(lk.multiplication_key) if lk.multiplication_key is not None else 0
+        gb = sum(key_bytes(k) for k in lk.galois_keys.values())
+        lv = RnsCkksLevelKeyStorage(level, mb, len(lk.galois_keys), gb, mb + gb)
+        rep.levels.append(lv)
+        rep.multiplication_bytes += mb
+        rep.galois_key_count += lv.galois_key_count
+        rep.galois_bytes += gb
+        rep.total_resident_bytes += lv.total_resident_bytes
+    return rep
+
+
 # ---------------------------------------------------------------- evaluator
 
 def rotation_exponent_left(degree: int, steps: int) -> int:

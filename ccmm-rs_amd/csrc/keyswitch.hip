@@ -1531,102 +1531,107 @@ static int key_switch_core(const ckks_ksk *k, const u64 *x, u64 x_stride, const 
     return rc;
 }
 
-static int key_switch_lane(const ckks_ksk *k, const u64 *x, u64 x_stride, const u64 *add0,
-                           const u64 *add1, u64 add_stride, u64 *out, u64 out_stride, size_t n_cts,
-                           u64 galois_kinv, const int64_t *hoisted, const HoistedMac *hm) {
-    ckks_context *ctx = k->ctx;
+// One key switch of a ciphertext batch, split into the phases that a
+// multi-key batch (key_switch_multi) interleaves with other entries.
+struct KsJob {
+    const ckks_ksk *k;
+    const u64 *x;
+    u64 x_stride;
+    const u64 *add0, *add1;
+    u64 add_stride;
+    u64 *out;
+    u64 out_stride;
+    size_t n_cts;
+    u64 galois_kinv;
+    const int64_t *hoisted;
+    const HoistedMac *hm;
+    u64 *digits, *acc, *mac_out;  // [ct][D][B][N] (or the hoisted digits), [ct][2][B][N], MAC output
+    bool mfma;
+};
+
+static bool ks_uses_mfma(const ckks_ksk *k) {
+    const ckks_context *ctx = k->ctx;
+    const int engine = ctx->base_conversion == CKKS_BASECONV_AUTO ? k->tuned_engine : ctx->base_conversion;
+    return k->mfma && engine == CKKS_BASECONV_MFMA;
+}
+
+// 1. decomposition: ModUp, bounded digits or hoisted-digit automorphism
+static int ks_decompose(const KsJob &j, hipStream_t s) {
+    const ckks_ksk *k = j.k;
     const ckks_rns_plan *plan = k->plan;
     const u64 N = k->degree, B = k->limbs, D = k->dnum;
-    const u32 logn = plan->logn;
-    hipStream_t s = ctx->stream;
-    int rc;
-    void *wd = nullptr, *wa;
-    // hoisted: slot 4 holds the unpermuted MAC output instead of the digits
-    const size_t slot4 = hm ? n_cts * 2 * B * N : n_cts * D * B * N;
-    if ((rc = ctx->workspace(4, slot4 * sizeof(u64), &wd))) return rc;
-    if ((rc = ctx->workspace(5, n_cts * 2 * B * N * sizeof(u64), &wa))) return rc;
-    u64 *digits = hm ? const_cast<u64 *>(hm->digits) : (u64 *)wd, *acc = (u64 *)wa;
-    u64 *mac_out = hm ? (u64 *)wd : acc;
     const u32 chunks = (u32)(N / kKsThreads);
-    // 1. ModUp
+    int rc;
     ModUpArgs mu{};
-    mu.in = x;
-    mu.in_stride = x_stride;
-    mu.kinv = galois_kinv;
-    mu.digits = digits;
+    mu.in = j.x;
+    mu.in_stride = j.x_stride;
+    mu.kinv = j.galois_kinv;
+    mu.digits = j.digits;
     mu.blocks = k->d_modup;
     mu.primes = plan->d_primes;
     mu.fp_targets = k->fp_targets;
     mu.dnum = (u32)D;
     mu.limbs = (u32)B;
-    mu.logn = logn;
+    mu.logn = plan->logn;
     mu.chunks = chunks;
-    const int engine = ctx->base_conversion == CKKS_BASECONV_AUTO ? k->tuned_engine : ctx->base_conversion;
-    const bool mfma = k->mfma && engine == CKKS_BASECONV_MFMA;
-    if (hm) {
+    if (j.hm) {
         // decomposed, lifted and transformed once (ckks_bounded_hoist)
     } else if (k->bounded) {
         BoundedArgs ba{};
-        ba.in = x;
-        ba.in_stride = x_stride;
-        ba.kinv = galois_kinv;
-        ba.digits = digits;
+        ba.in = j.x;
+        ba.in_stride = j.x_stride;
+        ba.kinv = j.galois_kinv;
+        ba.digits = j.digits;
         ba.t = bounded_table(k->d_bounded, k->q_limbs, k->bounded_words);
         ba.primes = plan->d_primes;
         ba.q_limbs = (u32)k->q_limbs;
         ba.limbs = (u32)B;
-        ba.logn = logn;
+        ba.logn = plan->logn;
         ba.digit_count = (u32)D;
         ba.base_log = k->base_log;
         ba.small = digits_below_primes(plan, B, k->base_log);
-        ba.total = (u64)n_cts * N;
-        if (hoisted) {
+        ba.total = (u64)j.n_cts * N;
+        if (j.hoisted) {
             HoistArgs ha{};
-            ha.digits = hoisted;
-            ha.lifted = digits;
+            ha.digits = j.hoisted;
+            ha.lifted = j.digits;
             ha.primes = plan->d_primes;
-            ha.kinv = galois_kinv;
+            ha.kinv = j.galois_kinv;
             ha.limbs = (u32)B;
-            ha.logn = logn;
+            ha.logn = plan->logn;
             ha.digit_count = (u32)D;
             ha.base_log = k->base_log;
             ha.small = ba.small;
-            ha.total = (u64)n_cts * N;
+            ha.total = (u64)j.n_cts * N;
             if ((rc = launch_hoist(s, ha))) return rc;
         } else if ((rc = launch_bounded(s, ba, k->bounded_words))) {
             return rc;
         }
-    } else if (mfma) {
+    } else if (j.mfma) {
         ModUpMfmaArgs mm{mu, k->d_bc};
-        hipLaunchKernelGGL(k_modup_mfma, dim3((u32)(n_cts * D * (N / kBcRows))), dim3(256), 0, s, mm);
+        hipLaunchKernelGGL(k_modup_mfma, dim3((u32)(j.n_cts * D * (N / kBcRows))), dim3(256), 0, s, mm);
     } else {
-        if ((rc = launch_modup(s, mu, k->block_size, k->block_fp, n_cts))) return rc;
+        if ((rc = launch_modup(s, mu, k->block_size, k->block_fp, j.n_cts))) return rc;
     }
     CK_HIP(hipGetLastError());
-    // 2+3. forward NTT of every digit limb, key MAC
-    const int fusion = ctx->ks_fusion == CKKS_KS_FUSION_AUTO ? k->tuned_fusion : ctx->ks_fusion;
-    const bool two_pass = logn > 12 && logn <= 17;
-    // CKKS_KS_FUSION_INVERSE: the MAC is computed inside the first inverse
-    // NTT pass (step 4) and the accumulators never reach HBM in the NTT domain
-    const bool mac_in_inverse = fusion == CKKS_KS_FUSION_INVERSE && two_pass && D <= 8 && !hm;
-    if ((fusion == CKKS_KS_FUSION_XCD || fusion == CKKS_KS_FUSION_LINEAR) && two_pass && !hm) {
-        if ((rc = launch_ntt_mac(ctx, plan, digits, n_cts, D, B, k->d_kb, k->d_ka, acc,
-                                 fusion == CKKS_KS_FUSION_XCD)))
-            return rc;
-    } else {
-    NttCall f{plan, digits, digits, n_cts * D, B, B * N, B * N};
-    if (!hm && (rc = launch_ntt(ctx, &f, 1, true))) return rc;
-    if (!mac_in_inverse) {
+    return CKKS_OK;
+}
+
+// 3. key MAC of the NTT-domain digits into mac_out
+static int ks_mac(const KsJob &j, hipStream_t s) {
+    const ckks_ksk *k = j.k;
+    const u64 N = k->degree, B = k->limbs, D = k->dnum;
+    const u32 chunks = (u32)(N / kKsThreads);
     MacArgs ma{};
-    ma.digits = digits;
-    ma.kb = hm ? hm->kb : k->d_kb;
-    ma.ka = hm ? hm->ka : k->d_ka;
-    ma.acc = mac_out;
-    ma.primes = plan->d_primes;
-    ma.n_cts = (u32)n_cts;
+    ma.digits = j.digits;
+    ma.kb = j.hm ? j.hm->kb : k->d_kb;
+    ma.ka = j.hm ? j.hm->ka : k->d_ka;
+    ma.acc = j.mac_out;
+    ma.primes = k->plan->d_primes;
+    ma.n_cts = (u32)j.n_cts;
     ma.dnum = (u32)D;
     ma.limbs = (u32)B;
-    ma.logn = logn;
+    ma.logn = k->plan->logn;
     ma.chunks = chunks;
     switch (D) {
 #define CK_MAC(DD) \
@@ -1634,44 +1639,43 @@ static int key_switch_lane(const ckks_ksk *k, const u64 *x, u64 x_stride, const 
         CK_MAC(1) CK_MAC(2) CK_MAC(3) CK_MAC(4) CK_MAC(5) CK_MAC(6) CK_MAC(7) CK_MAC(8)
 #undef CK_MAC
         default:
-            if (n_cts >= 2) {
+            if (j.n_cts >= 2) {
                 for (u32 j0 = 0; j0 < (u32)D; j0 += 8)
                     hipLaunchKernelGGL(k_ks_mac_chunk<8>, dim3((u32)(B * chunks)), dim3(kKsThreads), 0, s, ma, j0,
                                        j0 == 0 ? 1u : 0u);
             } else {
-                hipLaunchKernelGGL(k_ks_mac_any, dim3((u32)(n_cts * B * chunks)), dim3(kKsThreads), 0, s, ma);
+                hipLaunchKernelGGL(k_ks_mac_any, dim3((u32)(j.n_cts * B * chunks)), dim3(kKsThreads), 0, s, ma);
             }
     }
     CK_HIP(hipGetLastError());
-    }
-    }
-    // 4. inverse NTT of both accumulators.  Hoisted: the MAC ran over
-    // sigma_k^-1-permuted keys; the inverse NTT reads its output permuted by
-    // k, which is sum_j NTT(sigma_k(d_j)) * key_j
-    const NttMac mac{digits, k->d_kb, k->d_ka, (u32)D};
-    NttCall iv{plan, hm ? mac_out : acc, acc, n_cts * 2, B, B * N, B * N, mac_in_inverse ? &mac : nullptr,
-               hm ? hm->k : 0};
-    if ((rc = launch_ntt(ctx, &iv, 1, false))) return rc;
-    // 5. ModDown (or plain finalize) with pass-through adds
+    return CKKS_OK;
+}
+
+// 5. ModDown (or plain finalize) of the coefficient-domain accumulators
+// with the pass-through adds
+static int ks_moddown(const KsJob &j, hipStream_t s) {
+    const ckks_ksk *k = j.k;
+    const u64 N = k->degree, B = k->limbs, D = k->dnum;
+    const u32 chunks = (u32)(N / kKsThreads);
     DownArgs da{};
-    da.acc = acc;
-    da.add0 = add0;
-    da.add1 = add1;
-    da.add_stride = add_stride;
-    da.kinv = galois_kinv;
-    da.out = out;
-    da.out_stride = out_stride;
+    da.acc = j.acc;
+    da.add0 = j.add0;
+    da.add1 = j.add1;
+    da.add_stride = j.add_stride;
+    da.kinv = j.galois_kinv;
+    da.out = j.out;
+    da.out_stride = j.out_stride;
     da.md = k->d_moddown;
-    da.primes = plan->d_primes;
+    da.primes = k->plan->d_primes;
     da.fp_targets = k->fp_q_targets;
     da.limbs = (u32)B;
     da.q_limbs = (u32)k->q_limbs;
-    da.logn = logn;
+    da.logn = k->plan->logn;
     da.chunks = chunks;
-    dim3 g((u32)(n_cts * 2 * chunks));
-    if (mfma) {
+    dim3 g((u32)(j.n_cts * 2 * chunks));
+    if (j.mfma) {
         ModDownMfmaArgs dm{da, k->d_bc + D};
-        dim3 gm((u32)(n_cts * 2 * (N / kBcRows)));
+        dim3 gm((u32)(j.n_cts * 2 * (N / kBcRows)));
         switch (k->p_limbs) {
 #define CK_MDM(KK) \
     case KK: hipLaunchKernelGGL(k_moddown_mfma<KK>, gm, dim3(256), 0, s, dm); break;
@@ -1693,8 +1697,112 @@ static int key_switch_lane(const ckks_ksk *k, const u64 *x, u64 x_stride, const 
     return CKKS_OK;
 }
 
-// Device bytes a key holds: its entries and the permuted copies kept for
-// hoisted rotations (tables are a few KB and not counted).
+static int key_switch_lane(const ckks_ksk *k, const u64 *x, u64 x_stride, const u64 *add0,
+                           const u64 *add1, u64 add_stride, u64 *out, u64 out_stride, size_t n_cts,
+                           u64 galois_kinv, const int64_t *hoisted, const HoistedMac *hm) {
+    ckks_context *ctx = k->ctx;
+    const ckks_rns_plan *plan = k->plan;
+    const u64 N = k->degree, B = k->limbs, D = k->dnum;
+    const u32 logn = plan->logn;
+    hipStream_t s = ctx->stream;
+    int rc;
+    void *wd = nullptr, *wa;
+    // hoisted: slot 4 holds the unpermuted MAC output instead of the digits
+    const size_t slot4 = hm ? n_cts * 2 * B * N : n_cts * D * B * N;
+    if ((rc = ctx->workspace(4, slot4 * sizeof(u64), &wd))) return rc;
+    if ((rc = ctx->workspace(5, n_cts * 2 * B * N * sizeof(u64), &wa))) return rc;
+    KsJob j{k, x, x_stride, add0, add1, add_stride, out, out_stride, n_cts, galois_kinv, hoisted, hm};
+    j.digits = hm ? const_cast<u64 *>(hm->digits) : (u64 *)wd;
+    j.acc = (u64 *)wa;
+    j.mac_out = hm ? (u64 *)wd : j.acc;
+    j.mfma = ks_uses_mfma(k);
+    // 1. ModUp
+    if ((rc = ks_decompose(j, s))) return rc;
+    // 2+3. forward NTT of every digit limb, key MAC
+    const int fusion = ctx->ks_fusion == CKKS_KS_FUSION_AUTO ? k->tuned_fusion : ctx->ks_fusion;
+    const bool two_pass = logn > 12 && logn <= 17;
+    // CKKS_KS_FUSION_INVERSE: the MAC is computed inside the first inverse
+    // NTT pass (step 4) and the accumulators never reach HBM in the NTT domain
+    const bool mac_in_inverse = fusion == CKKS_KS_FUSION_INVERSE && two_pass && D <= 8 && !hm;
+    if ((fusion == CKKS_KS_FUSION_XCD || fusion == CKKS_KS_FUSION_LINEAR) && two_pass && !hm) {
+        if ((rc = launch_ntt_mac(ctx, plan, j.digits, n_cts, D, B, k->d_kb, k->d_ka, j.acc,
+                                 fusion == CKKS_KS_FUSION_XCD)))
+            return rc;
+    } else {
+        NttCall f{plan, j.digits, j.digits, n_cts * D, B, B * N, B * N};
+        if (!hm && (rc = launch_ntt(ctx, &f, 1, true))) return rc;
+        if (!mac_in_inverse && (rc = ks_mac(j, s))) return rc;
+    }
+    // 4. inverse NTT of both accumulators.  Hoisted: the MAC ran over
+    // sigma_k^-1-permuted keys; the inverse NTT reads its output permuted by
+    // k, which is sum_j NTT(sigma_k(d_j)) * key_j
+    const NttMac mac{j.digits, k->d_kb, k->d_ka, (u32)D};
+    NttCall iv{plan, hm ? j.mac_out : j.acc, j.acc, n_cts * 2, B, B * N, B * N, mac_in_inverse ? &mac : nullptr,
+               hm ? hm->k : 0};
+    if ((rc = launch_ntt(ctx, &iv, 1, false))) return rc;
+    // 5. ModDown (or plain finalize) with pass-through adds
+    return ks_moddown(j, s);
+}
+
+// Key switches of several ciphertext batches, each with its own key (keys
+// of different ring degrees, levels and kinds may mix): every phase runs for
+// all entries before the next, and the forward and inverse NTTs of all
+// entries go out as one launch per pass (mixed degrees share the descriptor
+// table), so small batches ride in the big batches' launches.  The plain
+// pipeline (no fusion modes, no hoisting, one stream); at most 16 entries.
+struct KsMultiEntry {
+    const ckks_ksk *k;
+    const u64 *x, *add0, *add1;
+    u64 x_stride, add_stride;
+    u64 *out;
+    u64 out_stride;
+    size_t n_cts;
+    u64 galois_kinv;
+};
+
+static int key_switch_multi(ckks_context *ctx, const KsMultiEntry *e, size_t count) {
+    if (count > 16) return fail(CKKS_ERR_INVALID, "a key-switch batch holds at most 16 entries");
+    size_t dig = 0, acc = 0;
+    for (size_t i = 0; i < count; ++i) {
+        dig += e[i].n_cts * e[i].k->dnum * e[i].k->limbs * e[i].k->degree;
+        acc += e[i].n_cts * 2 * e[i].k->limbs * e[i].k->degree;
+        ckks_execution_profile &pc = ctx->counts();
+        pc.key_switches += e[i].n_cts;
+        if (e[i].galois_kinv)
+            (e[i].galois_kinv == 2 * (u64)e[i].k->degree - 1 ? pc.automorphisms : pc.rotations) += e[i].n_cts;
+    }
+    int rc;
+    void *wd, *wa;
+    if ((rc = ctx->workspace(4, dig * sizeof(u64), &wd))) return rc;
+    if ((rc = ctx->workspace(5, acc * sizeof(u64), &wa))) return rc;
+    hipStream_t s = ctx->stream;
+    std::vector<KsJob> jobs(count);
+    std::vector<NttCall> fwd(count), inv(count);
+    u64 *pd = (u64 *)wd, *pa = (u64 *)wa;
+    for (size_t i = 0; i < count; ++i) {
+        const ckks_ksk *k = e[i].k;
+        const size_t B = k->limbs, N = k->degree;
+        KsJob &j = jobs[i];
+        j = KsJob{k, e[i].x, e[i].x_stride, e[i].add0, e[i].add1, e[i].add_stride, e[i].out, e[i].out_stride,
+                  e[i].n_cts, e[i].galois_kinv, nullptr, nullptr};
+        j.digits = pd;
+        j.acc = j.mac_out = pa;
+        j.mfma = ks_uses_mfma(k);
+        pd += e[i].n_cts * k->dnum * B * N;
+        pa += e[i].n_cts * 2 * B * N;
+        fwd[i] = NttCall{k->plan, j.digits, j.digits, e[i].n_cts * k->dnum, B, B * N, B * N};
+        inv[i] = NttCall{k->plan, j.acc, j.acc, e[i].n_cts * 2, B, B * N, B * N};
+        if ((rc = ks_decompose(j, s))) return rc;
+    }
+    if ((rc = launch_ntt(ctx, fwd.data(), count, true))) return rc;
+    for (size_t i = 0; i < count; ++i)
+        if ((rc = ks_mac(jobs[i], s))) return rc;
+    if ((rc = launch_ntt(ctx, inv.data(), count, false))) return rc;
+    for (size_t i = 0; i < count; ++i)
+        if ((rc = ks_moddown(jobs[i], s))) return rc;
+    return CKKS_OK;
+}
+
 size_t ksk_bytes(const ckks_ksk *k) {
     const size_t entries = 2 * k->dnum * k->limbs * k->degree * sizeof(u64);
     return entries * (1 + k->hoist_keys.size());
@@ -1887,6 +1995,12 @@ int ckks_mod_down(const ckks_rns_plan *qp_plan, size_t q_limbs, const uint64_t *
     return CKKS_OK;
 }
 
+int ckks_ksk_bytes(const ckks_ksk *k, size_t *bytes) {
+    if (!k || !bytes) return fail(CKKS_ERR_INVALID, "null argument");
+    *bytes = ksk_bytes(k);
+    return CKKS_OK;
+}
+
 int ckks_ksk_destroy(ckks_ksk *k) {
     if (!k) return CKKS_OK;
     hipSetDevice(k->ctx->device);
@@ -2024,6 +2138,28 @@ int ckks_apply_galois(const ckks_ksk *key, uint64_t exponent, const uint64_t *in
     const size_t poly = key->q_limbs * key->degree;
     return key_switch_core(key, in + poly, in_stride, in, nullptr, in_stride, out, out_stride, n_cts,
                            galois_inverse(k, two_n));
+}
+
+int ckks_apply_galois_batch(ckks_context *ctx, const ckks_galois_batch *entries, size_t count) {
+    if (!ctx || (count && !entries)) return fail(CKKS_ERR_INVALID, "null argument");
+    std::vector<KsMultiEntry> e;
+    for (size_t i = 0; i < count; ++i) {
+        const ckks_galois_batch &b = entries[i];
+        size_t in_stride = b.in_stride, out_stride = b.out_stride;
+        int rc = check_ct_args(b.key, b.in, b.out, b.n_cts, in_stride, out_stride, 2);
+        if (rc) return rc;
+        if (b.key->ctx != ctx) return fail(CKKS_ERR_INVALID, "batch keys must belong to the context");
+        if (b.n_cts == 0) continue;
+        const u64 two_n = 2 * (u64)b.key->degree;
+        const u64 k = b.exponent % two_n;
+        if ((k & 1) == 0) return fail(CKKS_ERR_INVALID, "CKKS automorphism exponent must be odd modulo 2N");
+        if (b.in == b.out) return fail(CKKS_ERR_INVALID, "rotation output must not alias its input");
+        const size_t poly = b.key->q_limbs * b.key->degree;
+        e.push_back(KsMultiEntry{b.key, b.in + poly, b.in, nullptr, in_stride, in_stride, b.out, out_stride, b.n_cts,
+                                 galois_inverse(k, two_n)});
+    }
+    if (e.empty()) return CKKS_OK;
+    return key_switch_multi(ctx, e.data(), e.size());
 }
 
 int ckks_bounded_apply_galois_hoisted(const ckks_ksk *key, uint64_t exponent, const uint64_t *in,

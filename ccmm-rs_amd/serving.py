@@ -124,6 +124,21 @@ class Program:
         final = regs[self.out_reg] if 0 <= self.out_reg < self.n_regs else 0
         return [(la, final) for la, _ in out] or [(0, final)]
 
+    def footprint(self, n_limbs: int, degree: int) -> int:
+        """Bytes the server charges to admit this program alone: its registers
+        (n_regs x 2 x n_limbs x N x 8) plus each distinct key and plaintext it
+        reads (ckks_server_submit's admission model)."""
+        regs = self.n_regs * 2 * int(n_limbs) * int(degree) * 8
+        seen, res = set(), 0
+        for (code, _d, _a, _b, key, _e, pt), (la, _) in zip(self.ops, self.register_limbs(n_limbs)):
+            if key is not None and code in (OP_ROTATE, OP_MUL_RELIN_RESCALE) and id(key) not in seen:
+                seen.add(id(key))
+                res += key.nbytes
+            if pt is not None and code == OP_MUL_PLAIN and id(pt) not in seen:
+                seen.add(id(pt))
+                res += la * int(degree) * 8
+        return regs + res
+
     def counts(self) -> dict:
         names = {OP_ROTATE: "rotate", OP_MUL_PLAIN: "mul_plain", OP_ADD: "add", OP_SUB: "sub", OP_RESCALE: "rescale",
                  OP_MUL_RELIN_RESCALE: "mul_relin_rescale", OP_COPY: "copy", OP_MOD_SWITCH: "mod_switch"}

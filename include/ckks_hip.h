@@ -420,6 +420,11 @@ int ckks_bounded_rotate_hoisted(const ckks_hoisted *hoisted, const ckks_ksk *key
 int ckks_bounded_hoist_destroy(ckks_hoisted *hoisted);
 
 int ckks_ksk_destroy(ckks_ksk *key);
+/* LogicalPayloadBytes::logical_payload_bytes of a key (ckks/key_accounting.rs:
+ * 13-120): the device bytes of its entries (2 * entries * limbs * N * 8) plus
+ * the permuted copies it caches for hoisted rotations, the figure the
+ * serving loop charges while a request reads the key. */
+int ckks_ksk_bytes(const ckks_ksk *key, size_t *bytes);
 
 /* rns_key_switch_with_prepared_dynamic_ntt (grafting/rns_key_switch.rs:
  * 2526-2608) for an RNS-gadget key, or the dnum hybrid key switch for a
@@ -434,6 +439,23 @@ int ckks_key_switch(const ckks_ksk *key, const uint64_t *in, uint64_t *out, size
  * both components then key switch with a key for sigma_k(s) -> s. */
 int ckks_apply_galois(const ckks_ksk *key, uint64_t exponent, const uint64_t *in, uint64_t *out,
                       size_t n_cts, size_t in_stride, size_t out_stride);
+
+/* Galois key switches of several ciphertext batches, each with its own key,
+ * in one launch sequence: the batched form of
+ * apply_hybrid_rns_galois_automorphism_* / apply_rns_galois_automorphism_*
+ * (ckks/hybrid_rns_galois_key.rs:202-780, ckks/rns_galois_key.rs:439-469),
+ * for ciphertexts of uneven ring degree and level.  Every entry's residues
+ * equal ckks_apply_galois on it alone.  Decomposition, key MAC and ModDown
+ * run per entry; the forward and inverse NTTs of all entries run as one
+ * launch per pass.  At most 16 entries, keys of this context. */
+typedef struct ckks_galois_batch {
+    const ckks_ksk *key;
+    uint64_t exponent;  /* odd modulo 2N */
+    const uint64_t *in; /* [n_cts] ciphertexts over the key's Q limbs */
+    uint64_t *out;      /* must not alias in */
+    size_t n_cts, in_stride, out_stride;  /* strides 0: 2 * q_limbs * N */
+} ckks_galois_batch;
+int ckks_apply_galois_batch(ckks_context *ctx, const ckks_galois_batch *entries, size_t count);
 
 /* rns_relinearize_with_prepared_dynamic_ntt (grafting/rns_key_switch.rs:
  * 2113-2259): (c0,c1,c2) -> (c0,c1) + switch(c2). */

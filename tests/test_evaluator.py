@@ -173,3 +173,34 @@ def test_gpu_evaluator_sequence_matches_oracle(pkg, ctx):
     assert (pkg.to_host(z.data) == cur[:, :L - 2]).all()
     kinds = [ev_.kind for ev_ in trace.events()]
     assert kinds.count(CK.ExecutionEventKind.ROTATE) == 1 and kinds.count(CK.ExecutionEventKind.MOD_SWITCH) == 1
+
+
+def test_evaluation_key_storage_report():
+    """evaluation_key_storage (ckks/key_accounting.rs:276-344) over the level
+    map: per-level multiplication and Galois bytes, totals, and the report
+    text layout."""
+    class FakeKey:
+        def __init__(self, moduli, q_limbs, nbytes):
+            self.plan = type("P", (), {"moduli": moduli})()
+            self.q_limbs = q_limbs
+            self.nbytes = nbytes
+
+    q = [11, 13, 17]
+    keys = CK.RnsCkksEvaluationKeys()
+    l0 = CK.RnsCkksLevelKeys(0, q)
+    l0.set_multiplication_key(FakeKey(q + [19], 3, 1000))
+    l0.insert_galois_key(5, FakeKey(q + [19], 3, 300))
+    l0.insert_galois_key(25, FakeKey(q + [19], 3, 300))
+    l1 = CK.RnsCkksLevelKeys(1, q[:2])
+    l1.insert_galois_key(5, FakeKey(q[:2] + [19], 2, 200))
+    keys.insert_level(l1)
+    keys.insert_level(l0)
+    rep = CK.evaluation_key_storage(keys)
+    assert [lv.level for lv in rep.levels] == [0, 1]
+    assert (rep.levels[0].multiplication_bytes, rep.levels[0].galois_key_count, rep.levels[0].galois_bytes,
+            rep.levels[0].total_resident_bytes) == (1000, 2, 600, 1600)
+    assert (rep.multiplication_bytes, rep.galois_key_count, rep.galois_bytes, rep.total_resident_bytes) == \
+        (1000, 3, 800, 1800)
+    text = rep.to_report_string()
+    assert text.startswith("CKKS_EVALUATION_KEY_STORAGE\nlevel=0 multiplication_bytes=1000 galois_key_count=2 ")
+    assert text.endswith("total_resident_bytes=1800\n")

@@ -128,3 +128,32 @@ def test_keygen_rejections(pkg, ctx):
     # the context still works afterwards
     kb, ka = pkg.keygen(plan, pkg.KEYGEN_HYBRID, 4, src, s, a, e, block_sizes=[2, 2])
     torch.cuda.synchronize()
+
+
+def test_key_bytes_and_storage_report(pkg, ctx):
+    """ckks_ksk_bytes: 2 x entries x limbs x N x 8 device bytes, growing by a
+    permuted copy per exponent cached for hoisted rotations; the level-map
+    report adds them up."""
+    CK = pkg.ckks
+    case = CASES["bounded_qp"]
+    basis, ql, w = case["basis"], case["q_limbs"], case["w"]
+    rng = np.random.default_rng(3)
+    plan = pkg.PreparedRnsNttPlan(ctx, basis, N)
+    count = pkg.bounded_digit_count(plan, ql, w)
+    key = pkg.BoundedKeySwitchKey(plan, ql, w, *[np.stack([np.stack([rng.integers(0, q, N, dtype=np.uint64)
+                                                                        for q in basis]) for _ in range(count)])
+                                                 for _ in range(2)])
+    base = 2 * count * len(basis) * N * 8
+    assert key.nbytes == base
+    ct = pkg.to_device(np.stack([np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in Q]) for _ in range(2)])[None])
+    h = key.hoist(ct)
+    key.rotate_hoisted(h, ct, O.rotation_exponent_left(N, 1))
+    key.rotate_hoisted(h, ct, O.rotation_exponent_left(N, 2))
+    h.close()
+    assert key.nbytes == 3 * base
+    keys = CK.RnsCkksEvaluationKeys()
+    lvl = CK.RnsCkksLevelKeys(0, Q)
+    lvl.insert_galois_key(5, key)
+    keys.insert_level(lvl)
+    rep = CK.evaluation_key_storage(keys)
+    assert rep.galois_bytes == rep.total_resident_bytes == 3 * base

@@ -153,7 +153,9 @@ def test_alone_and_in_mix_bit_identical(pkg, ctx, fx):
         server.close()
     # mix: arrivals interleaved with steps, duplicates of every job, budget
     # for about three jobs at once
-    foot = max(j[1].n_regs for j in jobs) * 2 * len(Q) * N * 8
+    # budget for about three jobs at once: registers plus the keys and
+    # diagonals each reads (shared ones are charged once)
+    foot = max(j[1].footprint(len(Q), N) for j in jobs)
     server = pkg.serving.Server(ctx, hbm_budget_bytes=3 * foot, max_active=3)
     order = np.random.default_rng(5).permutation([0, 1, 2] * 3)
     pending = list(order)
