@@ -74,6 +74,7 @@ _SIGNATURES = {
     "ckks_rns_plan_psi": (C.c_int, [_p, _p]),
     "ckks_ntt_forward": (C.c_int, [_p, _p, _p, _sz, _sz, _sz]),
     "ckks_ntt_inverse": (C.c_int, [_p, _p, _p, _sz, _sz, _sz]),
+    "ckks_ntt_forward_noalloc": (C.c_int, [_p, _p, _p, _sz, _sz, _sz]),
     "ckks_ntt_forward_batch": (C.c_int, [_p, _p, _sz]),
     "ckks_ntt_inverse_batch": (C.c_int, [_p, _p, _sz]),
     "ckks_poly_mul": (C.c_int, [_p, _p, _p, _p, _sz, _sz, _sz]),
@@ -386,6 +387,14 @@ class PreparedRnsNttPlan:
         out = x if out is None else out
         _check(lib().ckks_ntt_forward(self._h, _ptr(x, n_polys * per, name="x"),
                                       _ptr(out, n_polys * per, name="out"), n_polys, n_limbs, per))
+        return out
+
+    def forward_noalloc(self, x, out, n_limbs=None):
+        """ckks_ntt_forward_noalloc: forward() without any device allocation
+        (fixed per-device scratch; out must be given, and may be x)."""
+        n_polys, n_limbs, per = self._shape(x, n_limbs)
+        _check(lib().ckks_ntt_forward_noalloc(self._h, _ptr(x, n_polys * per, name="x"),
+                                              _ptr(out, n_polys * per, name="out"), n_polys, n_limbs, per))
         return out
 
     def inverse(self, x, out=None, n_limbs=None):

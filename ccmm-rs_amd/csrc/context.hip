@@ -68,6 +68,8 @@ int ckks_context_create(int device, void *stream, ckks_context **out) {
     std::unique_ptr<ckks_context> ctx(new ckks_context());
     ctx->device = device;
     ctx->stream = (hipStream_t)stream;
+    int rc = ntt_scratch_lookup(device, &ctx->ntt_scratch);
+    if (rc) return rc;
     *out = ctx.release();
     return CKKS_OK;
 }

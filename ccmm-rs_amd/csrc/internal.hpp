@@ -106,6 +106,7 @@ struct ckks_context {
     ckks_execution_profile profile = {}, captured = {};
     ckks_execution_profile &counts() { return capturing ? captured : profile; }
     std::vector<void *> owned;  // misc device allocations freed at destroy
+    uint64_t *ntt_scratch = nullptr;  // this device's no-allocation NTT scratch
 
     int workspace(int slot, size_t bytes, void **out);
 };
@@ -156,7 +157,13 @@ struct NttCall {
     // NTT-domain Galois permutation by this odd exponent (in != out)
     u64 galois_k = 0;
 };
-int launch_ntt(ckks_context *ctx, const NttCall *calls, size_t count, bool forward);
+// scratch / scratch_elems: an inter-pass buffer supplied by the caller
+// (the batch is cut into chunks that fit it) instead of workspace slot 0.
+int launch_ntt(ckks_context *ctx, const NttCall *calls, size_t count, bool forward, u64 *ext_scratch = nullptr,
+               size_t scratch_elems = 0);
+// Device address of the no-allocation NTT scratch array on the current
+// device (looked up once per device, at context creation).
+int ntt_scratch_lookup(int device, u64 **out);
 int ntt_variant_for(const ckks_context *ctx, u32 logn);
 // Forward NTT of key-switch digits [ct][dnum][limbs][N] fused with the key
 // inner product into acc [ct][2][limbs][N] (two-pass degrees only).

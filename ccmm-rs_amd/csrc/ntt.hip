@@ -11,6 +11,7 @@
 // batch is one chunk, which measured fastest on MI355X.
 #include <algorithm>
 #include <cstdlib>
+#include <mutex>
 
 #include "internal.hpp"
 #include "ntt_pass.hpp"
@@ -193,7 +194,30 @@ int ntt_variant_for(const ckks_context *ctx, u32 logn) {
 }
 
 
-int launch_ntt(ckks_context *ctx, const NttCall *calls, size_t count, bool forward) {
+// The inter-pass buffer of ckks_ntt_forward_noalloc: a fixed array each
+// device holds from the library's load to the process end, shared by every
+// call on that device (calls that overlap on one device race).  Host code
+// uses the address ntt_scratch_lookup returns, never the symbol itself.
+constexpr size_t kNttScratchBytes = 32u << 20;  // 64 limb transforms at N = 2^16
+__device__ __align__(256) u64 scratch[kNttScratchBytes / sizeof(u64)];
+
+int ntt_scratch_lookup(int device, u64 **out) {
+    static std::mutex mu;
+    static std::map<int, u64 *> by_device;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = by_device.find(device);
+    if (it == by_device.end()) {
+        // the caller made `device` current
+        void *p = nullptr;
+        CK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        it = by_device.emplace(device, (u64 *)p).first;
+    }
+    *out = it->second;
+    return CKKS_OK;
+}
+
+int launch_ntt(ckks_context *ctx, const NttCall *calls, size_t count, bool forward, u64 *ext_scratch,
+               size_t scratch_elems) {
     if (count == 0) return CKKS_OK;
     if (count > (size_t)kMaxNttDesc) return fail(CKKS_ERR_INVALID, "NTT batch holds at most 16 entries");
     size_t total_two = 0, max_n = 0;  // residues of all two-pass limb transforms
@@ -260,10 +284,17 @@ int launch_ntt(ckks_context *ctx, const NttCall *calls, size_t count, bool forwa
         size_t cap = ctx->ntt_chunk_bytes / sizeof(u64);
         if (cap == 0 || cap > total_two) cap = total_two;
         if (cap < max_n) cap = max_n;
-        void *ws;
-        int rc = ctx->workspace(0, cap * sizeof(u64), &ws);
-        if (rc) return rc;
-        u64 *tmp = (u64 *)ws;
+        u64 *tmp;
+        if (ext_scratch) {
+            if (scratch_elems < max_n) return fail(CKKS_ERR_INVALID, "one limb transform exceeds the scratch");
+            cap = std::min(cap, scratch_elems);
+            tmp = ext_scratch;
+        } else {
+            void *ws;
+            int rc = ctx->workspace(0, cap * sizeof(u64), &ws);
+            if (rc) return rc;
+            tmp = (u64 *)ws;
+        }
         // one sweep per kernel variant present (the variant is chosen per
         // degree); each sweep walks the limb transforms of its entries in
         // order, closing a chunk when the inter-pass buffer is full
@@ -488,6 +519,17 @@ int ckks_ntt_forward(const ckks_rns_plan *plan, const uint64_t *in, uint64_t *ou
     if (!plan) return fail(CKKS_ERR_INVALID, "null plan");
     NttCall c{plan, in, out, n_polys, n_limbs, poly_stride, poly_stride};
     return launch_ntt(plan->ctx, &c, 1, true);
+}
+
+int ckks_ntt_forward_noalloc(const ckks_rns_plan *plan, const uint64_t *in, uint64_t *out, size_t n_polys,
+                             size_t n_limbs, size_t poly_stride) {
+    if (!plan) return fail(CKKS_ERR_INVALID, "null plan");
+    const ckks_context *ctx = plan->ctx;
+    if (!ctx->ntt_scratch) return fail(CKKS_ERR_INVALID, "no scratch on this device");
+    if (plan->degree * sizeof(u64) > kNttScratchBytes)
+        return fail(CKKS_ERR_INVALID, "one limb transform exceeds the 32 MiB scratch");
+    NttCall c{plan, in, out, n_polys, n_limbs, poly_stride, poly_stride};
+    return launch_ntt(plan->ctx, &c, 1, true, ctx->ntt_scratch, kNttScratchBytes / sizeof(u64));
 }
 
 int ckks_ntt_inverse(const ckks_rns_plan *plan, const uint64_t *in, uint64_t *out,

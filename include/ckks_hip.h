@@ -156,6 +156,18 @@ int ckks_ntt_forward(const ckks_rns_plan *plan, const uint64_t *in, uint64_t *ou
                      size_t n_polys, size_t n_limbs, size_t poly_stride);
 int ckks_ntt_inverse(const ckks_rns_plan *plan, const uint64_t *in, uint64_t *out,
                      size_t n_polys, size_t n_limbs, size_t poly_stride);
+/* No counterpart in the reference: ckks_ntt_forward for callers that cannot
+ * allocate device memory (code under graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments and residues as
+ * ckks_ntt_forward; it allocates nothing: the inter-pass buffer of degrees
+ * above 4096 is a fixed 32 MiB array each device holds for the life of the
+ * process, and the batch runs in chunks of at most 32 MiB of limb
+ * transforms (64 at N = 2^16).  Every call on a device shares that array, so
+ * calls that overlap on one device (other streams, other contexts, other
+ * host threads) race; calls on one stream are ordered.  The array needs no
+ * clearing: each chunk writes it fully before reading it. */
+int ckks_ntt_forward_noalloc(const ckks_rns_plan *plan, const uint64_t *in, uint64_t *out,
+                             size_t n_polys, size_t n_limbs, size_t poly_stride);
 
 /* Mixed-degree / mixed-level batch in one launch per pass (at most 16
  * entries; every entry's plan must belong to the same context). */
