@@ -91,6 +91,7 @@ _SIGNATURES = {
     "ckks_mod_switch": (C.c_int, [_p, _p, _p, _sz, _sz, _sz, _sz]),
     "ckks_tensor": (C.c_int, [_p, _p, _p, _p, _sz, _sz, _sz, _sz]),
     "ckks_multiply_plain": (C.c_int, [_p, _p, _p, _p, _sz, _sz, _sz, _sz, _sz]),
+    "ckks_multiply_plain_accumulate_ntt": (C.c_int, [_p, _p, _p, _sz, _p, _sz]),
     "ckks_graph_begin": (C.c_int, [_p]),
     "ckks_graph_end": (C.c_int, [_p, C.POINTER(_p)]),
     "ckks_graph_launch": (C.c_int, [_p]),
@@ -122,6 +123,8 @@ _SIGNATURES = {
     "ckks_bounded_hoist": (C.c_int, [_p, _p, _sz, _sz, C.POINTER(_p)]),
     "ckks_bounded_rotate_hoisted": (C.c_int, [_p, _p, _u64, _p, _p, _sz, _sz]),
     "ckks_bounded_hoist_destroy": (C.c_int, [_p]),
+    "ckks_hoist": (C.c_int, [_p, _p, _sz, _sz, C.POINTER(_p)]),
+    "ckks_rotate_hoisted": (C.c_int, [_p, _p, _u64, _p, _p, _sz, _sz]),
     "ckks_ksk_destroy": (C.c_int, [_p]),
     "ckks_ksk_bytes": (C.c_int, [_p, C.POINTER(_sz)]),
     "ckks_key_switch": (C.c_int, [_p, _p, _p, _sz, _sz, _sz]),
@@ -243,6 +246,16 @@ class Context:
     @property
     def handle(self):
         return self._h
+
+    @property
+    def stream(self):
+        return self._stream
+
+    def set_stream(self, stream):
+        """ckks_context_set_stream: finish the work queued on the current
+        stream, then run on `stream` (a torch stream)."""
+        _check(lib().ckks_context_set_stream(self._h, _p(stream.cuda_stream or None)))
+        self._stream = stream
 
     def sync(self):
         _check(lib().ckks_sync(self._h))
@@ -527,6 +540,19 @@ class PreparedRnsNttPlan:
                                          2 * n_limbs * n))
         return out
 
+    def multiply_plain_accumulate_ntt(self, cts, pts, n_limbs: int, out=None):
+        """ckks_multiply_plain_accumulate_ntt: sum_t cts[t] * pts[t] over
+        NTT-domain ciphertexts [2][n_limbs][N] and plaintexts [n_limbs][N]."""
+        n = self.degree
+        if not cts or len(cts) != len(pts):
+            raise ValueError("one plaintext per ciphertext, at least one term")
+        out = empty((2, n_limbs, n)) if out is None else out
+        ca = (C.c_uint64 * len(cts))(*[_ptr(c, 2 * n_limbs * n, name="ct") for c in cts])
+        pa = (C.c_uint64 * len(pts))(*[_ptr(p, n_limbs * n, name="pt") for p in pts])
+        _check(lib().ckks_multiply_plain_accumulate_ntt(self._h, C.cast(ca, _p), C.cast(pa, _p), len(cts),
+                                                        _ptr(out, 2 * n_limbs * n, name="out"), int(n_limbs)))
+        return out
+
     def tensor(self, ct0, ct1, n_limbs: int, out=None):
         """rns_tensor_with_prepared_ntt (grafting/rns_key_switch.rs:1724)."""
         n = self.degree
@@ -654,6 +680,22 @@ class _KeySwitchKey:
         ptrs = self._ct_ptrs(n, [("ct0", ct0, 2), ("ct1", ct1, 2)], out, self.q_limbs - 1)
         _check(lib().ckks_multiply_relinearize_rescale(self._h, *ptrs, n,
                                                        0, 0))
+        return out
+
+    def hoist(self, ct) -> "HoistedDigits":
+        """ckks_hoist: digits of the a components of ct cut and transformed
+        once, for rotate_hoisted (RNS-gadget / dnum keys: the hoisted
+        variant's own residues; bounded keys: those of apply_galois)."""
+        return HoistedDigits(self, ct)
+
+    def rotate_hoisted(self, hoisted: "HoistedDigits", ct, exponent: int, out=None):
+        """ckks_rotate_hoisted: the rotation of the hoisted ciphertexts by
+        `exponent` (ct: the ciphertexts hoisted, whose b components are read)."""
+        if self._n_cts(ct, 2) != hoisted.n_cts:
+            raise ValueError("ct must hold the ciphertexts the digits were hoisted from")
+        out = empty((hoisted.n_cts, 2, self.q_limbs, self.plan.degree)) if out is None else out
+        _check(lib().ckks_rotate_hoisted(hoisted.handle, self._h, int(exponent),
+                                         *self._ct_ptrs(hoisted.n_cts, [("ct", ct, 2)], out), 0, 0))
         return out
 
     @property
@@ -872,7 +914,7 @@ class HoistedDigits:
         self.n_cts = key._n_cts(ct, 2)
         h = _p()
         p_ct = _ptr(ct, self.n_cts * 2 * key.q_limbs * key.plan.degree, name="ct")
-        _check(lib().ckks_bounded_hoist(key.handle, p_ct, self.n_cts, 0, C.byref(h)))
+        _check(lib().ckks_hoist(key.handle, p_ct, self.n_cts, 0, C.byref(h)))
         self._h = h
 
     @property

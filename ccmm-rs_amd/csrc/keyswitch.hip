@@ -113,6 +113,9 @@ struct ckks_hoisted {
     const ckks_rns_plan *plan = nullptr;
     size_t q_limbs = 0, digit_count = 0, n_cts = 0;
     unsigned base_log = 0;
+    // RNS-gadget / dnum-hybrid digits (ckks_hoist): the block layout they
+    // were cut with; empty for bounded digits
+    std::vector<size_t> blocks;
     ck::u64 *d_digits = nullptr;  // [ct][digit][B][N], NTT domain
     // (ct, coefficient) whose digit vector holds a -2^(w-1): negating it
     // carries, so a rotation that negates one takes the canonical path
@@ -2233,27 +2236,15 @@ int ckks_bounded_hoist(const ckks_ksk *key, const uint64_t *in, size_t n_cts, si
     return CKKS_OK;
 }
 
-int ckks_bounded_rotate_hoisted(const ckks_hoisted *h, const ckks_ksk *key, uint64_t exponent, const uint64_t *in,
-                                uint64_t *out, size_t in_stride, size_t out_stride) {
-    if (!h) return fail(CKKS_ERR_INVALID, "null argument");
-    const size_t n_cts = h->n_cts;
-    int rc = check_ct_args(key, in, out, n_cts, in_stride, out_stride, 2);
-    if (rc || n_cts == 0) return rc;
-    if (!key->bounded || key->plan != h->plan || key->q_limbs != h->q_limbs || key->base_log != h->base_log ||
-        key->dnum != h->digit_count)
-        return fail(CKKS_ERR_INVALID, "key does not match the hoisted digits (plan, Q limbs, base_log)");
+// The key entries permuted by k^-1 for hoisted rotations by k, built on
+// first use and cached (LRU, kHoistKeyCacheMax copies per key).
+constexpr int kNoRoomForKeyCopy = -1;
+
+static int hoisted_key(const ckks_ksk *key, u64 k, const u64 **out) {
     const u64 two_n = 2 * (u64)key->degree;
-    const u64 k = exponent % two_n;
-    if ((k & 1) == 0) return fail(CKKS_ERR_INVALID, "CKKS automorphism exponent must be odd modulo 2N");
-    if (in == out) return fail(CKKS_ERR_INVALID, "rotation output must not alias its input");
-    // sigma_k maps coefficient s to s k mod 2N, negated when that is >= N; a
-    // negated tie carries, and the hoisted digits would differ from the
-    // decomposition of the rotated a
-    for (const auto &t : h->ties)
-        if (((u64)t.second * k & (two_n - 1)) >= key->degree)
-            return ckks_apply_galois(key, exponent, in, out, n_cts, in_stride, out_stride);
     const u64 kinv = galois_inverse(k, two_n);
     const size_t DBN = key->dnum * key->limbs * key->degree;
+    int rc;
     auto it = key->hoist_keys.find(k);
     auto &lru = key->hoist_lru;
     if (it == key->hoist_keys.end()) {
@@ -2269,9 +2260,8 @@ int ckks_bounded_rotate_hoisted(const ckks_hoisted *h, const ckks_ksk *key, uint
         }
         u64 *pk = nullptr;
         if (hipMalloc(&pk, 2 * DBN * sizeof(u64)) != hipSuccess) {
-            // no room for a permuted copy: the canonical rotation needs none
             (void)hipGetLastError();
-            return ckks_apply_galois(key, exponent, in, out, n_cts, in_stride, out_stride);
+            return kNoRoomForKeyCopy;
         }
         if ((rc = launch_ntt_permute(ctx->stream, key->d_kb, pk, kinv, DBN, key->plan->logn)) ||
             (rc = launch_ntt_permute(ctx->stream, key->d_ka, pk + DBN, kinv, DBN, key->plan->logn))) {
@@ -2283,15 +2273,104 @@ int ckks_bounded_rotate_hoisted(const ckks_hoisted *h, const ckks_ksk *key, uint
         lru.erase(std::find(lru.begin(), lru.end(), k));
     }
     lru.insert(lru.begin(), k);
-    const HoistedMac hm{h->d_digits, it->second, it->second + DBN, k};
+    *out = it->second;
+    return CKKS_OK;
+}
+
+// The key switch of a hoisted rotation by k: MAC over the key permuted by
+// k^-1 (pk), inverse NTT reading the MAC output permuted by k, ModDown with
+// sigma_k(b) passed through.
+static int rotate_from_hoisted(const ckks_hoisted *h, const ckks_ksk *key, u64 k, const u64 *pk, const u64 *in,
+                               u64 *out, size_t in_stride, size_t out_stride) {
+    const size_t DBN = key->dnum * key->limbs * key->degree;
+    const HoistedMac hm{h->d_digits, pk, pk + DBN, k};
     const size_t poly = key->q_limbs * key->degree;
-    return key_switch_core(key, in + poly, in_stride, in, nullptr, in_stride, out, out_stride, n_cts, kinv,
-                           nullptr, &hm);
+    return key_switch_core(key, in + poly, in_stride, in, nullptr, in_stride, out, out_stride, h->n_cts,
+                           galois_inverse(k, 2 * (u64)key->degree), nullptr, &hm);
+}
+
+int ckks_bounded_rotate_hoisted(const ckks_hoisted *h, const ckks_ksk *key, uint64_t exponent, const uint64_t *in,
+                                uint64_t *out, size_t in_stride, size_t out_stride) {
+    if (!h) return fail(CKKS_ERR_INVALID, "null argument");
+    const size_t n_cts = h->n_cts;
+    int rc = check_ct_args(key, in, out, n_cts, in_stride, out_stride, 2);
+    if (rc || n_cts == 0) return rc;
+    if (!key->bounded || !h->blocks.empty() || key->plan != h->plan || key->q_limbs != h->q_limbs ||
+        key->base_log != h->base_log || key->dnum != h->digit_count)
+        return fail(CKKS_ERR_INVALID, "key does not match the hoisted digits (plan, Q limbs, base_log)");
+    const u64 two_n = 2 * (u64)key->degree;
+    const u64 k = exponent % two_n;
+    if ((k & 1) == 0) return fail(CKKS_ERR_INVALID, "CKKS automorphism exponent must be odd modulo 2N");
+    if (in == out) return fail(CKKS_ERR_INVALID, "rotation output must not alias its input");
+    // sigma_k maps coefficient s to s k mod 2N, negated when that is >= N; a
+    // negated tie carries, and the hoisted digits would differ from the
+    // decomposition of the rotated a
+    for (const auto &t : h->ties)
+        if (((u64)t.second * k & (two_n - 1)) >= key->degree)
+            return ckks_apply_galois(key, exponent, in, out, n_cts, in_stride, out_stride);
+    const u64 *pk = nullptr;
+    rc = hoisted_key(key, k, &pk);
+    if (rc == kNoRoomForKeyCopy)  // the canonical rotation needs no permuted copy, same residues
+        return ckks_apply_galois(key, exponent, in, out, n_cts, in_stride, out_stride);
+    if (rc) return rc;
+    return rotate_from_hoisted(h, key, k, pk, in, out, in_stride, out_stride);
 }
 
 int ckks_bounded_hoist_destroy(ckks_hoisted *h) {
     delete h;
     return CKKS_OK;
+}
+
+int ckks_hoist(const ckks_ksk *key, const uint64_t *in, size_t n_cts, size_t in_stride, ckks_hoisted **out) {
+    if (!key || !out || (n_cts && !in)) return fail(CKKS_ERR_INVALID, "null argument");
+    if (key->bounded) return ckks_bounded_hoist(key, in, n_cts, in_stride, out);
+    ckks_context *ctx = key->ctx;
+    if (ctx->capturing) return fail(CKKS_ERR_INVALID, "hoisting allocates: not capturable");
+    const size_t N = key->degree, B = key->limbs, D = key->dnum, poly = key->q_limbs * N;
+    if (in_stride == 0) in_stride = 2 * poly;
+    if (in_stride < 2 * poly) return fail(CKKS_ERR_INVALID, "ciphertext stride smaller than its limbs");
+    std::unique_ptr<ckks_hoisted> h(new ckks_hoisted());
+    h->ctx = ctx;
+    h->plan = key->plan;
+    h->q_limbs = key->q_limbs;
+    h->digit_count = D;
+    h->n_cts = n_cts;
+    h->blocks = key->block_size;
+    if (n_cts) {
+        CK_HIP(hipSetDevice(ctx->device));
+        CK_HIP(hipMalloc(&h->d_digits, n_cts * D * B * N * sizeof(u64)));
+        // digits of the unrotated a: the same decomposition as a key switch
+        KsJob j{key, in + poly, in_stride, nullptr, nullptr, 0, nullptr, 0, n_cts, 0, nullptr, nullptr};
+        j.digits = h->d_digits;
+        j.mfma = ks_uses_mfma(key);
+        int rc = ks_decompose(j, ctx->stream);
+        if (rc) return rc;
+        NttCall f{key->plan, h->d_digits, h->d_digits, n_cts * D, B, B * N, B * N};
+        if ((rc = launch_ntt(ctx, &f, 1, true))) return rc;
+    }
+    *out = h.release();
+    return CKKS_OK;
+}
+
+int ckks_rotate_hoisted(const ckks_hoisted *h, const ckks_ksk *key, uint64_t exponent, const uint64_t *in,
+                        uint64_t *out, size_t in_stride, size_t out_stride) {
+    if (!h) return fail(CKKS_ERR_INVALID, "null argument");
+    if (key && key->bounded) return ckks_bounded_rotate_hoisted(h, key, exponent, in, out, in_stride, out_stride);
+    const size_t n_cts = h->n_cts;
+    int rc = check_ct_args(key, in, out, n_cts, in_stride, out_stride, 2);
+    if (rc || n_cts == 0) return rc;
+    if (h->blocks.empty() || key->plan != h->plan || key->q_limbs != h->q_limbs || key->block_size != h->blocks)
+        return fail(CKKS_ERR_INVALID, "key does not match the hoisted digits (plan, Q limbs, gadget blocks)");
+    const u64 two_n = 2 * (u64)key->degree;
+    const u64 k = exponent % two_n;
+    if ((k & 1) == 0) return fail(CKKS_ERR_INVALID, "CKKS automorphism exponent must be odd modulo 2N");
+    if (in == out) return fail(CKKS_ERR_INVALID, "rotation output must not alias its input");
+    const u64 *pk = nullptr;
+    rc = hoisted_key(key, k, &pk);
+    // no canonical fallback here: it gives other residues
+    if (rc == kNoRoomForKeyCopy) return fail(CKKS_ERR_OOM, "no device memory for the permuted key copy");
+    if (rc) return rc;
+    return rotate_from_hoisted(h, key, k, pk, in, out, in_stride, out_stride);
 }
 
 int ckks_relinearize(const ckks_ksk *key, const uint64_t *in3, uint64_t *out, size_t n_cts,

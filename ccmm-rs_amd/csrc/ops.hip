@@ -167,6 +167,40 @@ static int permute(bool ntt_domain, const ckks_rns_plan *plan, const u64 *in, u6
     return CKKS_OK;
 }
 
+// ---- NTT-resident plaintext multiply-accumulate --------------------------------
+
+constexpr u32 kMaxMacTerms = 64;
+
+struct MacListArgs {
+    const u64 *ct[kMaxMacTerms];  // NTT-domain ciphertexts [2][limbs][N]
+    const u64 *pt[kMaxMacTerms];  // NTT-domain plaintexts [limbs][N]
+    u64 *out;                     // [2][limbs][N]
+    const DevPrime *primes;
+    u32 count, n_limbs, logn, chunks;
+};
+
+// out = sum_t ct_t * pt_t, limb by limb and evaluation point by point
+// (NttPolynomial::pointwise_mul_add_assign, ring/ntt_polynomial.rs:222-235,
+// over the NTT-resident ciphertexts of ckks/rns_plain_multiply.rs:119-191).
+__global__ __launch_bounds__(kEwThreads) void k_mac_list(MacListArgs a) {
+    const u32 b = blockIdx.x;
+    const u32 chunk = b % a.chunks;
+    const u32 pl = b / a.chunks;  // component * limbs + limb
+    const u32 limb = pl % a.n_limbs;
+    const DevPrime P = a.primes[limb];
+    const Barrett br{P.q, P.br_hi, P.br_lo};
+    const u64 off = ((u64)pl << a.logn) + (u64)chunk * kEwChunk + threadIdx.x * kEwPerThread;
+    const u64 poff = ((u64)limb << a.logn) + (u64)chunk * kEwChunk + threadIdx.x * kEwPerThread;
+    u64 acc[kEwPerThread] = {};
+    for (u32 t = 0; t < a.count; ++t) {
+#pragma unroll
+        for (int i = 0; i < kEwPerThread; ++i)
+            acc[i] = add_mod(acc[i], mul_mod(a.ct[t][off + i], a.pt[t][poff + i], br), P.q);
+    }
+#pragma unroll
+    for (int i = 0; i < kEwPerThread; ++i) a.out[off + i] = acc[i];
+}
+
 // ---- monomials and scalars -----------------------------------------------------
 
 struct MonoArgs {
@@ -867,6 +901,31 @@ int ckks_poly_sub(const ckks_rns_plan *plan, const uint64_t *x, const uint64_t *
 int ckks_poly_neg(const ckks_rns_plan *plan, const uint64_t *x, uint64_t *out, size_t n_polys,
                   size_t n_limbs, size_t poly_stride) {
     return elementwise(EW_NEG, plan, x, nullptr, out, n_polys, n_limbs, poly_stride);
+}
+
+int ckks_multiply_plain_accumulate_ntt(const ckks_rns_plan *plan, const uint64_t *const *cts,
+                                       const uint64_t *const *pts, size_t count, uint64_t *out, size_t n_limbs) {
+    int rc = check_plan_limbs(plan, n_limbs, "plaintext multiply-accumulate");
+    if (rc) return rc;
+    if (!cts || !pts || !out || count == 0) return fail(CKKS_ERR_INVALID, "null argument");
+    if (count > kMaxMacTerms) return fail(CKKS_ERR_UNSUPPORTED, "plaintext multiply-accumulate of at most 64 terms");
+    MacListArgs a{};
+    for (size_t t = 0; t < count; ++t) {
+        if (!cts[t] || !pts[t]) return fail(CKKS_ERR_INVALID, "null buffer");
+        a.ct[t] = cts[t];
+        a.pt[t] = pts[t];
+    }
+    a.out = out;
+    a.primes = plan->d_primes;
+    a.count = (u32)count;
+    a.n_limbs = (u32)n_limbs;
+    a.logn = plan->logn;
+    a.chunks = (u32)(plan->degree / kEwChunk);
+    hipLaunchKernelGGL(k_mac_list, dim3((u32)(2 * n_limbs * a.chunks)), dim3(kEwThreads), 0, plan->ctx->stream, a);
+    CK_HIP(hipGetLastError());
+    plan->ctx->counts().ciphertext_plaintext_multiplies += count;
+    plan->ctx->counts().additions += 2 * (count - 1);
+    return CKKS_OK;
 }
 
 int ckks_ntt_mul_monomial(const ckks_rns_plan *plan, const uint64_t *in, uint64_t *out, size_t n_polys,

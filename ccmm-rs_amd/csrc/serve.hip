@@ -141,6 +141,33 @@ class HipExec {
         const size_t per = 2 * L * N, per_out = 2 * (L - 1) * N;
         const ckks_rns_plan *plan = (const ckks_rns_plan *)g.plan;
         int rc;
+        if (g.code == serve::OP_MAC_PLAIN) {
+            // per member (each has its own plaintext table), straight on the
+            // registers: every output element is written by the thread that
+            // read its sources, so dst may be one of them
+            for (auto &m : g.members) {
+                const serve::Op &o = *m.second;
+                std::vector<const u64 *> src(o.b);
+                for (uint32_t t = 0; t < o.b; ++t) src[t] = reg(*m.first, o.a + t);
+                int r = ckks_multiply_plain_accumulate_ntt(plan, src.data(), o.pts, o.b, reg(*m.first, o.dst), L);
+                if (r) return r;
+            }
+            return CKKS_OK;
+        }
+        if (g.code == serve::OP_NTT || g.code == serve::OP_INTT) {
+            // straight on the registers, up to 16 requests per launch
+            for (size_t i0 = 0; i0 < n; i0 += 16) {
+                std::vector<NttCall> calls;
+                for (size_t i = i0; i < std::min(n, i0 + 16); ++i) {
+                    const serve::Op &o = *g.members[i].second;
+                    calls.push_back(NttCall{plan, reg(*g.members[i].first, o.a), reg(*g.members[i].first, o.dst),
+                                            2, L, L * N, L * N});
+                }
+                int r = launch_ntt(ctx_, calls.data(), calls.size(), g.code == serve::OP_NTT);
+                if (r) return r;
+            }
+            return CKKS_OK;
+        }
         if (g.code == serve::OP_COPY) {
             std::vector<const u64 *> s(n);
             std::vector<u64 *> d(n);
@@ -304,7 +331,19 @@ int ckks_server_submit(ckks_server *s, const ckks_job *job, uint64_t *ticket) {
     for (size_t i = 0; i < job->n_ops; ++i) {
         const ckks_op &o = job->ops[i];
         j.ops[i] = serve::Op{o.code, o.dst, o.a, o.b, o.key, o.exponent, o.plaintext};
+        if (o.code == serve::OP_MAC_PLAIN && o.plaintext && o.b && o.b <= serve::kMaxMacTerms) {
+            // the host table of the terms' plaintext pointers, copied
+            const auto *tab = reinterpret_cast<const uint64_t *const *>(o.plaintext);
+            j.tables.emplace_back(tab, tab + o.b);
+            j.ops[i].plaintext = nullptr;
+        }
     }
+    // tables are complete (no more reallocation of the outer vector): point
+    // the ops at them
+    for (size_t i = 0, t = 0; i < job->n_ops; ++i)
+        if (job->ops[i].code == serve::OP_MAC_PLAIN && job->ops[i].plaintext && job->ops[i].b &&
+            job->ops[i].b <= serve::kMaxMacTerms)
+            j.ops[i].pts = j.tables[t++].data();
     auto lookup = [&](const void *k) -> const serve::KeyInfo * {
         auto it = s->keys.find(k);
         if (it != s->keys.end()) return &it->second;

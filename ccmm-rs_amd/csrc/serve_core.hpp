@@ -25,6 +25,8 @@
 namespace ck {
 namespace serve {
 
+constexpr uint32_t kMaxMacTerms = 64;
+
 enum OpCode : uint32_t {
     OP_ROTATE = 1,             // dst = key switch of sigma_exponent(a)
     OP_MUL_PLAIN = 2,          // dst = a * plaintext (NTT-domain, a's level)
@@ -34,13 +36,22 @@ enum OpCode : uint32_t {
     OP_COPY = 6,               // dst = a
     OP_MOD_SWITCH = 7,         // dst = a without its last limb (scale kept)
     OP_SUB = 8,                // dst = a - b (same level)
+    OP_NTT = 9,                // dst = NTT(a): the register moves to the NTT domain
+    OP_INTT = 10,              // dst = NTT^-1(a): back to the coefficient domain
+    OP_MAC_PLAIN = 11,         // dst = sum_{i < b} reg[a + i] * pts[i], NTT domain
 };
+
+// Coefficient-domain registers feed every op except OP_INTT and
+// OP_MAC_PLAIN, which read NTT-domain ones; ADD / SUB / COPY keep the
+// domain of their operands; a program's output is coefficient domain.
+enum RegDomain : uint8_t { DOM_COEFF = 0, DOM_NTT = 1 };
 
 struct Op {
     uint32_t code, dst, a, b;
     const void *key;  // ROTATE: Galois key, MUL_RELIN_RESCALE: relinearization key
     uint64_t exponent;
     const uint64_t *plaintext;
+    const uint64_t *const *pts = nullptr;  // OP_MAC_PLAIN: b NTT-domain plaintexts (job-owned table)
 };
 
 // Key facts the validator needs (the executor resolves them from the key).
@@ -66,6 +77,7 @@ struct Job {
     uint64_t *output;
     std::vector<Op> ops;
     std::vector<Resource> resident;  // distinct keys and plaintexts the ops read
+    std::vector<std::vector<const uint64_t *>> tables;  // storage behind Op::pts
 };
 
 struct Request {
@@ -96,12 +108,22 @@ std::string validate(const Job &j, KeyLookup &&key_info) {
     if (j.n_limbs == 0 || j.degree == 0) return "job ciphertexts need limbs";
     if (j.n_inputs == 0 || j.n_inputs > j.n_regs) return "inputs must fit the register file";
     std::vector<size_t> limbs(j.n_regs, 0);
+    std::vector<uint8_t> dom(j.n_regs, DOM_COEFF);
     for (size_t i = 0; i < j.n_inputs; ++i) limbs[i] = j.n_limbs;
     for (size_t i = 0; i < j.ops.size(); ++i) {
         const Op &o = j.ops[i];
         const std::string at = " (op " + std::to_string(i) + ")";
         if (o.dst >= j.n_regs || o.a >= j.n_regs) return "register index outside the register file" + at;
         if (limbs[o.a] == 0) return "op reads a register that holds no ciphertext" + at;
+        const bool ntt_in = o.code == OP_INTT || o.code == OP_MAC_PLAIN;
+        const bool any_in = o.code == OP_ADD || o.code == OP_SUB || o.code == OP_COPY;
+        if (!any_in && dom[o.a] != (ntt_in ? DOM_NTT : DOM_COEFF))
+            return std::string(ntt_in ? "op needs an NTT-domain ciphertext" : "op needs a coefficient-domain ciphertext") + at;
+        if ((o.code == OP_ADD || o.code == OP_SUB) && o.b < j.n_regs && dom[o.b] != dom[o.a])
+            return "addition needs two ciphertexts in one domain" + at;
+        uint8_t dom_out = o.code == OP_NTT || o.code == OP_MAC_PLAIN ? DOM_NTT
+                          : any_in                                      ? dom[o.a]
+                                                                        : DOM_COEFF;
         switch (o.code) {
             case OP_ROTATE: {
                 if (!o.key || !(o.exponent & 1)) return "rotation needs a key and an odd exponent" + at;
@@ -136,11 +158,27 @@ std::string validate(const Job &j, KeyLookup &&key_info) {
                 limbs[o.dst] = limbs[o.a] - 1;
                 break;
             }
-            case OP_COPY: limbs[o.dst] = limbs[o.a]; break;
+            case OP_COPY:
+            case OP_NTT:
+            case OP_INTT: limbs[o.dst] = limbs[o.a]; break;
+            case OP_MAC_PLAIN: {
+                if (o.b == 0 || (size_t)o.a + o.b > j.n_regs || !o.pts)
+                    return "plaintext multiply-accumulate needs 1.. registers and a plaintext table" + at;
+                if (o.b > kMaxMacTerms) return "plaintext multiply-accumulate of at most 64 terms" + at;
+                for (uint32_t t = 0; t < o.b; ++t) {
+                    if (limbs[o.a + t] != limbs[o.a] || dom[o.a + t] != DOM_NTT)
+                        return "plaintext multiply-accumulate needs NTT-domain registers at one level" + at;
+                    if (!o.pts[t]) return "plaintext multiply-accumulate needs a plaintext per term" + at;
+                }
+                limbs[o.dst] = limbs[o.a];
+                break;
+            }
             default: return "unknown op code" + at;
         }
+        dom[o.dst] = dom_out;
     }
     if (limbs[j.out_reg] == 0) return "output register holds no ciphertext";
+    if (dom[j.out_reg] != DOM_COEFF) return "output register must be coefficient domain";
     return "";
 }
 
@@ -163,6 +201,8 @@ std::vector<Resource> resident_set(const Job &j, KeyLookup &&key_info) {
             add(o.key, ki ? ki->bytes : 0);
         }
         if (o.code == OP_MUL_PLAIN && o.plaintext) add(o.plaintext, la * j.degree * sizeof(uint64_t));
+        if (o.code == OP_MAC_PLAIN && o.pts)
+            for (uint32_t t = 0; t < o.b; ++t) add(o.pts[t], la * j.degree * sizeof(uint64_t));
         limbs[o.dst] = (o.code == OP_RESCALE || o.code == OP_MOD_SWITCH || o.code == OP_MUL_RELIN_RESCALE) ? la - 1 : la;
     }
     return out;

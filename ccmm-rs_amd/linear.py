@@ -71,11 +71,43 @@ def compose_diagonals(A: dict, B: dict, d: int) -> dict:
 
 
 class Builder:
-    def __init__(self, slots: int, key_for: Callable, encode: Callable, moduli):
+    """ntt_resident (default): every plaintext-diagonal inner sum runs in the
+    NTT domain -- the rotated copies are transformed once each, one
+    multiply-accumulate launch forms sum_i diag_i * rot_i(x), and one
+    inverse transform returns it -- instead of a forward transform, a
+    product and an inverse transform per diagonal.  Both forms give the same
+    residues (exact modular arithmetic, linear transforms)."""
+
+    def __init__(self, slots: int, key_for: Callable, encode: Callable, moduli, ntt_resident: bool = True):
         self.S = slots
         self.key_for = key_for
         self.encode = encode
         self.moduli = list(moduli)
+        self.ntt_resident = ntt_resident
+        self._zeros = {}
+
+    def _zero_pt(self, limbs):
+        if limbs not in self._zeros:
+            self._zeros[limbs] = self._pt(np.zeros(self.S, dtype=np.complex128), limbs)
+        return self._zeros[limbs]
+
+    def _mac_terms(self, prog: Program, x, y, terms: list, limbs: int):
+        """y = sum over (offset, slot values) terms of values * rot_offset(x),
+        NTT-resident: each rotated copy is transformed in a scratch register,
+        one multiply-accumulate, one inverse transform, then one rescale."""
+        regs = prog.scratch(len(terms))
+        pts, vals = [], []
+        for r, (o, v) in zip(regs, terms):
+            if o % self.S == 0:
+                prog.ntt(r, x)
+            else:
+                self._rot(prog, r, x, o, limbs)
+                prog.ntt(r, r)
+            pts.append(self._pt(v, limbs))
+            vals.append(v)
+        prog.mac_plain(y, regs[0], pts, values=vals)
+        prog.intt(y, y)
+        prog.rescale(y, y)
 
     def _rot(self, prog: Program, dst, src, steps, limbs):
         steps %= self.S
@@ -90,6 +122,9 @@ class Builder:
 
     def transform_direct(self, prog: Program, x, y, tmp, diags: dict, period: int, limbs: int):
         """y = M x from M's (few) diagonals, then one rescale (in place on y)."""
+        if self.ntt_resident:
+            return self._mac_terms(prog, x, y, [(i, tile(v, period, self.S)) for i, v in sorted(diags.items())],
+                                   limbs)
         first = True
         for i, v in sorted(diags.items()):
             vals = tile(v, period, self.S)
@@ -112,6 +147,9 @@ class Builder:
         unless s + o stays inside s's own row / block: no cyclic rotation
         couples independent rows (the masks of the reference's packed FFT2
         stages, eblas/fft.rs:2700-2800).  One rescale at the end."""
+        if self.ntt_resident:
+            return self._mac_terms(prog, x, y, [(o, np.asarray(v, dtype=np.complex128))
+                                                for o, v in sorted(sdiags.items())], limbs)
         first = True
         for o, vals in sorted(sdiags.items()):
             vals = np.asarray(vals, dtype=np.complex128)
@@ -140,13 +178,33 @@ class Builder:
         assert len(regs) >= g + 2
         baby, acc, tmp = regs[:g], regs[g], regs[g + 1]
         used_b = sorted({(i // unit) % g for i in offs})
+        if self.ntt_resident:
+            assert list(baby) == list(range(baby[0], baby[0] + g)), "baby-step registers must be consecutive"
+            used_b = list(range(used_b[0], used_b[-1] + 1))  # a multiply-accumulate reads a register range
         for b in used_b:
             self._rot(prog, baby[b], x, b * unit, limbs)
+            if self.ntt_resident:
+                prog.ntt(baby[b], baby[b])
         first_y = True
         for k in range((span + g - 1) // g):
             terms = [(b, diags[(g * k + b) * unit]) for b in range(g) if (g * k + b) * unit in diags]
             if not terms:
                 continue
+            if self.ntt_resident:
+                lo, hi = terms[0][0], terms[-1][0]
+                have = dict(terms)
+                pts, vals = [], []
+                for b in range(lo, hi + 1):
+                    if b in have:
+                        v = np.roll(tile(have[b], period, self.S), g * k * unit)
+                        pts.append(self._pt(v, limbs))
+                    else:
+                        v = np.zeros(self.S, dtype=np.complex128)
+                        pts.append(self._zero_pt(limbs))
+                    vals.append(v)
+                prog.mac_plain(acc, baby[lo], pts, values=vals)
+                prog.intt(acc, acc)
+                terms = []
             for t, (b, v) in enumerate(terms):
                 # rot_{-gk}(diag) so the giant rotation lands it on diag
                 vals = np.roll(tile(v, period, self.S), g * k * unit)

@@ -16,6 +16,7 @@ from typing import Optional
 from . import PreparedRnsNttPlan, _check, _p, _ptr, _sz, lib
 
 OP_ROTATE, OP_MUL_PLAIN, OP_ADD, OP_RESCALE, OP_MUL_RELIN_RESCALE, OP_COPY, OP_MOD_SWITCH, OP_SUB = 1, 2, 3, 4, 5, 6, 7, 8
+OP_NTT, OP_INTT, OP_MAC_PLAIN = 9, 10, 11
 
 
 class _Op(C.Structure):
@@ -49,6 +50,18 @@ class Program:
     keep: list = field(default_factory=list)
 
     meta: list = field(default_factory=list)  # per op: slot rotation or plaintext slot values
+    _scratch_base: int = -1
+    _scratch_n: int = 0
+
+    def scratch(self, k: int) -> list:
+        """k scratch registers after the program's own (one shared area,
+        grown as needed; consecutive indices)."""
+        if self._scratch_base < 0:
+            self._scratch_base = self.n_regs
+        if k > self._scratch_n:
+            self._scratch_n = k
+            self.n_regs = self._scratch_base + k
+        return list(range(self._scratch_base, self._scratch_base + k))
 
     def rotate(self, dst, a, key, exponent, steps=None):
         self.ops.append((OP_ROTATE, dst, a, 0, key, int(exponent), None))
@@ -63,6 +76,24 @@ class Program:
     def add(self, dst, a, b):
         self.ops.append((OP_ADD, dst, a, b, None, 0, None))
         self.meta.append(None)
+
+    def ntt(self, dst, a):
+        """dst = NTT(a): the register moves to the NTT domain."""
+        self.ops.append((OP_NTT, dst, a, 0, None, 0, None))
+        self.meta.append(None)
+
+    def intt(self, dst, a):
+        self.ops.append((OP_INTT, dst, a, 0, None, 0, None))
+        self.meta.append(None)
+
+    def mac_plain(self, dst, first, pts, values=None):
+        """dst = sum_t reg[first + t] * pts[t] over NTT-domain registers and
+        NTT-domain plaintexts (ckks_multiply_plain_accumulate_ntt): one
+        launch for a whole baby-step inner sum."""
+        pts = list(pts)
+        self.keep.extend(p for p in pts if p is not None)
+        self.ops.append((OP_MAC_PLAIN, dst, first, len(pts), None, 0, pts))
+        self.meta.append(values)
 
     def sub(self, dst, a, b):
         self.ops.append((OP_SUB, dst, a, b, None, 0, None))
@@ -102,6 +133,8 @@ class Program:
                 regs[dst] = regs[a] + regs[b]
             elif code == OP_SUB:
                 regs[dst] = regs[a] - regs[b]
+            elif code == OP_MAC_PLAIN:
+                regs[dst] = sum(regs[a + t] * meta[t] for t in range(b))
             elif code == OP_MUL_RELIN_RESCALE:
                 regs[dst] = regs[a] * regs[b]
             else:
@@ -134,13 +167,15 @@ class Program:
             if key is not None and code in (OP_ROTATE, OP_MUL_RELIN_RESCALE) and id(key) not in seen:
                 seen.add(id(key))
                 res += key.nbytes
-            if pt is not None and code == OP_MUL_PLAIN and id(pt) not in seen:
-                seen.add(id(pt))
-                res += la * int(degree) * 8
+            for p in (pt if code == OP_MAC_PLAIN else [pt] if code == OP_MUL_PLAIN else []):
+                if p is not None and id(p) not in seen:
+                    seen.add(id(p))
+                    res += la * int(degree) * 8
         return regs + res
 
     def counts(self) -> dict:
         names = {OP_ROTATE: "rotate", OP_MUL_PLAIN: "mul_plain", OP_ADD: "add", OP_SUB: "sub", OP_RESCALE: "rescale",
+                 OP_NTT: "ntt", OP_INTT: "intt", OP_MAC_PLAIN: "mac_plain",
                  OP_MUL_RELIN_RESCALE: "mul_relin_rescale", OP_COPY: "copy", OP_MOD_SWITCH: "mod_switch"}
         out: dict = {}
         for op in self.ops:
@@ -162,15 +197,22 @@ class Server:
         ops = (_Op * max(len(program.ops), 1))()
         n = plan.degree
         limbs = program.register_limbs(int(n_limbs))
+        tables = []
         for i, (code, dst, a, b, key, exp, pt) in enumerate(program.ops):
-            ops[i] = _Op(code, dst, a, b, key.handle if key is not None else None, exp,
-                         _ptr(pt, limbs[i][0] * n, name="plaintext") if pt is not None else None)
+            if code == OP_MAC_PLAIN:
+                # a host array of the terms' device pointers (copied at submission)
+                tab = (C.c_uint64 * len(pt))(*[_ptr(p, limbs[i][0] * n, name="plaintext") for p in pt])
+                tables.append(tab)
+                ptr = C.cast(tab, _p)
+            else:
+                ptr = _ptr(pt, limbs[i][0] * n, name="plaintext") if pt is not None else None
+            ops[i] = _Op(code, dst, a, b, key.handle if key is not None else None, exp, ptr)
         job = _Job(plan.handle, int(n_limbs), program.n_regs, program.out_reg, program.n_inputs,
                    _ptr(inputs, program.n_inputs * 2 * int(n_limbs) * n, name="inputs"),
                    _ptr(output, 2 * limbs[-1][1] * n, name="output"), C.cast(ops, _p), len(program.ops))
         t = C.c_uint64()
         _check(_fn("ckks_server_submit")(self._h, C.addressof(job), C.byref(t)))
-        self._jobs[t.value] = (job, ops, program, inputs, output)
+        self._jobs[t.value] = (job, ops, program, inputs, output, tables)
         return t.value
 
     def step(self) -> int:
@@ -209,3 +251,124 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+class ProgramRunner:
+    """Runs one Program directly -- one library call per op on registers it
+    owns -- outside the serving loop, so that a whole rotation schedule can
+    be captured once as a hipGraph (capture()) and replayed (replay())
+    without per-op host work.  Registers hold compact [2][limbs][N]
+    ciphertexts at their current level (as the server's).  The program's
+    keys must belong to the plan's context; each run switches that context
+    to the runner's own (non-null, default-priority) stream and back, which
+    waits for the run.  The first run() sizes every library workspace, so
+    run() must precede capture(); nothing is allocated while capturing."""
+
+    def __init__(self, plan: PreparedRnsNttPlan, n_limbs: int, program: Program, stream=None):
+        import torch
+
+        self.ctx, self.plan, self.prog = plan.ctx, plan, program
+        self.stream = stream if stream is not None else torch.cuda.Stream()
+        self.L = int(n_limbs)
+        n = plan.degree
+        self.per = 2 * self.L * n
+        # two extra registers: the output of ops that cannot run in place
+        self.regs = torch.empty((program.n_regs + 2, self.per), dtype=torch.int64, device="cuda")
+        self.tmp = program.n_regs
+        self.limbs = program.register_limbs(self.L)
+        self.graph = None
+
+    def _reg(self, i, limbs):
+        return self.regs[i, : 2 * limbs * self.plan.degree]
+
+    def _ct(self, i, limbs):
+        return self._reg(i, limbs).view(1, 2, limbs, self.plan.degree)
+
+    def _run_ops(self):
+        plan, n = self.plan, self.plan.degree
+        regs = [self.L if i < self.prog.n_inputs else 0 for i in range(self.prog.n_regs)]
+        for (code, dst, a, b, key, exp, pt) in self.prog.ops:
+            la = regs[a]
+            out_l = la - 1 if code in (OP_RESCALE, OP_MOD_SWITCH, OP_MUL_RELIN_RESCALE) else la
+            # ops whose output may not alias an input write the spare register
+            spare = code in (OP_ROTATE, OP_RESCALE, OP_MOD_SWITCH, OP_MUL_RELIN_RESCALE) and \
+                (dst == a or (code == OP_MUL_RELIN_RESCALE and dst == b))
+            d = self.tmp if spare else dst
+            if code == OP_ROTATE:
+                key.apply_galois(self._ct(a, la), exp, out=self._ct(d, la))
+            elif code == OP_MUL_PLAIN:
+                plan.multiply_plain(self._ct(a, la), pt, la, out=self._ct(d, la))
+            elif code in (OP_ADD, OP_SUB):
+                (plan.add if code == OP_ADD else plan.sub)(self._reg(a, la), self._reg(b, la),
+                                                           out=self._reg(d, la), n_limbs=la)
+            elif code == OP_RESCALE:
+                plan.rescale_to_next(self._ct(a, la), la, out=self._ct(d, out_l))
+            elif code == OP_MOD_SWITCH:
+                plan.mod_switch_to_next(self._ct(a, la), la, out=self._ct(d, out_l))
+            elif code == OP_MUL_RELIN_RESCALE:
+                key.multiply_relinearize_rescale(self._ct(a, la), self._ct(b, la), out=self._ct(d, out_l))
+            elif code == OP_COPY:
+                if d != a:
+                    self._reg(d, la).copy_(self._reg(a, la))
+            elif code == OP_NTT:
+                plan.forward(self._reg(a, la), out=self._reg(d, la), n_limbs=la)
+            elif code == OP_INTT:
+                plan.inverse(self._reg(a, la), out=self._reg(d, la), n_limbs=la)
+            elif code == OP_MAC_PLAIN:
+                plan.multiply_plain_accumulate_ntt([self._reg(a + t, la) for t in range(b)], pt, la,
+                                                   out=self._reg(d, la))
+            else:
+                raise ValueError(f"unknown op code {code}")
+            if spare:
+                self._reg(dst, out_l).copy_(self._reg(self.tmp, out_l))
+            regs[dst] = out_l
+
+    def _on_stream(self):
+        import contextlib
+
+        import torch
+
+        @contextlib.contextmanager
+        def cm():
+            prev = self.ctx.stream
+            torch.cuda.current_stream().synchronize()
+            self.ctx.set_stream(self.stream)
+            try:
+                with torch.cuda.stream(self.stream):
+                    yield
+            finally:
+                self.ctx.set_stream(prev)  # waits for the runner's stream
+        return cm()
+
+    def _load(self, inputs):
+        self.regs[: self.prog.n_inputs].copy_(inputs.reshape(self.prog.n_inputs, self.per))
+
+    def output(self):
+        lo = self.limbs[-1][1]
+        return self._ct(self.prog.out_reg, lo)[0]
+
+    def run(self, inputs=None):
+        """Run the program on `inputs` ([n_inputs][2][n_limbs][N]; None: the
+        registers as they are); returns the output register (complete)."""
+        with self._on_stream():
+            if inputs is not None:
+                self._load(inputs)
+            self._run_ops()
+        return self.output()
+
+    def capture(self):
+        """Record one run of the ops (not the input load) as a graph."""
+        with self._on_stream():
+            self.ctx.graph_begin()
+            try:
+                self._run_ops()
+            finally:
+                self.graph = self.ctx.graph_end()
+        return self.graph
+
+    def replay(self, inputs=None):
+        with self._on_stream():
+            if inputs is not None:
+                self._load(inputs)
+            self.graph.launch()
+        return self.output()

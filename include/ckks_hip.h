@@ -256,6 +256,16 @@ int ckks_mod_switch(const ckks_rns_plan *plan, const uint64_t *in, uint64_t *out
 int ckks_multiply_plain(const ckks_rns_plan *plan, const uint64_t *in, const uint64_t *pt,
                         uint64_t *out, size_t n_cts, size_t n_limbs, size_t in_stride,
                         size_t pt_stride, size_t out_stride);
+/* multiply_ntt_resident_rlwe_with_prepared_plaintext(_resident)
+ * (ckks/rns_plain_multiply.rs:119-191) summed with
+ * pointwise_mul_add_assign (ring/ntt_polynomial.rs:222-235): the
+ * plaintext-diagonal inner sum of a baby-step / giant-step schedule in one
+ * launch,  out = sum_{t < count} cts[t] * pts[t]  over NTT-domain
+ * ciphertexts ([2][n_limbs][N] each) and NTT-domain plaintexts ([n_limbs][N]),
+ * limb by limb.  cts and pts are host arrays of device pointers, count in
+ * 1..64; out may be one of the cts. */
+int ckks_multiply_plain_accumulate_ntt(const ckks_rns_plan *plan, const uint64_t *const *cts,
+                                       const uint64_t *const *pts, size_t count, uint64_t *out, size_t n_limbs);
 
 /* encrypt_rns_raw_with_distribution_ntt_rng (grafting/rns_key_switch.rs:
  * 245-383) after its sampling, which stays on the host (ChaCha20, Gaussian
@@ -421,15 +431,37 @@ int ckks_bounded_apply_galois_hoisted(const ckks_ksk *key, uint64_t exponent, co
  * the call takes the ckks_apply_galois path.  Same residues as
  * ckks_apply_galois either way.  The first call with a given key and
  * exponent builds a copy of that key permuted by exponent^-1 (as much device
- * memory as the key; kept until ckks_ksk_destroy; refused under graph
- * capture), so calls with one key from several host threads must be
- * serialised by the caller.  Destroy the handle before the context. */
+ * memory as the key; at most 8 such copies per key, least recently used
+ * dropped first; refused under graph capture; without device memory for it
+ * the call takes the ckks_apply_galois path), so calls with one key from
+ * several host threads must be serialised by the caller.  Destroy the handle
+ * before the context. */
 typedef struct ckks_hoisted ckks_hoisted;
 int ckks_bounded_hoist(const ckks_ksk *key, const uint64_t *in, size_t n_cts, size_t in_stride,
                        ckks_hoisted **out);
 int ckks_bounded_rotate_hoisted(const ckks_hoisted *hoisted, const ckks_ksk *key, uint64_t exponent,
                                 const uint64_t *in, uint64_t *out, size_t in_stride, size_t out_stride);
 int ckks_bounded_hoist_destroy(ckks_hoisted *hoisted);
+
+/* Hoisted rotations with RNS-gadget and dnum-hybrid keys (build-defined:
+ * the reference hoists bounded digits only).  ckks_hoist cuts the digits of
+ * the UNROTATED a components once -- the canonical lift of a mod Q_j, as a key
+ * switch does -- and transforms them; ckks_rotate_hoisted(k) then switches
+ * with sigma_k of those digits (a permutation in the NTT domain, read by the
+ * key MAC over the key permuted by k^-1), so no decomposition or forward NTT
+ * runs per rotation.  sigma_k(d_j) is congruent to sigma_k(a) mod Q_j with
+ * coefficients of the same size as the canonical digit's, so the result is
+ * a key switch of sigma_k(a) with the same noise bound; its residues differ
+ * from ckks_apply_galois (which cuts the canonical digits of sigma_k(a))
+ * and are those of the oracle's hoisted composition.  For bounded keys both
+ * calls are ckks_bounded_hoist / ckks_bounded_rotate_hoisted.  The key must
+ * have the plan, Q limbs and gadget blocks the digits were cut with; the
+ * permuted key copies are cached as for bounded keys (without device memory
+ * for a copy the call fails with CKKS_ERR_OOM).  Destroy the handle with
+ * ckks_bounded_hoist_destroy, before the context. */
+int ckks_hoist(const ckks_ksk *key, const uint64_t *in, size_t n_cts, size_t in_stride, ckks_hoisted **out);
+int ckks_rotate_hoisted(const ckks_hoisted *hoisted, const ckks_ksk *key, uint64_t exponent, const uint64_t *in,
+                        uint64_t *out, size_t in_stride, size_t out_stride);
 
 int ckks_ksk_destroy(ckks_ksk *key);
 /* LogicalPayloadBytes::logical_payload_bytes of a key (ckks/key_accounting.rs:
@@ -516,8 +548,17 @@ enum {
     CKKS_OP_MUL_RELIN_RESCALE = 5,  /* dst = rescale(relin(a x b)), relinearization key */
     CKKS_OP_COPY = 6,               /* dst = a */
     CKKS_OP_MOD_SWITCH = 7,         /* dst = a without its last limb */
-    CKKS_OP_SUB = 8                 /* dst = a - b */
+    CKKS_OP_SUB = 8,                /* dst = a - b */
+    CKKS_OP_NTT = 9,                /* dst = NTT(a): the register moves to the NTT domain */
+    CKKS_OP_INTT = 10,              /* dst = NTT^-1(a): back to the coefficient domain */
+    CKKS_OP_MAC_PLAIN = 11          /* dst = sum_{t < b} reg[a + t] * pt_t (NTT domain); `plaintext`
+                                       points to a host array of b device pointers to NTT-domain
+                                       plaintexts, copied at submission; 1 <= b <= 64 */
 };
+/* Registers hold coefficient-domain ciphertexts except after CKKS_OP_NTT or
+ * CKKS_OP_MAC_PLAIN; CKKS_OP_INTT and CKKS_OP_MAC_PLAIN read NTT-domain
+ * registers, every other op reads coefficient-domain ones (add, sub and copy
+ * keep their operands' domain); the output register ends coefficient domain. */
 typedef struct ckks_op {
     uint32_t code, dst, a, b;
     const ckks_ksk *key;

@@ -218,6 +218,11 @@ int serve_emu_validate(int which, char *msg, int cap) {
                 j.ops.push_back(Op{OP_RESCALE, 1, 1, 0, nullptr, 0, nullptr}); break;  // past the last level
         case 5: j.ops.push_back(Op{OP_ROTATE, 1, 1, 0, &kKey, 5, nullptr}); break;  // key level mismatch
         case 6: break;                                            // valid
+        case 7: j.ops.insert(j.ops.begin(), Op{OP_INTT, 1, 0, 0, nullptr, 0, nullptr}); break;  // INTT of a coefficient reg
+        case 8: j.ops.push_back(Op{OP_NTT, 1, 1, 0, nullptr, 0, nullptr});
+                j.ops.push_back(Op{OP_ROTATE, 1, 1, 0, &kKey, 5, nullptr}); break;  // rotation of an NTT reg
+        case 9: j.ops.push_back(Op{OP_NTT, (uint32_t)(j.n_regs - 1), 1, 0, nullptr, 0, nullptr}); break;  // NTT output
+        case 10: j.ops.push_back(Op{OP_MAC_PLAIN, 1, 0, 1, nullptr, 0, nullptr}); break;  // no table
         default: return -1;
     }
     std::string m = validate(j, lookup);

@@ -232,3 +232,89 @@ def test_cancel_waiting_and_active_requests(pkg, ctx, fx):
     with pytest.raises(pkg.CkksError, match="finished"):
         server.cancel(t1)
     server.close()
+
+
+def test_mac_accumulate_ntt_bit_exact(pkg, ctx):
+    """ckks_multiply_plain_accumulate_ntt: sum_t ct_t * pt_t in the NTT domain,
+    against the oracle's pointwise products and sums."""
+    from conftest import rand_residues as rr
+    rng = np.random.default_rng(12)
+    L, n = 4, N
+    q = Q[:L]
+    plan = pkg.PreparedRnsNttPlan(ctx, q, n)
+    cts = [rr(rng, q, n, polys=2) for _ in range(5)]
+    pts = [rr(rng, q, n)[0] for _ in range(5)]
+    out = pkg.to_host(plan.multiply_plain_accumulate_ntt([pkg.to_device(c) for c in cts],
+                                                         [pkg.to_device(p) for p in pts], L))
+    for comp in range(2):
+        for i, qi in enumerate(q):
+            want = np.zeros(n, dtype=np.uint64)
+            for c, p in zip(cts, pts):
+                want = O.pointwise_mac(want, c[comp, i], p[i], qi)
+            acc = np.zeros(n, dtype=object)
+            for c, p in zip(cts, pts):
+                acc = (acc + c[comp, i].astype(object) * p[i].astype(object)) % qi
+            assert (out[comp, i] == acc.astype(np.uint64)).all()
+            assert (want == out[comp, i]).all()
+
+
+def test_ntt_resident_programs_match_classic_ones(pkg, ctx, fx):
+    """The NTT-resident form of every program (one multiply-accumulate per
+    inner sum) gives the residues of the classic form (a transform pair per
+    diagonal) through the serving loop, with fewer ops."""
+    lin = pkg.linear
+    rng = np.random.default_rng(21)
+    L = len(Q)
+    d = 32
+    M = rng.uniform(-1, 1, (d, d))
+    x = rng.uniform(-1, 1, d)
+    progs = {}
+    for nr in (False, True):
+        b = lin.Builder(SLOTS, fx.key_for, fx.encode_pt, Q, ntt_resident=nr)
+        progs[nr] = [("gemv", lin.gemv_program(b, M, L), [lin.tile(x, d, SLOTS)], L - 1),
+                     ("fft", lin.fft_program(b, 64, L), [lin.tile(rng.uniform(-1, 1, 64), 64, SLOTS)], L - 3),
+                     ("ccmm", lin.ccmm_program(b, 16, L, fx.relin_key(L - 2)),
+                      [lin.tile(rng.uniform(-1, 1, 256), 256, SLOTS)] * 2, L - 3)]
+    inputs = [torch.stack([fx.encrypt(v) for v in job[2]]).contiguous() for job in progs[False]]
+    outs = {}
+    for nr in (False, True):
+        server = pkg.serving.Server(ctx, hbm_budget_bytes=8 << 30)
+        res = []
+        for (name, prog, _, ol), cts in zip(progs[nr], inputs):
+            out = torch.zeros((2, ol, N), dtype=torch.int64, device="cuda")
+            res.append((server.submit(fx.plan, L, cts, out, prog), out))
+        server.drain()
+        outs[nr] = [o for _, o in res]
+        server.close()
+    for (name, p0, _, _), (_, p1, _, _), o0, o1 in zip(progs[False], progs[True], outs[False], outs[True]):
+        assert torch.equal(o0, o1), name
+        assert len(p1.ops) <= len(p0.ops), name
+    assert len(progs[True][0][1].ops) * 2 < len(progs[False][0][1].ops)  # GEMV: far fewer ops
+
+
+def test_program_runner_graph_replays_match_eager(pkg, ctx, fx):
+    """A GEMV program run directly (one library call per op) equals its
+    serving-loop result; captured once as a hipGraph and replayed with new
+    inputs and eager work between the replays, every replay equals the eager
+    run on the same inputs."""
+    lin = pkg.linear
+    rng = np.random.default_rng(33)
+    L = len(Q)
+    d = 32
+    M = rng.uniform(-1, 1, (d, d))
+    prog = lin.gemv_program(fx.builder(), M, L)
+    runner = pkg.serving.ProgramRunner(fx.plan, L, prog)
+    xs = [torch.stack([fx.encrypt(lin.tile(rng.uniform(-1, 1, d), d, SLOTS))]).contiguous() for _ in range(4)]
+    eager = [runner.run(x).clone() for x in xs]
+    server = pkg.serving.Server(fx.ctx, hbm_budget_bytes=8 << 30)
+    out = torch.zeros((2, L - 1, N), dtype=torch.int64, device="cuda")
+    server.submit(fx.plan, L, xs[0], out, prog)
+    server.drain()
+    assert torch.equal(out, eager[0])
+    server.close()
+    runner.capture()
+    for i in (1, 3, 0, 2):
+        got = runner.replay(xs[i]).clone()
+        assert torch.equal(got, eager[i]), f"replay {i}"
+        assert torch.equal(runner.run(xs[(i + 1) % 4]), eager[(i + 1) % 4])  # eager work between replays
+    runner.graph.close()

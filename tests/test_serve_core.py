@@ -116,6 +116,10 @@ def test_executor_error_stops_the_step(emu):
     (4, "cannot leave the final level"),
     (5, "key basis must match the ciphertext level"),
     (6, ""),
+    (7, "op needs an NTT-domain ciphertext"),
+    (8, "op needs a coefficient-domain ciphertext"),
+    (9, "output register must be coefficient domain"),
+    (10, "op needs an NTT-domain ciphertext"),
 ])
 def test_validation_messages(emu, which, msg):
     buf = C.create_string_buffer(256)
