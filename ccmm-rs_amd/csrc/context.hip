@@ -85,6 +85,8 @@ int ckks_context_destroy(ckks_context *ctx) {
     for (auto &kv : ctx->tables) {
         hipFree(kv.second.fwd);
         hipFree(kv.second.inv);
+        if (kv.second.rowf) hipFree(kv.second.rowf);
+        if (kv.second.rowf_inv) hipFree(kv.second.rowf_inv);
     }
     for (void *p : ctx->owned) hipFree(p);
     for (int i = 0; i < 2 * ckks_context::kSlots; ++i)
@@ -317,12 +319,24 @@ int ckks_rns_plan_create(ckks_context *ctx, const uint64_t *moduli, size_t n_lim
             CK_HIP(hipMalloc(&tt.inv, bytes));
             CK_HIP(hipMemcpy(tt.fwd, fwd.data(), bytes, hipMemcpyHostToDevice));
             CK_HIP(hipMemcpy(tt.inv, inv.data(), bytes, hipMemcpyHostToDevice));
+            if (q < host::kFpModulusBound && logn > 12) {
+                std::vector<uint64_t> rf, rfi;
+                host::row_factor_table_fp(q, psi, logn, rf);
+                host::row_factor_table_fp(q, host::invmod(psi, q), logn, rfi);
+                const size_t rb = rf.size() * sizeof(uint64_t);
+                CK_HIP(hipMalloc(&tt.rowf, rb));
+                CK_HIP(hipMalloc(&tt.rowf_inv, rb));
+                CK_HIP(hipMemcpy(tt.rowf, rf.data(), rb, hipMemcpyHostToDevice));
+                CK_HIP(hipMemcpy(tt.rowf_inv, rfi.data(), rb, hipMemcpyHostToDevice));
+            }
             it = ctx->tables.emplace(key, tt).first;
         }
         DevPrime p{};
         p.q = q;
         p.fwd = it->second.fwd;
         p.inv = it->second.inv;
+        p.rowf = it->second.rowf;
+        p.rowf_inv = it->second.rowf_inv;
         p.ninv = host::invmod(degree % q, q);
         p.ninv_q = host::shoup_quotient(p.ninv, q);
         host::barrett(q, p.br_hi, p.br_lo);

@@ -92,6 +92,29 @@ inline void twiddle_table_fp(uint64_t q, uint64_t root, uint32_t logn, std::vect
     }
 }
 
+// Row factors of the two-pass NTT (n = n1 + n2, rows u < 2^n1, row stages
+// s < n2), FP64 layout like twiddle_table_fp, entry u * n2 + s.  Row u's
+// twiddle at stage s and offset j < 2^s is T[(2^n1 + bitrev_n1(u)) 2^s + j]
+// with T[k] = root^bitrev_n(k); bit-reversing that index gives the exponent
+//   bitrev_s(j) 2^(n-s) + u 2^(n2-s) + 2^(n2-1-s),
+// while T[2^s + j] = root^(bitrev_s(j) 2^(n-s) + 2^(n-1-s)), so
+//   T[(2^n1 + bitrev_n1(u)) 2^s + j] = T[2^s + j] * root^((2u+1) 2^(n2-1-s) - 2^(n-1-s)):
+// the first factor is shared by every row, the second is this table.
+inline void row_factor_table_fp(uint64_t q, uint64_t root, uint32_t logn, std::vector<uint64_t> &out) {
+    const uint32_t n2 = (logn + 1) / 2, n1 = logn - n2;
+    const uint64_t two_n = 2ull << logn;  // root^(2N) = 1
+    out.assign(2 * ((size_t)1 << n1) * n2, 0);
+    for (uint64_t u = 0; u < (1ull << n1); ++u)
+        for (uint32_t s = 0; s < n2; ++s) {
+            const uint64_t e = ((2 * u + 1) << (n2 - 1 - s)) % two_n;
+            const uint64_t sub = (1ull << (logn - 1 - s)) % two_n;
+            const uint64_t w = powmod(root, (e + two_n - sub) % two_n, q);
+            const size_t k = (size_t)u * n2 + s;
+            out[2 * k] = f64_bits((double)w);
+            out[2 * k + 1] = f64_bits((double)w / (double)q);
+        }
+}
+
 // Deterministic Miller-Rabin for 64-bit values (witness set of
 // ring/ntt_prime.rs:8-33).
 inline bool is_prime(uint64_t v) {

@@ -19,6 +19,9 @@ struct DevPrime {
     u64 q;
     const Tw *fwd;  // psi^bitrev(k) with Shoup quotients, k in [0, N)
     const Tw *inv;  // psi^-bitrev(k)
+    // FP64 primes of a two-pass degree: row factors (row_factor_table_fp)
+    // of the forward and inverse tables, null otherwise
+    const Tw *rowf, *rowf_inv;
     u64 ninv, ninv_q;
     u64 br_hi, br_lo;  // floor(2^128 / q)
     double qd, qinv;      // FP64 path: q and 1/q
@@ -64,6 +67,7 @@ struct TwiddleTables {
     u64 q;
     u32 logn;
     Tw *fwd = nullptr, *inv = nullptr;  // device
+    Tw *rowf = nullptr, *rowf_inv = nullptr;
 };
 
 }  // namespace ck

@@ -130,7 +130,7 @@ namespace ck {
 // ---- device helpers --------------------------------------------------------
 
 CK_HD u64 reduce128(const Acc128 &a, const DevPrime &P) {
-    return barrett_reduce128(a.hi, a.lo, Barrett{P.q, P.br_hi, P.br_lo});
+    return a.reduce(Barrett{P.q, P.br_hi, P.br_lo});
 }
 
 // ---- 1. ModUp -------------------------------------------------------------------
@@ -190,7 +190,7 @@ __device__ void modup_thread(const ModUpArgs &a, const CK_CONST_AS ModUpBlock *b
         u64 v = src[(u64)m * N];
         x[m] = gi.neg ? neg_mod(v, primes[start + m].q) : v;
         // inside the block the canonical digit reduces to the residue
-        dst[(u64)(start + m) * N] = x[m];
+        st_stream(dst + (u64)(start + m) * N, x[m]);
     }
     const CK_CONST_AS u64 *wt = const_table(blk->w);
     if (MODE == 0) {
@@ -205,7 +205,7 @@ __device__ void modup_thread(const ModUpArgs &a, const CK_CONST_AS ModUpBlock *b
                 t.zero();
 #pragma unroll
                 for (int m = 0; m < k; ++m) t.mac(c[m], blk->pm[k][m]);
-                u64 tr = barrett_reduce128(t.hi, t.lo, bk);
+                u64 tr = t.reduce(bk);
                 c[k] = shoup(sub_mod(x[k], tr, bk.q), blk->inv[k], blk->inv_q[k], bk.q);
             }
         }
@@ -221,9 +221,9 @@ __device__ void modup_thread(const ModUpArgs &a, const CK_CONST_AS ModUpBlock *b
                 t.zero();
 #pragma unroll
                 for (int m = 0; m < A; ++m) t.mac(c[m], w[m]);
-                v = barrett_reduce128(t.hi, t.lo, bi);
+                v = t.reduce(bi);
             }
-            dst[(u64)i * N] = v;
+            st_stream(dst + (u64)i * N, v);
         }
         return;
     }
@@ -262,9 +262,9 @@ __device__ void modup_thread(const ModUpArgs &a, const CK_CONST_AS ModUpBlock *b
             t.mac(x[0], w[0]);
 #pragma unroll
             for (int m = 1; m < A; ++m) t.mac(f64_to_small(cl[m]), w[m]);
-            v = barrett_reduce128(t.hi, t.lo, bi);
+            v = t.reduce(bi);
         }
-        dst[(u64)i * N] = v;
+        st_stream(dst + (u64)i * N, v);
     }
 }
 
@@ -501,6 +501,36 @@ static int launch_ntt_permute(hipStream_t s, const u64 *in, u64 *out, u64 k, u64
 // Each thread owns one residue position (i, n) and keeps the key entries of
 // every digit in registers while it walks the batch, so a key is read from
 // HBM once per launch, not once per ciphertext.
+//
+// FP64 limbs (q < 2^41): each product is fp_mulmod(digit, key), |term| <=
+// 0.51 q, so D <= 8 terms sum below 2^44 (exact) and fp_residue makes the sum
+// canonical; the key entries enter as doubles with fl(w / q) divided once
+// per thread.
+template <int D>
+__device__ __forceinline__ void ks_mac_fp(const MacArgs &a, const DevPrime &P, u32 i, u64 N, u64 n) {
+    double kb[D], kbp[D], ka[D], kap[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+        kb[j] = small_to_f64(a.kb[((u64)j * a.limbs + i) * N + n]);
+        ka[j] = small_to_f64(a.ka[((u64)j * a.limbs + i) * N + n]);
+        kbp[j] = kb[j] / P.qd;
+        kap[j] = ka[j] / P.qd;
+    }
+    for (u32 ct = 0; ct < a.n_cts; ++ct) {
+        const u64 *d = a.digits + (((u64)ct * D) * a.limbs + i) * N + n;
+        double sb = 0.0, sa = 0.0;
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            const double dv = small_to_f64(ld_stream(d + (u64)j * a.limbs * N));
+            sb += fp_mulmod(dv, kb[j], kbp[j], P.qd);
+            sa += fp_mulmod(dv, ka[j], kap[j], P.qd);
+        }
+        u64 *o = a.acc + ((u64)ct * 2 * a.limbs + i) * N + n;
+        st_stream(o, fp_residue(sb, P.qd, P.qinv));
+        st_stream(o + (u64)a.limbs * N, fp_residue(sa, P.qd, P.qinv));
+    }
+}
+
 template <int D>
 __global__ __launch_bounds__(kKsThreads) void k_ks_mac(MacArgs a) {
     const u32 b = blockIdx.x;
@@ -509,6 +539,10 @@ __global__ __launch_bounds__(kKsThreads) void k_ks_mac(MacArgs a) {
     const u64 N = (u64)1 << a.logn;
     const u64 n = (u64)chunk * kKsThreads + threadIdx.x;
     const DevPrime P = a.primes[i];
+    if (P.fp) {
+        ks_mac_fp<D>(a, P, i, N, n);
+        return;
+    }
     u64 kb[D], ka[D];
 #pragma unroll
     for (int j = 0; j < D; ++j) {
@@ -522,13 +556,13 @@ __global__ __launch_bounds__(kKsThreads) void k_ks_mac(MacArgs a) {
         sa.zero();
 #pragma unroll
         for (int j = 0; j < D; ++j) {
-            u64 dv = d[(u64)j * a.limbs * N];
+            u64 dv = ld_stream(d + (u64)j * a.limbs * N);
             sb.mac(dv, kb[j]);
             sa.mac(dv, ka[j]);
         }
         u64 *o = a.acc + ((u64)ct * 2 * a.limbs + i) * N + n;
-        o[0] = reduce128(sb, P);
-        o[(u64)a.limbs * N] = reduce128(sa, P);
+        st_stream(o, reduce128(sb, P));
+        st_stream(o + (u64)a.limbs * N, reduce128(sa, P));
     }
 }
 
@@ -667,7 +701,7 @@ __global__ __launch_bounds__(kKsThreads) void k_moddown(DownArgs a) {
     if (K > 0) {
         u64 xp[K > 0 ? K : 1];
 #pragma unroll
-        for (int m = 0; m < K; ++m) xp[m] = x[(u64)(a.q_limbs + m) * N];
+        for (int m = 0; m < K; ++m) xp[m] = ld_stream(x + (u64)(a.q_limbs + m) * N);
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             if (k == 0) {
@@ -678,7 +712,7 @@ __global__ __launch_bounds__(kKsThreads) void k_moddown(DownArgs a) {
                 t.zero();
 #pragma unroll
                 for (int m = 0; m < k; ++m) t.mac(c[m], md->pm[k][m]);
-                c[k] = shoup(sub_mod(xp[k], barrett_reduce128(t.hi, t.lo, bk), bk.q), md->inv[k],
+                c[k] = shoup(sub_mod(xp[k], t.reduce(bk), bk.q), md->inv[k],
                              md->inv_q[k], bk.q);
             }
         }
@@ -713,14 +747,14 @@ __global__ __launch_bounds__(kKsThreads) void k_moddown(DownArgs a) {
             for (int m = 1; m < K; ++m)
                 acc += fp_mulmod(cl[m], r[6 + 4 * m], r[7 + 4 * m], q) +
                        fp_mulmod(ch[m], r[8 + 4 * m], r[9 + 4 * m], q);
-            double t = small_to_f64(x[(u64)i * N]) - acc;  // x_i - (x_P mod q_i)
+            double t = small_to_f64(ld_stream(x + (u64)i * N)) - acc;  // x_i - (x_P mod q_i)
             if (neg) t += r[2];                             // x_P - P
             double v = fp_mulmod(t, r[3], r[4], q);
             if (add) {
                 const double ad = small_to_f64(add[(u64)i * N]);
                 v = ga.neg ? v - ad : v + ad;
             }
-            o[(u64)i * N] = fp_residue(v, q, r[1]);
+            st_stream(o + (u64)i * N, fp_residue(v, q, r[1]));
             continue;
         }
         const Barrett bi = barrett_at(primes, i);
@@ -729,14 +763,14 @@ __global__ __launch_bounds__(kKsThreads) void k_moddown(DownArgs a) {
         t.zero();
 #pragma unroll
         for (int m = 0; m < K; ++m) t.mac(c[m], w[m]);
-        u64 r = barrett_reduce128(t.hi, t.lo, bi);  // x_P mod q_i
+        u64 r = t.reduce(bi);  // x_P mod q_i
         if (neg) r = sub_mod(r, pmod[i], bi.q);  // (x_P - P) mod q_i
-        u64 v = shoup(sub_mod(x[(u64)i * N], r, bi.q), pinv[i], pinv_q[i], bi.q);
+        u64 v = shoup(sub_mod(ld_stream(x + (u64)i * N), r, bi.q), pinv[i], pinv_q[i], bi.q);
         if (add) {
             u64 ad = add[(u64)i * N];
             v = ga.neg ? sub_mod(v, ad, bi.q) : add_mod(v, ad, bi.q);
         }
-        o[(u64)i * N] = v;
+        st_stream(o + (u64)i * N, v);
     }
 }
 

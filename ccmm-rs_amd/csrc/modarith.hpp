@@ -65,25 +65,19 @@ struct Barrett {
     u64 r_hi, r_lo;
 };
 
-// Reduce a 128-bit value (hi, lo) with hi < q (true for any sum of a few
-// products of canonical residues when q < 2^62) to [0, q).
-// Quotient estimate floor(x * r / 2^128) is short by at most 2.
+// Reduce a 128-bit value hi 2^64 + lo to [0, q) (q < 2^62) as two Shoup
+// multiplications by constants the Barrett pair already holds:
+// floor(2^64 / q) = r_hi, and with W = 2^64 mod q = -r_hi q (mod 2^64),
+// floor(W 2^64 / q) = floor(2^128 / q) - r_hi 2^64 = r_lo.  So
+//   hi W - floor(hi r_lo / 2^64) q  is hi 2^64 mod q, in [0, 2q),
+//   lo   - floor(lo r_hi / 2^64) q  is lo mod q,      in [0, 2q),
+// for any hi, lo < 2^64; their sum is below 4q < 2^64.
 CK_HD u64 barrett_reduce128(u64 hi, u64 lo, const Barrett &b) {
-    // x * r = (hi 2^64 + lo)(r_hi 2^64 + r_lo); need bits >= 128.
-    u64 p0h = mulhi64(lo, b.r_lo);
-    u64 p1l = lo * b.r_hi, p1h = mulhi64(lo, b.r_hi);
-    u64 p2l = hi * b.r_lo, p2h = mulhi64(hi, b.r_lo);
-    u64 p3l = hi * b.r_hi;
-    // middle column sum with carries
-    u64 m = p0h + p1l;
-    u64 c = m < p0h;
-    u64 m2 = m + p2l;
-    c += m2 < m;
-    u64 quot = p3l + p1h + p2h + c;
-    u64 r = lo - quot * b.q;  // true remainder < 3q fits in 64 bits
-    if (r >= b.q) r -= b.q;
-    if (r >= b.q) r -= b.q;
-    return r;
+    const u64 w = 0 - b.r_hi * b.q;
+    u64 r = (hi * w - mulhi64(hi, b.r_lo) * b.q) + (lo - mulhi64(lo, b.r_hi) * b.q);
+    const u64 two_q = b.q << 1;
+    r = r >= two_q ? r - two_q : r;
+    return r >= b.q ? r - b.q : r;
 }
 
 CK_HD u64 mul_mod(u64 a, u64 b, const Barrett &br) {
@@ -160,19 +154,78 @@ CK_HD u64 fp_residue(double x, double q, double qinv) {
     return f64_to_small(r);
 }
 
-// 128-bit accumulator for sums of products before one reduction.
+// Streaming accesses: data a kernel reads or writes exactly once go through
+// the caches with the nontemporal hint (the `nt` bit of global loads and
+// stores), so they do not evict what the kernel reuses from its XCD's L2:
+// the source limbs of a Galois gather, twiddle and constant tables.
+CK_HD u64 ld_stream(const u64 *p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_nontemporal_load(p);
+#else
+    return *p;
+#endif
+}
+
+CK_HD void st_stream(u64 *p, u64 v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    __builtin_nontemporal_store(v, p);
+#else
+    *p = v;
+#endif
+}
+
+// 128-bit accumulator for sums of products of 64-bit words before one
+// reduction.  The second factor is cut into 21-bit pieces b = b0 + b1 2^21 +
+// b2 2^42 (b < 2^63) and the first into its 32-bit words, so each of the six
+// partial products a_i b_j is below 2^53 and goes straight into its own
+// 64-bit sum by one v_mad_u64_u32, with no carries: up to 1024 products fit.
+// A full 128-bit product per term would recompute its cross terms for the
+// low and the high word and track two carries.  fold() recombines the sums
+// s_ij 2^(32 i + 21 j) modulo 2^128.  A factor that stays fixed over many
+// products (a key entry, a table constant) is cut once with split21.
+struct Split21 {
+    u64 b0, b1, b2;
+};
+
+CK_HD Split21 split21(u64 b) { return Split21{b & 0x1fffffull, (b >> 21) & 0x1fffffull, b >> 42}; }
+
 struct Acc128 {
-    u64 hi, lo;
-    CK_HD void zero() { hi = lo = 0; }
-    CK_HD void mac(u64 a, u64 b) {
-        u64 h, l;
-        mul128(a, b, h, l);
-        lo += l;
-        hi += h + (lo < l);
+    u64 s00, s01, s02, s10, s11, s12;  // s_ij = sum a_i b_j
+    CK_HD void zero() { s00 = s01 = s02 = s10 = s11 = s12 = 0; }
+    CK_HD void mac(u64 a, const Split21 &b) {
+        const u64 a0 = (u32)a, a1 = a >> 32;
+        s00 += a0 * b.b0;
+        s01 += a0 * b.b1;
+        s02 += a0 * b.b2;
+        s10 += a1 * b.b0;
+        s11 += a1 * b.b1;
+        s12 += a1 * b.b2;
     }
-    CK_HD void add(u64 v) {
-        lo += v;
-        hi += (lo < v);
+    CK_HD void mac(u64 a, u64 b) { mac(a, split21(b)); }
+    CK_HD void fold(u64 &hi, u64 &lo) const {
+        // weights 2^0, 2^21, 2^42, 2^32, 2^53, 2^74
+        u64 l = s00, h = 0, t;
+        t = s01 << 21;
+        l += t;
+        h += (s01 >> 43) + (l < t);
+        t = s02 << 42;
+        l += t;
+        h += (s02 >> 22) + (l < t);
+        t = s10 << 32;
+        l += t;
+        h += (s10 >> 32) + (l < t);
+        t = s11 << 53;
+        l += t;
+        h += (s11 >> 11) + (l < t);
+        h += s12 << 10;
+        hi = h;
+        lo = l;
+    }
+    // sum mod q for a sum below q 2^64 (barrett_reduce128's input range)
+    CK_HD u64 reduce(const Barrett &b) const {
+        u64 hi, lo;
+        fold(hi, lo);
+        return barrett_reduce128(hi, lo, b);
     }
 };
 

@@ -62,7 +62,7 @@ struct MacSrc {
         Acc128 t;
         t.zero();
         for (u32 j = 0; j < dnum; ++j) t.mac(d[j * d_stride + off], k[j * k_stride + off]);
-        return barrett_reduce128(t.hi, t.lo, br);
+        return t.reduce(br);
     }
 };
 
@@ -103,6 +103,7 @@ __global__ __launch_bounds__(1 << (TB - 4)) void k_ntt(NttLaunch launch) {
 
     TileJob job;
     job.tw = FWD ? P.fwd : P.inv;
+    job.rowf = FWD ? P.rowf : P.rowf_inv;
     job.q = P.q;
     job.ninv = P.ninv;
     job.ninv_q = P.ninv_q;
@@ -377,6 +378,13 @@ __device__ __forceinline__ void opaque_tid(BlockExec &ex) {
 #endif
 }
 
+// FP64 limbs multiply the row pass's lazy doubles (|x| < 10 q < 2^45, the
+// input range of fp_mulmod) by the key entries without making them
+// canonical first; a key entry w enters with wp = fl(fl(w) * fl(1/q)), at
+// most two ulps from w / q, which moves y wp by under 2^-6 for |y| < 2^45,
+// so each term stays an exact integer within 0.52 q and dnum <= 8 terms sum
+// below 2^45 before fp_residue.  Integer limbs make the digit canonical and
+// multiply by Barrett reduction.
 template <int TB, bool BLK, bool FP, int LG>
 __device__ void row_mac_body(const RowMacArgs &a, const DevPrime &P, u32 ct, u32 tile, u32 i,
                              u64 *lds) {
@@ -387,6 +395,7 @@ __device__ void row_mac_body(const RowMacArgs &a, const DevPrime &P, u32 ct, u32
     const u64 N = (u64)1 << LG;
     TileJob job;
     job.tw = P.fwd;
+    job.rowf = P.rowf;
     job.q = P.q;
     job.ninv = P.ninv;
     job.ninv_q = P.ninv_q;
@@ -402,7 +411,7 @@ __device__ void row_mac_body(const RowMacArgs &a, const DevPrime &P, u32 ct, u32
     u64 regs[16];
     BlockExec ex{T};
     auto off = [&](int r) { return row_natural_offset<N2, TB - N2>(pa, Pass::stage_coord((u32)r * T + ex.tid)); };
-    u64 sb[16], sa[16];
+    u64 sb[16], sa[16];  // FP: the sums as double bits
 #pragma unroll
     for (int r = 0; r < 16; ++r) sb[r] = sa[r] = 0;
     auto regs_of = [&](u32) -> u64(&)[16] { return regs; };
@@ -419,10 +428,18 @@ __device__ void row_mac_body(const RowMacArgs &a, const DevPrime &P, u32 ct, u32
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const u32 e = Pass::stage_coord((u32)r * T + ex.tid);
-            const u64 d = Pass::finish(lds[lds_index<N2, TB - N2>(e)], job);
+            const u64 x = lds[lds_index<N2, TB - N2>(e)];
             const u32 o = off(r);
-            sb[r] = add_mod(sb[r], mul_mod(d, kb[o], br), P.q);
-            sa[r] = add_mod(sa[r], mul_mod(d, ka[o], br), P.q);
+            if (FP) {
+                const double y = u64_as_f64(x);
+                const double wb = small_to_f64(kb[o]), wa = small_to_f64(ka[o]);
+                sb[r] = f64_as_u64(u64_as_f64(sb[r]) + fp_mulmod(y, wb, wb * P.qinv, P.qd));
+                sa[r] = f64_as_u64(u64_as_f64(sa[r]) + fp_mulmod(y, wa, wa * P.qinv, P.qd));
+            } else {
+                const u64 d = Pass::finish(x, job);
+                sb[r] = add_mod(sb[r], mul_mod(d, kb[o], br), P.q);
+                sa[r] = add_mod(sa[r], mul_mod(d, ka[o], br), P.q);
+            }
             // keep the scheduler from hoisting all 48 loads of the epilogue
             if ((r & 3) == 3) __builtin_amdgcn_sched_barrier(0);
         }
@@ -434,8 +451,13 @@ __device__ void row_mac_body(const RowMacArgs &a, const DevPrime &P, u32 ct, u32
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const u32 o = off(r);
-        ob[o] = sb[r];
-        oa[o] = sa[r];
+        if (FP) {
+            st_stream(ob + o, fp_residue(u64_as_f64(sb[r]), P.qd, P.qinv));
+            st_stream(oa + o, fp_residue(u64_as_f64(sa[r]), P.qd, P.qinv));
+        } else {
+            st_stream(ob + o, sb[r]);
+            st_stream(oa + o, sa[r]);
+        }
     }
 }
 

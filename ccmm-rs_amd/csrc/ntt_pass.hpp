@@ -41,7 +41,7 @@ struct BlockExec {
 // pass works on.
 struct PlainSrc {
     CK_HD u64 operator()(const TileJob &job, u32 off, bool fp_initial) const {
-        const u64 x = job.src[off];
+        const u64 x = ld_stream(job.src + off);
         return fp_initial ? f64_as_u64((double)x) : x;
     }
 };
@@ -77,10 +77,13 @@ struct NttPass {
         constexpr int s0 = round_start(L, I);
         constexpr bool row = (KIND == PASS_ROW);
         constexpr u32 UM = uniform_twiddle_mask(g, L, CL, K, s0, FWD, row);
+        // row-factored twiddles: the shared index reads t bits only
+        constexpr u32 SM = uniform_twiddle_mask(g, L, CL, K, s0, FWD, false);
+        constexpr bool RU = wave_uniform_bits(g, L, L + CL);
         if (FWD)
-            fwd_round<L, CL, K, FP, UM>(v, g, tid, s0, row, pa, job);
+            fwd_round<L, CL, K, FP, UM, SM, RU>(v, g, tid, s0, row, pa, job);
         else
-            inv_round<L, CL, K, FP, UM>(v, g, tid, s0, row, pa, job);
+            inv_round<L, CL, K, FP, UM, SM, RU>(v, g, tid, s0, row, pa, job);
     }
 
     // global offset (within the limb) of the element at tile coordinate e on
@@ -241,7 +244,7 @@ struct NttPass {
                     u32 e = stage_coord(x);
                     u32 off = KIND == PASS_FULL ? x : row_natural_offset<L, CL>(pa, e);
                     CK_BOUNDS(off, 1u << n);
-                    job.dst[off] = finish(lds[lds_index<L, CL>(e)], job);
+                    st_stream(job.dst + off, finish(lds[lds_index<L, CL>(e)], job));
                 }
             });
         } else {
@@ -254,7 +257,7 @@ struct NttPass {
                     u64 x = FINAL ? finish(v[i], job) : v[i];
                     u32 off = write_off(pa, coord(g, tid, i));
                     CK_BOUNDS(off, 1u << n);
-                    job.dst[off] = x;
+                    st_stream(job.dst + off, x);
                 }
             });
         }

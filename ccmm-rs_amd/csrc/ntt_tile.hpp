@@ -56,6 +56,10 @@ struct TileJob {
     const u64 *src;  // input residues of this limb transform
     u64 *dst;        // output residues
     const Tw *tw;    // forward or inverse twiddle table (index 1..N-1)
+    // FP64 primes of two-pass degrees: row factors of tw
+    // (row_factor_table_fp); row passes build each twiddle as tw[2^s + j]
+    // times its row's factor
+    const Tw *rowf;
     u64 q;
     u64 ninv, ninv_q;  // N^-1 and its Shoup quotient (inverse only)
     // FP64 path (q < 2^42): twiddle entries are (double w, double w/q)
@@ -318,10 +322,29 @@ CK_HD Tw load_twiddle(const Tw *tw, u32 idx) {
     return tw[idx];
 }
 
+// Row-pass twiddle of an FP64 prime as the product of the shared entry
+// tw[sidx] (sidx = 2^level + j, the column pass's index) and the factor of
+// (row, level): fp_mulmod gives it exactly, a signed residue within 0.52 q,
+// and wp = fl(w * fl(1/q)) is at most two ulps from w / q, which moves y wp by
+// under 2^-7 for |y| < 2^45.  The butterfly products then stay within 0.52 q
+// (0.51 q with table twiddles), inside the margins of the lazy ranges above:
+// forward |x| < q + 17 * 0.52 q < 10 q, inverse sums below 16 q.
+// SU / RU: the shared index / the row is wave-uniform.
+template <bool SU, bool RU>
+CK_HD Tw row_twiddle_fp(const TileJob &job, u32 sidx, u32 fidx) {
+    const Tw ts = load_twiddle<SU>(job.tw, sidx);
+    const Tw f = load_twiddle<RU>(job.rowf, fidx);
+    const double w = fp_mulmod(u64_as_f64(ts.w), u64_as_f64(f.w), u64_as_f64(f.wq), job.qd);
+    return Tw{f64_as_u64(w), f64_as_u64(w * job.qinv)};
+}
+
 // Forward round: stages s0..s0+k-1 (butterfly bit of stage s is L-1-s).
-template <int L, int CL, int K, bool FP, u32 UM = 0>
+// SM: stages whose shared index (row-factored twiddles) is wave-uniform;
+// RU: the row of an element is wave-uniform.
+template <int L, int CL, int K, bool FP, u32 UM = 0, u32 SM = 0, bool RU = false>
 CK_HD void fwd_round(u64 (&v)[16], const Geom &g, u32 tid, int s0, bool row_base,
                      const PassAddr &pa, const TileJob &job) {
+    const bool fac = FP && row_base;  // FP64 row passes always read row factors
 #pragma unroll
     for (int j = 0; j < K; ++j) {
         const int s = s0 + j;
@@ -331,10 +354,19 @@ CK_HD void fwd_round(u64 (&v)[16], const Geom &g, u32 tid, int s0, bool row_base
             if (i & (1 << rb)) continue;
             u32 e = coord(g, tid, i);
             u32 t = e & ((1u << L) - 1);
-            u32 base = row_base ? row_twiddle_base<L, CL>(pa, e) : 1u;
-            u32 idx = (base << s) + (t >> (L - s));
-            CK_BOUNDS(idx, 1u << pa.logn);
-            const Tw w = (UM >> j) & 1u ? load_twiddle<true>(job.tw, idx) : load_twiddle<false>(job.tw, idx);
+            Tw w;
+            if (fac) {
+                const u32 sidx = (1u << s) + (t >> (L - s));
+                const u32 fidx = (pa.tile * (1u << CL) + (e >> L)) * L + (u32)s;
+                CK_BOUNDS(sidx, 1u << L);
+                CK_BOUNDS(fidx, (1u << (pa.logn - L)) * L);
+                w = (SM >> j) & 1u ? row_twiddle_fp<true, RU>(job, sidx, fidx) : row_twiddle_fp<false, RU>(job, sidx, fidx);
+            } else {
+                u32 base = row_base ? row_twiddle_base<L, CL>(pa, e) : 1u;
+                u32 idx = (base << s) + (t >> (L - s));
+                CK_BOUNDS(idx, 1u << pa.logn);
+                w = (UM >> j) & 1u ? load_twiddle<true>(job.tw, idx) : load_twiddle<false>(job.tw, idx);
+            }
             if (FP)
                 ct_bfly_fp(v[i], v[i | (1 << rb)], w, job.qd);
             else
@@ -343,10 +375,12 @@ CK_HD void fwd_round(u64 (&v)[16], const Geom &g, u32 tid, int s0, bool row_base
     }
 }
 
-// Inverse round: GS stages u0..u0+k-1 (butterfly bit of stage u is u).
-template <int L, int CL, int K, bool FP, u32 UM = 0>
+// Inverse round: GS stages u0..u0+k-1 (butterfly bit of stage u is u; its
+// twiddle level is L - 1 - u).
+template <int L, int CL, int K, bool FP, u32 UM = 0, u32 SM = 0, bool RU = false>
 CK_HD void inv_round(u64 (&v)[16], const Geom &g, u32 tid, int u0, bool row_base,
                      const PassAddr &pa, const TileJob &job) {
+    const bool fac = FP && row_base;  // FP64 row passes always read row factors
 #pragma unroll
     for (int j = 0; j < K; ++j) {
         const int u = u0 + j;
@@ -356,10 +390,19 @@ CK_HD void inv_round(u64 (&v)[16], const Geom &g, u32 tid, int u0, bool row_base
             if (i & (1 << rb)) continue;
             u32 e = coord(g, tid, i);
             u32 t = e & ((1u << L) - 1);
-            u32 base = row_base ? row_twiddle_base<L, CL>(pa, e) : 1u;
-            u32 idx = (base << (L - 1 - u)) + (t >> (u + 1));
-            CK_BOUNDS(idx, 1u << pa.logn);
-            const Tw w = (UM >> j) & 1u ? load_twiddle<true>(job.tw, idx) : load_twiddle<false>(job.tw, idx);
+            Tw w;
+            if (fac) {
+                const u32 sidx = (1u << (L - 1 - u)) + (t >> (u + 1));
+                const u32 fidx = (pa.tile * (1u << CL) + (e >> L)) * L + (u32)(L - 1 - u);
+                CK_BOUNDS(sidx, 1u << L);
+                CK_BOUNDS(fidx, (1u << (pa.logn - L)) * L);
+                w = (SM >> j) & 1u ? row_twiddle_fp<true, RU>(job, sidx, fidx) : row_twiddle_fp<false, RU>(job, sidx, fidx);
+            } else {
+                u32 base = row_base ? row_twiddle_base<L, CL>(pa, e) : 1u;
+                u32 idx = (base << (L - 1 - u)) + (t >> (u + 1));
+                CK_BOUNDS(idx, 1u << pa.logn);
+                w = (UM >> j) & 1u ? load_twiddle<true>(job.tw, idx) : load_twiddle<false>(job.tw, idx);
+            }
             if (FP && j == K - 1)
                 gs_bfly_fp<true>(v[i], v[i | (1 << rb)], w, job.qd, job.qinv);
             else if (FP)

@@ -445,7 +445,7 @@ __global__ __launch_bounds__(kEwThreads) void k_tensor(TensorArgs a) {
         acc.zero();
         acc.mac(b0, a1);
         acc.mac(a0, b1);
-        y[poly + i] = barrett_reduce128(acc.hi, acc.lo, br);
+        y[poly + i] = acc.reduce(br);
         y[2 * poly + i] = mul_mod(a0, a1, br);
     }
 }

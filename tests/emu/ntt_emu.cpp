@@ -39,7 +39,8 @@ static void run_variant(const TileJob &job, u32 logn, int tb, int blk) {
 
 // Transforms one limb of 2^logn residues modulo q (< 2^62) with kernel
 // variant (tb = log2 two-pass tile size 12 or 13, blk = blocked inter-pass
-// layout).
+// layout).  FP64 primes of two-pass degrees build their row twiddles from
+// the row factors, as the kernels do.
 extern "C" int emu_ntt_variant(const uint64_t *in, uint64_t *out, uint64_t q, uint32_t logn,
                                int forward, int tb, int blk) {
     if (logn < 10 || logn > 17 || (tb != 12 && tb != 13)) return -1;
@@ -53,9 +54,12 @@ extern "C" int emu_ntt_variant(const uint64_t *in, uint64_t *out, uint64_t q, ui
     else
         host::twiddle_table(q, forward ? psi : host::invmod(psi, q), logn, tw);
     uint32_t n = 1u << logn;
-    std::vector<uint64_t> tmp(n, 0xDEADBEEFull), src(in, in + n);
+    std::vector<uint64_t> tmp(n, 0xDEADBEEFull), src(in, in + n), rowf;
+    if (fp && ntt_two_pass(logn))
+        host::row_factor_table_fp(q, forward ? psi : host::invmod(psi, q), logn, rowf);
     TileJob job{};
     job.tw = reinterpret_cast<const Tw *>(tw.data());
+    job.rowf = rowf.empty() ? nullptr : reinterpret_cast<const Tw *>(rowf.data());
     job.q = q;
     job.ninv = host::invmod(n % q, q);
     job.ninv_q = host::shoup_quotient(job.ninv, q);
@@ -179,8 +183,54 @@ static void uniform_pass(u32 logn, int &flagged, int &bad) {
     }
 }
 
+// Row passes of FP64 primes build twiddles from row factors: the shared
+// index (stage flagged in the mask without row bits) and, when the geometry
+// says so, the row itself are read through the scalar unit.
+template <int L, int CL, PassKind KIND, bool FWD>
+static void uniform_row_factors(u32 logn, int &flagged, int &bad) {
+    typedef NttPass<L, CL, KIND, FWD, true> P;
+    const int T = (1 << (L + CL)) / 16;
+    const u32 tiles = (1u << logn) >> (L + CL);
+    for (int I = 0; I < n_rounds(L); ++I) {
+        const Geom g = P::geom(I);
+        const int K = round_k(L, I), s0 = round_start(L, I);
+        const u32 sm = uniform_twiddle_mask(g, L, CL, K, s0, FWD, false);
+        const bool ru = wave_uniform_bits(g, L, L + CL);
+        for (int j = 0; j < K; ++j) {
+            const bool s_flag = (sm >> j) & 1;
+            if (!s_flag && !ru) continue;
+            flagged += (int)s_flag + (int)ru;
+            const int s = s0 + j, rb = FWD ? K - 1 - j : j;
+            for (u32 tile = 0; tile < tiles; ++tile)
+                for (int i = 0; i < 16; ++i) {
+                    if (i & (1 << rb)) continue;
+                    for (int w0 = 0; w0 < T; w0 += 64) {
+                        long first_s = -1, first_r = -1;
+                        for (int tid = w0; tid < w0 + 64 && tid < T; ++tid) {
+                            const u32 e = coord(g, tid, i), t = e & ((1u << L) - 1);
+                            const u32 sidx = FWD ? (1u << s) + (t >> (L - s)) : (1u << (L - 1 - s)) + (t >> (s + 1));
+                            const u32 r = tile * (1u << CL) + (e >> L);
+                            if (s_flag) {
+                                if (first_s < 0) first_s = sidx;
+                                else if ((long)sidx != first_s) ++bad;
+                            }
+                            if (ru) {
+                                if (first_r < 0) first_r = r;
+                                else if ((long)r != first_r) ++bad;
+                            }
+                        }
+                    }
+                }
+        }
+    }
+}
+
 template <int L, int CL, PassKind KIND>
 static void uniform_both(u32 logn, int &flagged, int &bad) {
+    if (KIND == PASS_ROW) {
+        uniform_row_factors<L, CL, KIND, true>(logn, flagged, bad);
+        uniform_row_factors<L, CL, KIND, false>(logn, flagged, bad);
+    }
     uniform_pass<L, CL, KIND, true>(logn, flagged, bad);
     uniform_pass<L, CL, KIND, false>(logn, flagged, bad);
 }

@@ -196,6 +196,30 @@ def test_emulated_fp64_inverse_extremes(emu, logn):
         assert (back == plan.inverse(y)).all()
 
 
+@pytest.mark.parametrize("logn", [13, 14, 15, 16, 17])
+@pytest.mark.parametrize("tb,blk", [(12, 0), (13, 1)])
+def test_row_factor_twiddles_at_the_fp64_bound(emu, logn, tb, blk):
+    """FP64 row passes build each twiddle from the shared column-pass entry
+    and a per-(row, stage) factor; with the largest NTT prime below the FP64
+    bound and inputs that grow the lazy ranges fastest, both directions give
+    the oracle's residues."""
+    if tb == 13 and logn < 14:
+        pytest.skip("8192-residue tiles serve N >= 2^14")
+    emu.emu_ntt_variant.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32, C.c_int,
+                                    C.c_int, C.c_int]
+    n = 1 << logn
+    q = O.ntt_primes(41, n, 1)[0]
+    plan = O.Plan(q, n)
+    rng = np.random.default_rng(300 + logn)
+    for x in (rng.integers(0, q, n, dtype=np.uint64), np.full(n, q - 1, dtype=np.uint64),
+              np.where(np.arange(n) % 3 == 0, q - 1, 0).astype(np.uint64)):
+        out = np.zeros(n, dtype=np.uint64)
+        assert emu.emu_ntt_variant(x.ctypes.data, out.ctypes.data, q, logn, 1, tb, blk) == 0
+        assert (out == plan.forward(x)).all()
+        assert emu.emu_ntt_variant(x.ctypes.data, out.ctypes.data, q, logn, 0, tb, blk) == 0
+        assert (out == plan.inverse(x)).all()
+
+
 def test_scalar_twiddle_stages_are_wave_uniform(emu):
     """The NTT stages whose twiddle the kernels read through the scalar unit
     (readfirstlane of the index) have one twiddle index per wave64 in every
