@@ -265,3 +265,47 @@ extern "C" int emu_twiddle_uniform(int *flagged_out) {
     *flagged_out = flagged;
     return bad;
 }
+
+// ---- 128-bit accumulation and reduction ------------------------------------
+// Acc128 (six carry-free partial sums) against unsigned __int128 sums of up to
+// 16 products of operands below 2^bits (bits <= 62, all-ones operands
+// included), and barrett_reduce128 (two Shoup products) against the 128-bit
+// remainder for odd q < 2^62 and arbitrary hi, lo.  Returns the mismatches.
+extern "C" long emu_wide_arith_check(uint64_t seed, long iters) {
+    uint64_t st = seed * 0x9E3779B97F4A7C15ull + 1;
+    auto next = [&]() {  // splitmix64
+        uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    };
+    typedef unsigned __int128 u128;
+    long bad = 0;
+    for (long it = 0; it < iters; ++it) {
+        const int n = 1 + (int)(next() % 16), bits = 40 + (int)(next() % 23);
+        Acc128 a;
+        a.zero();
+        u128 ref = 0;
+        for (int k = 0; k < n; ++k) {
+            u64 x = next() >> (64 - bits), y = next() >> (64 - bits);
+            if (it % 7 == 0) x = y = (1ull << bits) - 1;
+            a.mac(x, y);
+            ref += (u128)x * y;
+        }
+        u64 hi, lo;
+        a.fold(hi, lo);
+        bad += hi != (u64)(ref >> 64) || lo != (u64)ref;
+        u64 q = (next() >> (2 + next() % 61)) | 1;
+        if (q < 3) q = 3;
+        if (it % 5 == 0) q = (1ull << 62) - 57;
+        u64 bh, bl;
+        host::barrett(q, bh, bl);
+        const Barrett b{q, bh, bl};
+        u64 h = next(), l = next();
+        if (it % 3 == 0) h %= q;
+        if (it % 11 == 0) h = l = ~0ull;
+        const u128 x = ((u128)h << 64) | l;
+        bad += barrett_reduce128(h, l, b) != (u64)(x % q);
+    }
+    return bad;
+}

@@ -228,3 +228,12 @@ def test_scalar_twiddle_stages_are_wave_uniform(emu):
     bad = emu.emu_twiddle_uniform(C.byref(flagged))
     assert flagged.value > 0
     assert bad == 0
+
+
+def test_wide_accumulation_and_reduction(emu):
+    """The carry-free 128-bit accumulator and the two-Shoup wide reduction
+    that every integer-limb kernel uses agree with exact 128-bit arithmetic
+    on random and all-ones operands, for odd moduli up to 2^62."""
+    emu.emu_wide_arith_check.restype = C.c_long
+    emu.emu_wide_arith_check.argtypes = [C.c_uint64, C.c_long]
+    assert emu.emu_wide_arith_check(1, 400000) == 0
