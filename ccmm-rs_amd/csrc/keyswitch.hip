@@ -1785,8 +1785,9 @@ static int key_switch_lane(const ckks_ksk *k, const u64 *x, u64 x_stride, const 
 // of different ring degrees, levels and kinds may mix): every phase runs for
 // all entries before the next, and the forward and inverse NTTs of all
 // entries go out as one launch per pass (mixed degrees share the descriptor
-// table), so small batches ride in the big batches' launches.  The plain
-// pipeline (no fusion modes, no hoisting, one stream); at most 16 entries.
+// table), so small batches ride in the big batches' launches.  Forward
+// row-pass fusion of the key MAC applies per entry (its key's mode); no
+// inverse-pass fusion, hoisting or stream split; at most 16 entries.
 struct KsMultiEntry {
     const ckks_ksk *k;
     const u64 *x, *add0, *add1;
@@ -1831,9 +1832,30 @@ static int key_switch_multi(ckks_context *ctx, const KsMultiEntry *e, size_t cou
         inv[i] = NttCall{k->plan, j.acc, j.acc, e[i].n_cts * 2, B, B * N, B * N};
         if ((rc = ks_decompose(j, s))) return rc;
     }
-    if ((rc = launch_ntt(ctx, fwd.data(), count, true))) return rc;
-    for (size_t i = 0; i < count; ++i)
-        if ((rc = ks_mac(jobs[i], s))) return rc;
+    // entries whose key runs the MAC fused into the digits' forward row pass
+    // (tuned, or set on the context) take launch_ntt_mac; the others share
+    // one forward launch per pass
+    std::vector<char> fused(count, 0);
+    std::vector<NttCall> plain;
+    for (size_t i = 0; i < count; ++i) {
+        const ckks_ksk *k = e[i].k;
+        const int f = ctx->ks_fusion == CKKS_KS_FUSION_AUTO ? k->tuned_fusion : ctx->ks_fusion;
+        const u32 logn = k->plan->logn;
+        fused[i] = (f == CKKS_KS_FUSION_XCD || f == CKKS_KS_FUSION_LINEAR) && logn > 12 && logn <= 17;
+        if (!fused[i]) plain.push_back(fwd[i]);
+    }
+    if ((rc = launch_ntt(ctx, plain.data(), plain.size(), true))) return rc;
+    for (size_t i = 0; i < count; ++i) {
+        const ckks_ksk *k = e[i].k;
+        if (!fused[i]) {
+            if ((rc = ks_mac(jobs[i], s))) return rc;
+            continue;
+        }
+        const int f = ctx->ks_fusion == CKKS_KS_FUSION_AUTO ? k->tuned_fusion : ctx->ks_fusion;
+        if ((rc = launch_ntt_mac(ctx, k->plan, jobs[i].digits, e[i].n_cts, k->dnum, k->limbs, k->d_kb, k->d_ka,
+                                 jobs[i].acc, f == CKKS_KS_FUSION_XCD)))
+            return rc;
+    }
     if ((rc = launch_ntt(ctx, inv.data(), count, false))) return rc;
     for (size_t i = 0; i < count; ++i)
         if ((rc = ks_moddown(jobs[i], s))) return rc;

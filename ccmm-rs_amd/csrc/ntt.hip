@@ -8,7 +8,7 @@
 // Degrees above 4096 take two passes (column pass, row pass) through an
 // inter-pass buffer.  The batch may be cut into chunks whose buffer is
 // reused chunk after chunk (ckks_set_ntt_chunk_bytes); by default the whole
-// batch is one chunk, which measured fastest on MI355X.
+// batch is one chunk.
 #include <algorithm>
 #include <cstdlib>
 #include <mutex>
@@ -382,9 +382,10 @@ __device__ __forceinline__ void opaque_tid(BlockExec &ex) {
 // input range of fp_mulmod) by the key entries without making them
 // canonical first; a key entry w enters with wp = fl(fl(w) * fl(1/q)), at
 // most two ulps from w / q, which moves y wp by under 2^-6 for |y| < 2^45,
-// so each term stays an exact integer within 0.52 q and dnum <= 8 terms sum
-// below 2^45 before fp_residue.  Integer limbs make the digit canonical and
-// multiply by Barrett reduction.
+// so each term stays an exact integer within 0.52 q; the sums are reduced
+// into half a q after every 8 digits, so they stay below 2^45 (fp_reduce's
+// and fp_residue's range) for any digit count.  Integer limbs make the digit
+// canonical and multiply by Barrett reduction.
 template <int TB, bool BLK, bool FP, int LG>
 __device__ void row_mac_body(const RowMacArgs &a, const DevPrime &P, u32 ct, u32 tile, u32 i,
                              u64 *lds) {
@@ -442,6 +443,13 @@ __device__ void row_mac_body(const RowMacArgs &a, const DevPrime &P, u32 ct, u32
             }
             // keep the scheduler from hoisting all 48 loads of the epilogue
             if ((r & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+        }
+        if (FP && (j & 7) == 7) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                sb[r] = f64_as_u64(fp_reduce(u64_as_f64(sb[r]), P.qd, P.qinv));
+                sa[r] = f64_as_u64(fp_reduce(u64_as_f64(sa[r]), P.qd, P.qinv));
+            }
         }
         __syncthreads();  // the next digit's pass reuses lds
     }

@@ -23,7 +23,11 @@ def _dev_batch(moduli, n, n_cts, seed):
     return out
 
 
-def test_mixed_degree_batch_matches_single_calls(pkg, ctx):
+@pytest.mark.parametrize("fusion", ["off", "xcd", "linear"])
+def test_mixed_degree_batch_matches_single_calls(pkg, ctx, fusion):
+    """Also with the key MAC fused into the forward row pass of the entries
+    whose degree takes two passes (the bounded and RNS-gadget keys stay
+    unfused at their degrees or with more than 8 digits' registers)."""
     rng = np.random.default_rng(404)
     entries, keys, mats = [], [], []
     # the benchmark's pair: 64 ct at N = 2^16 (25 limbs, [9, 8, 8], K = 7) and
@@ -51,10 +55,16 @@ def test_mixed_degree_batch_matches_single_calls(pkg, ctx):
         k = O.rotation_exponent_left(n, 1 << idx)
         keys.append((key, k, ct, q, basis, blocks, kind))
         entries.append((key, k, ct, torch.empty_like(ct)))
-    pkg.apply_galois_batch(ctx, entries)
-    torch.cuda.synchronize()
-    for (key, k, ct, out) in entries:
-        assert torch.equal(out, key.apply_galois(ct, k)), "batched entry != single call"
+    mode = {"off": ctx.KS_FUSION_OFF, "xcd": ctx.KS_FUSION_XCD, "linear": ctx.KS_FUSION_LINEAR}[fusion]
+    try:
+        ctx.set_key_switch_fusion(mode)
+        pkg.apply_galois_batch(ctx, entries)
+        torch.cuda.synchronize()
+        ctx.set_key_switch_fusion(ctx.KS_FUSION_OFF)
+        for (key, k, ct, out) in entries:
+            assert torch.equal(out, key.apply_galois(ct, k)), "batched entry != single call"
+    finally:
+        ctx.set_key_switch_fusion(ctx.KS_FUSION_AUTO)
     # one ciphertext of the N = 2^15 entry against the oracle
     key, k, ct, q, basis, blocks, _ = keys[1]
     kb, ka = mats[1]

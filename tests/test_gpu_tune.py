@@ -124,6 +124,31 @@ def test_fused_ntt_mac_matches_unfused(pkg, ctx, logn, variant):
         assert (g == want).all(), f"fusion mode {mode}"
 
 
+def test_fused_ntt_mac_with_many_digits(pkg, ctx):
+    """A bounded key with more than 8 digits (its FP64 sums are reduced
+    every 8 digits) gives the unfused residues with the MAC fused into the
+    forward row pass, and the unfused path matches the oracle."""
+    n, L, K, base_log = 1 << 13, 5, 2, 16
+    q_moduli = [Q0[0]] + Q40[:L - 1]
+    qp = q_moduli + P60[:K]
+    cnt = O.bounded_digit_count(q_moduli, base_log)
+    assert cnt > 8
+    rng = np.random.default_rng(77)
+    plan = pkg.PreparedRnsNttPlan(ctx, qp, n)
+    kb, ka = rand_residues(rng, qp, n, cnt), rand_residues(rng, qp, n, cnt)
+    key = pkg.BoundedKeySwitchKey(plan, L, base_log, kb, ka)
+    ct = pkg.to_device(rand_residues(rng, q_moduli, n, polys=4).reshape(2, 2, L, n))
+    k = O.rotation_exponent_left(n, 5)
+    try:
+        ctx.set_key_switch_fusion(ctx.KS_FUSION_OFF)
+        want = pkg.to_host(key.apply_galois(ct, k))
+        for mode in (ctx.KS_FUSION_XCD, ctx.KS_FUSION_LINEAR):
+            ctx.set_key_switch_fusion(mode)
+            assert (pkg.to_host(key.apply_galois(ct, k)) == want).all(), f"fusion mode {mode}"
+    finally:
+        ctx.set_key_switch_fusion(ctx.KS_FUSION_AUTO)
+
+
 @pytest.mark.parametrize("n_cts", [8, 9, 16])
 def test_split_key_switch_matches_single_stream(pkg, ctx, n_cts):
     """Halves of the batch on two streams give the single-stream residues,
