@@ -72,6 +72,11 @@ _SIGNATURES = {
     "ckks_rns_plan_create": (C.c_int, [_p, _p, _sz, _sz, C.POINTER(_p)]),
     "ckks_rns_plan_destroy": (C.c_int, [_p]),
     "ckks_rns_plan_psi": (C.c_int, [_p, _p]),
+    "ckks_ntt64_plan_create": (C.c_int, [_p, C.c_uint64, _sz, C.POINTER(_p)]),
+    "ckks_ntt64_plan_destroy": (C.c_int, [_p]),
+    "ckks_ntt64_plan_psi": (C.c_int, [_p, C.POINTER(C.c_uint64)]),
+    "ckks_ntt64_forward": (C.c_int, [_p, _p, _p, _sz]),
+    "ckks_ntt64_inverse": (C.c_int, [_p, _p, _p, _sz]),
     "ckks_ntt_forward": (C.c_int, [_p, _p, _p, _sz, _sz, _sz]),
     "ckks_ntt_inverse": (C.c_int, [_p, _p, _p, _sz, _sz, _sz]),
     "ckks_ntt_forward_noalloc": (C.c_int, [_p, _p, _p, _sz, _sz, _sz]),
@@ -612,6 +617,54 @@ class PreparedRnsNttPlan:
         # the context releases its plans itself when it is closed first
         if getattr(self, "_h", None) and self.ctx.handle:
             lib().ckks_rns_plan_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class PreparedNtt64:
+    """PreparedNttPlan (ring/prepared_ntt.rs:34, forward :200, inverse :370)
+    for one prime of any width up to 64 bits, the reference's 64-bit
+    parameter set included (ring/ntt_parameter_sets.rs:11-12).  Residues are
+    u64 values carried in int64 tensors ([polys][N], values >= 2^63 as their
+    two's-complement bit patterns)."""
+
+    def __init__(self, ctx: Context, q: int, degree: int):
+        self.ctx = ctx
+        self.q, self.degree = int(q), int(degree)
+        h = _p()
+        _check(lib().ckks_ntt64_plan_create(ctx.handle, self.q, self.degree, C.byref(h)))
+        self._h = h
+
+    @property
+    def psi(self) -> int:
+        v = C.c_uint64()
+        _check(lib().ckks_ntt64_plan_psi(self._h, C.byref(v)))
+        return int(v.value)
+
+    def _run(self, fn, x, out):
+        if x.numel() == 0 or x.numel() % self.degree:
+            raise ValueError("tensor size must be a positive multiple of N")
+        n = x.numel() // self.degree
+        out = x if out is None else out
+        _check(fn(self._h, _ptr(x, x.numel(), name="x"), _ptr(out, x.numel(), name="out"), n))
+        return out
+
+    def forward(self, x, out=None):
+        """Forward negacyclic NTT of every polynomial in x; out may be x."""
+        return self._run(lib().ckks_ntt64_forward, x, out)
+
+    def inverse(self, x, out=None):
+        return self._run(lib().ckks_ntt64_inverse, x, out)
+
+    def close(self):
+        # the context releases its plans itself when it is closed first
+        if getattr(self, "_h", None) and self.ctx.handle:
+            lib().ckks_ntt64_plan_destroy(self._h)
         self._h = None
 
     def __del__(self):

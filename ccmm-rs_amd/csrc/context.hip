@@ -82,6 +82,7 @@ int ckks_context_destroy(ckks_context *ctx) {
     while (!ctx->graphs.empty()) ckks_graph_destroy(*ctx->graphs.begin());
     while (!ctx->keys.empty()) ckks_ksk_destroy(*ctx->keys.begin());
     while (!ctx->plans.empty()) ckks_rns_plan_destroy(*ctx->plans.begin());
+    while (!ctx->ntt64_plans.empty()) ckks_ntt64_plan_destroy(*ctx->ntt64_plans.begin());
     for (auto &kv : ctx->tables) {
         hipFree(kv.second.fwd);
         hipFree(kv.second.inv);

@@ -73,6 +73,7 @@ struct TwiddleTables {
 }  // namespace ck
 
 struct ckks_rns_plan;
+struct ckks_ntt64_plan;
 struct ckks_ksk;
 struct ckks_graph;
 struct ckks_server;
@@ -82,6 +83,7 @@ struct ckks_context {
     hipStream_t stream = nullptr;  // nullptr = the null stream
     // plans and keys created from this context, released with it
     std::set<ckks_rns_plan *> plans;
+    std::set<ckks_ntt64_plan *> ntt64_plans;
     std::set<ckks_ksk *> keys;
     std::set<ckks_graph *> graphs;
     std::set<ckks_server *> servers;  // destroyed before the keys and plans they use
@@ -125,6 +127,16 @@ struct ProfileGuard {
     ckks_execution_profile saved;
     explicit ProfileGuard(ckks_context *c) : ctx(c), saved(c->profile) {}
     ~ProfileGuard() { ctx->profile = saved; }
+};
+
+// Single-modulus plan for primes up to 64 bits (canonical butterflies).
+struct ckks_ntt64_plan {
+    ckks_context *ctx = nullptr;
+    uint64_t q = 0, psi = 0;
+    uint32_t logn = 0;
+    size_t degree = 0;
+    ck::DevPrime prime{};  // device twiddle tables, N^-1
+    ck::Tw *d_fwd = nullptr, *d_inv = nullptr;
 };
 
 struct ckks_rns_plan {

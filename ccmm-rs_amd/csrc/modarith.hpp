@@ -48,6 +48,26 @@ CK_HD u64 shoup(u64 x, u64 w, u64 wq, u64 q) {
     return r >= q ? r - q : r;
 }
 
+// ---- moduli up to 2^64 (the reference's 64-bit NTT parameter set) ---------
+// Canonical residues throughout.  Shoup's remainder x w - floor(x wq / 2^64) q
+// lies in [0, 2q) for any q < 2^64 and x < 2^64, which can exceed 2^64 when
+// q > 2^63, so its high word (0 or 1) is formed from the two products' high
+// words and decides the one correction.
+CK_HD u64 shoup_wide(u64 x, u64 w, u64 wq, u64 q) {
+    const u64 qh = mulhi64(x, wq);
+    const u64 pl = x * w, ql = qh * q;
+    const u64 lo = pl - ql;
+    const u64 hi = mulhi64(x, w) - mulhi64(qh, q) - (pl < ql ? 1u : 0u);
+    return (hi != 0 || lo >= q) ? lo - q : lo;
+}
+
+// a + b mod q for a, b < q < 2^64: a carry out of 64 bits means the sum
+// exceeds q.
+CK_HD u64 add_mod_wide(u64 a, u64 b, u64 q) {
+    const u64 s = a + b;
+    return (s < a || s >= q) ? s - q : s;
+}
+
 CK_HD u64 add_mod(u64 a, u64 b, u64 q) {
     u64 s = a + b;
     return s >= q ? s - q : s;

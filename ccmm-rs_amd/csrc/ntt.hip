@@ -11,8 +11,11 @@
 // batch is one chunk.
 #include <algorithm>
 #include <cstdlib>
+#include <memory>
 #include <mutex>
+#include <vector>
 
+#include "host_math.hpp"
 #include "internal.hpp"
 #include "ntt_pass.hpp"
 
@@ -538,11 +541,145 @@ int launch_ntt_mac(ckks_context *ctx, const ckks_rns_plan *plan, const u64 *digi
     return CKKS_OK;
 }
 
+// ---- single-modulus NTT for primes up to 64 bits ----------------------------------
+//
+// The tile passes of the RNS kernels with canonical-residue butterflies
+// (NttPass<..., WIDE>): one block per (polynomial, tile), 4096-residue tiles,
+// the plain inter-pass layout.
+
+struct NttWideArgs {
+    const u64 *src;
+    u64 *dst;
+    u64 *tmp;  // inter-pass buffer, one polynomial after another
+    DevPrime p;
+    u32 logn;
+};
+
+template <bool FWD, bool FIRST>
+__global__ __launch_bounds__(256) void k_ntt_wide(NttWideArgs a) {
+    __shared__ u64 lds[1 << 12];
+    const u32 tiles = ntt_tiles(a.logn);
+    const u32 poly = blockIdx.x / tiles, tile = blockIdx.x % tiles;
+    const size_t N = (size_t)1 << a.logn;
+    TileJob job;
+    job.tw = FWD ? a.p.fwd : a.p.inv;
+    job.rowf = nullptr;
+    job.q = a.p.q;
+    job.ninv = a.p.ninv;
+    job.ninv_q = a.p.ninv_q;
+    job.qd = job.qinv = job.ninv_d = job.ninv_p = 0.0;
+    job.logn = a.logn;
+    job.tile = tile;
+    const bool two = ntt_two_pass(a.logn);
+    job.src = FIRST ? a.src + poly * N : a.tmp + poly * N;
+    job.dst = FIRST && two ? a.tmp + poly * N : a.dst + poly * N;
+    u64 regs[16];
+    BlockExec ex{256};
+    auto regs_of = [&](u32) -> u64(&)[16] { return regs; };
+    ntt_dispatch<FWD, FIRST, false, 12, false, true>(ex, regs_of, job, lds);
+}
+
+static int ntt64_run(const ckks_ntt64_plan *plan, const u64 *in, u64 *out, size_t n_polys, bool fwd) {
+    if (!plan || !in || !out) return fail(CKKS_ERR_INVALID, "null argument");
+    if (n_polys == 0) return CKKS_OK;
+    ckks_context *ctx = plan->ctx;
+    const size_t N = plan->degree;
+    const u32 tiles = ntt_tiles(plan->logn);
+    if (n_polys * tiles >= (1ull << 31)) return fail(CKKS_ERR_INVALID, "NTT batch too large for one launch");
+    NttWideArgs a{in, out, nullptr, plan->prime, plan->logn};
+    const bool two = ntt_two_pass(plan->logn);
+    if (two) {
+        void *ws;
+        int rc = ctx->workspace(0, n_polys * N * sizeof(u64), &ws);
+        if (rc) return rc;
+        a.tmp = (u64 *)ws;
+    }
+    ckks_execution_profile &pc = ctx->counts();
+    (fwd ? pc.ntt_forwards : pc.ntt_inverses) += n_polys;
+    pc.ntt_limb_transforms += n_polys;
+    const dim3 g((u32)(n_polys * tiles)), t(256);
+    if (fwd) {
+        hipLaunchKernelGGL((k_ntt_wide<true, true>), g, t, 0, ctx->stream, a);
+        if (two) hipLaunchKernelGGL((k_ntt_wide<true, false>), g, t, 0, ctx->stream, a);
+    } else {
+        hipLaunchKernelGGL((k_ntt_wide<false, true>), g, t, 0, ctx->stream, a);
+        if (two) hipLaunchKernelGGL((k_ntt_wide<false, false>), g, t, 0, ctx->stream, a);
+    }
+    CK_HIP(hipGetLastError());
+    return CKKS_OK;
+}
+
 }  // namespace ck
 
 using namespace ck;
 
 extern "C" {
+
+int ckks_ntt64_plan_create(ckks_context *ctx, uint64_t q, size_t degree, ckks_ntt64_plan **out) {
+    if (!ctx || !out) return fail(CKKS_ERR_INVALID, "null argument");
+    if (degree == 0 || (degree & (degree - 1))) return fail(CKKS_ERR_INVALID, "NTT degree must be a positive power of two");
+    uint32_t logn = 0;
+    while (((size_t)1 << logn) < degree) ++logn;
+    if (logn < 10 || logn > 17) return fail(CKKS_ERR_UNSUPPORTED, "GPU NTT supports degrees 2^10 .. 2^17");
+    if (q < 3 || (q - 1) % (2 * (uint64_t)degree) != 0) return fail(CKKS_ERR_INVALID, "2N must divide q - 1");
+    if (!host::is_prime(q)) return fail(CKKS_ERR_INVALID, "NTT modulus must be prime");
+    const uint64_t psi = host::negacyclic_root(q, degree);
+    if (!psi) return fail(CKKS_ERR_INVALID, "no primitive 2N-th root found");
+    CK_HIP(hipSetDevice(ctx->device));
+    std::unique_ptr<ckks_ntt64_plan> plan(new ckks_ntt64_plan());
+    plan->ctx = ctx;
+    plan->q = q;
+    plan->psi = psi;
+    plan->logn = logn;
+    plan->degree = degree;
+    std::vector<uint64_t> fwd, inv;
+    host::twiddle_table(q, psi, logn, fwd);
+    host::twiddle_table(q, host::invmod(psi, q), logn, inv);
+    const size_t bytes = fwd.size() * sizeof(uint64_t);
+    CK_HIP(hipMalloc(&plan->d_fwd, bytes));
+    if (hipMalloc(&plan->d_inv, bytes) != hipSuccess) {
+        hipFree(plan->d_fwd);
+        return fail(CKKS_ERR_OOM, "twiddle table allocation failed");
+    }
+    CK_HIP(hipMemcpy(plan->d_fwd, fwd.data(), bytes, hipMemcpyHostToDevice));
+    CK_HIP(hipMemcpy(plan->d_inv, inv.data(), bytes, hipMemcpyHostToDevice));
+    DevPrime &p = plan->prime;
+    p.q = q;
+    p.fwd = plan->d_fwd;
+    p.inv = plan->d_inv;
+    p.ninv = host::invmod(degree % q, q);
+    p.ninv_q = host::shoup_quotient(p.ninv, q);
+    p.logn = logn;
+    ctx->ntt64_plans.insert(plan.get());
+    *out = plan.release();
+    return CKKS_OK;
+}
+
+int ckks_ntt64_plan_destroy(ckks_ntt64_plan *plan) {
+    if (!plan) return CKKS_OK;
+    hipSetDevice(plan->ctx->device);
+    hipStreamSynchronize(plan->ctx->stream);
+    plan->ctx->ntt64_plans.erase(plan);
+    hipFree(plan->d_fwd);
+    hipFree(plan->d_inv);
+    delete plan;
+    return CKKS_OK;
+}
+
+int ckks_ntt64_plan_psi(const ckks_ntt64_plan *plan, uint64_t *psi) {
+    if (!plan || !psi) return fail(CKKS_ERR_INVALID, "null argument");
+    *psi = plan->psi;
+    return CKKS_OK;
+}
+
+int ckks_ntt64_forward(const ckks_ntt64_plan *plan, const uint64_t *in, uint64_t *out, size_t n_polys) {
+    return ntt64_run(plan, in, out, n_polys, true);
+}
+
+int ckks_ntt64_inverse(const ckks_ntt64_plan *plan, const uint64_t *in, uint64_t *out, size_t n_polys) {
+    return ntt64_run(plan, in, out, n_polys, false);
+}
+
 
 int ckks_ntt_forward(const ckks_rns_plan *plan, const uint64_t *in, uint64_t *out,
                      size_t n_polys, size_t n_limbs, size_t poly_stride) {

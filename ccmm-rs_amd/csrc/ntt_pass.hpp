@@ -47,7 +47,9 @@ struct PlainSrc {
 };
 
 // BLK: the inter-pass buffer uses the blocked layout (internal_offset).
-template <int L, int CL, PassKind KIND, bool FWD, bool FP, bool BLK = false>
+// WIDE: integer arithmetic on canonical residues for moduli in [2^62, 2^64)
+// (the lazy [0, 4q) butterflies need q < 2^62).
+template <int L, int CL, PassKind KIND, bool FWD, bool FP, bool BLK = false, bool WIDE = false>
 struct NttPass {
     typedef TileShape<L, CL> S;
     static constexpr int NR = n_rounds(L);
@@ -81,9 +83,9 @@ struct NttPass {
         constexpr u32 SM = uniform_twiddle_mask(g, L, CL, K, s0, FWD, false);
         constexpr bool RU = wave_uniform_bits(g, L, L + CL);
         if (FWD)
-            fwd_round<L, CL, K, FP, UM, SM, RU>(v, g, tid, s0, row, pa, job);
+            fwd_round<L, CL, K, FP, UM, SM, RU, WIDE>(v, g, tid, s0, row, pa, job);
         else
-            inv_round<L, CL, K, FP, UM, SM, RU>(v, g, tid, s0, row, pa, job);
+            inv_round<L, CL, K, FP, UM, SM, RU, WIDE>(v, g, tid, s0, row, pa, job);
     }
 
     // global offset (within the limb) of the element at tile coordinate e on
@@ -133,6 +135,7 @@ struct NttPass {
             return fp_canonical(d, job.qd, job.qinv);
         }
         u64 q = job.q;
+        if (WIDE) return FWD ? x : shoup_wide(x, job.ninv, job.ninv_q, q);
         if (FWD) {
             u64 two_q = q << 1;
             x = x >= two_q ? x - two_q : x;
@@ -301,13 +304,13 @@ CK_HD u32 ntt_tiles(u32 logn, u32 tb = 12) { return logn > 12 ? (1u << (logn - t
 // transform (forward: column/full, inverse: row/full).  TB = log2 of the
 // two-pass tile size (12: 256 threads, 13: 512 threads; the full pass for
 // N <= 4096 always uses 256 threads); BLK = blocked inter-pass layout.
-template <bool FWD, bool FIRST, bool FP, int TB = 12, bool BLK = false, class Exec, class Regs,
+template <bool FWD, bool FIRST, bool FP, int TB = 12, bool BLK = false, bool WIDE = false, class Exec, class Regs,
           class Src = PlainSrc>
 CK_HD void ntt_dispatch(Exec &ex, Regs &&regs_of, const TileJob &job, u64 *lds, const Src &src = Src{}) {
     switch (job.logn) {
 #define CK_FULL(LG)                                                   \
     case LG:                                                          \
-        if (FIRST) NttPass<LG, 0, PASS_FULL, FWD, FP>::run(ex, regs_of, job, lds, src); \
+        if (FIRST) NttPass<LG, 0, PASS_FULL, FWD, FP, false, WIDE>::run(ex, regs_of, job, lds, src); \
         break;
         CK_FULL(10)
         CK_FULL(11)
@@ -317,9 +320,9 @@ CK_HD void ntt_dispatch(Exec &ex, Regs &&regs_of, const TileJob &job, u64 *lds, 
     case LG: {                                                                              \
         constexpr int N1 = LG - (LG + 1) / 2, N2 = (LG + 1) / 2;                            \
         if (FWD == FIRST)                                                                   \
-            NttPass<N1, TB - N1, PASS_COL, FWD, FP, BLK>::run(ex, regs_of, job, lds, src);  \
+            NttPass<N1, TB - N1, PASS_COL, FWD, FP, BLK, WIDE>::run(ex, regs_of, job, lds, src);  \
         else                                                                                \
-            NttPass<N2, TB - N2, PASS_ROW, FWD, FP, BLK>::run(ex, regs_of, job, lds, src);  \
+            NttPass<N2, TB - N2, PASS_ROW, FWD, FP, BLK, WIDE>::run(ex, regs_of, job, lds, src);  \
         break;                                                                              \
     }
         CK_TWO(13)

@@ -194,6 +194,21 @@ CK_HD void gs_bfly(u64 &x, u64 &y, const Tw &w, u64 q) {
     y = shoup_lazy(d, w.w, w.wq, q);
 }
 
+// Butterflies of moduli in [2^62, 2^64) on canonical residues (shoup_wide);
+// sub_mod is exact for any q < 2^64.
+CK_HD void ct_bfly_wide(u64 &x, u64 &y, const Tw &w, u64 q) {
+    const u64 t = shoup_wide(y, w.w, w.wq, q);
+    const u64 a = x;
+    x = add_mod_wide(a, t, q);
+    y = sub_mod(a, t, q);
+}
+
+CK_HD void gs_bfly_wide(u64 &x, u64 &y, const Tw &w, u64 q) {
+    const u64 a = x, b = y;
+    x = add_mod_wide(a, b, q);
+    y = shoup_wide(sub_mod(a, b, q), w.w, w.wq, q);
+}
+
 // FP64 butterflies on bit-cast doubles (see fp_mulmod).  Forward: no
 // reduction; |x| grows by at most 0.51 q per stage, so after 17 stages it is
 // below 10 q < 2^46 (q < 2^42).  Inverse: the product output is within
@@ -341,7 +356,7 @@ CK_HD Tw row_twiddle_fp(const TileJob &job, u32 sidx, u32 fidx) {
 // Forward round: stages s0..s0+k-1 (butterfly bit of stage s is L-1-s).
 // SM: stages whose shared index (row-factored twiddles) is wave-uniform;
 // RU: the row of an element is wave-uniform.
-template <int L, int CL, int K, bool FP, u32 UM = 0, u32 SM = 0, bool RU = false>
+template <int L, int CL, int K, bool FP, u32 UM = 0, u32 SM = 0, bool RU = false, bool WIDE = false>
 CK_HD void fwd_round(u64 (&v)[16], const Geom &g, u32 tid, int s0, bool row_base,
                      const PassAddr &pa, const TileJob &job) {
     const bool fac = FP && row_base;  // FP64 row passes always read row factors
@@ -369,6 +384,8 @@ CK_HD void fwd_round(u64 (&v)[16], const Geom &g, u32 tid, int s0, bool row_base
             }
             if (FP)
                 ct_bfly_fp(v[i], v[i | (1 << rb)], w, job.qd);
+            else if (WIDE)
+                ct_bfly_wide(v[i], v[i | (1 << rb)], w, job.q);
             else
                 ct_bfly(v[i], v[i | (1 << rb)], w, job.q);
         }
@@ -377,7 +394,7 @@ CK_HD void fwd_round(u64 (&v)[16], const Geom &g, u32 tid, int s0, bool row_base
 
 // Inverse round: GS stages u0..u0+k-1 (butterfly bit of stage u is u; its
 // twiddle level is L - 1 - u).
-template <int L, int CL, int K, bool FP, u32 UM = 0, u32 SM = 0, bool RU = false>
+template <int L, int CL, int K, bool FP, u32 UM = 0, u32 SM = 0, bool RU = false, bool WIDE = false>
 CK_HD void inv_round(u64 (&v)[16], const Geom &g, u32 tid, int u0, bool row_base,
                      const PassAddr &pa, const TileJob &job) {
     const bool fac = FP && row_base;  // FP64 row passes always read row factors
@@ -407,6 +424,8 @@ CK_HD void inv_round(u64 (&v)[16], const Geom &g, u32 tid, int u0, bool row_base
                 gs_bfly_fp<true>(v[i], v[i | (1 << rb)], w, job.qd, job.qinv);
             else if (FP)
                 gs_bfly_fp<false>(v[i], v[i | (1 << rb)], w, job.qd, job.qinv);
+            else if (WIDE)
+                gs_bfly_wide(v[i], v[i | (1 << rb)], w, job.q);
             else
                 gs_bfly(v[i], v[i | (1 << rb)], w, job.q);
         }

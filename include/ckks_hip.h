@@ -169,6 +169,21 @@ int ckks_ntt_inverse(const ckks_rns_plan *plan, const uint64_t *in, uint64_t *ou
 int ckks_ntt_forward_noalloc(const ckks_rns_plan *plan, const uint64_t *in, uint64_t *out,
                              size_t n_polys, size_t n_limbs, size_t poly_stride);
 
+/* PreparedNttPlan::new / forward / inverse (ring/prepared_ntt.rs:34,200,370)
+ * for one prime of any width up to 64 bits, the reference's 64-bit
+ * parameter set included (ring/ntt_parameter_sets.rs:11-12: q = 2^64 -
+ * 114687, psi at N = 4096).  degree 2^10 .. 2^17 with 2N | q-1; psi as
+ * find_negacyclic_root.  Residues stay canonical in every butterfly (the RNS
+ * plan's lazy butterflies need q < 2^62), so this plan serves moduli the RNS
+ * plan refuses.  in may equal out; n_polys polynomials of `degree`
+ * residues each, contiguous. */
+typedef struct ckks_ntt64_plan ckks_ntt64_plan;
+int ckks_ntt64_plan_create(ckks_context *ctx, uint64_t q, size_t degree, ckks_ntt64_plan **out);
+int ckks_ntt64_plan_destroy(ckks_ntt64_plan *plan);
+int ckks_ntt64_plan_psi(const ckks_ntt64_plan *plan, uint64_t *psi);
+int ckks_ntt64_forward(const ckks_ntt64_plan *plan, const uint64_t *in, uint64_t *out, size_t n_polys);
+int ckks_ntt64_inverse(const ckks_ntt64_plan *plan, const uint64_t *in, uint64_t *out, size_t n_polys);
+
 /* Mixed-degree / mixed-level batch in one launch per pass (at most 16
  * entries; every entry's plan must belong to the same context). */
 typedef struct ckks_ntt_batch {

@@ -11,7 +11,7 @@
 
 using namespace ck;
 
-template <bool FWD, bool FIRST, bool FP, int TB, bool BLK>
+template <bool FWD, bool FIRST, bool FP, int TB, bool BLK, bool WIDE = false>
 static void run_pass(const TileJob &base, u32 tiles) {
     static u64 regs[1 << (TB - 4)][16];
     static u64 lds[1 << TB];
@@ -21,23 +21,23 @@ static void run_pass(const TileJob &base, u32 tiles) {
         BlockExec ex{1u << (TB - 4)};
         std::memset(lds, 0xA5, sizeof lds);
         auto regs_of = [&](u32 tid) -> u64(&)[16] { return regs[tid]; };
-        ntt_dispatch<FWD, FIRST, FP, TB, BLK>(ex, regs_of, job, lds);
+        ntt_dispatch<FWD, FIRST, FP, TB, BLK, WIDE>(ex, regs_of, job, lds);
     }
 }
 
-template <bool FWD, bool FIRST, bool FP>
+template <bool FWD, bool FIRST, bool FP, bool WIDE = false>
 static void run_variant(const TileJob &job, u32 logn, int tb, int blk) {
     const bool two = ntt_two_pass(logn);
     if (two && tb == 13) {
-        if (blk) run_pass<FWD, FIRST, FP, 13, true>(job, ntt_tiles(logn, 13));
-        else run_pass<FWD, FIRST, FP, 13, false>(job, ntt_tiles(logn, 13));
+        if (blk) run_pass<FWD, FIRST, FP, 13, true, WIDE>(job, ntt_tiles(logn, 13));
+        else run_pass<FWD, FIRST, FP, 13, false, WIDE>(job, ntt_tiles(logn, 13));
     } else {
-        if (blk && two) run_pass<FWD, FIRST, FP, 12, true>(job, ntt_tiles(logn));
-        else run_pass<FWD, FIRST, FP, 12, false>(job, ntt_tiles(logn));
+        if (blk && two) run_pass<FWD, FIRST, FP, 12, true, WIDE>(job, ntt_tiles(logn));
+        else run_pass<FWD, FIRST, FP, 12, false, WIDE>(job, ntt_tiles(logn));
     }
 }
 
-// Transforms one limb of 2^logn residues modulo q (< 2^62) with kernel
+// Transforms one limb of 2^logn residues modulo q (< 2^64) with kernel
 // variant (tb = log2 two-pass tile size 12 or 13, blk = blocked inter-pass
 // layout).  FP64 primes of two-pass degrees build their row twiddles from
 // the row factors, as the kernels do.
@@ -71,10 +71,13 @@ extern "C" int emu_ntt_variant(const uint64_t *in, uint64_t *out, uint64_t q, ui
     bool two = ntt_two_pass(logn);
     job.src = src.data();
     job.dst = two ? tmp.data() : out;
-#define CK_RUN(F, S)                                              \
-    do {                                                          \
-        if (fp) run_variant<F, S, true>(job, logn, tb, blk);      \
-        else run_variant<F, S, false>(job, logn, tb, blk);        \
+    // moduli from 2^62 up take the canonical-residue (wide) butterflies
+    const bool wide = q >= (1ull << 62);
+#define CK_RUN(F, S)                                                   \
+    do {                                                               \
+        if (fp) run_variant<F, S, true>(job, logn, tb, blk);           \
+        else if (wide) run_variant<F, S, false, true>(job, logn, tb, blk); \
+        else run_variant<F, S, false>(job, logn, tb, blk);             \
     } while (0)
     if (forward) CK_RUN(true, true);
     else CK_RUN(false, true);
@@ -309,3 +312,6 @@ extern "C" long emu_wide_arith_check(uint64_t seed, long iters) {
     }
     return bad;
 }
+
+// host::negacyclic_root, the root every plan is built with.
+extern "C" uint64_t emu_negacyclic_root(uint64_t q, uint64_t degree) { return host::negacyclic_root(q, degree); }

@@ -237,3 +237,37 @@ def test_wide_accumulation_and_reduction(emu):
     emu.emu_wide_arith_check.restype = C.c_long
     emu.emu_wide_arith_check.argtypes = [C.c_uint64, C.c_long]
     assert emu.emu_wide_arith_check(1, 400000) == 0
+
+
+@pytest.mark.parametrize("logn", [10, 12, 13, 14, 16, 17])
+def test_emulated_wide_modulus_ntt_matches_oracle(emu, logn):
+    """Moduli from 2^62 to 2^64 run canonical-residue butterflies (the
+    64-bit parameter set's prime where 2N | q - 1, else a 63-bit NTT prime),
+    in every kernel variant, against the oracle; inputs at q - 1 included."""
+    emu.emu_ntt_variant.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32, C.c_int,
+                                    C.c_int, C.c_int]
+    n = 1 << logn
+    qs = [O.ntt_primes(63, n, 1)[0]]
+    if logn <= 13:
+        qs.append(18446744073709436929)
+    rng = np.random.default_rng(500 + logn)
+    for q in qs:
+        assert q >= 1 << 62
+        plan = O.Plan(q, n)
+        variants = [(12, 0)] if logn <= 12 else [(12, 0), (12, 1)] + ([(13, 1)] if logn >= 14 else [])
+        for x in (rng.integers(0, q, n, dtype=np.uint64), np.full(n, q - 1, dtype=np.uint64)):
+            for tb, blk in variants:
+                out = np.zeros(n, dtype=np.uint64)
+                assert emu.emu_ntt_variant(x.ctypes.data, out.ctypes.data, q, logn, 1, tb, blk) == 0
+                assert (out == plan.forward(x)).all()
+                assert emu.emu_ntt_variant(x.ctypes.data, out.ctypes.data, q, logn, 0, tb, blk) == 0
+                assert (out == plan.inverse(x)).all()
+
+
+def test_root_selection_reproduces_the_64_bit_parameter_set(emu):
+    """The plans' psi search gives the reference's 64-bit parameter (and the
+    32-bit one): ring/ntt_parameter_sets.rs:8-12."""
+    emu.emu_negacyclic_root.restype = C.c_uint64
+    emu.emu_negacyclic_root.argtypes = [C.c_uint64, C.c_uint64]
+    assert emu.emu_negacyclic_root(18446744073709436929, 4096) == 5975861664659593359
+    assert emu.emu_negacyclic_root(4294828033, 4096) == 1953722822
