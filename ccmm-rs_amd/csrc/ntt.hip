@@ -472,8 +472,10 @@ __device__ void row_mac_body(const RowMacArgs &a, const DevPrime &P, u32 ct, u32
     }
 }
 
+// At most 128 VGPRs (4 waves per SIMD): the unbounded build takes 144
+// (occupancy 3) and the bound costs no spills.
 template <int TB, bool BLK, int LG>
-__global__ __launch_bounds__(1 << (TB - 4)) void k_ntt_row_mac(RowMacArgs a) {
+__global__ __launch_bounds__(1 << (TB - 4), 4) void k_ntt_row_mac(RowMacArgs a) {
     __shared__ u64 lds[1 << TB];
     const u32 tiles = 1u << (LG - TB);
     const u32 lb = a.xcd_order ? xcd_contiguous_block(blockIdx.x, gridDim.x) : blockIdx.x;
