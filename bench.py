@@ -10,10 +10,19 @@ Workload (batched core primitives, N = 2^16):
   (uniform, canonical) and keys are synthetic NTT-domain material: the work
   done per rotation does not depend on the values.
 
+  The step (--rotation hoisted, the default) cuts and transforms each
+  batch's canonical digits once (ckks_hoist) and then runs one
+  ckks_rotate_hoisted_batch per exponent; the hoisted MAC plus a
+  per-exponent correction of the negated coefficients' lift gives the
+  residues of the per-rotation key switch, which --rotation canonical times
+  (ckks_apply_galois_batch per exponent) and which is reported beside it.
+
 Reported (one JSON line):
-  value           batched hybrid-key-switched rotations per second
+  value           batched hybrid-key-switched rotations per second of the step
   batched_n65536_rotations_per_s
-                  the N = 2^16 calls of the step alone (HIP events around each)
+                  one step over the N = 2^16 batch alone
+  hoisted / canonical
+                  both steps: rates, N = 2^16 rates, call latencies
   plain_*         the plain per-op GPU path: the reference's own rotation
                   algorithm (RNS-gadget key switch with singleton blocks,
                   rotate_left_rns_ckks_with_prepared_ntt) one ciphertext per
@@ -26,8 +35,9 @@ Reported (one JSON line):
                   the reference's bounded-digit rotation shape, canonical
                   and hoisted
   ntt_*           forward NTT over the batch's 96 digit limbs per ciphertext
-  p50/p99_ms      latency of one batched rotation call (64 ciphertexts at
-                  N = 2^16 and the 16 at N = 2^15 in one launch sequence)
+  p50/p99_ms      latency of one batched rotation call of the step (64
+                  ciphertexts at N = 2^16 and the 16 at N = 2^15 in one
+                  launch sequence; hoisted: after the step's hoist)
   roofline        the forward NTT (dominant kernel pair) against HBM peak
   cpu_baseline    the CPU oracle's hybrid rotation on all the host cores the
                   job may use (one ciphertext per thread), bounded sample;
@@ -393,6 +403,10 @@ def main():
     ap.add_argument("--mixed-launch", choices=["batch", "separate"], default="batch",
                     help="the N=2^15 ciphertexts ride in the N=2^16 call (ckks_apply_galois_batch) "
                          "or run as calls of their own")
+    ap.add_argument("--rotation", choices=["hoisted", "canonical"], default="hoisted",
+                    help="timed step: each batch's digits hoisted once per step and rotated 15 times "
+                         "(ckks_hoist + ckks_rotate_hoisted_batch, the canonical residues), or every "
+                         "rotation decomposing its own digits (ckks_apply_galois_batch)")
     ap.add_argument("--plain-sample", type=int, default=24, help="plain-path rotations timed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -506,7 +520,9 @@ def main():
 
     batched = args.mixed and args.mixed_launch == "batch"
 
-    def step(lat=None, only16=False):
+    def step_canonical(lat=None, only16=False):
+        """15 calls, each rotating the 64 + 16 ciphertexts (one launch
+        sequence) and decomposing their rotated digits itself."""
         for idx, (k, key) in enumerate(keys):
             if lat is not None:
                 e0 = torch.cuda.Event(enable_timing=True)
@@ -525,53 +541,82 @@ def main():
                 k2, key2 = keys2[idx]
                 key2.apply_galois(cts2, k2, out2)
 
-    print("# warmup", file=sys.stderr, flush=True)
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
+    def step_hoisted(lat=None, only16=False, hoist_lat=None):
+        """The canonical digits of each batch cut and transformed once per
+        step (ckks_hoist), then 15 calls each rotating the 64 + 16
+        ciphertexts from them (ckks_rotate_hoisted_batch: the inverse NTTs
+        of both degrees in one launch per pass); the residues are those of
+        step_canonical."""
+        if hoist_lat is not None:
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+        h16 = keys[0][1].hoist(cts)
+        h15 = keys2[0][1].hoist(cts2) if args.mixed and not only16 else None
+        if hoist_lat is not None:
+            e1.record()
+            hoist_lat.append((e0, e1))
+        for idx, (k, key) in enumerate(keys):
+            if lat is not None:
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record()
+            entries = [(h16, key, k, cts, out)]
+            if h15 is not None and batched:
+                k2, key2 = keys2[idx]
+                entries.append((h15, key2, k2, cts2, out2))
+            pkg.rotate_hoisted_batch(ctx, entries)
+            if lat is not None:
+                e1.record()
+                lat.append((e0, e1))
+            if h15 is not None and not batched:
+                k2, key2 = keys2[idx]
+                key2.rotate_hoisted(h15, cts2, k2, out2)
+        h16.close()
+        if h15 is not None:
+            h15.close()
 
-    lat_events = []
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    with Timer(torch) as tm:
-        for _ in range(args.steps):
-            step(lat_events)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_step = tm.ms / args.steps
+    def timed(step_fn, label):
+        print(f"# warmup ({label})", file=sys.stderr, flush=True)
+        for _ in range(args.warmup):
+            step_fn()
+        torch.cuda.synchronize()
+        lat_events, hoist_events = [], []
+        t0 = time.perf_counter()
+        with Timer(torch) as tm:
+            for _ in range(args.steps):
+                if step_fn is step_hoisted:
+                    step_fn(lat_events, hoist_lat=hoist_events)
+                else:
+                    step_fn(lat_events)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        ms = tm.ms / args.steps
+        lats = sorted(e0.elapsed_time(e1) for e0, e1 in lat_events)
+        hoists = [e0.elapsed_time(e1) for e0, e1 in hoist_events]
+        # the N=2^16 batch alone (one step of it), the like-for-like figure
+        # for the comparison with the plain path, which runs at N=2^16
+        t16 = time.perf_counter()
+        with Timer(torch) as t16ev:
+            step_fn(only16=True)
+        torch.cuda.synchronize()
+        n16 = B * len(keys) / (t16ev.ms / 1e3)
+        print(f"# {label}: {ms:.1f} ms/step (wall {(wall / args.steps) * 1e3:.1f}); N=2^16 batch alone "
+              f"{n16:.0f} rot/s ({(time.perf_counter() - t16) * 1e3:.0f} ms wall)", file=sys.stderr, flush=True)
+        return ms, lats, hoists, n16
+
     rot_per_step = len(rot_steps) * (B + args.mixed)
-    value = rot_per_step / (ms_step / 1e3)
-    lats = sorted(e0.elapsed_time(e1) for e0, e1 in lat_events)
+    can_ms, can_lats, _, can_n16 = timed(step_canonical, "canonical")
+    can_value = rot_per_step / (can_ms / 1e3)
+    hoi_ms, hoi_lats, hoi_hoists, hoi_n16 = timed(step_hoisted, "hoisted")
+    hoi_value = rot_per_step / (hoi_ms / 1e3)
+    if args.rotation == "hoisted":
+        ms_step, lats, n16_value, value = hoi_ms, hoi_lats, hoi_n16, hoi_value
+    else:
+        ms_step, lats, n16_value, value = can_ms, can_lats, can_n16, can_value
     p50 = float(np.percentile(lats, 50))
     p99 = float(np.percentile(lats, 99))
-    # the N=2^16 calls alone (events bracket each one), the like-for-like
-    # figure for the comparison with the plain path, which runs at N=2^16
-    lat16 = []
-    step(lat16, only16=True)
-    torch.cuda.synchronize()
-    lat16 = [e0.elapsed_time(e1) for e0, e1 in lat16]
-    n16_value = B * len(lat16) / (sum(lat16) / 1e3)
-    print(f"# batched: {ms_step:.1f} ms/step, {value:.0f} rot/s (wall {wall:.2f}s); "
-          f"N=2^16 calls alone {n16_value:.0f} rot/s", file=sys.stderr, flush=True)
-
-    # ---- hoisted dnum rotations (build-defined variant: the digits of the
-    # unrotated batch are cut and transformed once, each rotation permutes
-    # them in the NTT domain; other residues than the canonical path, same
-    # noise bound): one hoist of the 64 ciphertexts, then the 15 rotations
-    h = keys[0][1].hoist(cts)
-    for k, key in keys:
-        key.rotate_hoisted(h, cts, k, out)  # builds each key's permuted copy
-    h.close()
-    torch.cuda.synchronize()
-    with Timer(torch) as th:
-        h = keys[0][1].hoist(cts)
-    with Timer(torch) as tr:
-        for k, key in keys:
-            key.rotate_hoisted(h, cts, k, out)
-    h.close()
-    hoisted_value = B * len(keys) / ((th.ms + tr.ms) / 1e3)
-    print(f"# hoisted dnum: hoist {th.ms:.2f} ms + {len(keys)} rotations {tr.ms:.2f} ms: "
-          f"{hoisted_value:.0f} rot/s", file=sys.stderr, flush=True)
+    print(f"# step: canonical {can_value:.0f} rot/s, hoisted {hoi_value:.0f} rot/s", file=sys.stderr, flush=True)
 
     # ---- plain per-op path: reference algorithm (RNS gadget, singleton
     # blocks), one ciphertext per call
@@ -629,25 +674,35 @@ def main():
             "ring_degree": N, "limbs": L, "dnum": 3, "special_primes": K,
             "batch": B, "mixed_batch": args.mixed, "rotations_per_step": rot_per_step,
         },
+        "rotation": args.rotation,
         "batched_n65536_rotations_per_s": round(n16_value, 1),
         "plain_rotations_per_s": round(plain_value, 1),
         "speedup_vs_plain": round(n16_value / plain_value, 2),
         "hybrid_one_ct_per_call_rotations_per_s": round(single_hybrid_value, 1),
-        "hoisted_dnum_n65536": {
-            "rotations_per_s": round(hoisted_value, 1),
-            "hoist_ms": round(th.ms, 3), "rotations_ms": round(tr.ms, 3),
-            "speedup_vs_plain": round(hoisted_value / plain_value, 2),
-            "note": f"{B} ct at N=2^16 hoisted once (allocation included), then rotated by the "
-                    f"{len(keys)} exponents: build-defined hoisted digits, residues of the oracle's "
-                    "hoisted composition (not the canonical path's)"},
         "speedup_vs_hybrid_one_ct_per_call": round(n16_value / single_hybrid_value, 2),
+        "hoisted": {
+            "rotations_per_s": round(hoi_value, 1), "n65536_rotations_per_s": round(hoi_n16, 1),
+            "hoist_ms_p50": round(float(np.percentile(hoi_hoists, 50)), 3) if hoi_hoists else None,
+            "p50_ms": round(float(np.percentile(hoi_lats, 50)), 3),
+            "p99_ms": round(float(np.percentile(hoi_lats, 99)), 3),
+            "note": "per step: ckks_hoist of each batch (allocation and the zero-block check included), "
+                    "then one ckks_rotate_hoisted_batch per exponent; residues equal the canonical "
+                    "rotation's (tests compare them)"},
+        "canonical": {
+            "rotations_per_s": round(can_value, 1), "n65536_rotations_per_s": round(can_n16, 1),
+            "speedup_vs_plain": round(can_n16 / plain_value, 2),
+            "speedup_vs_hybrid_one_ct_per_call": round(can_n16 / single_hybrid_value, 2),
+            "p50_ms": round(float(np.percentile(can_lats, 50)), 3),
+            "p99_ms": round(float(np.percentile(can_lats, 99)), 3),
+            "note": "every rotation decomposes its own rotated digits (ckks_apply_galois_batch per exponent)"},
         "ntt_forward_gbps": round(ntt_gbps, 1),
         "ntt_inverse_gbps": round(intt_gbps, 1),
         "autotune": tuned,
         "p50_ms": round(p50, 3),
         "p99_ms": round(p99, 3),
-        "latency_call": (f"one rotation call of {B} ct at N=2^16" +
-                         (f" and {args.mixed} ct at N=2^15 (one launch sequence)" if batched else "")),
+        "latency_call": (f"one {args.rotation} rotation call of {B} ct at N=2^16" +
+                         (f" and {args.mixed} ct at N=2^15 (one launch sequence)" if batched else "") +
+                         (" from digits hoisted once per step" if args.rotation == "hoisted" else "")),
         "roofline": {
             "bound": "hbm",
             "kernel": "forward NTT (column + row pass), 2*N*8 bytes per limb transform",

@@ -72,6 +72,7 @@ _SIGNATURES = {
     "ckks_rns_plan_create": (C.c_int, [_p, _p, _sz, _sz, C.POINTER(_p)]),
     "ckks_rns_plan_destroy": (C.c_int, [_p]),
     "ckks_rns_plan_psi": (C.c_int, [_p, _p]),
+    "ckks_rotate_hoisted_batch": (C.c_int, [_p, _p, _sz]),
     "ckks_ntt64_plan_create": (C.c_int, [_p, C.c_uint64, _sz, C.POINTER(_p)]),
     "ckks_ntt64_plan_destroy": (C.c_int, [_p]),
     "ckks_ntt64_plan_psi": (C.c_int, [_p, C.POINTER(C.c_uint64)]),
@@ -737,13 +738,14 @@ class _KeySwitchKey:
 
     def hoist(self, ct) -> "HoistedDigits":
         """ckks_hoist: digits of the a components of ct cut and transformed
-        once, for rotate_hoisted (RNS-gadget / dnum keys: the hoisted
-        variant's own residues; bounded keys: those of apply_galois)."""
+        once, for rotate_hoisted, whose residues are those of apply_galois."""
         return HoistedDigits(self, ct)
 
     def rotate_hoisted(self, hoisted: "HoistedDigits", ct, exponent: int, out=None):
-        """ckks_rotate_hoisted: the rotation of the hoisted ciphertexts by
-        `exponent` (ct: the ciphertexts hoisted, whose b components are read)."""
+        """ckks_rotate_hoisted: apply_galois(ct, exponent) from the hoisted
+        digits, residue for residue (ct: the ciphertexts hoisted, whose b
+        components are read; the first call per exponent builds the key's
+        permuted copy and correction, kept with the key)."""
         if self._n_cts(ct, 2) != hoisted.n_cts:
             raise ValueError("ct must hold the ciphertexts the digits were hoisted from")
         out = empty((hoisted.n_cts, 2, self.q_limbs, self.plan.degree)) if out is None else out
@@ -1046,6 +1048,25 @@ def apply_galois_batch(ctx: Context, entries):
         p_ct, p_out = key._ct_ptrs(n, [("ct", ct, 2)], out)
         arr[i] = GaloisBatchEntry(key.handle, int(exponent), p_ct, p_out, n, 0, 0)
     _check(lib().ckks_apply_galois_batch(ctx.handle, C.cast(arr, _p), len(entries)))
+
+
+class HoistedRotationEntry(C.Structure):
+    _fields_ = [("hoisted", _p), ("key", _p), ("exponent", _u64), ("input", _p), ("output", _p),
+                ("in_stride", _sz), ("out_stride", _sz)]
+
+
+def rotate_hoisted_batch(ctx: Context, entries):
+    """ckks_rotate_hoisted_batch over [(hoisted, key, exponent, ct, out)]:
+    hoisted rotations of uneven degree in one launch sequence; each out
+    equals key.rotate_hoisted(hoisted, ct, exponent), i.e.
+    key.apply_galois(ct, exponent)."""
+    arr = (HoistedRotationEntry * max(len(entries), 1))()
+    for i, (hoisted, key, exponent, ct, out) in enumerate(entries):
+        if key._n_cts(ct, 2) != hoisted.n_cts:
+            raise ValueError("ct must hold the ciphertexts the digits were hoisted from")
+        p_ct, p_out = key._ct_ptrs(hoisted.n_cts, [("ct", ct, 2)], out)
+        arr[i] = HoistedRotationEntry(hoisted.handle, key.handle, int(exponent), p_ct, p_out, 0, 0)
+    _check(lib().ckks_rotate_hoisted_batch(ctx.handle, C.cast(arr, _p), len(entries)))
 
 
 class NttBatchEntry(C.Structure):

@@ -120,6 +120,9 @@ struct ckks_hoisted {
     // (ct, coefficient) whose digit vector holds a -2^(w-1): negating it
     // carries, so a rotation that negates one takes the canonical path
     std::vector<std::pair<ck::u32, ck::u32>> ties;
+    // canonical digits: some coefficient has every residue of a gadget block
+    // zero, so rotations take the canonical path
+    bool zero_block = false;
     ~ckks_hoisted() {
         if (d_digits) hipFree(d_digits);
     }
@@ -469,6 +472,9 @@ struct MacArgs {
     u64 *acc;            // [ct][2][B][N]
     const DevPrime *primes;
     u32 n_cts, dnum, limbs, logn, chunks;
+    // hoisted canonical rotations: the key-side correction [B][N] of each
+    // component, added to every ciphertext's sums (null otherwise)
+    const u64 *tb, *ta;
 };
 
 // NTT-domain Galois permutation (ring/ntt_galois_permutation.rs:5-19) over
@@ -503,9 +509,9 @@ static int launch_ntt_permute(hipStream_t s, const u64 *in, u64 *out, u64 k, u64
 // HBM once per launch, not once per ciphertext.
 //
 // FP64 limbs (q < 2^41): each product is fp_mulmod(digit, key), |term| <=
-// 0.51 q, so D <= 8 terms sum below 2^44 (exact) and fp_residue makes the sum
-// canonical; the key entries enter as doubles with fl(w / q) divided once
-// per thread.
+// 0.51 q, so D <= 8 terms and a canonical correction sum below 2^44 (exact)
+// and fp_residue makes the sum canonical; the key entries enter as doubles
+// with fl(w / q) divided once per thread.
 template <int D>
 __device__ __forceinline__ void ks_mac_fp(const MacArgs &a, const DevPrime &P, u32 i, u64 N, u64 n) {
     double kb[D], kbp[D], ka[D], kap[D];
@@ -516,9 +522,12 @@ __device__ __forceinline__ void ks_mac_fp(const MacArgs &a, const DevPrime &P, u
         kbp[j] = kb[j] / P.qd;
         kap[j] = ka[j] / P.qd;
     }
+    // the correction enters as the sums' starting value (canonical, < 2^41)
+    const double t0b = a.tb ? small_to_f64(a.tb[(u64)i * N + n]) : 0.0;
+    const double t0a = a.ta ? small_to_f64(a.ta[(u64)i * N + n]) : 0.0;
     for (u32 ct = 0; ct < a.n_cts; ++ct) {
         const u64 *d = a.digits + (((u64)ct * D) * a.limbs + i) * N + n;
-        double sb = 0.0, sa = 0.0;
+        double sb = t0b, sa = t0a;
 #pragma unroll
         for (int j = 0; j < D; ++j) {
             const double dv = small_to_f64(ld_stream(d + (u64)j * a.limbs * N));
@@ -549,6 +558,7 @@ __global__ __launch_bounds__(kKsThreads) void k_ks_mac(MacArgs a) {
         kb[j] = a.kb[((u64)j * a.limbs + i) * N + n];
         ka[j] = a.ka[((u64)j * a.limbs + i) * N + n];
     }
+    const u64 tb = a.tb ? a.tb[(u64)i * N + n] : 0, ta = a.ta ? a.ta[(u64)i * N + n] : 0;
     for (u32 ct = 0; ct < a.n_cts; ++ct) {
         const u64 *d = a.digits + (((u64)ct * D) * a.limbs + i) * N + n;
         Acc128 sb, sa;
@@ -560,6 +570,8 @@ __global__ __launch_bounds__(kKsThreads) void k_ks_mac(MacArgs a) {
             sb.mac(dv, kb[j]);
             sa.mac(dv, ka[j]);
         }
+        sb.mac(tb, 1);
+        sa.mac(ta, 1);
         u64 *o = a.acc + ((u64)ct * 2 * a.limbs + i) * N + n;
         st_stream(o, reduce128(sb, P));
         st_stream(o + (u64)a.limbs * N, reduce128(sa, P));
@@ -594,6 +606,8 @@ __global__ __launch_bounds__(kKsThreads) void k_ks_mac_chunk(MacArgs a, u32 j0, 
     u64 dv[C];
 #pragma unroll
     for (int j = 0; j < C; ++j) dv[j] = (u32)j < cnt ? d[j * d_j] : 0;
+    const bool corr = first && a.tb;
+    const u64 tbv = corr ? a.tb[(u64)i * N + n] : 0, tav = corr ? a.ta[(u64)i * N + n] : 0;
     for (u32 ct = 0; ct < a.n_cts; ++ct) {
         u64 nx[C];
         const bool more = ct + 1 < a.n_cts;
@@ -611,6 +625,10 @@ __global__ __launch_bounds__(kKsThreads) void k_ks_mac_chunk(MacArgs a, u32 j0, 
 #pragma unroll
         for (int j = 0; j < C; ++j) dv[j] = nx[j];
         u64 *o = a.acc + ((u64)ct * 2 * a.limbs + i) * N + n;
+        if (corr) {
+            sb.mac(tbv, 1);
+            sa.mac(tav, 1);
+        }
         u64 rb = reduce128(sb, P), ra = reduce128(sa, P);
         if (!first) {
             rb = add_mod(rb, o[0], P.q);
@@ -633,7 +651,7 @@ __global__ __launch_bounds__(kKsThreads) void k_ks_mac_any(MacArgs a) {
     const u64 n = (u64)chunk * kKsThreads + threadIdx.x;
     const DevPrime P = a.primes[i];
     const u64 *d = a.digits + (((u64)ct * a.dnum) * a.limbs + i) * N + n;
-    u64 rb = 0, ra = 0;
+    u64 rb = a.tb ? a.tb[(u64)i * N + n] : 0, ra = a.ta ? a.ta[(u64)i * N + n] : 0;
     for (u32 j0 = 0; j0 < a.dnum; j0 += 8) {
         Acc128 sb, sa;
         sb.zero();
@@ -1508,6 +1526,7 @@ struct HoistedMac {
     const u64 *digits;   // [ct][D][B][N]
     const u64 *kb, *ka;  // [D][B][N]
     u64 k;
+    const u64 *tb, *ta;  // canonical-lift correction [B][N] (null for bounded digits)
 };
 
 static int key_switch_lane(const ckks_ksk *k, const u64 *x, u64 x_stride, const u64 *add0,
@@ -1663,6 +1682,8 @@ static int ks_mac(const KsJob &j, hipStream_t s) {
     ma.digits = j.digits;
     ma.kb = j.hm ? j.hm->kb : k->d_kb;
     ma.ka = j.hm ? j.hm->ka : k->d_ka;
+    ma.tb = j.hm ? j.hm->tb : nullptr;
+    ma.ta = j.hm ? j.hm->ta : nullptr;
     ma.acc = j.mac_out;
     ma.primes = k->plan->d_primes;
     ma.n_cts = (u32)j.n_cts;
@@ -1864,7 +1885,10 @@ static int key_switch_multi(ckks_context *ctx, const KsMultiEntry *e, size_t cou
 
 size_t ksk_bytes(const ckks_ksk *k) {
     const size_t entries = 2 * k->dnum * k->limbs * k->degree * sizeof(u64);
-    return entries * (1 + k->hoist_keys.size());
+    // hoisting tables: a permuted key copy per exponent, plus the correction
+    // of canonical-digit keys
+    const size_t corr = k->bounded ? 0 : 2 * k->limbs * k->degree * sizeof(u64);
+    return entries * (1 + k->hoist_keys.size()) + corr * k->hoist_keys.size();
 }
 
 int ksk_info(const ckks_ksk *k, size_t *q_limbs, size_t *degree) {
@@ -2292,10 +2316,137 @@ int ckks_bounded_hoist(const ckks_ksk *key, const uint64_t *in, size_t n_cts, si
     return CKKS_OK;
 }
 
-// The key entries permuted by k^-1 for hoisted rotations by k, built on
-// first use and cached (LRU, kHoistKeyCacheMax copies per key).
+// Return code of the per-exponent tables below when device memory is short.
 constexpr int kNoRoomForKeyCopy = -1;
 
+// ---- exact hoisting of canonical (RNS-gadget / dnum) digits -----------------------
+//
+// The canonical digit of block j is the lift L_j(x) in [0, Q_j) reduced mod
+// every key-basis prime p_i.  sigma_k moves coefficient g to n with a sign;
+// for a negated position, L_j(-v) = Q_j - L_j(v) when v != 0 mod Q_j, so
+//   digit_j(sigma_k(x))_i = sigma_k(digit_j(x)_i) + (Q_j mod p_i) [n negated]
+// unless the block value at the source is 0 mod Q_j (every residue of the
+// block zero).  In the NTT domain sigma_k is the Galois permutation, so the
+// key MAC of the rotated digits is the MAC of the permuted hoisted digits
+// plus T_i = NTT_i(neg_k) * sum_j (Q_j mod p_i) key_j,i, a per-key, per-exponent
+// constant: added to the sums of every ciphertext, it makes the hoisted
+// rotation give the residues of the canonical one.  A batch with a zero
+// block value takes the canonical path.
+
+// neg_k over every key-basis limb: 1 where sigma_k negates (n k^-1 mod 2N >= N).
+__global__ __launch_bounds__(kKsThreads) void k_neg_pattern(u64 *out, u64 kinv, u32 logn, u64 total) {
+    const u64 x = (u64)blockIdx.x * kKsThreads + threadIdx.x;
+    if (x >= total) return;
+    const u64 N = (u64)1 << logn, n = x & (N - 1);
+    out[x] = ((n * kinv) & (2 * N - 1)) >= N ? 1u : 0u;
+}
+
+// T[c][i][n] = NEG[i][n] * sum_j qmod[j][i] key_c[j][i][n] mod p_i.
+struct CorrArgs {
+    const u64 *neg;      // [B][N], NTT domain
+    const u64 *kb, *ka;  // [D][B][N]
+    const u64 *qmod;     // [D][B]: Q_j mod p_i
+    u64 *out;            // [2][B][N]
+    const DevPrime *primes;
+    u32 dnum, limbs, logn;
+    u64 total;  // B * N
+};
+
+__global__ __launch_bounds__(kKsThreads) void k_hoist_correction(CorrArgs a) {
+    const u64 x = (u64)blockIdx.x * kKsThreads + threadIdx.x;
+    if (x >= a.total) return;
+    const u32 i = (u32)(x >> a.logn);
+    const CK_CONST_AS DevPrime *pr = const_table(a.primes);
+    const Barrett br = barrett_at(pr, i);
+    Acc128 sb, sa;
+    sb.zero();
+    sa.zero();
+    for (u32 j = 0; j < a.dnum; ++j) {
+        const u64 c = a.qmod[(u64)j * a.limbs + i];
+        sb.mac(a.kb[(u64)j * a.total + x], c);
+        sa.mac(a.ka[(u64)j * a.total + x], c);
+    }
+    const u64 ng = a.neg[x];
+    a.out[x] = mul_mod(ng, sb.reduce(br), br);
+    a.out[a.total + x] = mul_mod(ng, sa.reduce(br), br);
+}
+
+// Sets *flag when some coefficient of some ciphertext has every residue of a
+// gadget block zero (its canonical digit cannot be hoisted exactly).
+struct ZeroArgs {
+    const u64 *in;  // the a components: limb m of ct c at in + c * stride + m * N
+    u64 stride;
+    const ModUpBlock *blocks;
+    int *flag;
+    u32 dnum, logn, chunks;
+};
+
+__global__ __launch_bounds__(kKsThreads) void k_zero_blocks(ZeroArgs a) {
+    const u32 b = blockIdx.x;
+    const u32 chunk = b % a.chunks, ct = b / a.chunks;
+    const u64 N = (u64)1 << a.logn;
+    const u64 n = (u64)chunk * kKsThreads + threadIdx.x;
+    const CK_CONST_AS ModUpBlock *blk = const_table(a.blocks);
+    const u64 *x = a.in + ct * a.stride + n;
+    for (u32 j = 0; j < a.dnum; ++j) {
+        const u32 start = blk[j].start, size = blk[j].size;
+        bool zero = true;
+        for (u32 m = 0; m < size; ++m) zero = zero && x[(u64)(start + m) * N] == 0;
+        if (zero) *a.flag = 1;  // every writer stores the same value
+    }
+}
+
+// Q_j mod p_i for the key's blocks and basis, [D][B].
+static std::vector<u64> block_moduli_mod(const ckks_ksk *key) {
+    const std::vector<uint64_t> &mods = key->plan->moduli;
+    std::vector<u64> out(key->dnum * key->limbs);
+    for (size_t j = 0; j < key->dnum; ++j)
+        for (size_t i = 0; i < key->limbs; ++i) {
+            unsigned __int128 r = 1 % mods[i];
+            for (size_t m = 0; m < key->block_size[j]; ++m) r = r * (mods[key->block_start[j] + m] % mods[i]) % mods[i];
+            out[j * key->limbs + i] = (u64)r;
+        }
+    return out;
+}
+
+// The correction of exponent k permuted by k^-1 (it joins the MAC output
+// before the inverse NTT reads that permuted by k) into out[2][B][N].
+static int hoist_correction(const ckks_ksk *key, u64 k, u64 *out) {
+    ckks_context *ctx = key->ctx;
+    const u64 two_n = 2 * (u64)key->degree, kinv = galois_inverse(k, two_n);
+    const size_t B = key->limbs, N = key->degree, BN = B * N;
+    const std::vector<u64> qm = block_moduli_mod(key);
+    u64 *tmp = nullptr;  // NEG [B][N], T [2][B][N], qmod [D][B]
+    if (hipMalloc(&tmp, (3 * BN + qm.size()) * sizeof(u64)) != hipSuccess) {
+        (void)hipGetLastError();
+        return kNoRoomForKeyCopy;
+    }
+    u64 *neg = tmp, *t = tmp + BN, *qmod = tmp + 3 * BN;
+    int rc = CKKS_OK;
+    hipStream_t s = ctx->stream;
+    const dim3 g((u32)((BN + kKsThreads - 1) / kKsThreads));
+    if (hipMemcpyAsync(qmod, qm.data(), qm.size() * sizeof(u64), hipMemcpyHostToDevice, s) != hipSuccess)
+        rc = fail(CKKS_ERR_HIP, "correction table upload failed");
+    if (!rc) {
+        hipLaunchKernelGGL(k_neg_pattern, g, dim3(kKsThreads), 0, s, neg, kinv, key->plan->logn, (u64)BN);
+        ProfileGuard pg(ctx);  // table setup, not an operation of the caller
+        NttCall f{key->plan, neg, neg, 1, B, BN, BN};
+        rc = launch_ntt(ctx, &f, 1, true);
+    }
+    if (!rc) {
+        CorrArgs a{neg, key->d_kb, key->d_ka, qmod, t, key->plan->d_primes, (u32)key->dnum, (u32)B,
+                   key->plan->logn, (u64)BN};
+        hipLaunchKernelGGL(k_hoist_correction, g, dim3(kKsThreads), 0, s, a);
+        rc = launch_ntt_permute(s, t, out, kinv, 2 * BN, key->plan->logn);
+    }
+    if (hipStreamSynchronize(s) != hipSuccess && !rc) rc = fail(CKKS_ERR_HIP, "correction table build failed");
+    hipFree(tmp);
+    return rc;
+}
+
+// The key entries permuted by k^-1 for hoisted rotations by k (canonical
+// digits: and the correction above), built on first use and cached (LRU,
+// kHoistKeyCacheMax copies per key).
 static int hoisted_key(const ckks_ksk *key, u64 k, const u64 **out) {
     const u64 two_n = 2 * (u64)key->degree;
     const u64 kinv = galois_inverse(k, two_n);
@@ -2314,13 +2465,17 @@ static int hoisted_key(const ckks_ksk *key, u64 k, const u64 **out) {
             key->hoist_keys.erase(lru.back());
             lru.pop_back();
         }
+        // canonical-digit keys also keep the correction [2][B][N] after the copy
+        const size_t extra = key->bounded ? 0 : 2 * key->limbs * key->degree;
         u64 *pk = nullptr;
-        if (hipMalloc(&pk, 2 * DBN * sizeof(u64)) != hipSuccess) {
+        if (hipMalloc(&pk, (2 * DBN + extra) * sizeof(u64)) != hipSuccess) {
             (void)hipGetLastError();
             return kNoRoomForKeyCopy;
         }
         if ((rc = launch_ntt_permute(ctx->stream, key->d_kb, pk, kinv, DBN, key->plan->logn)) ||
-            (rc = launch_ntt_permute(ctx->stream, key->d_ka, pk + DBN, kinv, DBN, key->plan->logn))) {
+            (rc = launch_ntt_permute(ctx->stream, key->d_ka, pk + DBN, kinv, DBN, key->plan->logn)) ||
+            (extra && (rc = hoist_correction(key, k, pk + 2 * DBN)))) {
+            hipStreamSynchronize(ctx->stream);
             hipFree(pk);
             return rc;
         }
@@ -2338,8 +2493,9 @@ static int hoisted_key(const ckks_ksk *key, u64 k, const u64 **out) {
 // sigma_k(b) passed through.
 static int rotate_from_hoisted(const ckks_hoisted *h, const ckks_ksk *key, u64 k, const u64 *pk, const u64 *in,
                                u64 *out, size_t in_stride, size_t out_stride) {
-    const size_t DBN = key->dnum * key->limbs * key->degree;
-    const HoistedMac hm{h->d_digits, pk, pk + DBN, k};
+    const size_t DBN = key->dnum * key->limbs * key->degree, BN = key->limbs * key->degree;
+    const u64 *tb = key->bounded ? nullptr : pk + 2 * DBN;
+    const HoistedMac hm{h->d_digits, pk, pk + DBN, k, tb, tb ? tb + BN : nullptr};
     const size_t poly = key->q_limbs * key->degree;
     return key_switch_core(key, in + poly, in_stride, in, nullptr, in_stride, out, out_stride, h->n_cts,
                            galois_inverse(k, 2 * (u64)key->degree), nullptr, &hm);
@@ -2403,6 +2559,21 @@ int ckks_hoist(const ckks_ksk *key, const uint64_t *in, size_t n_cts, size_t in_
         if (rc) return rc;
         NttCall f{key->plan, h->d_digits, h->d_digits, n_cts * D, B, B * N, B * N};
         if ((rc = launch_ntt(ctx, &f, 1, true))) return rc;
+        // exactness of the hoisted rotations (see hoist_correction)
+        int *flag = nullptr;
+        CK_HIP(hipMalloc(&flag, sizeof(int)));
+        int host_flag = 0;
+        ZeroArgs za{in + poly, in_stride, key->d_modup, flag, (u32)D, key->plan->logn, (u32)(N / kKsThreads)};
+        hipError_t e = hipMemsetAsync(flag, 0, sizeof(int), ctx->stream);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_zero_blocks, dim3((u32)(n_cts * za.chunks)), dim3(kKsThreads), 0, ctx->stream, za);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipMemcpyAsync(&host_flag, flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        hipFree(flag);
+        if (e != hipSuccess) return fail(CKKS_ERR_HIP, std::string("zero-block check: ") + hipGetErrorString(e));
+        h->zero_block = host_flag != 0;
     }
     *out = h.release();
     return CKKS_OK;
@@ -2421,12 +2592,105 @@ int ckks_rotate_hoisted(const ckks_hoisted *h, const ckks_ksk *key, uint64_t exp
     const u64 k = exponent % two_n;
     if ((k & 1) == 0) return fail(CKKS_ERR_INVALID, "CKKS automorphism exponent must be odd modulo 2N");
     if (in == out) return fail(CKKS_ERR_INVALID, "rotation output must not alias its input");
+    // a zero block value, or no room for the per-exponent tables: the
+    // canonical key switch, whose residues the hoisted one reproduces
+    const u64 kinv = galois_inverse(k, two_n);
+    const size_t poly = key->q_limbs * key->degree;
+    if (h->zero_block)
+        return key_switch_core(key, in + poly, in_stride, in, nullptr, in_stride, out, out_stride, n_cts, kinv);
     const u64 *pk = nullptr;
     rc = hoisted_key(key, k, &pk);
-    // no canonical fallback here: it gives other residues
-    if (rc == kNoRoomForKeyCopy) return fail(CKKS_ERR_OOM, "no device memory for the permuted key copy");
+    if (rc == kNoRoomForKeyCopy)
+        return key_switch_core(key, in + poly, in_stride, in, nullptr, in_stride, out, out_stride, n_cts, kinv);
     if (rc) return rc;
     return rotate_from_hoisted(h, key, k, pk, in, out, in_stride, out_stride);
+}
+
+int ckks_rotate_hoisted_batch(ckks_context *ctx, const ckks_hoisted_rotation *e, size_t count) {
+    if (!ctx || (count && !e)) return fail(CKKS_ERR_INVALID, "null argument");
+    if (count > 16) return fail(CKKS_ERR_INVALID, "a hoisted rotation batch holds at most 16 entries");
+    struct Fast {
+        const ckks_hoisted *h;
+        const ckks_ksk *key;
+        const u64 *in;
+        u64 *out;
+        size_t in_stride, out_stride;
+        u64 k;
+        const u64 *pk;
+    };
+    std::vector<Fast> fast;
+    int rc;
+    for (size_t i = 0; i < count; ++i) {
+        const ckks_hoisted_rotation &r = e[i];
+        if (!r.hoisted || !r.key) return fail(CKKS_ERR_INVALID, "null argument");
+        const ckks_ksk *key = r.key;
+        const ckks_hoisted *h = r.hoisted;
+        if (key->ctx != ctx) return fail(CKKS_ERR_INVALID, "hoisted rotation batch entries must share the context");
+        if (key->bounded) {
+            if ((rc = ckks_bounded_rotate_hoisted(h, key, r.exponent, r.in, r.out, r.in_stride, r.out_stride)))
+                return rc;
+            continue;
+        }
+        size_t in_stride = r.in_stride, out_stride = r.out_stride;
+        if ((rc = check_ct_args(key, r.in, r.out, h->n_cts, in_stride, out_stride, 2))) return rc;
+        if (h->n_cts == 0) continue;
+        if (h->blocks.empty() || key->plan != h->plan || key->q_limbs != h->q_limbs || key->block_size != h->blocks)
+            return fail(CKKS_ERR_INVALID, "key does not match the hoisted digits (plan, Q limbs, gadget blocks)");
+        const u64 two_n = 2 * (u64)key->degree, k = r.exponent % two_n;
+        if ((k & 1) == 0) return fail(CKKS_ERR_INVALID, "CKKS automorphism exponent must be odd modulo 2N");
+        if (r.in == r.out) return fail(CKKS_ERR_INVALID, "rotation output must not alias its input");
+        const u64 kinv = galois_inverse(k, two_n);
+        const size_t poly = key->q_limbs * key->degree;
+        const u64 *pk = nullptr;
+        rc = h->zero_block ? kNoRoomForKeyCopy : hoisted_key(key, k, &pk);
+        if (rc == kNoRoomForKeyCopy) {  // the canonical key switch, same residues
+            if ((rc = key_switch_core(key, r.in + poly, in_stride, r.in, nullptr, in_stride, r.out, out_stride,
+                                      h->n_cts, kinv)))
+                return rc;
+            continue;
+        }
+        if (rc) return rc;
+        fast.push_back(Fast{h, key, r.in, r.out, in_stride, out_stride, k, pk});
+    }
+    if (fast.empty()) return CKKS_OK;
+    // the MAC outputs (slot 4) and the accumulators (slot 5) of every entry
+    size_t total = 0;
+    for (const Fast &f : fast) total += f.h->n_cts * 2 * f.key->limbs * f.key->degree;
+    void *wd, *wa;
+    if ((rc = ctx->workspace(4, total * sizeof(u64), &wd))) return rc;
+    if ((rc = ctx->workspace(5, total * sizeof(u64), &wa))) return rc;
+    hipStream_t s = ctx->stream;
+    std::vector<HoistedMac> hms(fast.size());
+    std::vector<KsJob> jobs(fast.size());
+    std::vector<NttCall> inv(fast.size());
+    u64 *pd = (u64 *)wd, *pa = (u64 *)wa;
+    for (size_t i = 0; i < fast.size(); ++i) {
+        const Fast &f = fast[i];
+        const ckks_ksk *key = f.key;
+        const size_t B = key->limbs, N = key->degree, DBN = key->dnum * B * N, poly = key->q_limbs * N;
+        hms[i] = HoistedMac{f.h->d_digits, f.pk, f.pk + DBN, f.k, f.pk + 2 * DBN, f.pk + 2 * DBN + B * N};
+        const u64 kinv = galois_inverse(f.k, 2 * (u64)N);
+        KsJob &j = jobs[i];
+        j = KsJob{key, f.in + poly, f.in_stride, f.in, nullptr, f.in_stride, f.out, f.out_stride, f.h->n_cts, kinv,
+                  nullptr, &hms[i]};
+        j.digits = const_cast<u64 *>(f.h->d_digits);
+        j.mac_out = pd;
+        j.acc = pa;
+        j.mfma = ks_uses_mfma(key);
+        pd += f.h->n_cts * 2 * B * N;
+        pa += f.h->n_cts * 2 * B * N;
+        inv[i] = NttCall{key->plan, j.mac_out, j.acc, f.h->n_cts * 2, B, B * N, B * N, nullptr, f.k};
+        ckks_execution_profile &pc = ctx->counts();
+        pc.key_switches += f.h->n_cts;
+        (f.k == 2 * (u64)N - 1 ? pc.automorphisms : pc.rotations) += f.h->n_cts;
+        if ((rc = ks_mac(j, s))) return rc;
+    }
+    // one inverse NTT launch per pass for every entry, each reading its MAC
+    // output through its own Galois permutation
+    if ((rc = launch_ntt(ctx, inv.data(), inv.size(), false))) return rc;
+    for (KsJob &j : jobs)
+        if ((rc = ks_moddown(j, s))) return rc;
+    return CKKS_OK;
 }
 
 int ckks_relinearize(const ckks_ksk *key, const uint64_t *in3, uint64_t *out, size_t n_cts,

@@ -238,9 +238,11 @@ int launch_ntt(ckks_context *ctx, const NttCall *calls, size_t count, bool forwa
         if (c.mac && (forward || count != 1 || !ntt_two_pass(c.plan->logn) || c.n_polys % 2 ||
                       c.mac->dnum == 0 || c.mac->dnum > 8))
             return fail(CKKS_ERR_INVALID, "fused key inner product: one inverse two-pass call, 1..8 digits");
-        if (c.galois_k && (forward || count != 1 || c.mac || (c.galois_k & 1) == 0 ||
-                           c.galois_k >= 2 * (u64)c.plan->degree || c.in == c.out))
-            return fail(CKKS_ERR_INVALID, "Galois-permuted load: one inverse call, odd exponent below 2N, in != out");
+        if ((c.galois_k != 0) != (calls[0].galois_k != 0))
+            return fail(CKKS_ERR_INVALID, "Galois-permuted loads: every entry of a launch or none");
+        if (c.galois_k && (forward || c.mac || (c.galois_k & 1) == 0 || c.galois_k >= 2 * (u64)c.plan->degree ||
+                           c.in == c.out))
+            return fail(CKKS_ERR_INVALID, "Galois-permuted load: inverse only, odd exponent below 2N, in != out");
         if (ntt_two_pass(c.plan->logn)) {
             total_two += c.n_polys * c.n_limbs * c.plan->degree;
             max_n = std::max(max_n, c.plan->degree);

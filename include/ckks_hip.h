@@ -458,31 +458,51 @@ int ckks_bounded_rotate_hoisted(const ckks_hoisted *hoisted, const ckks_ksk *key
                                 const uint64_t *in, uint64_t *out, size_t in_stride, size_t out_stride);
 int ckks_bounded_hoist_destroy(ckks_hoisted *hoisted);
 
-/* Hoisted rotations with RNS-gadget and dnum-hybrid keys (build-defined:
- * the reference hoists bounded digits only).  ckks_hoist cuts the digits of
- * the UNROTATED a components once -- the canonical lift of a mod Q_j, as a key
- * switch does -- and transforms them; ckks_rotate_hoisted(k) then switches
- * with sigma_k of those digits (a permutation in the NTT domain, read by the
- * key MAC over the key permuted by k^-1), so no decomposition or forward NTT
- * runs per rotation.  sigma_k(d_j) is congruent to sigma_k(a) mod Q_j with
- * coefficients of the same size as the canonical digit's, so the result is
- * a key switch of sigma_k(a) with the same noise bound; its residues differ
- * from ckks_apply_galois (which cuts the canonical digits of sigma_k(a))
- * and are those of the oracle's hoisted composition.  For bounded keys both
- * calls are ckks_bounded_hoist / ckks_bounded_rotate_hoisted.  The key must
- * have the plan, Q limbs and gadget blocks the digits were cut with; the
- * permuted key copies are cached as for bounded keys (without device memory
- * for a copy the call fails with CKKS_ERR_OOM).  Destroy the handle with
- * ckks_bounded_hoist_destroy, before the context. */
+/* Hoisted rotations with RNS-gadget and dnum-hybrid keys, residue for
+ * residue those of ckks_apply_galois (rotate_left_rns_ckks_with_prepared_ntt,
+ * ckks/rns_galois_key.rs:730; the reference hoists bounded digits only).
+ * ckks_hoist cuts the canonical digits of the UNROTATED a components once
+ * and transforms them; ckks_rotate_hoisted(k) runs the key MAC on them
+ * permuted by sigma_k (read through the key permuted by k^-1), so no
+ * decomposition or forward NTT runs per rotation.  The canonical digit of a
+ * negated coefficient is Q_j - L, not -L: the difference, (Q_j mod p_i) at
+ * every position sigma_k negates, enters as a per-key, per-exponent
+ * correction T = NTT(neg_k) * sum_j (Q_j mod p_i) key_j added to every
+ * ciphertext's MAC sums.  It holds unless a coefficient has every residue
+ * of a gadget block zero; ckks_hoist checks for that (one pass over the a
+ * components and a host synchronisation) and such a batch, like a call
+ * without device memory for the per-exponent tables, runs the canonical
+ * key switch.  For bounded keys both calls are ckks_bounded_hoist /
+ * ckks_bounded_rotate_hoisted.  The key must have the plan, Q limbs and
+ * gadget blocks the digits were cut with; the permuted key copies and
+ * corrections are cached per exponent as for bounded keys.  Destroy the
+ * handle with ckks_bounded_hoist_destroy, before the context. */
 int ckks_hoist(const ckks_ksk *key, const uint64_t *in, size_t n_cts, size_t in_stride, ckks_hoisted **out);
 int ckks_rotate_hoisted(const ckks_hoisted *hoisted, const ckks_ksk *key, uint64_t exponent, const uint64_t *in,
                         uint64_t *out, size_t in_stride, size_t out_stride);
+/* Several hoisted rotations in one launch sequence (no reference
+ * counterpart; the same residues as ckks_rotate_hoisted entry by entry):
+ * the key MACs run per entry, the inverse NTTs of all entries are one
+ * launch per pass (mixed degrees, each entry read through its own Galois
+ * permutation), then ModDown per entry.  Entries of bounded keys, and
+ * entries that take the canonical path, run on their own.  At most 16
+ * entries; in and out follow ckks_rotate_hoisted (strides 0 = dense). */
+typedef struct ckks_hoisted_rotation {
+    const ckks_hoisted *hoisted;
+    const ckks_ksk *key;
+    uint64_t exponent;
+    const uint64_t *in;
+    uint64_t *out;
+    size_t in_stride, out_stride;
+} ckks_hoisted_rotation;
+int ckks_rotate_hoisted_batch(ckks_context *ctx, const ckks_hoisted_rotation *entries, size_t count);
 
 int ckks_ksk_destroy(ckks_ksk *key);
 /* LogicalPayloadBytes::logical_payload_bytes of a key (ckks/key_accounting.rs:
  * 13-120): the device bytes of its entries (2 * entries * limbs * N * 8) plus
- * the permuted copies it caches for hoisted rotations, the figure the
- * serving loop charges while a request reads the key. */
+ * the tables it caches for hoisted rotations (a permuted copy per exponent;
+ * canonical-digit keys add 2 * limbs * N * 8 of correction per exponent),
+ * the figure the serving loop charges while a request reads the key. */
 int ckks_ksk_bytes(const ckks_ksk *key, size_t *bytes);
 
 /* rns_key_switch_with_prepared_dynamic_ntt (grafting/rns_key_switch.rs:

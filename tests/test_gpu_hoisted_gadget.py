@@ -1,11 +1,12 @@
 """Hoisted rotations with dnum-hybrid and RNS-gadget keys (ckks_hoist /
-ckks_rotate_hoisted): digits of the unrotated a cut and transformed once,
-each rotation a key MAC over the key permuted by k^-1.  Bit-exact against
-the oracle's composition of the same steps -- canonical digits of a
-(basis_transition.rs lift per block), sigma_k of every digit limb, NTT, key
-MAC, inverse NTT, ModDown by P, sigma_k(b) added -- at toy sizes and at the
-N = 2^16, 25-limb, [9, 8, 8], K = 7 shape; and, with device-generated keys,
-a hoisted rotation decrypts to the rotated slots like the canonical one."""
+ckks_rotate_hoisted): the canonical digits of the unrotated a are cut and
+transformed once; each rotation is a key MAC over the key permuted by k^-1
+plus the per-exponent correction of the canonical lift of negated
+coefficients, so every residue equals the canonical rotation
+(ckks_apply_galois, itself checked against the oracle here): at toy sizes,
+for every key kind and for conjugation, at the N = 2^16, 25-limb,
+[9, 8, 8], K = 7 shape, and for batches holding a zero block value (which
+take the canonical path)."""
 from __future__ import annotations
 
 import numpy as np
@@ -17,38 +18,21 @@ from conftest import O, P60, Q0, Q40, rand_residues
 pytestmark = pytest.mark.gpu
 
 
-def hoisted_switch_oracle(qp_moduli, q_limbs, blocks, b, a, k, kb, ka):
-    n = a.shape[-1]
+def canonical_rotation_oracle(qp_moduli, q_limbs, blocks, ct, k, kb, ka):
+    """rotate_left via the canonical key switch: sigma_k of both components,
+    canonical digits of sigma_k(a), MAC, inverse NTT, ModDown by P."""
+    n = ct.shape[-1]
     q = qp_moduli[:q_limbs]
-    p = qp_moduli[q_limbs:]
-    orc = O.RnsPlan(qp_moduli, n)
-    acc_b = [np.zeros(n, dtype=np.uint64) for _ in qp_moduli]
-    acc_a = [np.zeros(n, dtype=np.uint64) for _ in qp_moduli]
-    start = 0
-    for j, sz in enumerate(blocks):
-        blk = list(range(start, start + sz))
-        others = [i for i in range(len(qp_moduli)) if i not in blk]
-        ext = O.extend_basis(a[start:start + sz], [qp_moduli[i] for i in blk], [qp_moduli[i] for i in others])
-        digit = np.empty((len(qp_moduli), n), dtype=np.uint64)
-        digit[blk] = a[start:start + sz]
-        digit[others] = ext
-        rot = np.stack([O.apply_automorphism(digit[i], qi, k) for i, qi in enumerate(qp_moduli)])
-        f = orc.forward(rot)
-        for i, qi in enumerate(qp_moduli):
-            acc_b[i] = O.pointwise_mac(acc_b[i], f[i], kb[j, i], qi)
-            acc_a[i] = O.pointwise_mac(acc_a[i], f[i], ka[j, i], qi)
-        start += sz
-    cb, ca = orc.inverse(np.stack(acc_b)), orc.inverse(np.stack(acc_a))
-    if p:
-        cb, ca = O.mod_down(cb, q, p), O.mod_down(ca, q, p)
-    rb = np.stack([O.apply_automorphism(b[i], qi, k) for i, qi in enumerate(q)])
-    bo = np.array([(cb[i].astype(object) + rb[i]) % qi for i, qi in enumerate(q)], dtype=np.uint64)
-    return bo, ca
+    rb = np.stack([O.apply_automorphism(ct[0, i], qi, k) for i, qi in enumerate(q)])
+    ra = np.stack([O.apply_automorphism(ct[1, i], qi, k) for i, qi in enumerate(q)])
+    if len(qp_moduli) > q_limbs:
+        return O.hybrid_key_switch(O.RnsPlan(qp_moduli, n), q_limbs, blocks, rb, ra, kb, ka)
+    return O.rns_key_switch(O.RnsPlan(qp_moduli, n), blocks, rb, ra, kb, ka)
 
 
 @pytest.mark.parametrize("kind,logn,L,K,blocks", [("hybrid", 12, 4, 2, [2, 2]), ("hybrid", 13, 9, 4, [5, 4]),
                                                   ("rns", 12, 4, 0, [1, 1, 1, 1]), ("rns", 12, 4, 0, [2, 2])])
-def test_hoisted_rotation_matches_oracle_composition(pkg, ctx, kind, logn, L, K, blocks):
+def test_hoisted_rotation_equals_canonical(pkg, ctx, kind, logn, L, K, blocks):
     n = 1 << logn
     q = [Q0[0]] + Q40[:L - 1]
     basis = q + P60[:K]
@@ -57,16 +41,53 @@ def test_hoisted_rotation_matches_oracle_composition(pkg, ctx, kind, logn, L, K,
     plan = pkg.PreparedRnsNttPlan(ctx, basis, n)
     key = (pkg.HybridKeySwitchKey(plan, L, blocks, kb, ka) if kind == "hybrid"
            else pkg.RnsKeySwitchKey(plan, blocks, kb, ka))
-    ct = rand_residues(rng, q, n, polys=4).reshape(2, 2, L, n)
+    ct = rand_residues(rng, q, n, polys=6).reshape(3, 2, L, n)
     ct[0, 1, :, :8] = np.array(q, dtype=np.uint64)[:, None] - 1
     dct = pkg.to_device(ct)
     h = key.hoist(dct)
     try:
-        for k in (O.rotation_exponent_left(n, 1), O.rotation_exponent_left(n, 5), 2 * n - 1):
-            got = pkg.to_host(key.rotate_hoisted(h, dct, k))
-            for c in range(2):
-                bo, ao = hoisted_switch_oracle(basis, L, blocks, ct[c, 0], ct[c, 1], k, kb, ka)
-                assert (got[c, 0] == bo).all() and (got[c, 1] == ao).all(), (k, c)
+        for k in (O.rotation_exponent_left(n, 1), O.rotation_exponent_left(n, 5), 2 * n - 1,
+                  O.rotation_exponent_left(n, n // 4 - 1)):
+            ctx.reset_execution_profile()
+            got = key.rotate_hoisted(h, dct, k)
+            # the hoisted path itself: no decomposition, no forward NTT
+            assert ctx.execution_profile().ntt_forwards == 0
+            assert torch.equal(got, key.apply_galois(dct, k)), k
+        k = O.rotation_exponent_left(n, 3)
+        got = pkg.to_host(key.rotate_hoisted(h, dct, k))
+        bo, ao = canonical_rotation_oracle(basis, L, blocks, ct[2], k, kb, ka)
+        assert (got[2, 0] == bo).all() and (got[2, 1] == ao).all()
+    finally:
+        h.close()
+
+
+@pytest.mark.parametrize("kind", ["hybrid", "rns"])
+def test_zero_block_values_take_the_canonical_path(pkg, ctx, kind):
+    """A coefficient whose every residue in a gadget block is zero has a
+    canonical digit that does not follow the negation; such a batch is
+    rotated canonically, so the residues still equal ckks_apply_galois for
+    every exponent (including ones that negate that coefficient)."""
+    n, L = 1 << 12, 4
+    blocks, K = ([2, 2], 2) if kind == "hybrid" else ([1, 1, 1, 1], 0)
+    q = [Q0[0]] + Q40[:L - 1]
+    basis = q + P60[:K]
+    rng = np.random.default_rng(91)
+    kb, ka = rand_residues(rng, basis, n, len(blocks)), rand_residues(rng, basis, n, len(blocks))
+    plan = pkg.PreparedRnsNttPlan(ctx, basis, n)
+    key = (pkg.HybridKeySwitchKey(plan, L, blocks, kb, ka) if kind == "hybrid"
+           else pkg.RnsKeySwitchKey(plan, blocks, kb, ka))
+    ct = rand_residues(rng, q, n, polys=4).reshape(2, 2, L, n)
+    ct[1, 1, 0:2, 17] = 0      # block 0 of a at coefficient 17 is zero
+    ct[1, 1, 0:2, n - 3] = 0   # and at a coefficient near the top
+    dct = pkg.to_device(ct)
+    h = key.hoist(dct)
+    try:
+        for r in (1, 2, 7, 100):
+            k = O.rotation_exponent_left(n, r)
+            ctx.reset_execution_profile()
+            got = key.rotate_hoisted(h, dct, k)
+            assert ctx.execution_profile().ntt_forwards == 2 * len(blocks)  # the canonical path ran
+            assert torch.equal(got, key.apply_galois(dct, k)), r
     finally:
         h.close()
 
@@ -82,18 +103,22 @@ def test_hoisted_rotation_at_the_benchmark_shape(pkg, ctx):
     ct = rand_residues(rng, q, n, polys=4).reshape(2, 2, L, n)
     dct = pkg.to_device(ct)
     h = key.hoist(dct)
-    k = O.rotation_exponent_left(n, 1 << 7)
-    got = pkg.to_host(key.rotate_hoisted(h, dct, k))
-    h.close()
-    bo, ao = hoisted_switch_oracle(basis, L, blocks, ct[1, 0], ct[1, 1], k, kb, ka)
+    try:
+        for i in (0, 7, 14):
+            k = O.rotation_exponent_left(n, 1 << i)
+            assert torch.equal(key.rotate_hoisted(h, dct, k), key.apply_galois(dct, k)), i
+        k = O.rotation_exponent_left(n, 1 << 7)
+        got = pkg.to_host(key.rotate_hoisted(h, dct, k))
+    finally:
+        h.close()
+    bo, ao = canonical_rotation_oracle(basis, L, blocks, ct[1], k, kb, ka)
     assert (got[1, 0] == bo).all() and (got[1, 1] == ao).all()
 
 
-def test_hoisted_rotation_decrypts_like_the_canonical_one(pkg, ctx):
-    """With keys generated on the device, the hoisted rotation and the
-    canonical one both decrypt to the rotated slots (FP64, 1e-6 of the
-    scale); the hoisted variant is a key switch of sigma_k(a) of the same
-    noise size."""
+def test_hoisted_rotation_decrypts_to_the_rotated_slots(pkg, ctx):
+    """With keys generated on the device, the hoisted rotation is the
+    canonical one, residue for residue, and decrypts to the rotated slots
+    (FP64, 1e-6 of the scale)."""
     from workloads import Workload
     wl = Workload(pkg, ctx, logn=13, L=6, K=3, dnum=3, seed=77)
     L = len(wl.Q)
@@ -106,8 +131,46 @@ def test_hoisted_rotation_decrypts_like_the_canonical_one(pkg, ctx):
         hoisted = key.rotate_hoisted(h, ct, k)
         canonical = key.apply_galois(ct, k)
         h.close()
-        want = np.roll(z, -steps)
-        for out in (hoisted, canonical):
-            err = np.max(np.abs(wl.decrypt(out[0], L, wl.scale) - want))
-            assert err < 1e-6, (steps, err)
-        assert not torch.equal(hoisted, canonical)  # different digits, different residues
+        assert torch.equal(hoisted, canonical), steps
+        err = np.max(np.abs(wl.decrypt(hoisted[0], L, wl.scale) - np.roll(z, -steps)))
+        assert err < 1e-6, (steps, err)
+
+
+def test_hoisted_rotation_batch_of_uneven_degrees(pkg, ctx):
+    """ckks_rotate_hoisted_batch: the benchmark's two shapes (N = 2^16, 25
+    limbs, [9, 8, 8], K = 7 and N = 2^15, 13 limbs, [5, 4, 4], K = 4), an
+    RNS-gadget key and a bounded key in one call; every entry equals
+    apply_galois, and its inverse NTTs run as one launch per pass."""
+    rng = np.random.default_rng(2024)
+    specs = [("hybrid", 16, 25, 7, [9, 8, 8], 3), ("hybrid", 15, 13, 4, [5, 4, 4], 2),
+             ("rns", 12, 4, 0, [2, 1, 1], 2), ("bounded", 13, 6, 2, 32, 2)]
+    entries, hoisted = [], []
+    for idx, (kind, logn, L, K, blocks, n_cts) in enumerate(specs):
+        n = 1 << logn
+        q = [Q0[0]] + Q40[:L - 1]
+        basis = q + P60[:K]
+        plan = pkg.PreparedRnsNttPlan(ctx, basis, n)
+        if kind == "hybrid":
+            key = pkg.HybridKeySwitchKey(plan, L, blocks, rand_residues(rng, basis, n, len(blocks)),
+                                         rand_residues(rng, basis, n, len(blocks)))
+        elif kind == "rns":
+            key = pkg.RnsKeySwitchKey(plan, blocks, rand_residues(rng, basis, n, len(blocks)),
+                                      rand_residues(rng, basis, n, len(blocks)))
+        else:
+            cnt = O.bounded_digit_count(q, blocks)
+            key = pkg.BoundedKeySwitchKey(plan, L, blocks, rand_residues(rng, basis, n, cnt),
+                                          rand_residues(rng, basis, n, cnt))
+        ct = pkg.to_device(rand_residues(rng, q, n, polys=2 * n_cts).reshape(n_cts, 2, L, n))
+        h = key.hoist(ct)
+        hoisted.append(h)
+        entries.append((h, key, O.rotation_exponent_left(n, 3 + idx), ct, torch.empty_like(ct)))
+    try:
+        ctx.reset_execution_profile()
+        pkg.rotate_hoisted_batch(ctx, entries)
+        prof = ctx.execution_profile()
+        assert prof.ntt_forwards == 0  # no decomposition of canonical digits ran again
+        for h, key, k, ct, out in entries:
+            assert torch.equal(out, key.apply_galois(ct, k)), k
+    finally:
+        for h in hoisted:
+            h.close()
