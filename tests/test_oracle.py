@@ -597,3 +597,64 @@ def test_samplers_ranges_and_determinism():
     t = O.Rng(2).ternary(3000)
     assert set(np.unique(t).tolist()) <= {-1, 0, 1}
     assert all(abs((t == v).mean() - 1 / 3) < 0.04 for v in (-1, 0, 1))
+
+
+@pytest.mark.parametrize("blocks,K", [([2, 2], 2), ([1, 1, 1, 1], 0), ([3, 1], 1)])
+def test_hoisted_canonical_digit_identity(blocks, K):
+    """The identity exact hoisting of canonical digits rests on, in the
+    oracle: NTT of the canonical digits of sigma_k(x) = the NTT-domain Galois
+    permutation of the NTT of the digits of x, plus (Q_j mod p_i) times
+    NTT(neg_k) (neg_k = 1 where sigma_k negates), unless a block value is
+    zero; with a zero block value at a negated source the identity fails
+    there, which is why such batches take the canonical path."""
+    n = 64
+    L = sum(blocks)
+    q = [Q0[0]] + Q40[:L - 1]
+    basis = q + P60[:K]
+    orc = O.RnsPlan(basis, n)
+    rng = np.random.default_rng(sum(blocks) * 10 + K)
+    x = np.stack([rng.integers(0, qi, n, dtype=np.uint64) for qi in q])
+
+    def digits(v):
+        out, start = [], 0
+        for sz in blocks:
+            blk = list(range(start, start + sz))
+            others = [i for i in range(len(basis)) if i not in blk]
+            d = np.empty((len(basis), n), dtype=np.uint64)
+            d[blk] = v[start:start + sz]
+            d[others] = O.extend_basis(v[start:start + sz], [basis[i] for i in blk], [basis[i] for i in others])
+            out.append(d)
+            start += sz
+        return out
+
+    for r in (1, 3, 7):
+        k = O.rotation_exponent_left(n, r)
+        kinv = pow(k, -1, 2 * n)
+        neg = np.array([1 if (t * kinv) % (2 * n) >= n else 0 for t in range(n)], dtype=np.uint64)
+        rot = np.stack([O.apply_automorphism(x[i], qi, k) for i, qi in enumerate(q)])
+        want = [orc.forward(d) for d in digits(rot)]
+        base = [orc.forward(d) for d in digits(x)]
+        start = 0
+        for j, sz in enumerate(blocks):
+            Qj = int(np.prod([int(v) for v in q[start:start + sz]], dtype=object))
+            for i, p in enumerate(basis):
+                c = Qj % p
+                got = (O.permute_ntt_galois(base[j][i], k).astype(object) +
+                       c * O.Plan(p, n).forward(neg).astype(object)) % p
+                assert (got.astype(np.uint64) == want[j][i]).all(), (r, j, i)
+            start += sz
+    # a zero block value at a negated source breaks it for that block
+    x[0:blocks[0], 5] = 0
+    k = O.rotation_exponent_left(n, 1)
+    kinv = pow(k, -1, 2 * n)
+    neg = np.array([1 if (t * kinv) % (2 * n) >= n else 0 for t in range(n)], dtype=np.uint64)
+    negated_sources = {(t * kinv) % (2 * n) - n for t in range(n) if (t * kinv) % (2 * n) >= n}
+    rot = np.stack([O.apply_automorphism(x[i], qi, k) for i, qi in enumerate(q)])
+    want = orc.forward(digits(rot)[0])
+    base = orc.forward(digits(x)[0])
+    Q0b = int(np.prod([int(v) for v in q[:blocks[0]]], dtype=object))
+    i = len(basis) - 1 if K else blocks[0]  # a limb outside block 0
+    p = basis[i]
+    got = (O.permute_ntt_galois(base[i], k).astype(object) +
+           (Q0b % p) * O.Plan(p, n).forward(neg).astype(object)) % p
+    assert (got.astype(np.uint64) == want[i]).all() == (5 not in negated_sources)
