@@ -91,7 +91,11 @@ enum { kSrcPlain = 0, kSrcMac = 1, kSrcGalois = 2 };
 template <bool FWD, bool FIRST, int TB, bool BLK, int SRC = kSrcPlain>
 __global__ __launch_bounds__(1 << (TB - 4)) void k_ntt(NttLaunch launch) {
     __shared__ u64 lds[1 << TB];
-    const u32 b = blockIdx.x;
+    // A Galois-permuted load gathers each tile's residues from the whole
+    // limb: the tiles of one limb transform run on one XCD (consecutive
+    // logical blocks), so that XCD's L2 fetches the limb once instead of
+    // every XCD fetching the lines its tiles touch.
+    const u32 b = SRC == kSrcGalois ? xcd_contiguous_block(blockIdx.x, gridDim.x) : blockIdx.x;
     u32 di = 0;
     while (di + 1 < launch.n && b >= launch.d[di + 1].block_start) ++di;
     const DevNttDesc &D = launch.d[di];
