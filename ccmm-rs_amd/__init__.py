@@ -73,6 +73,7 @@ _SIGNATURES = {
     "ckks_rns_plan_destroy": (C.c_int, [_p]),
     "ckks_rns_plan_psi": (C.c_int, [_p, _p]),
     "ckks_rotate_hoisted_batch": (C.c_int, [_p, _p, _sz]),
+    "ckks_rehoist": (C.c_int, [_p, _p, _p, _sz]),
     "ckks_ntt64_plan_create": (C.c_int, [_p, C.c_uint64, _sz, C.POINTER(_p)]),
     "ckks_ntt64_plan_destroy": (C.c_int, [_p]),
     "ckks_ntt64_plan_psi": (C.c_int, [_p, C.POINTER(C.c_uint64)]),
@@ -971,6 +972,15 @@ class HoistedDigits:
         p_ct = _ptr(ct, self.n_cts * 2 * key.q_limbs * key.plan.degree, name="ct")
         _check(lib().ckks_hoist(key.handle, p_ct, self.n_cts, 0, C.byref(h)))
         self._h = h
+
+    def rehoist(self, key, ct):
+        """ckks_rehoist: the digits of a new batch of as many ciphertexts,
+        in place (RNS-gadget / dnum keys); no device allocation."""
+        if key._n_cts(ct, 2) != self.n_cts:
+            raise ValueError("ct must hold as many ciphertexts as were hoisted")
+        p_ct = _ptr(ct, self.n_cts * 2 * key.q_limbs * key.plan.degree, name="ct")
+        _check(lib().ckks_rehoist(self._h, key.handle, p_ct, 0))
+        return self
 
     @property
     def handle(self):

@@ -123,8 +123,11 @@ struct ckks_hoisted {
     // canonical digits: some coefficient has every residue of a gadget block
     // zero, so rotations take the canonical path
     bool zero_block = false;
+    int *d_flag = nullptr, *h_flag = nullptr;  // the zero-block check's result (device, pinned host)
     ~ckks_hoisted() {
         if (d_digits) hipFree(d_digits);
+        if (d_flag) hipFree(d_flag);
+        if (h_flag) hipHostFree(h_flag);
     }
 };
 
@@ -2533,6 +2536,36 @@ int ckks_bounded_hoist_destroy(ckks_hoisted *h) {
     return CKKS_OK;
 }
 
+// The canonical digits of the a components of `in` into h (sized for
+// n_cts ciphertexts of key's shape), their forward NTT, and the zero-block
+// check (one host synchronisation).
+static int hoist_into(ckks_hoisted *h, const ckks_ksk *key, const uint64_t *in, size_t n_cts, size_t in_stride) {
+    ckks_context *ctx = key->ctx;
+    const size_t N = key->degree, B = key->limbs, D = key->dnum, poly = key->q_limbs * N;
+    if (n_cts == 0) return CKKS_OK;
+    CK_HIP(hipSetDevice(ctx->device));
+    // digits of the unrotated a: the same decomposition as a key switch
+    KsJob j{key, in + poly, in_stride, nullptr, nullptr, 0, nullptr, 0, n_cts, 0, nullptr, nullptr};
+    j.digits = h->d_digits;
+    j.mfma = ks_uses_mfma(key);
+    int rc = ks_decompose(j, ctx->stream);
+    if (rc) return rc;
+    NttCall f{key->plan, h->d_digits, h->d_digits, n_cts * D, B, B * N, B * N};
+    if ((rc = launch_ntt(ctx, &f, 1, true))) return rc;
+    // exactness of the hoisted rotations (see hoist_correction)
+    ZeroArgs za{in + poly, in_stride, key->d_modup, h->d_flag, (u32)D, key->plan->logn, (u32)(N / kKsThreads)};
+    hipError_t e = hipMemsetAsync(h->d_flag, 0, sizeof(int), ctx->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_zero_blocks, dim3((u32)(n_cts * za.chunks)), dim3(kKsThreads), 0, ctx->stream, za);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(h->h_flag, h->d_flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return fail(CKKS_ERR_HIP, std::string("zero-block check: ") + hipGetErrorString(e));
+    h->zero_block = *h->h_flag != 0;
+    return CKKS_OK;
+}
+
 int ckks_hoist(const ckks_ksk *key, const uint64_t *in, size_t n_cts, size_t in_stride, ckks_hoisted **out) {
     if (!key || !out || (n_cts && !in)) return fail(CKKS_ERR_INVALID, "null argument");
     if (key->bounded) return ckks_bounded_hoist(key, in, n_cts, in_stride, out);
@@ -2551,32 +2584,26 @@ int ckks_hoist(const ckks_ksk *key, const uint64_t *in, size_t n_cts, size_t in_
     if (n_cts) {
         CK_HIP(hipSetDevice(ctx->device));
         CK_HIP(hipMalloc(&h->d_digits, n_cts * D * B * N * sizeof(u64)));
-        // digits of the unrotated a: the same decomposition as a key switch
-        KsJob j{key, in + poly, in_stride, nullptr, nullptr, 0, nullptr, 0, n_cts, 0, nullptr, nullptr};
-        j.digits = h->d_digits;
-        j.mfma = ks_uses_mfma(key);
-        int rc = ks_decompose(j, ctx->stream);
+        CK_HIP(hipMalloc(&h->d_flag, sizeof(int)));
+        CK_HIP(hipHostMalloc(&h->h_flag, sizeof(int)));
+        int rc = hoist_into(h.get(), key, in, n_cts, in_stride);
         if (rc) return rc;
-        NttCall f{key->plan, h->d_digits, h->d_digits, n_cts * D, B, B * N, B * N};
-        if ((rc = launch_ntt(ctx, &f, 1, true))) return rc;
-        // exactness of the hoisted rotations (see hoist_correction)
-        int *flag = nullptr;
-        CK_HIP(hipMalloc(&flag, sizeof(int)));
-        int host_flag = 0;
-        ZeroArgs za{in + poly, in_stride, key->d_modup, flag, (u32)D, key->plan->logn, (u32)(N / kKsThreads)};
-        hipError_t e = hipMemsetAsync(flag, 0, sizeof(int), ctx->stream);
-        if (e == hipSuccess) {
-            hipLaunchKernelGGL(k_zero_blocks, dim3((u32)(n_cts * za.chunks)), dim3(kKsThreads), 0, ctx->stream, za);
-            e = hipGetLastError();
-        }
-        if (e == hipSuccess) e = hipMemcpyAsync(&host_flag, flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-        hipFree(flag);
-        if (e != hipSuccess) return fail(CKKS_ERR_HIP, std::string("zero-block check: ") + hipGetErrorString(e));
-        h->zero_block = host_flag != 0;
     }
     *out = h.release();
     return CKKS_OK;
+}
+
+int ckks_rehoist(ckks_hoisted *h, const ckks_ksk *key, const uint64_t *in, size_t in_stride) {
+    if (!h || !key || (h->n_cts && !in)) return fail(CKKS_ERR_INVALID, "null argument");
+    if (key->bounded || h->blocks.empty())
+        return fail(CKKS_ERR_UNSUPPORTED, "re-hoisting serves RNS-gadget and dnum-hybrid digits");
+    if (key->plan != h->plan || key->q_limbs != h->q_limbs || key->block_size != h->blocks)
+        return fail(CKKS_ERR_INVALID, "key does not match the hoisted digits (plan, Q limbs, gadget blocks)");
+    if (key->ctx->capturing) return fail(CKKS_ERR_INVALID, "re-hoisting synchronises: not capturable");
+    const size_t poly = key->q_limbs * key->degree;
+    if (in_stride == 0) in_stride = 2 * poly;
+    if (in_stride < 2 * poly) return fail(CKKS_ERR_INVALID, "ciphertext stride smaller than its limbs");
+    return hoist_into(h, key, in, h->n_cts, in_stride);
 }
 
 int ckks_rotate_hoisted(const ckks_hoisted *h, const ckks_ksk *key, uint64_t exponent, const uint64_t *in,

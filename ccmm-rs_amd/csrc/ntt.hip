@@ -305,18 +305,30 @@ int launch_ntt(ckks_context *ctx, const NttCall *calls, size_t count, bool forwa
             if (rc) return rc;
             tmp = (u64 *)ws;
         }
-        // one sweep per kernel variant present (the variant is chosen per
-        // degree); each sweep walks the limb transforms of its entries in
-        // order, closing a chunk when the inter-pass buffer is full
+        // The variant is chosen per degree; in a batch of several degrees the
+        // entry with the most residues lends its variant to every entry it
+        // applies to (8192-residue tiles need N >= 2^14), so the batch runs
+        // as one launch per pass instead of a small trailing launch per
+        // variant.  One sweep per variant left; each sweep walks the limb
+        // transforms of its entries in order, closing a chunk when the
+        // inter-pass buffer is full.
+        int vv[kMaxNttDesc] = {};
+        size_t dom = count, dom_res = 0;
+        for (size_t i = 0; i < count; ++i) {
+            if (!two_pass(i)) continue;
+            vv[i] = ntt_variant_for(ctx, calls[i].plan->logn);
+            const size_t res = calls[i].n_polys * calls[i].n_limbs * calls[i].plan->degree;
+            if (res > dom_res) dom = i, dom_res = res;
+        }
+        for (size_t i = 0; i < count && dom < count; ++i)
+            if (two_pass(i) && (!(vv[dom] & CKKS_NTT_VARIANT_TILE8K) || calls[i].plan->logn >= 14)) vv[i] = vv[dom];
         bool seen[CKKS_NTT_VARIANT_COUNT] = {};
         for (size_t i = 0; i < count; ++i)
-            if (two_pass(i)) seen[ntt_variant_for(ctx, calls[i].plan->logn)] = true;
+            if (two_pass(i)) seen[vv[i]] = true;
         for (int var = 0; var < CKKS_NTT_VARIANT_COUNT; ++var) {
         if (!seen[var]) continue;
         const u32 tb = (var & CKKS_NTT_VARIANT_TILE8K) ? 13 : 12;
-        auto in_sweep = [&](size_t i) {
-            return two_pass(i) && ntt_variant_for(ctx, calls[i].plan->logn) == var;
-        };
+        auto in_sweep = [&](size_t i) { return two_pass(i) && vv[i] == var; };
         size_t ci = 0, lt = 0;
         while (ci < count) {
             NttLaunch l{};

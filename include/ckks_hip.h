@@ -478,6 +478,11 @@ int ckks_bounded_hoist_destroy(ckks_hoisted *hoisted);
  * corrections are cached per exponent as for bounded keys.  Destroy the
  * handle with ckks_bounded_hoist_destroy, before the context. */
 int ckks_hoist(const ckks_ksk *key, const uint64_t *in, size_t n_cts, size_t in_stride, ckks_hoisted **out);
+/* No reference counterpart: ckks_hoist of a new batch of the same count and
+ * shape into an existing handle (RNS-gadget / dnum keys), reusing its
+ * device buffers, for loops that hoist batch after batch without
+ * allocating; the handle's digits of the previous batch are replaced. */
+int ckks_rehoist(ckks_hoisted *hoisted, const ckks_ksk *key, const uint64_t *in, size_t in_stride);
 int ckks_rotate_hoisted(const ckks_hoisted *hoisted, const ckks_ksk *key, uint64_t exponent, const uint64_t *in,
                         uint64_t *out, size_t in_stride, size_t out_stride);
 /* Several hoisted rotations in one launch sequence (no reference

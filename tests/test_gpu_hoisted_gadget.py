@@ -174,3 +174,34 @@ def test_hoisted_rotation_batch_of_uneven_degrees(pkg, ctx):
     finally:
         for h in hoisted:
             h.close()
+
+
+def test_rehoist_replaces_the_digits_in_place(pkg, ctx):
+    """ckks_rehoist: a second batch hoisted into the first one's buffers
+    rotates to apply_galois of the second batch; a batch with a zero block
+    value re-hoisted after a clean one takes the canonical path, and a clean
+    one after it the hoisted path again."""
+    n, L, K, blocks = 1 << 13, 9, 4, [5, 4]
+    q = [Q0[0]] + Q40[:L - 1]
+    basis = q + P60[:K]
+    rng = np.random.default_rng(4242)
+    plan = pkg.PreparedRnsNttPlan(ctx, basis, n)
+    key = pkg.HybridKeySwitchKey(plan, L, blocks, rand_residues(rng, basis, n, 2), rand_residues(rng, basis, n, 2))
+    batches = [rand_residues(rng, q, n, polys=6).reshape(3, 2, L, n) for _ in range(3)]
+    batches[1][2, 1, 5:9, 100] = 0  # block 1 of a zero at coefficient 100
+    k = O.rotation_exponent_left(n, 9)
+    d0 = pkg.to_device(batches[0])
+    h = key.hoist(d0)
+    try:
+        for i, b in enumerate(batches):
+            d = pkg.to_device(b)
+            if i:
+                h.rehoist(key, d)
+            ctx.reset_execution_profile()
+            got = key.rotate_hoisted(h, d, k)
+            assert (ctx.execution_profile().ntt_forwards > 0) == (i == 1), i
+            assert torch.equal(got, key.apply_galois(d, k)), i
+        with pytest.raises(ValueError, match="as many"):
+            h.rehoist(key, pkg.to_device(batches[0][:2]))
+    finally:
+        h.close()
