@@ -47,6 +47,7 @@ Reported (one JSON line):
 from __future__ import annotations
 
 import argparse
+import gc
 import importlib.util
 import json
 import os
@@ -550,6 +551,8 @@ def main():
         held[name] = key.hoist(batch)
         return held[name]
 
+    host_call_ms = []  # host time of each ckks_rotate_hoisted_batch call
+
     def step_hoisted(lat=None, only16=False, hoist_lat=None):
         """The canonical digits of each batch cut and transformed once per
         step (ckks_hoist the first time, ckks_rehoist into the same buffers
@@ -574,7 +577,9 @@ def main():
             if h15 is not None and batched:
                 k2, key2 = keys2[idx]
                 entries.append((h15, key2, k2, cts2, out2))
+            th = time.perf_counter()
             pkg.rotate_hoisted_batch(ctx, entries)
+            host_call_ms.append((time.perf_counter() - th) * 1e3)
             if lat is not None:
                 e1.record()
                 lat.append((e0, e1))
@@ -588,6 +593,21 @@ def main():
             step_fn()
         torch.cuda.synchronize()
         lat_events, hoist_events = [], []
+        # The host enqueues far ahead of the GPU except right after the
+        # per-step host synchronisation of the hoisted step's zero-block
+        # check; a full cyclic garbage collection there (tens of ms over
+        # torch's objects) left the GPU idle.  Objects that exist before the
+        # timed steps are moved out of the collector's reach.
+        host_call_ms.clear()
+        gc.collect()
+        gc.freeze()
+        pauses = []
+        def on_gc(phase, info, _t=[0.0]):
+            if phase == "start":
+                _t[0] = time.perf_counter()
+            else:
+                pauses.append((info["generation"], (time.perf_counter() - _t[0]) * 1e3))
+        gc.callbacks.append(on_gc)
         t0 = time.perf_counter()
         with Timer(torch) as tm:
             for _ in range(args.steps):
@@ -597,6 +617,14 @@ def main():
                     step_fn(lat_events)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
+        gc.callbacks.remove(on_gc)
+        gc.unfreeze()
+        print(f"# {label}: {len(pauses)} garbage collections in the timed steps, longest "
+              f"{max((p[1] for p in pauses), default=0.0):.2f} ms", file=sys.stderr, flush=True)
+        if step_fn is step_hoisted and host_call_ms:
+            hc = sorted(host_call_ms)
+            print(f"# {label}: host time per rotation call p50 {hc[len(hc) // 2]:.3f} ms, max {hc[-1]:.3f} ms",
+                  file=sys.stderr, flush=True)
         ms = tm.ms / args.steps
         lats = sorted(e0.elapsed_time(e1) for e0, e1 in lat_events)
         hoists = [e0.elapsed_time(e1) for e0, e1 in hoist_events]

@@ -31,6 +31,7 @@ struct DevNttDesc {
     u32 lt_begin;  // first limb transform (limb-major index) covered
     NttMac mac;    // fused key inner product (k_ntt<.., kSrcMac> only)
     u32 galois_k;  // NTT-domain Galois permutation on the load (k_ntt<.., kSrcGalois> only)
+    u32 galois_kinv;  // galois_k^-1 mod N
 };
 
 constexpr int kMaxNttDesc = 16;
@@ -72,7 +73,8 @@ struct MacSrc {
 // Input of the first inverse pass read through the NTT-domain Galois
 // permutation (ring/ntt_galois_permutation.rs:5-19): element n of the limb is
 // in[((k (2n + 1)) mod 2N - 1) / 2], k odd; 2N divides 2^32, so the 32-bit
-// product may wrap.
+// product may wrap.  Single-pass degrees gather with it; two-pass degrees
+// read whole source segments and permute in LDS (row_u).
 struct GaloisSrc {
     u32 k;
     __device__ __forceinline__ u64 operator()(const TileJob &job, u32 off, bool fp_initial) const {
@@ -91,11 +93,7 @@ enum { kSrcPlain = 0, kSrcMac = 1, kSrcGalois = 2 };
 template <bool FWD, bool FIRST, int TB, bool BLK, int SRC = kSrcPlain>
 __global__ __launch_bounds__(1 << (TB - 4)) void k_ntt(NttLaunch launch) {
     __shared__ u64 lds[1 << TB];
-    // A Galois-permuted load gathers each tile's residues from the whole
-    // limb: the tiles of one limb transform run on one XCD (consecutive
-    // logical blocks), so that XCD's L2 fetches the limb once instead of
-    // every XCD fetching the lines its tiles touch.
-    const u32 b = SRC == kSrcGalois ? xcd_contiguous_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    const u32 b = blockIdx.x;
     u32 di = 0;
     while (di + 1 < launch.n && b >= launch.d[di + 1].block_start) ++di;
     const DevNttDesc &D = launch.d[di];
@@ -108,7 +106,7 @@ __global__ __launch_bounds__(1 << (TB - 4)) void k_ntt(NttLaunch launch) {
     const DevPrime &P = D.primes[limb];
     const size_t N = (size_t)1 << D.logn;
 
-    TileJob job;
+    TileJob job{};
     job.tw = FWD ? P.fwd : P.inv;
     job.rowf = FWD ? P.rowf : P.rowf_inv;
     job.q = P.q;
@@ -135,6 +133,15 @@ __global__ __launch_bounds__(1 << (TB - 4)) void k_ntt(NttLaunch launch) {
     BlockExec ex{1u << (TB - 4)};
     auto regs_of = [&](u32) -> u64(&)[16] { return regs; };
     static_assert(SRC == kSrcPlain || (!FWD && FIRST), "load variants feed the first inverse pass");
+    if (SRC == kSrcGalois && two) {
+        job.gk = D.galois_k;
+        job.gkinv = D.galois_kinv;
+        if (P.fp)
+            ntt_dispatch<FWD, FIRST, true, TB, BLK, false, true>(ex, regs_of, job, lds);
+        else
+            ntt_dispatch<FWD, FIRST, false, TB, BLK, false, true>(ex, regs_of, job, lds);
+        return;
+    }
     if (SRC == kSrcGalois) {
         const GaloisSrc gs{D.galois_k};
         if (P.fp)
@@ -272,6 +279,13 @@ int launch_ntt(ckks_context *ctx, const NttCall *calls, size_t count, bool forwa
         d.logn = c.plan->logn;
         if (c.mac) d.mac = *c.mac;
         d.galois_k = (u32)c.galois_k;
+        if (c.galois_k) {
+            // inverse modulo 2^32 by Newton steps (each doubles the correct
+            // low bits; k k == 1 mod 8 seeds three), reduced mod N
+            u32 inv = d.galois_k;
+            for (int it = 0; it < 4; ++it) inv *= 2u - d.galois_k * inv;
+            d.galois_kinv = inv & (u32)(c.plan->degree - 1);
+        }
         return d;
     };
     const int src = calls[0].mac ? kSrcMac : calls[0].galois_k ? kSrcGalois : kSrcPlain;
@@ -415,7 +429,7 @@ __device__ void row_mac_body(const RowMacArgs &a, const DevPrime &P, u32 ct, u32
     constexpr u32 T = 1u << (TB - 4);
     (void)N1;
     const u64 N = (u64)1 << LG;
-    TileJob job;
+    TileJob job{};
     job.tw = P.fwd;
     job.rowf = P.rowf;
     job.q = P.q;
@@ -581,7 +595,7 @@ __global__ __launch_bounds__(256) void k_ntt_wide(NttWideArgs a) {
     const u32 tiles = ntt_tiles(a.logn);
     const u32 poly = blockIdx.x / tiles, tile = blockIdx.x % tiles;
     const size_t N = (size_t)1 << a.logn;
-    TileJob job;
+    TileJob job{};
     job.tw = FWD ? a.p.fwd : a.p.inv;
     job.rowf = nullptr;
     job.q = a.p.q;

@@ -49,8 +49,12 @@ struct PlainSrc {
 // BLK: the inter-pass buffer uses the blocked layout (internal_offset).
 // WIDE: integer arithmetic on canonical residues for moduli in [2^62, 2^64)
 // (the lazy [0, 4q) butterflies need q < 2^62).
-template <int L, int CL, PassKind KIND, bool FWD, bool FP, bool BLK = false, bool WIDE = false>
+// GU: inverse row pass whose input is read through the NTT-domain Galois
+// map of job.gk (row_u): the load reads whole source segments and places
+// each residue at its permuted position in LDS.
+template <int L, int CL, PassKind KIND, bool FWD, bool FP, bool BLK = false, bool WIDE = false, bool GU = false>
 struct NttPass {
+    static_assert(!GU || (KIND == PASS_ROW && !FWD), "Galois-read rows belong to the inverse row pass");
     typedef TileShape<L, CL> S;
     static constexpr int NR = n_rounds(L);
 
@@ -83,9 +87,9 @@ struct NttPass {
         constexpr u32 SM = uniform_twiddle_mask(g, L, CL, K, s0, FWD, false);
         constexpr bool RU = wave_uniform_bits(g, L, L + CL);
         if (FWD)
-            fwd_round<L, CL, K, FP, UM, SM, RU, WIDE>(v, g, tid, s0, row, pa, job);
+            fwd_round<L, CL, K, FP, UM, SM, RU, WIDE, GU>(v, g, tid, s0, row, pa, job);
         else
-            inv_round<L, CL, K, FP, UM, SM, RU, WIDE>(v, g, tid, s0, row, pa, job);
+            inv_round<L, CL, K, FP, UM, SM, RU, WIDE, GU>(v, g, tid, s0, row, pa, job);
     }
 
     // global offset (within the limb) of the element at tile coordinate e on
@@ -94,14 +98,14 @@ struct NttPass {
         if (KIND == PASS_COL)
             return FWD ? col_pass_offset<L, CL>(pa, e) : col_internal_offset<L, CL, BLK>(pa, e);
         if (KIND == PASS_ROW)
-            return FWD ? row_internal_offset<L, CL, BLK>(pa, e) : row_natural_offset<L, CL>(pa, e);
+            return FWD ? row_internal_offset<L, CL, BLK, GU>(pa, e) : row_natural_offset<L, CL>(pa, e);
         return e;  // PASS_FULL reads are staged linearly (see run)
     }
     CK_HD static u32 write_off(const PassAddr &pa, u32 e) {
         if (KIND == PASS_COL)
             return FWD ? col_internal_offset<L, CL, BLK>(pa, e) : col_pass_offset<L, CL>(pa, e);
         if (KIND == PASS_ROW)
-            return FWD ? row_natural_offset<L, CL>(pa, e) : row_internal_offset<L, CL, BLK>(pa, e);
+            return FWD ? row_natural_offset<L, CL>(pa, e) : row_internal_offset<L, CL, BLK, GU>(pa, e);
         return e;
     }
 
@@ -151,6 +155,9 @@ struct NttPass {
         pa.n2 = (job.logn + 1) / 2;
         pa.n1 = job.logn - pa.n2;
         pa.tile = job.tile;
+        pa.gk = job.gk;
+        pa.gc = (job.gk - 1) >> 1;
+        pa.gkinv = job.gkinv;
         return pa;
     }
 
@@ -195,6 +202,15 @@ struct NttPass {
                     } else {
                         e = stage_coord(x);
                         off = row_natural_offset<L, CL>(pa, e);
+                        if (GU) {
+                            // off = R 2^n1 + v is source element gk n + gc of
+                            // destination n = h 2^n1 + u, u = row_u(col):
+                            // gk h + ((gk u + gc) >> n1) = R (mod 2^n2)
+                            const u32 col = e >> L;
+                            const u32 z = pa.gk * row_u<CL, true>(pa, col) + pa.gc;
+                            const u32 h = (((off >> pa.n1) - (z >> pa.n1)) * pa.gkinv) & ((1u << pa.n2) - 1);
+                            e = bitrev(h, pa.n2) | (col << L);
+                        }
                     }
                     CK_BOUNDS(off, 1u << n);
                     lds[lds_index<L, CL>(e)] = src(job, off, FP && INITIAL);
@@ -304,8 +320,10 @@ CK_HD u32 ntt_tiles(u32 logn, u32 tb = 12) { return logn > 12 ? (1u << (logn - t
 // transform (forward: column/full, inverse: row/full).  TB = log2 of the
 // two-pass tile size (12: 256 threads, 13: 512 threads; the full pass for
 // N <= 4096 always uses 256 threads); BLK = blocked inter-pass layout.
-template <bool FWD, bool FIRST, bool FP, int TB = 12, bool BLK = false, bool WIDE = false, class Exec, class Regs,
-          class Src = PlainSrc>
+// GU: the inverse first (row) pass of a two-pass degree reads its input
+// through the Galois map of job.gk (NttPass); other passes ignore it.
+template <bool FWD, bool FIRST, bool FP, int TB = 12, bool BLK = false, bool WIDE = false, bool GU = false,
+          class Exec, class Regs, class Src = PlainSrc>
 CK_HD void ntt_dispatch(Exec &ex, Regs &&regs_of, const TileJob &job, u64 *lds, const Src &src = Src{}) {
     switch (job.logn) {
 #define CK_FULL(LG)                                                   \
@@ -322,7 +340,7 @@ CK_HD void ntt_dispatch(Exec &ex, Regs &&regs_of, const TileJob &job, u64 *lds, 
         if (FWD == FIRST)                                                                   \
             NttPass<N1, TB - N1, PASS_COL, FWD, FP, BLK, WIDE>::run(ex, regs_of, job, lds, src);  \
         else                                                                                \
-            NttPass<N2, TB - N2, PASS_ROW, FWD, FP, BLK, WIDE>::run(ex, regs_of, job, lds, src);  \
+            NttPass<N2, TB - N2, PASS_ROW, FWD, FP, BLK, WIDE, GU && !FWD>::run(ex, regs_of, job, lds, src); \
         break;                                                                              \
     }
         CK_TWO(13)

@@ -67,6 +67,9 @@ struct TileJob {
     double ninv_d, ninv_p;  // N^-1 and N^-1/q as doubles
     u32 logn;
     u32 tile;  // tile index within the limb transform
+    // Galois-read inverse row pass: input element n is src[(gk n + (gk - 1) / 2) mod N]
+    // for the odd exponent gk < 2N; gkinv = gk^-1 mod N
+    u32 gk, gkinv;
 };
 
 CK_HD u32 bitrev(u32 x, u32 bits) {
@@ -235,10 +238,25 @@ CK_HD void gs_bfly_fp(u64 &x, u64 &y, const Tw &w, double q, double qinv) {
 // Global element offsets (within the limb) for tile coordinate e.
 // Column pass: t indexes rows of a column, col indexes columns.
 // Row pass: col indexes rows u (natural side), t = r.
+// Galois-read inverse row passes (GU below) also carry the permutation
+// n -> gk n + gc (mod N) of the source and gkinv = gk^-1 mod N.
 struct PassAddr {
     u32 logn, n1, n2;
     u32 tile;
+    u32 gk, gc, gkinv;
 };
+
+// Row u of tile column col in a row pass.  Plain: the tile's 2^CL
+// consecutive rows.  GU (inverse row pass reading its input through the
+// NTT-domain Galois map n -> gk n + gc): the rows whose images have low
+// n1 bits v = tile 2^CL + col, so the tile's source is 2^CL whole source
+// columns (contiguous segments of every source row, see the staged load);
+// u = (v - gc) gk^-1 mod 2^n1.
+template <int CL, bool GU>
+CK_HD u32 row_u(const PassAddr &pa, u32 col) {
+    const u32 v = pa.tile * (1u << CL) + col;
+    return GU ? ((v - pa.gc) * pa.gkinv) & ((1u << pa.n1) - 1) : v;
+}
 
 template <int L, int CL>
 CK_HD u32 col_pass_offset(const PassAddr &pa, u32 e) {
@@ -265,10 +283,10 @@ CK_HD u32 internal_offset(const PassAddr &pa, u32 m, u32 c) {
 }
 
 // Row pass, internal side: element (m, r) with m = bitrev_n1(u).
-template <int L, int CL, bool BLK>
+template <int L, int CL, bool BLK, bool GU = false>
 CK_HD u32 row_internal_offset(const PassAddr &pa, u32 e) {
     u32 r = e & ((1u << L) - 1), col = e >> L;
-    u32 u = pa.tile * (1u << CL) + col;
+    u32 u = row_u<CL, GU>(pa, col);
     u32 m = bitrev(u, pa.n1);
     return internal_offset<L + CL, BLK>(pa, m, r);
 }
@@ -289,10 +307,10 @@ CK_HD u32 row_natural_offset(const PassAddr &pa, u32 e) {
     return (bitrev(r, pa.n2) << pa.n1) + u;
 }
 
-template <int L, int CL>
+template <int L, int CL, bool GU = false>
 CK_HD u32 row_twiddle_base(const PassAddr &pa, u32 e) {
     u32 col = e >> L;
-    u32 u = pa.tile * (1u << CL) + col;
+    u32 u = row_u<CL, GU>(pa, col);
     return (1u << pa.n1) + bitrev(u, pa.n1);
 }
 
@@ -356,7 +374,8 @@ CK_HD Tw row_twiddle_fp(const TileJob &job, u32 sidx, u32 fidx) {
 // Forward round: stages s0..s0+k-1 (butterfly bit of stage s is L-1-s).
 // SM: stages whose shared index (row-factored twiddles) is wave-uniform;
 // RU: the row of an element is wave-uniform.
-template <int L, int CL, int K, bool FP, u32 UM = 0, u32 SM = 0, bool RU = false, bool WIDE = false>
+template <int L, int CL, int K, bool FP, u32 UM = 0, u32 SM = 0, bool RU = false, bool WIDE = false,
+          bool GU = false>
 CK_HD void fwd_round(u64 (&v)[16], const Geom &g, u32 tid, int s0, bool row_base,
                      const PassAddr &pa, const TileJob &job) {
     const bool fac = FP && row_base;  // FP64 row passes always read row factors
@@ -372,12 +391,12 @@ CK_HD void fwd_round(u64 (&v)[16], const Geom &g, u32 tid, int s0, bool row_base
             Tw w;
             if (fac) {
                 const u32 sidx = (1u << s) + (t >> (L - s));
-                const u32 fidx = (pa.tile * (1u << CL) + (e >> L)) * L + (u32)s;
+                const u32 fidx = row_u<CL, GU>(pa, e >> L) * L + (u32)s;
                 CK_BOUNDS(sidx, 1u << L);
                 CK_BOUNDS(fidx, (1u << (pa.logn - L)) * L);
                 w = (SM >> j) & 1u ? row_twiddle_fp<true, RU>(job, sidx, fidx) : row_twiddle_fp<false, RU>(job, sidx, fidx);
             } else {
-                u32 base = row_base ? row_twiddle_base<L, CL>(pa, e) : 1u;
+                u32 base = row_base ? row_twiddle_base<L, CL, GU>(pa, e) : 1u;
                 u32 idx = (base << s) + (t >> (L - s));
                 CK_BOUNDS(idx, 1u << pa.logn);
                 w = (UM >> j) & 1u ? load_twiddle<true>(job.tw, idx) : load_twiddle<false>(job.tw, idx);
@@ -394,7 +413,8 @@ CK_HD void fwd_round(u64 (&v)[16], const Geom &g, u32 tid, int s0, bool row_base
 
 // Inverse round: GS stages u0..u0+k-1 (butterfly bit of stage u is u; its
 // twiddle level is L - 1 - u).
-template <int L, int CL, int K, bool FP, u32 UM = 0, u32 SM = 0, bool RU = false, bool WIDE = false>
+template <int L, int CL, int K, bool FP, u32 UM = 0, u32 SM = 0, bool RU = false, bool WIDE = false,
+          bool GU = false>
 CK_HD void inv_round(u64 (&v)[16], const Geom &g, u32 tid, int u0, bool row_base,
                      const PassAddr &pa, const TileJob &job) {
     const bool fac = FP && row_base;  // FP64 row passes always read row factors
@@ -410,12 +430,12 @@ CK_HD void inv_round(u64 (&v)[16], const Geom &g, u32 tid, int u0, bool row_base
             Tw w;
             if (fac) {
                 const u32 sidx = (1u << (L - 1 - u)) + (t >> (u + 1));
-                const u32 fidx = (pa.tile * (1u << CL) + (e >> L)) * L + (u32)(L - 1 - u);
+                const u32 fidx = row_u<CL, GU>(pa, e >> L) * L + (u32)(L - 1 - u);
                 CK_BOUNDS(sidx, 1u << L);
                 CK_BOUNDS(fidx, (1u << (pa.logn - L)) * L);
                 w = (SM >> j) & 1u ? row_twiddle_fp<true, RU>(job, sidx, fidx) : row_twiddle_fp<false, RU>(job, sidx, fidx);
             } else {
-                u32 base = row_base ? row_twiddle_base<L, CL>(pa, e) : 1u;
+                u32 base = row_base ? row_twiddle_base<L, CL, GU>(pa, e) : 1u;
                 u32 idx = (base << (L - 1 - u)) + (t >> (u + 1));
                 CK_BOUNDS(idx, 1u << pa.logn);
                 w = (UM >> j) & 1u ? load_twiddle<true>(job.tw, idx) : load_twiddle<false>(job.tw, idx);

@@ -11,7 +11,7 @@
 
 using namespace ck;
 
-template <bool FWD, bool FIRST, bool FP, int TB, bool BLK, bool WIDE = false>
+template <bool FWD, bool FIRST, bool FP, int TB, bool BLK, bool WIDE = false, bool GU = false>
 static void run_pass(const TileJob &base, u32 tiles) {
     static u64 regs[1 << (TB - 4)][16];
     static u64 lds[1 << TB];
@@ -21,19 +21,19 @@ static void run_pass(const TileJob &base, u32 tiles) {
         BlockExec ex{1u << (TB - 4)};
         std::memset(lds, 0xA5, sizeof lds);
         auto regs_of = [&](u32 tid) -> u64(&)[16] { return regs[tid]; };
-        ntt_dispatch<FWD, FIRST, FP, TB, BLK, WIDE>(ex, regs_of, job, lds);
+        ntt_dispatch<FWD, FIRST, FP, TB, BLK, WIDE, GU>(ex, regs_of, job, lds);
     }
 }
 
-template <bool FWD, bool FIRST, bool FP, bool WIDE = false>
+template <bool FWD, bool FIRST, bool FP, bool WIDE = false, bool GU = false>
 static void run_variant(const TileJob &job, u32 logn, int tb, int blk) {
     const bool two = ntt_two_pass(logn);
     if (two && tb == 13) {
-        if (blk) run_pass<FWD, FIRST, FP, 13, true, WIDE>(job, ntt_tiles(logn, 13));
-        else run_pass<FWD, FIRST, FP, 13, false, WIDE>(job, ntt_tiles(logn, 13));
+        if (blk) run_pass<FWD, FIRST, FP, 13, true, WIDE, GU>(job, ntt_tiles(logn, 13));
+        else run_pass<FWD, FIRST, FP, 13, false, WIDE, GU>(job, ntt_tiles(logn, 13));
     } else {
-        if (blk && two) run_pass<FWD, FIRST, FP, 12, true, WIDE>(job, ntt_tiles(logn));
-        else run_pass<FWD, FIRST, FP, 12, false, WIDE>(job, ntt_tiles(logn));
+        if (blk && two) run_pass<FWD, FIRST, FP, 12, true, WIDE, GU>(job, ntt_tiles(logn));
+        else run_pass<FWD, FIRST, FP, 12, false, WIDE, GU>(job, ntt_tiles(logn));
     }
 }
 
@@ -87,6 +87,55 @@ extern "C" int emu_ntt_variant(const uint64_t *in, uint64_t *out, uint64_t q, ui
         if (forward) CK_RUN(true, false);
         else CK_RUN(false, false);
     }
+    return 0;
+}
+
+// Inverse transform of a two-pass degree whose first pass reads its input
+// through the NTT-domain Galois map of the odd exponent k < 2N (element n
+// of the transformed limb is in[(k n + (k - 1) / 2) mod N]), as the rotation
+// kernels run it: whole source segments loaded, permuted in LDS.  The
+// inverse of k mod N is computed the way the launch code does.
+extern "C" int emu_intt_galois(const uint64_t *in, uint64_t *out, uint64_t q, uint32_t logn, int tb, int blk,
+                               uint32_t k) {
+    if (logn < 13 || logn > 17 || (tb != 12 && tb != 13) || (tb == 13 && logn < 14)) return -1;
+    const uint32_t n = 1u << logn;
+    if ((k & 1u) == 0 || k >= 2 * n) return -1;
+    uint64_t psi = host::negacyclic_root(q, n);
+    if (!psi) return -2;
+    const bool fp = q < host::kFpModulusBound;
+    if (q >= (1ull << 62)) return -1;
+    const uint64_t ipsi = host::invmod(psi, q);
+    std::vector<uint64_t> tw, rowf, tmp(n, 0xDEADBEEFull), src(in, in + n);
+    if (fp) {
+        host::twiddle_table_fp(q, ipsi, logn, tw);
+        host::row_factor_table_fp(q, ipsi, logn, rowf);
+    } else {
+        host::twiddle_table(q, ipsi, logn, tw);
+    }
+    TileJob job{};
+    job.tw = reinterpret_cast<const Tw *>(tw.data());
+    job.rowf = rowf.empty() ? nullptr : reinterpret_cast<const Tw *>(rowf.data());
+    job.q = q;
+    job.ninv = host::invmod(n % q, q);
+    job.ninv_q = host::shoup_quotient(job.ninv, q);
+    job.qd = (double)q;
+    job.qinv = 1.0 / (double)q;
+    job.ninv_d = (double)job.ninv;
+    job.ninv_p = (double)job.ninv / (double)q;
+    job.logn = logn;
+    u32 inv = k;
+    for (int it = 0; it < 4; ++it) inv *= 2u - k * inv;
+    job.gk = k;
+    job.gkinv = inv & (n - 1);
+    job.src = src.data();
+    job.dst = tmp.data();
+    if (fp) run_variant<false, true, true, false, true>(job, logn, tb, blk);
+    else run_variant<false, true, false, false, true>(job, logn, tb, blk);
+    job.gk = job.gkinv = 0;
+    job.src = tmp.data();
+    job.dst = out;
+    if (fp) run_variant<false, false, true>(job, logn, tb, blk);
+    else run_variant<false, false, false>(job, logn, tb, blk);
     return 0;
 }
 

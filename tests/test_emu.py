@@ -69,6 +69,31 @@ def test_emulated_kernel_variants_match_oracle(emu, logn, tb, blk):
         assert (back == plan.inverse(x)).all()
 
 
+@pytest.mark.parametrize("logn", [13, 14, 15, 16, 17])
+@pytest.mark.parametrize("tb,blk", [(12, 0), (12, 1), (13, 0), (13, 1)])
+def test_emulated_galois_read_inverse_matches_oracle(emu, logn, tb, blk):
+    """The rotation's first inverse pass, which loads whole source segments
+    and places each residue at its Galois-permuted position in LDS, gives
+    the oracle's inverse NTT of the oracle's NTT-domain permutation, for
+    the identity, the rotation generators, conjugation and odd exponents."""
+    if tb == 13 and logn < 14:
+        pytest.skip("8192-residue tiles serve N >= 2^14")
+    emu.emu_intt_galois.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32, C.c_int, C.c_int,
+                                    C.c_uint32]
+    n = 1 << logn
+    rng = np.random.default_rng(300 + logn)
+    primes = [Q40[2], P60[2]] if logn < 17 else [O.ntt_primes(60, 1 << 17, 1)[0]]
+    exps = [1, 5, pow(5, 7, 2 * n), pow(5, n // 4 - 1, 2 * n), 2 * n - 1, 2 * n - 5,
+            int(rng.integers(0, n)) * 2 + 1]
+    for q in primes:
+        plan = O.Plan(q, n)
+        x = rng.integers(0, q, n, dtype=np.uint64)
+        for k in exps:
+            out = np.zeros(n, dtype=np.uint64)
+            assert emu.emu_intt_galois(x.ctypes.data, out.ctypes.data, q, logn, tb, blk, k) == 0
+            assert (out == plan.inverse(O.permute_ntt_galois(x, k))).all(), (q, k)
+
+
 def _is_prime(n):
     if n < 2:
         return False
