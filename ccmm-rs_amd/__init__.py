@@ -297,8 +297,9 @@ class Context:
 
     def set_key_switch_fusion(self, mode: int):
         """Key switch: digits' forward row pass fused with the key MAC
-        (XCD or LINEAR block order), separate kernels (OFF), or the key's
-        tuned choice (AUTO, the default)."""
+        (XCD or LINEAR block order), the MAC inside the first inverse pass
+        (INVERSE; also for hoisted rotations), separate kernels (OFF), or
+        the key's tuned choice (AUTO, the default)."""
         _check(lib().ckks_set_key_switch_fusion(self._h, int(mode)))
 
     KS_SPLIT_AUTO, KS_SPLIT_OFF, KS_SPLIT_ON = -1, 0, 1

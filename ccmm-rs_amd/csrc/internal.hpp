@@ -163,6 +163,9 @@ struct NttMac {
     const u64 *digits;   // [ct][dnum][limbs][N]
     const u64 *kb, *ka;  // [dnum][limbs][N]
     u32 dnum;            // 1..8
+    // hoisted rotations: a [limbs][N] table per component added to every
+    // ciphertext's sum (the canonical-lift correction), null otherwise
+    const u64 *tb = nullptr, *ta = nullptr;
 };
 
 // Launch helpers (ntt.hip).
@@ -174,7 +177,8 @@ struct NttCall {
     size_t in_stride, out_stride;  // elements between consecutive polynomials
     const NttMac *mac = nullptr;   // inverse two-pass degrees only: see NttMac
     // inverse only, 0 = off: the transform reads its input through the
-    // NTT-domain Galois permutation by this odd exponent (in != out)
+    // NTT-domain Galois permutation by this odd exponent (in != out); with
+    // mac, the key inner product is what the permutation reads
     u64 galois_k = 0;
 };
 // scratch / scratch_elems: an inter-pass buffer supplied by the caller

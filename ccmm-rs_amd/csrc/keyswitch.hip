@@ -1758,6 +1758,18 @@ static int ks_moddown(const KsJob &j, hipStream_t s) {
     return CKKS_OK;
 }
 
+// With CKKS_KS_FUSION_INVERSE set on the context, hoisted rotations of
+// two-pass degrees with dnum <= 8 compute the key MAC inside the
+// Galois-read first inverse NTT pass (the NTT-domain accumulators never
+// reach HBM).  Not the default: every output then reads its dnum digit,
+// key and correction entries (7 loads at dnum = 3) where the separate MAC
+// keeps the key entries in registers across the batch; on MI355X at
+// N = 2^16, 64 + 16 ciphertexts, the fused pass took 3.27 ms against
+// 1.26 ms (MAC) + 1.30 ms (Galois-read pass).
+static bool hoisted_mac_fused(const ckks_context *ctx, u32 logn, size_t dnum) {
+    return logn > 12 && logn <= 17 && dnum <= 8 && ctx->ks_fusion == CKKS_KS_FUSION_INVERSE;
+}
+
 static int key_switch_lane(const ckks_ksk *k, const u64 *x, u64 x_stride, const u64 *add0,
                            const u64 *add1, u64 add_stride, u64 *out, u64 out_stride, size_t n_cts,
                            u64 galois_kinv, const int64_t *hoisted, const HoistedMac *hm) {
@@ -1768,9 +1780,13 @@ static int key_switch_lane(const ckks_ksk *k, const u64 *x, u64 x_stride, const 
     hipStream_t s = ctx->stream;
     int rc;
     void *wd = nullptr, *wa;
+    // hoisted rotations of two-pass degrees with dnum <= 8: the MAC runs in
+    // the first inverse NTT pass, which reads it through the permutation
+    const bool two_pass = logn > 12 && logn <= 17;
+    const bool hoist_fused = hm && hoisted_mac_fused(ctx, logn, D);
     // hoisted: slot 4 holds the unpermuted MAC output instead of the digits
     const size_t slot4 = hm ? n_cts * 2 * B * N : n_cts * D * B * N;
-    if ((rc = ctx->workspace(4, slot4 * sizeof(u64), &wd))) return rc;
+    if (!hoist_fused && (rc = ctx->workspace(4, slot4 * sizeof(u64), &wd))) return rc;
     if ((rc = ctx->workspace(5, n_cts * 2 * B * N * sizeof(u64), &wa))) return rc;
     KsJob j{k, x, x_stride, add0, add1, add_stride, out, out_stride, n_cts, galois_kinv, hoisted, hm};
     j.digits = hm ? const_cast<u64 *>(hm->digits) : (u64 *)wd;
@@ -1781,7 +1797,6 @@ static int key_switch_lane(const ckks_ksk *k, const u64 *x, u64 x_stride, const 
     if ((rc = ks_decompose(j, s))) return rc;
     // 2+3. forward NTT of every digit limb, key MAC
     const int fusion = ctx->ks_fusion == CKKS_KS_FUSION_AUTO ? k->tuned_fusion : ctx->ks_fusion;
-    const bool two_pass = logn > 12 && logn <= 17;
     // CKKS_KS_FUSION_INVERSE: the MAC is computed inside the first inverse
     // NTT pass (step 4) and the accumulators never reach HBM in the NTT domain
     const bool mac_in_inverse = fusion == CKKS_KS_FUSION_INVERSE && two_pass && D <= 8 && !hm;
@@ -1792,14 +1807,15 @@ static int key_switch_lane(const ckks_ksk *k, const u64 *x, u64 x_stride, const 
     } else {
         NttCall f{plan, j.digits, j.digits, n_cts * D, B, B * N, B * N};
         if (!hm && (rc = launch_ntt(ctx, &f, 1, true))) return rc;
-        if (!mac_in_inverse && (rc = ks_mac(j, s))) return rc;
+        if (!mac_in_inverse && !hoist_fused && (rc = ks_mac(j, s))) return rc;
     }
-    // 4. inverse NTT of both accumulators.  Hoisted: the MAC ran over
+    // 4. inverse NTT of both accumulators.  Hoisted: the MAC runs over
     // sigma_k^-1-permuted keys; the inverse NTT reads its output permuted by
     // k, which is sum_j NTT(sigma_k(d_j)) * key_j
-    const NttMac mac{j.digits, k->d_kb, k->d_ka, (u32)D};
-    NttCall iv{plan, hm ? j.mac_out : j.acc, j.acc, n_cts * 2, B, B * N, B * N, mac_in_inverse ? &mac : nullptr,
-               hm ? hm->k : 0};
+    const NttMac mac = hm ? NttMac{hm->digits, hm->kb, hm->ka, (u32)D, hm->tb, hm->ta}
+                          : NttMac{j.digits, k->d_kb, k->d_ka, (u32)D};
+    NttCall iv{plan, hm ? (hoist_fused ? hm->digits : j.mac_out) : j.acc, j.acc, n_cts * 2, B, B * N, B * N,
+               mac_in_inverse || hoist_fused ? &mac : nullptr, hm ? hm->k : 0};
     if ((rc = launch_ntt(ctx, &iv, 1, false))) return rc;
     // 5. ModDown (or plain finalize) with pass-through adds
     return ks_moddown(j, s);
@@ -2680,16 +2696,20 @@ int ckks_rotate_hoisted_batch(ckks_context *ctx, const ckks_hoisted_rotation *e,
         fast.push_back(Fast{h, key, r.in, r.out, in_stride, out_stride, k, pk});
     }
     if (fast.empty()) return CKKS_OK;
-    // the MAC outputs (slot 4) and the accumulators (slot 5) of every entry
+    // every entry's MAC in the first inverse pass, or none
+    bool fused = true;
+    for (const Fast &f : fast) fused = fused && hoisted_mac_fused(ctx, f.key->plan->logn, f.key->dnum);
+    // the MAC outputs (slot 4, unfused only) and the accumulators (slot 5)
     size_t total = 0;
     for (const Fast &f : fast) total += f.h->n_cts * 2 * f.key->limbs * f.key->degree;
-    void *wd, *wa;
-    if ((rc = ctx->workspace(4, total * sizeof(u64), &wd))) return rc;
+    void *wd = nullptr, *wa;
+    if (!fused && (rc = ctx->workspace(4, total * sizeof(u64), &wd))) return rc;
     if ((rc = ctx->workspace(5, total * sizeof(u64), &wa))) return rc;
     hipStream_t s = ctx->stream;
     std::vector<HoistedMac> hms(fast.size());
     std::vector<KsJob> jobs(fast.size());
     std::vector<NttCall> inv(fast.size());
+    std::vector<NttMac> macs(fast.size());
     u64 *pd = (u64 *)wd, *pa = (u64 *)wa;
     for (size_t i = 0; i < fast.size(); ++i) {
         const Fast &f = fast[i];
@@ -2701,19 +2721,23 @@ int ckks_rotate_hoisted_batch(ckks_context *ctx, const ckks_hoisted_rotation *e,
         j = KsJob{key, f.in + poly, f.in_stride, f.in, nullptr, f.in_stride, f.out, f.out_stride, f.h->n_cts, kinv,
                   nullptr, &hms[i]};
         j.digits = const_cast<u64 *>(f.h->d_digits);
-        j.mac_out = pd;
+        j.mac_out = fused ? nullptr : pd;
         j.acc = pa;
         j.mfma = ks_uses_mfma(key);
-        pd += f.h->n_cts * 2 * B * N;
+        if (!fused) pd += f.h->n_cts * 2 * B * N;
         pa += f.h->n_cts * 2 * B * N;
-        inv[i] = NttCall{key->plan, j.mac_out, j.acc, f.h->n_cts * 2, B, B * N, B * N, nullptr, f.k};
+        const HoistedMac &hm = hms[i];
+        macs[i] = NttMac{hm.digits, hm.kb, hm.ka, (u32)key->dnum, hm.tb, hm.ta};
+        inv[i] = NttCall{key->plan, fused ? hm.digits : j.mac_out, j.acc, f.h->n_cts * 2, B, B * N, B * N,
+                         fused ? &macs[i] : nullptr, f.k};
         ckks_execution_profile &pc = ctx->counts();
         pc.key_switches += f.h->n_cts;
         (f.k == 2 * (u64)N - 1 ? pc.automorphisms : pc.rotations) += f.h->n_cts;
-        if ((rc = ks_mac(j, s))) return rc;
+        if (!fused && (rc = ks_mac(j, s))) return rc;
     }
     // one inverse NTT launch per pass for every entry, each reading its MAC
-    // output through its own Galois permutation
+    // (computed in the first pass when fused) through its own Galois
+    // permutation
     if ((rc = launch_ntt(ctx, inv.data(), inv.size(), false))) return rc;
     for (KsJob &j : jobs)
         if ((rc = ks_moddown(j, s))) return rc;

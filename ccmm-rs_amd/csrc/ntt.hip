@@ -29,6 +29,7 @@ struct DevNttDesc {
     u64 in_stride, out_stride;
     u32 n_polys, n_limbs, logn, block_start;
     u32 lt_begin;  // first limb transform (limb-major index) covered
+    u32 lt_count;  // limb transforms covered (two-pass launches)
     NttMac mac;    // fused key inner product (k_ntt<.., kSrcMac> only)
     u32 galois_k;  // NTT-domain Galois permutation on the load (k_ntt<.., kSrcGalois> only)
     u32 galois_kinv;  // galois_k^-1 mod N
@@ -47,25 +48,42 @@ struct NttLaunch {
 // sum into 0.51 q, the input range of the inverse FP64 butterflies; integer
 // limbs accumulate 128-bit products and reduce once (D q^2 < 2^64 q for
 // q < 2^61).  Digits and key entries are canonical residues.
+// A correction table c (hoisted rotations: canonical residues) joins the
+// sum as one more term.
 struct MacSrc {
     const u64 *d;  // digit 0 of this (ct, limb); digit j at d + j * d_stride
     const u64 *k;  // key entry 0 of this (comp, limb); entry j at k + j * k_stride
+    const u64 *c;  // correction of this (comp, limb) or null
     u64 d_stride, k_stride;
     u32 dnum;
     Barrett br;
     double qd, qinv;
     __device__ __forceinline__ u64 operator()(const TileJob &, u32 off, bool fp) const {
-        if (fp) {
-            double acc = 0.0;
-            for (u32 j = 0; j < dnum; ++j) {
-                const double w = small_to_f64(k[j * k_stride + off]);
-                acc += fp_mulmod(small_to_f64(d[j * d_stride + off]), w, w * qinv, qd);
+        // every load of the element issued before the first use
+        u64 dv[8], kv[8];
+        const u64 cv = c ? c[off] : 0;
+#pragma unroll
+        for (u32 j = 0; j < 8; ++j)
+            if (j < dnum) {
+                dv[j] = d[j * d_stride + off];
+                kv[j] = k[j * k_stride + off];
             }
+        if (fp) {
+            double acc = small_to_f64(cv);
+#pragma unroll
+            for (u32 j = 0; j < 8; ++j)
+                if (j < dnum) {
+                    const double w = small_to_f64(kv[j]);
+                    acc += fp_mulmod(small_to_f64(dv[j]), w, w * qinv, qd);
+                }
             return f64_as_u64(fp_reduce(acc, qd, qinv));
         }
         Acc128 t;
         t.zero();
-        for (u32 j = 0; j < dnum; ++j) t.mac(d[j * d_stride + off], k[j * k_stride + off]);
+        t.mac(cv, 1);
+#pragma unroll
+        for (u32 j = 0; j < 8; ++j)
+            if (j < dnum) t.mac(dv[j], kv[j]);
         return t.reduce(br);
     }
 };
@@ -84,7 +102,13 @@ struct GaloisSrc {
     }
 };
 
-enum { kSrcPlain = 0, kSrcMac = 1, kSrcGalois = 2 };
+// kSrcMacGalois: the key inner product read through the Galois permutation
+// (two-pass degrees).  Its blocks run tile-major, limb transform fastest,
+// remapped so each XCD runs a contiguous range: the two components of one
+// ciphertext (adjacent limb transforms) read the same digit segments, and
+// every ciphertext of one limb the same key and correction segments, from
+// one L2.
+enum { kSrcPlain = 0, kSrcMac = 1, kSrcGalois = 2, kSrcMacGalois = 3 };
 
 // TB = log2 of the two-pass tile (12: 256 threads, 13: 512 threads);
 // BLK = blocked inter-pass layout.  Single-pass launches use <.., 12, false>.
@@ -93,14 +117,14 @@ enum { kSrcPlain = 0, kSrcMac = 1, kSrcGalois = 2 };
 template <bool FWD, bool FIRST, int TB, bool BLK, int SRC = kSrcPlain>
 __global__ __launch_bounds__(1 << (TB - 4)) void k_ntt(NttLaunch launch) {
     __shared__ u64 lds[1 << TB];
-    const u32 b = blockIdx.x;
+    const u32 b = SRC == kSrcMacGalois ? xcd_contiguous_block(blockIdx.x, gridDim.x) : blockIdx.x;
     u32 di = 0;
     while (di + 1 < launch.n && b >= launch.d[di + 1].block_start) ++di;
     const DevNttDesc &D = launch.d[di];
     const u32 tiles = ntt_tiles(D.logn, TB);
     const u32 local = b - D.block_start;
-    const u32 tile = local % tiles;
-    const u32 rel = local / tiles;
+    const u32 tile = SRC == kSrcMacGalois ? local / D.lt_count : local % tiles;
+    const u32 rel = SRC == kSrcMacGalois ? local % D.lt_count : local / tiles;
     const u32 lt = D.lt_begin + rel;
     const u32 limb = lt / D.n_polys, poly = lt % D.n_polys;
     const DevPrime &P = D.primes[limb];
@@ -150,17 +174,28 @@ __global__ __launch_bounds__(1 << (TB - 4)) void k_ntt(NttLaunch launch) {
             ntt_dispatch<FWD, FIRST, false, TB, BLK>(ex, regs_of, job, lds, gs);
         return;
     }
-    if (SRC == kSrcMac) {
+    if (SRC == kSrcMac || SRC == kSrcMacGalois) {
         const u32 ct = poly >> 1, comp = poly & 1;
         MacSrc ms;
         ms.d_stride = (u64)D.n_limbs * N;
         ms.k_stride = (u64)D.n_limbs * N;
         ms.d = D.mac.digits + (u64)ct * D.mac.dnum * ms.d_stride + (u64)limb * N;
         ms.k = (comp ? D.mac.ka : D.mac.kb) + (u64)limb * N;
+        const u64 *corr = comp ? D.mac.ta : D.mac.tb;
+        ms.c = corr ? corr + (u64)limb * N : nullptr;
         ms.dnum = D.mac.dnum;
         ms.br = Barrett{P.q, P.br_hi, P.br_lo};
         ms.qd = P.qd;
         ms.qinv = P.qinv;
+        if (SRC == kSrcMacGalois) {
+            job.gk = D.galois_k;
+            job.gkinv = D.galois_kinv;
+            if (P.fp)
+                ntt_dispatch<FWD, FIRST, true, TB, BLK, false, true>(ex, regs_of, job, lds, ms);
+            else
+                ntt_dispatch<FWD, FIRST, false, TB, BLK, false, true>(ex, regs_of, job, lds, ms);
+            return;
+        }
         if (P.fp)
             ntt_dispatch<FWD, FIRST, true, TB, BLK>(ex, regs_of, job, lds, ms);
         else
@@ -181,6 +216,8 @@ static void launch_variant(hipStream_t s, bool forward, bool first, const NttLau
         else hipLaunchKernelGGL((k_ntt<true, false, TB, BLK>), g, t, 0, s, l);
     } else {
         if (first && src == kSrcMac) hipLaunchKernelGGL((k_ntt<false, true, TB, BLK, kSrcMac>), g, t, 0, s, l);
+        else if (first && src == kSrcMacGalois)
+            hipLaunchKernelGGL((k_ntt<false, true, TB, BLK, kSrcMacGalois>), g, t, 0, s, l);
         else if (first && src == kSrcGalois)
             hipLaunchKernelGGL((k_ntt<false, true, TB, BLK, kSrcGalois>), g, t, 0, s, l);
         else if (first) hipLaunchKernelGGL((k_ntt<false, true, TB, BLK>), g, t, 0, s, l);
@@ -246,13 +283,14 @@ int launch_ntt(ckks_context *ctx, const NttCall *calls, size_t count, bool forwa
         if (c.n_polys > 1 && (c.in_stride < c.n_limbs * c.plan->degree ||
                               c.out_stride < c.n_limbs * c.plan->degree))
             return fail(CKKS_ERR_INVALID, "polynomial stride smaller than its limbs");
-        if (c.mac && (forward || count != 1 || !ntt_two_pass(c.plan->logn) || c.n_polys % 2 ||
+        if (c.mac && (forward || (count != 1 && !c.galois_k) || !ntt_two_pass(c.plan->logn) || c.n_polys % 2 ||
                       c.mac->dnum == 0 || c.mac->dnum > 8))
-            return fail(CKKS_ERR_INVALID, "fused key inner product: one inverse two-pass call, 1..8 digits");
-        if ((c.galois_k != 0) != (calls[0].galois_k != 0))
-            return fail(CKKS_ERR_INVALID, "Galois-permuted loads: every entry of a launch or none");
-        if (c.galois_k && (forward || c.mac || (c.galois_k & 1) == 0 || c.galois_k >= 2 * (u64)c.plan->degree ||
-                           c.in == c.out))
+            return fail(CKKS_ERR_INVALID,
+                        "fused key inner product: inverse two-pass calls (several only Galois-read), 1..8 digits");
+        if ((c.galois_k != 0) != (calls[0].galois_k != 0) || (c.mac != nullptr) != (calls[0].mac != nullptr))
+            return fail(CKKS_ERR_INVALID, "Galois-permuted loads and key products: every entry of a launch or none");
+        if (c.galois_k && (forward || (c.galois_k & 1) == 0 || c.galois_k >= 2 * (u64)c.plan->degree ||
+                           (!c.mac && c.in == c.out)))
             return fail(CKKS_ERR_INVALID, "Galois-permuted load: inverse only, odd exponent below 2N, in != out");
         if (ntt_two_pass(c.plan->logn)) {
             total_two += c.n_polys * c.n_limbs * c.plan->degree;
@@ -288,7 +326,8 @@ int launch_ntt(ckks_context *ctx, const NttCall *calls, size_t count, bool forwa
         }
         return d;
     };
-    const int src = calls[0].mac ? kSrcMac : calls[0].galois_k ? kSrcGalois : kSrcPlain;
+    const int src = calls[0].mac ? (calls[0].galois_k ? kSrcMacGalois : kSrcMac)
+                                 : calls[0].galois_k ? kSrcGalois : kSrcPlain;
     auto two_pass = [&](size_t i) { return calls[i].n_polys != 0 && ntt_two_pass(calls[i].plan->logn); };
     // single-pass entries (N <= 4096): one launch
     {
@@ -362,6 +401,7 @@ int launch_ntt(ckks_context *ctx, const NttCall *calls, size_t count, bool forwa
                 DevNttDesc d = base_desc(c);
                 d.block_start = blocks;
                 d.lt_begin = (u32)lt;
+                d.lt_count = (u32)take;
                 d.tmp = tmp + used;
                 l.d[l.n++] = d;
                 blocks += (u32)(take * ntt_tiles(d.logn, tb));

@@ -89,7 +89,10 @@ int ckks_get_ntt_variant(const ckks_context *ctx, uint32_t logn, int *variant);
  * fused with the MAC (the NTT-domain digits never reach HBM), with the
  * blocks sharing a key tile placed on one XCD or in plain order.
  * CKKS_KS_FUSION_INVERSE: the MAC is computed inside the first inverse NTT
- * pass (the NTT-domain accumulators never reach HBM; dnum <= 8).
+ * pass (the NTT-domain accumulators never reach HBM; dnum <= 8); set on the
+ * context, it also fuses the MAC of hoisted rotations of degrees above
+ * 2^12 into their Galois-read first inverse pass (off otherwise: slower
+ * than the separate MAC, which reads each key entry once per batch).
  * CKKS_KS_FUSION_AUTO (default): the choice ckks_autotune_key_switch made
  * for the key, OFF for an untuned key.  Identical residues in every mode.
  * No reference counterpart. */

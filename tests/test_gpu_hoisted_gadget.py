@@ -176,6 +176,37 @@ def test_hoisted_rotation_batch_of_uneven_degrees(pkg, ctx):
             h.close()
 
 
+def test_hoisted_mac_inside_the_inverse_pass(pkg, ctx):
+    """With KS_FUSION_INVERSE, hoisted rotations of two-pass degrees compute
+    the key MAC (and the canonical-lift correction) inside the Galois-read
+    first inverse pass: single calls and a batch of the benchmark's two
+    shapes give the residues of apply_galois."""
+    rng = np.random.default_rng(99)
+    entries, hoisted = [], []
+    for logn, L, K, blocks, n_cts in ((16, 25, 7, [9, 8, 8], 2), (15, 13, 4, [5, 4, 4], 3), (13, 9, 4, [5, 4], 2)):
+        n = 1 << logn
+        q = [Q0[0]] + Q40[:L - 1]
+        basis = q + P60[:K]
+        plan = pkg.PreparedRnsNttPlan(ctx, basis, n)
+        key = pkg.HybridKeySwitchKey(plan, L, blocks, rand_residues(rng, basis, n, len(blocks)),
+                                     rand_residues(rng, basis, n, len(blocks)))
+        ct = pkg.to_device(rand_residues(rng, q, n, polys=2 * n_cts).reshape(n_cts, 2, L, n))
+        hoisted.append(key.hoist(ct))
+        entries.append((hoisted[-1], key, O.rotation_exponent_left(n, 1 << (logn - 4)), ct, torch.empty_like(ct)))
+    try:
+        want = [key.apply_galois(ct, k) for h, key, k, ct, out in entries]
+        ctx.set_key_switch_fusion(ctx.KS_FUSION_INVERSE)
+        for (h, key, k, ct, out), w in zip(entries, want):
+            assert torch.equal(key.rotate_hoisted(h, ct, k), w), k
+        pkg.rotate_hoisted_batch(ctx, entries[:2])
+        for (h, key, k, ct, out), w in zip(entries[:2], want):
+            assert torch.equal(out, w), k
+    finally:
+        ctx.set_key_switch_fusion(ctx.KS_FUSION_AUTO)
+        for h in hoisted:
+            h.close()
+
+
 def test_rehoist_replaces_the_digits_in_place(pkg, ctx):
     """ckks_rehoist: a second batch hoisted into the first one's buffers
     rotates to apply_galois of the second batch; a batch with a zero block
