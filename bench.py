@@ -409,6 +409,9 @@ def main():
                     help="timed step: each batch's digits hoisted once per step and rotated 15 times "
                          "(ckks_hoist + ckks_rotate_hoisted_batch, the canonical residues), or every "
                          "rotation decomposing its own digits (ckks_apply_galois_batch)")
+    ap.add_argument("--hoist-b", action=argparse.BooleanOptionalAction, default=True,
+                    help="hoist with b (ckks_hoist_with_b: P NTT(b) added inside the key MAC, no sigma_k "
+                         "gather of b in ModDown; same residues)")
     ap.add_argument("--plain-sample", type=int, default=24, help="plain-path rotations timed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -548,7 +551,7 @@ def main():
     def hoist(name, key, batch):
         if name in held:
             return held[name].rehoist(key, batch)
-        held[name] = key.hoist(batch)
+        held[name] = key.hoist(batch, with_b=args.hoist_b)
         return held[name]
 
     host_call_ms = []  # host time of each ckks_rotate_hoisted_batch call
@@ -721,10 +724,12 @@ def main():
             "hoist_ms_p50": round(float(np.percentile(hoi_hoists, 50)), 3) if hoi_hoists else None,
             "p50_ms": round(float(np.percentile(hoi_lats, 50)), 3),
             "p99_ms": round(float(np.percentile(hoi_lats, 99)), 3),
+            "with_b": args.hoist_b,
             "note": "per step: each batch's digits re-hoisted (ckks_rehoist into the buffers of the "
-                    "warmup step's ckks_hoist; the zero-block check and its host synchronisation "
-                    "included), then one ckks_rotate_hoisted_batch per exponent; residues equal the "
-                    "canonical rotation's (tests compare them)"},
+                    "warmup step's ckks_hoist" + ("_with_b, which also transforms P b" if args.hoist_b else "") +
+                    "; the zero-block check and its host synchronisation included), then one "
+                    "ckks_rotate_hoisted_batch per exponent; residues equal the canonical rotation's "
+                    "(tests compare them)"},
         "canonical": {
             "rotations_per_s": round(can_value, 1), "n65536_rotations_per_s": round(can_n16, 1),
             "speedup_vs_plain": round(can_n16 / plain_value, 2),

@@ -131,6 +131,7 @@ _SIGNATURES = {
     "ckks_bounded_rotate_hoisted": (C.c_int, [_p, _p, _u64, _p, _p, _sz, _sz]),
     "ckks_bounded_hoist_destroy": (C.c_int, [_p]),
     "ckks_hoist": (C.c_int, [_p, _p, _sz, _sz, C.POINTER(_p)]),
+    "ckks_hoist_with_b": (C.c_int, [_p, _p, _sz, _sz, C.POINTER(_p)]),
     "ckks_rotate_hoisted": (C.c_int, [_p, _p, _u64, _p, _p, _sz, _sz]),
     "ckks_ksk_destroy": (C.c_int, [_p]),
     "ckks_ksk_bytes": (C.c_int, [_p, C.POINTER(_sz)]),
@@ -738,10 +739,13 @@ class _KeySwitchKey:
                                                        0, 0))
         return out
 
-    def hoist(self, ct) -> "HoistedDigits":
+    def hoist(self, ct, with_b: bool = False) -> "HoistedDigits":
         """ckks_hoist: digits of the a components of ct cut and transformed
-        once, for rotate_hoisted, whose residues are those of apply_galois."""
-        return HoistedDigits(self, ct)
+        once, for rotate_hoisted, whose residues are those of apply_galois.
+        with_b (ckks_hoist_with_b): P NTT(b) is kept too and rotations add
+        it inside the key MAC instead of reading sigma_k(b) from their ct
+        argument, which must then still hold these ciphertexts."""
+        return HoistedDigits(self, ct, with_b)
 
     def rotate_hoisted(self, hoisted: "HoistedDigits", ct, exponent: int, out=None):
         """ckks_rotate_hoisted: apply_galois(ct, exponent) from the hoisted
@@ -966,12 +970,13 @@ class HoistedDigits:
     """ckks_bounded_hoist: the NTT-domain digits of a ciphertext batch's a
     components, decomposed once for many rotations (rotate_hoisted)."""
 
-    def __init__(self, key: "BoundedKeySwitchKey", ct):
+    def __init__(self, key: "BoundedKeySwitchKey", ct, with_b: bool = False):
         self.ctx = key.ctx
         self.n_cts = key._n_cts(ct, 2)
         h = _p()
         p_ct = _ptr(ct, self.n_cts * 2 * key.q_limbs * key.plan.degree, name="ct")
-        _check(lib().ckks_hoist(key.handle, p_ct, self.n_cts, 0, C.byref(h)))
+        fn = lib().ckks_hoist_with_b if with_b else lib().ckks_hoist
+        _check(fn(key.handle, p_ct, self.n_cts, 0, C.byref(h)))
         self._h = h
 
     def rehoist(self, key, ct):

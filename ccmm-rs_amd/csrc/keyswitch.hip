@@ -124,8 +124,12 @@ struct ckks_hoisted {
     // zero, so rotations take the canonical path
     bool zero_block = false;
     int *d_flag = nullptr, *h_flag = nullptr;  // the zero-block check's result (device, pinned host)
+    // ckks_hoist_with_b: P NTT(b) of the b components, [ct][q_limbs][N]
+    // (P the product of the key's special primes, 1 without them)
+    ck::u64 *d_bhat = nullptr;
     ~ckks_hoisted() {
         if (d_digits) hipFree(d_digits);
+        if (d_bhat) hipFree(d_bhat);
         if (d_flag) hipFree(d_flag);
         if (h_flag) hipHostFree(h_flag);
     }
@@ -478,6 +482,10 @@ struct MacArgs {
     // hoisted canonical rotations: the key-side correction [B][N] of each
     // component, added to every ciphertext's sums (null otherwise)
     const u64 *tb, *ta;
+    // hoisted with b: P NTT(b) [ct][q_limbs][N], added to component 0's sums
+    // on the Q limbs (null otherwise; fixed-D kernels only)
+    const u64 *bhat;
+    u32 q_limbs;
 };
 
 // NTT-domain Galois permutation (ring/ntt_galois_permutation.rs:5-19) over
@@ -528,9 +536,10 @@ __device__ __forceinline__ void ks_mac_fp(const MacArgs &a, const DevPrime &P, u
     // the correction enters as the sums' starting value (canonical, < 2^41)
     const double t0b = a.tb ? small_to_f64(a.tb[(u64)i * N + n]) : 0.0;
     const double t0a = a.ta ? small_to_f64(a.ta[(u64)i * N + n]) : 0.0;
+    const u64 *bh = a.bhat && i < a.q_limbs ? a.bhat + (u64)i * N + n : nullptr;
     for (u32 ct = 0; ct < a.n_cts; ++ct) {
         const u64 *d = a.digits + (((u64)ct * D) * a.limbs + i) * N + n;
-        double sb = t0b, sa = t0a;
+        double sb = bh ? t0b + small_to_f64(ld_stream(bh + (u64)ct * a.q_limbs * N)) : t0b, sa = t0a;
 #pragma unroll
         for (int j = 0; j < D; ++j) {
             const double dv = small_to_f64(ld_stream(d + (u64)j * a.limbs * N));
@@ -562,11 +571,13 @@ __global__ __launch_bounds__(kKsThreads) void k_ks_mac(MacArgs a) {
         ka[j] = a.ka[((u64)j * a.limbs + i) * N + n];
     }
     const u64 tb = a.tb ? a.tb[(u64)i * N + n] : 0, ta = a.ta ? a.ta[(u64)i * N + n] : 0;
+    const u64 *bh = a.bhat && i < a.q_limbs ? a.bhat + (u64)i * N + n : nullptr;
     for (u32 ct = 0; ct < a.n_cts; ++ct) {
         const u64 *d = a.digits + (((u64)ct * D) * a.limbs + i) * N + n;
         Acc128 sb, sa;
         sb.zero();
         sa.zero();
+        if (bh) sb.mac(ld_stream(bh + (u64)ct * a.q_limbs * N), 1);
 #pragma unroll
         for (int j = 0; j < D; ++j) {
             u64 dv = ld_stream(d + (u64)j * a.limbs * N);
@@ -1530,6 +1541,9 @@ struct HoistedMac {
     const u64 *kb, *ka;  // [D][B][N]
     u64 k;
     const u64 *tb, *ta;  // canonical-lift correction [B][N] (null for bounded digits)
+    // P NTT(b) [ct][q_limbs][N] of a handle hoisted with b (null otherwise):
+    // the MAC adds it and ModDown reads no pass-through for component 0
+    const u64 *bhat = nullptr;
 };
 
 static int key_switch_lane(const ckks_ksk *k, const u64 *x, u64 x_stride, const u64 *add0,
@@ -1576,6 +1590,7 @@ static int key_switch_core(const ckks_ksk *k, const u64 *x, u64 x_stride, const 
     if (hm) {
         hm2 = *hm;
         hm2.digits += n1 * k->dnum * k->limbs * k->degree;
+        if (hm2.bhat) hm2.bhat += n1 * k->q_limbs * k->degree;
     }
     int rc = key_switch_lane(k, x + n1 * x_stride, x_stride, add0 ? add0 + n1 * add_stride : nullptr,
                              add1 ? add1 + n1 * add_stride : nullptr, add_stride, out + n1 * out_stride,
@@ -1677,6 +1692,10 @@ static int ks_decompose(const KsJob &j, hipStream_t s) {
 }
 
 // 3. key MAC of the NTT-domain digits into mac_out
+// The MAC adds the hoisted P NTT(b) (the fixed-D kernels, dnum <= 8); the
+// ModDown of such a job reads no pass-through for component 0.
+static bool ks_mac_takes_b(const KsJob &j) { return j.hm && j.hm->bhat && j.k->dnum <= 8; }
+
 static int ks_mac(const KsJob &j, hipStream_t s) {
     const ckks_ksk *k = j.k;
     const u64 N = k->degree, B = k->limbs, D = k->dnum;
@@ -1687,6 +1706,8 @@ static int ks_mac(const KsJob &j, hipStream_t s) {
     ma.ka = j.hm ? j.hm->ka : k->d_ka;
     ma.tb = j.hm ? j.hm->tb : nullptr;
     ma.ta = j.hm ? j.hm->ta : nullptr;
+    ma.bhat = ks_mac_takes_b(j) ? j.hm->bhat : nullptr;
+    ma.q_limbs = (u32)k->q_limbs;
     ma.acc = j.mac_out;
     ma.primes = k->plan->d_primes;
     ma.n_cts = (u32)j.n_cts;
@@ -1720,7 +1741,7 @@ static int ks_moddown(const KsJob &j, hipStream_t s) {
     const u32 chunks = (u32)(N / kKsThreads);
     DownArgs da{};
     da.acc = j.acc;
-    da.add0 = j.add0;
+    da.add0 = ks_mac_takes_b(j) ? nullptr : j.add0;
     da.add1 = j.add1;
     da.add_stride = j.add_stride;
     da.kinv = j.galois_kinv;
@@ -1783,7 +1804,7 @@ static int key_switch_lane(const ckks_ksk *k, const u64 *x, u64 x_stride, const 
     // hoisted rotations of two-pass degrees with dnum <= 8: the MAC runs in
     // the first inverse NTT pass, which reads it through the permutation
     const bool two_pass = logn > 12 && logn <= 17;
-    const bool hoist_fused = hm && hoisted_mac_fused(ctx, logn, D);
+    const bool hoist_fused = hm && !hm->bhat && hoisted_mac_fused(ctx, logn, D);
     // hoisted: slot 4 holds the unpermuted MAC output instead of the digits
     const size_t slot4 = hm ? n_cts * 2 * B * N : n_cts * D * B * N;
     if (!hoist_fused && (rc = ctx->workspace(4, slot4 * sizeof(u64), &wd))) return rc;
@@ -2514,7 +2535,7 @@ static int rotate_from_hoisted(const ckks_hoisted *h, const ckks_ksk *key, u64 k
                                u64 *out, size_t in_stride, size_t out_stride) {
     const size_t DBN = key->dnum * key->limbs * key->degree, BN = key->limbs * key->degree;
     const u64 *tb = key->bounded ? nullptr : pk + 2 * DBN;
-    const HoistedMac hm{h->d_digits, pk, pk + DBN, k, tb, tb ? tb + BN : nullptr};
+    const HoistedMac hm{h->d_digits, pk, pk + DBN, k, tb, tb ? tb + BN : nullptr, h->d_bhat};
     const size_t poly = key->q_limbs * key->degree;
     return key_switch_core(key, in + poly, in_stride, in, nullptr, in_stride, out, out_stride, h->n_cts,
                            galois_inverse(k, 2 * (u64)key->degree), nullptr, &hm);
@@ -2568,6 +2589,23 @@ static int hoist_into(ckks_hoisted *h, const ckks_ksk *key, const uint64_t *in, 
     if (rc) return rc;
     NttCall f{key->plan, h->d_digits, h->d_digits, n_cts * D, B, B * N, B * N};
     if ((rc = launch_ntt(ctx, &f, 1, true))) return rc;
+    if (h->d_bhat) {
+        // P NTT(b) on the Q limbs: the reference adds P b before ModDown,
+        // which the MAC does here in the NTT domain
+        const size_t L = key->q_limbs;
+        NttCall fb{key->plan, in, h->d_bhat, n_cts, L, in_stride, L * N};
+        if ((rc = launch_ntt(ctx, &fb, 1, true))) return rc;
+        if (key->p_limbs) {
+            const std::vector<uint64_t> &mods = key->plan->moduli;
+            std::vector<uint64_t> pm(L);
+            for (size_t i = 0; i < L; ++i) {
+                unsigned __int128 r = 1;
+                for (size_t m = L; m < key->limbs; ++m) r = r * (mods[m] % mods[i]) % mods[i];
+                pm[i] = (uint64_t)r;
+            }
+            if ((rc = ckks_poly_scalar_mul(key->plan, h->d_bhat, pm.data(), h->d_bhat, n_cts, L, L * N))) return rc;
+        }
+    }
     // exactness of the hoisted rotations (see hoist_correction)
     ZeroArgs za{in + poly, in_stride, key->d_modup, h->d_flag, (u32)D, key->plan->logn, (u32)(N / kKsThreads)};
     hipError_t e = hipMemsetAsync(h->d_flag, 0, sizeof(int), ctx->stream);
@@ -2582,9 +2620,13 @@ static int hoist_into(ckks_hoisted *h, const ckks_ksk *key, const uint64_t *in, 
     return CKKS_OK;
 }
 
-int ckks_hoist(const ckks_ksk *key, const uint64_t *in, size_t n_cts, size_t in_stride, ckks_hoisted **out) {
+static int hoist_new(const ckks_ksk *key, const uint64_t *in, size_t n_cts, size_t in_stride, bool with_b,
+                     ckks_hoisted **out) {
     if (!key || !out || (n_cts && !in)) return fail(CKKS_ERR_INVALID, "null argument");
-    if (key->bounded) return ckks_bounded_hoist(key, in, n_cts, in_stride, out);
+    if (key->bounded) {
+        if (with_b) return fail(CKKS_ERR_UNSUPPORTED, "hoisting with b serves RNS-gadget and dnum-hybrid keys");
+        return ckks_bounded_hoist(key, in, n_cts, in_stride, out);
+    }
     ckks_context *ctx = key->ctx;
     if (ctx->capturing) return fail(CKKS_ERR_INVALID, "hoisting allocates: not capturable");
     const size_t N = key->degree, B = key->limbs, D = key->dnum, poly = key->q_limbs * N;
@@ -2600,6 +2642,7 @@ int ckks_hoist(const ckks_ksk *key, const uint64_t *in, size_t n_cts, size_t in_
     if (n_cts) {
         CK_HIP(hipSetDevice(ctx->device));
         CK_HIP(hipMalloc(&h->d_digits, n_cts * D * B * N * sizeof(u64)));
+        if (with_b) CK_HIP(hipMalloc(&h->d_bhat, n_cts * key->q_limbs * N * sizeof(u64)));
         CK_HIP(hipMalloc(&h->d_flag, sizeof(int)));
         CK_HIP(hipHostMalloc(&h->h_flag, sizeof(int)));
         int rc = hoist_into(h.get(), key, in, n_cts, in_stride);
@@ -2607,6 +2650,15 @@ int ckks_hoist(const ckks_ksk *key, const uint64_t *in, size_t n_cts, size_t in_
     }
     *out = h.release();
     return CKKS_OK;
+}
+
+int ckks_hoist(const ckks_ksk *key, const uint64_t *in, size_t n_cts, size_t in_stride, ckks_hoisted **out) {
+    return hoist_new(key, in, n_cts, in_stride, false, out);
+}
+
+int ckks_hoist_with_b(const ckks_ksk *key, const uint64_t *in, size_t n_cts, size_t in_stride,
+                      ckks_hoisted **out) {
+    return hoist_new(key, in, n_cts, in_stride, true, out);
 }
 
 int ckks_rehoist(ckks_hoisted *h, const ckks_ksk *key, const uint64_t *in, size_t in_stride) {
@@ -2698,7 +2750,8 @@ int ckks_rotate_hoisted_batch(ckks_context *ctx, const ckks_hoisted_rotation *e,
     if (fast.empty()) return CKKS_OK;
     // every entry's MAC in the first inverse pass, or none
     bool fused = true;
-    for (const Fast &f : fast) fused = fused && hoisted_mac_fused(ctx, f.key->plan->logn, f.key->dnum);
+    for (const Fast &f : fast)
+        fused = fused && !f.h->d_bhat && hoisted_mac_fused(ctx, f.key->plan->logn, f.key->dnum);
     // the MAC outputs (slot 4, unfused only) and the accumulators (slot 5)
     size_t total = 0;
     for (const Fast &f : fast) total += f.h->n_cts * 2 * f.key->limbs * f.key->degree;
@@ -2715,7 +2768,8 @@ int ckks_rotate_hoisted_batch(ckks_context *ctx, const ckks_hoisted_rotation *e,
         const Fast &f = fast[i];
         const ckks_ksk *key = f.key;
         const size_t B = key->limbs, N = key->degree, DBN = key->dnum * B * N, poly = key->q_limbs * N;
-        hms[i] = HoistedMac{f.h->d_digits, f.pk, f.pk + DBN, f.k, f.pk + 2 * DBN, f.pk + 2 * DBN + B * N};
+        hms[i] = HoistedMac{f.h->d_digits, f.pk, f.pk + DBN, f.k, f.pk + 2 * DBN, f.pk + 2 * DBN + B * N,
+                            f.h->d_bhat};
         const u64 kinv = galois_inverse(f.k, 2 * (u64)N);
         KsJob &j = jobs[i];
         j = KsJob{key, f.in + poly, f.in_stride, f.in, nullptr, f.in_stride, f.out, f.out_stride, f.h->n_cts, kinv,

@@ -481,6 +481,18 @@ int ckks_bounded_hoist_destroy(ckks_hoisted *hoisted);
  * corrections are cached per exponent as for bounded keys.  Destroy the
  * handle with ckks_bounded_hoist_destroy, before the context. */
 int ckks_hoist(const ckks_ksk *key, const uint64_t *in, size_t n_cts, size_t in_stride, ckks_hoisted **out);
+/* No reference counterpart: ckks_hoist that also keeps P NTT(b) of the b
+ * components (Q limbs; P the product of the key's special primes, 1 for an
+ * RNS-gadget key).  The reference adds P b to the key-switch sums before
+ * ModDown; rotations from this handle add P NTT(b) inside the key MAC,
+ * where the Galois-read inverse NTT turns it into P sigma_k(b), so ModDown
+ * reads no sigma_k gather of b.  Same residues as ckks_hoist, provided the
+ * in argument of each rotation still holds the hoisted ciphertexts (b is
+ * taken as it was at the hoist; a batch that takes the canonical path reads
+ * it from in).  The MAC adds it for dnum <= 8 (larger dnum: b read from in
+ * as for ckks_hoist).  Not for bounded keys; ckks_rehoist refreshes it. */
+int ckks_hoist_with_b(const ckks_ksk *key, const uint64_t *in, size_t n_cts, size_t in_stride,
+                      ckks_hoisted **out);
 /* No reference counterpart: ckks_hoist of a new batch of the same count and
  * shape into an existing handle (RNS-gadget / dnum keys), reusing its
  * device buffers, for loops that hoist batch after batch without

@@ -1,5 +1,6 @@
 """Hoisted rotations with dnum-hybrid and RNS-gadget keys (ckks_hoist /
-ckks_rotate_hoisted): the canonical digits of the unrotated a are cut and
+ckks_hoist_with_b / ckks_rotate_hoisted): the canonical digits of the
+unrotated a are cut and
 transformed once; each rotation is a key MAC over the key permuted by k^-1
 plus the per-exponent correction of the canonical lift of negated
 coefficients, so every residue equals the canonical rotation
@@ -30,9 +31,10 @@ def canonical_rotation_oracle(qp_moduli, q_limbs, blocks, ct, k, kb, ka):
     return O.rns_key_switch(O.RnsPlan(qp_moduli, n), blocks, rb, ra, kb, ka)
 
 
+@pytest.mark.parametrize("with_b", [False, True])
 @pytest.mark.parametrize("kind,logn,L,K,blocks", [("hybrid", 12, 4, 2, [2, 2]), ("hybrid", 13, 9, 4, [5, 4]),
                                                   ("rns", 12, 4, 0, [1, 1, 1, 1]), ("rns", 12, 4, 0, [2, 2])])
-def test_hoisted_rotation_equals_canonical(pkg, ctx, kind, logn, L, K, blocks):
+def test_hoisted_rotation_equals_canonical(pkg, ctx, kind, logn, L, K, blocks, with_b):
     n = 1 << logn
     q = [Q0[0]] + Q40[:L - 1]
     basis = q + P60[:K]
@@ -43,8 +45,9 @@ def test_hoisted_rotation_equals_canonical(pkg, ctx, kind, logn, L, K, blocks):
            else pkg.RnsKeySwitchKey(plan, blocks, kb, ka))
     ct = rand_residues(rng, q, n, polys=6).reshape(3, 2, L, n)
     ct[0, 1, :, :8] = np.array(q, dtype=np.uint64)[:, None] - 1
+    ct[1, 0, :, :8] = np.array(q, dtype=np.uint64)[:, None] - 1
     dct = pkg.to_device(ct)
-    h = key.hoist(dct)
+    h = key.hoist(dct, with_b=with_b)
     try:
         for k in (O.rotation_exponent_left(n, 1), O.rotation_exponent_left(n, 5), 2 * n - 1,
                   O.rotation_exponent_left(n, n // 4 - 1)):
@@ -92,7 +95,8 @@ def test_zero_block_values_take_the_canonical_path(pkg, ctx, kind):
         h.close()
 
 
-def test_hoisted_rotation_at_the_benchmark_shape(pkg, ctx):
+@pytest.mark.parametrize("with_b", [False, True])
+def test_hoisted_rotation_at_the_benchmark_shape(pkg, ctx, with_b):
     n, L, K, blocks = 1 << 16, 25, 7, [9, 8, 8]
     q = [Q0[0]] + Q40[:L - 1]
     basis = q + P60[:K]
@@ -102,7 +106,7 @@ def test_hoisted_rotation_at_the_benchmark_shape(pkg, ctx):
     key = pkg.HybridKeySwitchKey(plan, L, blocks, kb, ka)
     ct = rand_residues(rng, q, n, polys=4).reshape(2, 2, L, n)
     dct = pkg.to_device(ct)
-    h = key.hoist(dct)
+    h = key.hoist(dct, with_b=with_b)
     try:
         for i in (0, 7, 14):
             k = O.rotation_exponent_left(n, 1 << i)
@@ -136,7 +140,8 @@ def test_hoisted_rotation_decrypts_to_the_rotated_slots(pkg, ctx):
         assert err < 1e-6, (steps, err)
 
 
-def test_hoisted_rotation_batch_of_uneven_degrees(pkg, ctx):
+@pytest.mark.parametrize("with_b", [False, True])
+def test_hoisted_rotation_batch_of_uneven_degrees(pkg, ctx, with_b):
     """ckks_rotate_hoisted_batch: the benchmark's two shapes (N = 2^16, 25
     limbs, [9, 8, 8], K = 7 and N = 2^15, 13 limbs, [5, 4, 4], K = 4), an
     RNS-gadget key and a bounded key in one call; every entry equals
@@ -161,7 +166,7 @@ def test_hoisted_rotation_batch_of_uneven_degrees(pkg, ctx):
             key = pkg.BoundedKeySwitchKey(plan, L, blocks, rand_residues(rng, basis, n, cnt),
                                           rand_residues(rng, basis, n, cnt))
         ct = pkg.to_device(rand_residues(rng, q, n, polys=2 * n_cts).reshape(n_cts, 2, L, n))
-        h = key.hoist(ct)
+        h = key.hoist(ct, with_b=True) if with_b and kind != "bounded" else key.hoist(ct)
         hoisted.append(h)
         entries.append((h, key, O.rotation_exponent_left(n, 3 + idx), ct, torch.empty_like(ct)))
     try:
@@ -207,7 +212,8 @@ def test_hoisted_mac_inside_the_inverse_pass(pkg, ctx):
             h.close()
 
 
-def test_rehoist_replaces_the_digits_in_place(pkg, ctx):
+@pytest.mark.parametrize("with_b", [False, True])
+def test_rehoist_replaces_the_digits_in_place(pkg, ctx, with_b):
     """ckks_rehoist: a second batch hoisted into the first one's buffers
     rotates to apply_galois of the second batch; a batch with a zero block
     value re-hoisted after a clean one takes the canonical path, and a clean
@@ -222,7 +228,7 @@ def test_rehoist_replaces_the_digits_in_place(pkg, ctx):
     batches[1][2, 1, 5:9, 100] = 0  # block 1 of a zero at coefficient 100
     k = O.rotation_exponent_left(n, 9)
     d0 = pkg.to_device(batches[0])
-    h = key.hoist(d0)
+    h = key.hoist(d0, with_b=with_b)
     try:
         for i, b in enumerate(batches):
             d = pkg.to_device(b)
