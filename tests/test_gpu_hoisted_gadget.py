@@ -1,7 +1,7 @@
 """Hoisted rotations with dnum-hybrid and RNS-gadget keys (ckks_hoist /
 ckks_hoist_with_b / ckks_rotate_hoisted): the canonical digits of the
-unrotated a are cut and
-transformed once; each rotation is a key MAC over the key permuted by k^-1
+unrotated a are cut and transformed once; each rotation is a key MAC over
+the key permuted by k^-1
 plus the per-exponent correction of the canonical lift of negated
 coefficients, so every residue equals the canonical rotation
 (ckks_apply_galois, itself checked against the oracle here): at toy sizes,
