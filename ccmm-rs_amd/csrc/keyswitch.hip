@@ -592,6 +592,105 @@ __global__ __launch_bounds__(kKsThreads) void k_ks_mac(MacArgs a) {
     }
 }
 
+// Hoisted rotations of one batch by up to kMacMaxExp exponents in one MAC:
+// every exponent's key entries (permuted by its k^-1) and correction stay
+// in registers while the thread walks the batch, and each digit (and the
+// hoisted P NTT(b)) is read once for all of them instead of once per
+// exponent.  Same sums, products and reductions as k_ks_mac per exponent.
+constexpr int kMacMaxExp = 4;
+
+struct MacMultiArgs {
+    const u64 *digits;  // [ct][D][B][N] NTT domain
+    const u64 *kb[kMacMaxExp], *ka[kMacMaxExp];  // per exponent [D][B][N]
+    const u64 *tb[kMacMaxExp], *ta[kMacMaxExp];  // per exponent [B][N], or null
+    u64 *acc[kMacMaxExp];                        // per exponent [ct][2][B][N]
+    const u64 *bhat;  // P NTT(b) [ct][q_limbs][N], or null
+    const DevPrime *primes;
+    u32 n_cts, limbs, q_limbs, logn, chunks;
+};
+
+template <int D, int E>
+__global__ __launch_bounds__(kKsThreads) void k_ks_mac_multi(MacMultiArgs a) {
+    const u32 b = blockIdx.x;
+    const u32 chunk = b % a.chunks;
+    const u32 i = b / a.chunks;
+    const u64 N = (u64)1 << a.logn;
+    const u64 n = (u64)chunk * kKsThreads + threadIdx.x;
+    const DevPrime P = a.primes[i];
+    const u64 *bh = a.bhat && i < a.q_limbs ? a.bhat + (u64)i * N + n : nullptr;
+    const u64 kstride = (u64)a.limbs * N;
+    if (P.fp) {
+        double kb[E][D], kbp[E][D], ka[E][D], kap[E][D], t0b[E], t0a[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+#pragma unroll
+            for (int j = 0; j < D; ++j) {
+                kb[e][j] = small_to_f64(a.kb[e][(u64)j * kstride + (u64)i * N + n]);
+                ka[e][j] = small_to_f64(a.ka[e][(u64)j * kstride + (u64)i * N + n]);
+                kbp[e][j] = kb[e][j] / P.qd;
+                kap[e][j] = ka[e][j] / P.qd;
+            }
+            t0b[e] = a.tb[e] ? small_to_f64(a.tb[e][(u64)i * N + n]) : 0.0;
+            t0a[e] = a.ta[e] ? small_to_f64(a.ta[e][(u64)i * N + n]) : 0.0;
+        }
+        for (u32 ct = 0; ct < a.n_cts; ++ct) {
+            const u64 *d = a.digits + (((u64)ct * D) * a.limbs + i) * N + n;
+            double dv[D];
+#pragma unroll
+            for (int j = 0; j < D; ++j) dv[j] = small_to_f64(ld_stream(d + (u64)j * kstride));
+            const double bv = bh ? small_to_f64(ld_stream(bh + (u64)ct * a.q_limbs * N)) : 0.0;
+            const u64 o = ((u64)ct * 2 * a.limbs + i) * N + n;
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                double sb = t0b[e] + bv, sa = t0a[e];
+#pragma unroll
+                for (int j = 0; j < D; ++j) {
+                    sb += fp_mulmod(dv[j], kb[e][j], kbp[e][j], P.qd);
+                    sa += fp_mulmod(dv[j], ka[e][j], kap[e][j], P.qd);
+                }
+                st_stream(a.acc[e] + o, fp_residue(sb, P.qd, P.qinv));
+                st_stream(a.acc[e] + o + kstride, fp_residue(sa, P.qd, P.qinv));
+            }
+        }
+        return;
+    }
+    u64 kb[E][D], ka[E][D], tb[E], ta[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            kb[e][j] = a.kb[e][(u64)j * kstride + (u64)i * N + n];
+            ka[e][j] = a.ka[e][(u64)j * kstride + (u64)i * N + n];
+        }
+        tb[e] = a.tb[e] ? a.tb[e][(u64)i * N + n] : 0;
+        ta[e] = a.ta[e] ? a.ta[e][(u64)i * N + n] : 0;
+    }
+    for (u32 ct = 0; ct < a.n_cts; ++ct) {
+        const u64 *d = a.digits + (((u64)ct * D) * a.limbs + i) * N + n;
+        u64 dv[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) dv[j] = ld_stream(d + (u64)j * kstride);
+        const u64 bv = bh ? ld_stream(bh + (u64)ct * a.q_limbs * N) : 0;
+        const u64 o = ((u64)ct * 2 * a.limbs + i) * N + n;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            Acc128 sb, sa;
+            sb.zero();
+            sa.zero();
+#pragma unroll
+            for (int j = 0; j < D; ++j) {
+                sb.mac(dv[j], kb[e][j]);
+                sa.mac(dv[j], ka[e][j]);
+            }
+            sb.mac(tb[e], 1);
+            sa.mac(ta[e], 1);
+            sb.mac(bv, 1);
+            st_stream(a.acc[e] + o, reduce128(sb, P));
+            st_stream(a.acc[e] + o + kstride, reduce128(sa, P));
+        }
+    }
+}
+
 // More than 8 digits over a batch (the bounded gadget: 23 digits at the
 // reference's N=2^16 shape): chunks of C digits, one launch each, with the
 // chunk's key entries in registers while the thread walks the batch, so a
@@ -1733,6 +1832,51 @@ static int ks_mac(const KsJob &j, hipStream_t s) {
     return CKKS_OK;
 }
 
+// The MACs of hoisted jobs jobs[g] (g in grp) that share their digits:
+// one k_ks_mac_multi when 2..kMacMaxExp of them have dnum <= 8, else one
+// ks_mac each.
+static int ks_mac_group(const std::vector<KsJob> &jobs, const std::vector<size_t> &grp, hipStream_t s) {
+    const KsJob &j0 = jobs[grp[0]];
+    const ckks_ksk *k = j0.k;
+    const u64 D = k->dnum;
+    if (grp.size() < 2 || D > 8 || !j0.hm) {
+        for (size_t g : grp)
+            if (int rc = ks_mac(jobs[g], s)) return rc;
+        return CKKS_OK;
+    }
+    const u64 N = k->degree, B = k->limbs;
+    MacMultiArgs ma{};
+    ma.digits = j0.digits;
+    for (size_t e = 0; e < grp.size(); ++e) {
+        const KsJob &j = jobs[grp[e]];
+        ma.kb[e] = j.hm->kb;
+        ma.ka[e] = j.hm->ka;
+        ma.tb[e] = j.hm->tb;
+        ma.ta[e] = j.hm->ta;
+        ma.acc[e] = j.mac_out;
+    }
+    ma.bhat = ks_mac_takes_b(j0) ? j0.hm->bhat : nullptr;
+    ma.primes = k->plan->d_primes;
+    ma.n_cts = (u32)j0.n_cts;
+    ma.limbs = (u32)B;
+    ma.q_limbs = (u32)k->q_limbs;
+    ma.logn = k->plan->logn;
+    ma.chunks = (u32)(N / kKsThreads);
+    const dim3 g((u32)(B * ma.chunks)), t(kKsThreads);
+    const int E = (int)grp.size();
+    switch (D * 8 + E) {
+#define CK_MM(DD, EE) \
+    case DD * 8 + EE: hipLaunchKernelGGL((k_ks_mac_multi<DD, EE>), g, t, 0, s, ma); break;
+#define CK_MM_D(DD) CK_MM(DD, 2) CK_MM(DD, 3) CK_MM(DD, 4)
+        CK_MM_D(1) CK_MM_D(2) CK_MM_D(3) CK_MM_D(4) CK_MM_D(5) CK_MM_D(6) CK_MM_D(7) CK_MM_D(8)
+#undef CK_MM_D
+#undef CK_MM
+        default: return fail(CKKS_ERR_INVALID, "hoisted MAC group: 2..4 exponents, dnum <= 8");
+    }
+    CK_HIP(hipGetLastError());
+    return CKKS_OK;
+}
+
 // 5. ModDown (or plain finalize) of the coefficient-domain accumulators
 // with the pass-through adds
 static int ks_moddown(const KsJob &j, hipStream_t s) {
@@ -2787,7 +2931,23 @@ int ckks_rotate_hoisted_batch(ckks_context *ctx, const ckks_hoisted_rotation *e,
         ckks_execution_profile &pc = ctx->counts();
         pc.key_switches += f.h->n_cts;
         (f.k == 2 * (u64)N - 1 ? pc.automorphisms : pc.rotations) += f.h->n_cts;
-        if (!fused && (rc = ks_mac(j, s))) return rc;
+    }
+    // the MACs: entries of one hoisted batch share their digit reads, up to
+    // kMacMaxExp exponents per launch
+    if (!fused) {
+        std::vector<bool> done(fast.size(), false);
+        for (size_t i = 0; i < fast.size(); ++i) {
+            if (done[i]) continue;
+            std::vector<size_t> grp{i};
+            done[i] = true;
+            for (size_t m = i + 1; m < fast.size() && grp.size() < (size_t)kMacMaxExp; ++m)
+                if (!done[m] && fast[m].h == fast[i].h && fast[m].key->dnum == fast[i].key->dnum &&
+                    fast[m].key->limbs == fast[i].key->limbs) {
+                    grp.push_back(m);
+                    done[m] = true;
+                }
+            if ((rc = ks_mac_group(jobs, grp, s))) return rc;
+        }
     }
     // one inverse NTT launch per pass for every entry, each reading its MAC
     // (computed in the first pass when fused) through its own Galois

@@ -502,11 +502,13 @@ int ckks_rotate_hoisted(const ckks_hoisted *hoisted, const ckks_ksk *key, uint64
                         uint64_t *out, size_t in_stride, size_t out_stride);
 /* Several hoisted rotations in one launch sequence (no reference
  * counterpart; the same residues as ckks_rotate_hoisted entry by entry):
- * the key MACs run per entry, the inverse NTTs of all entries are one
- * launch per pass (mixed degrees, each entry read through its own Galois
- * permutation), then ModDown per entry.  Entries of bounded keys, and
- * entries that take the canonical path, run on their own.  At most 16
- * entries; in and out follow ckks_rotate_hoisted (strides 0 = dense). */
+ * the key MACs of entries that share a hoisted handle (one batch rotated by
+ * several exponents) run up to 4 per launch, reading each digit once; the
+ * inverse NTTs of all entries are one launch per pass (mixed degrees, each
+ * entry read through its own Galois permutation), then ModDown per entry.
+ * Entries of bounded keys, and entries that take the canonical path, run on
+ * their own.  At most 16 entries, each with its own out; in and out follow
+ * ckks_rotate_hoisted (strides 0 = dense). */
 typedef struct ckks_hoisted_rotation {
     const ckks_hoisted *hoisted;
     const ckks_ksk *key;

@@ -181,6 +181,45 @@ def test_hoisted_rotation_batch_of_uneven_degrees(pkg, ctx, with_b):
             h.close()
 
 
+@pytest.mark.parametrize("with_b", [False, True])
+def test_hoisted_batch_of_exponents_shares_the_digits(pkg, ctx, with_b):
+    """Several exponents of one hoisted batch in one ckks_rotate_hoisted_batch
+    call (their MACs read each digit once, up to 4 exponents per launch):
+    5 exponents of an N = 2^15 batch, interleaved with 2 of an N = 2^13
+    batch and a lone RNS-gadget one, each equal to apply_galois."""
+    rng = np.random.default_rng(515)
+    shapes = [("hybrid", 15, 13, 4, [5, 4, 4], 2, 5), ("hybrid", 13, 9, 4, [5, 4], 3, 2), ("rns", 12, 4, 0, [2, 2], 2, 1)]
+    entries, hoisted = [], []
+    per = []
+    for kind, logn, L, K, blocks, n_cts, n_exp in shapes:
+        n = 1 << logn
+        q = [Q0[0]] + Q40[:L - 1]
+        basis = q + P60[:K]
+        plan = pkg.PreparedRnsNttPlan(ctx, basis, n)
+        make = pkg.HybridKeySwitchKey if kind == "hybrid" else (lambda p, L_, b, kb, ka: pkg.RnsKeySwitchKey(p, b, kb, ka))
+        ct = pkg.to_device(rand_residues(rng, q, n, polys=2 * n_cts).reshape(n_cts, 2, L, n))
+        keys = [make(plan, L, blocks, rand_residues(rng, basis, n, len(blocks)),
+                     rand_residues(rng, basis, n, len(blocks))) for _ in range(n_exp)]
+        h = keys[0].hoist(ct, with_b=with_b)
+        hoisted.append(h)
+        exps = [O.rotation_exponent_left(n, 1 << e) for e in range(n_exp - 1)] + [2 * n - 1]
+        per.append([(h, key, k, ct, torch.empty_like(ct)) for key, k in zip(keys, exps)])
+    # interleave the entries of the three batches
+    for i in range(max(len(p) for p in per)):
+        for p in per:
+            if i < len(p):
+                entries.append(p[i])
+    try:
+        ctx.reset_execution_profile()
+        pkg.rotate_hoisted_batch(ctx, entries)
+        assert ctx.execution_profile().ntt_forwards == 0
+        for h, key, k, ct, out in entries:
+            assert torch.equal(out, key.apply_galois(ct, k)), k
+    finally:
+        for h in hoisted:
+            h.close()
+
+
 def test_hoisted_mac_inside_the_inverse_pass(pkg, ctx):
     """With KS_FUSION_INVERSE, hoisted rotations of two-pass degrees compute
     the key MAC (and the canonical-lift correction) inside the Galois-read
