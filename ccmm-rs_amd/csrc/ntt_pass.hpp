@@ -206,9 +206,9 @@ struct NttPass {
                             // off = R 2^n1 + v is source element gk n + gc of
                             // destination n = h 2^n1 + u, u = row_u(col):
                             // gk h + ((gk u + gc) >> n1) = R (mod 2^n2)
-                            const u32 col = e >> L;
-                            const u32 z = pa.gk * row_u<CL, true>(pa, col) + pa.gc;
-                            const u32 h = (((off >> pa.n1) - (z >> pa.n1)) * pa.gkinv) & ((1u << pa.n2) - 1);
+                            const u32 col = e >> L, m2 = (1u << pa.n2) - 1;
+                            const u32 z = mul24(pa.gk, row_u<CL, true>(pa, col)) + pa.gc;
+                            const u32 h = mul24(((off >> pa.n1) - (z >> pa.n1)) & m2, pa.gkinv) & m2;
                             e = bitrev(h, pa.n2) | (col << L);
                         }
                     }

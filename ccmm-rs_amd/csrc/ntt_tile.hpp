@@ -246,6 +246,18 @@ struct PassAddr {
     u32 gk, gc, gkinv;
 };
 
+// 24-bit product: v_mul_u32_u24 issues at full rate where the 32-bit
+// v_mul_lo_u32 is quarter rate.  Both operands below 2^24.
+CK_HD u32 mul24(u32 a, u32 b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umul24(a, b);
+#else
+    CK_BOUNDS(a, 1u << 24);
+    CK_BOUNDS(b, 1u << 24);
+    return a * b;
+#endif
+}
+
 // Row u of tile column col in a row pass.  Plain: the tile's 2^CL
 // consecutive rows.  GU (inverse row pass reading its input through the
 // NTT-domain Galois map n -> gk n + gc): the rows whose images have low
@@ -255,7 +267,8 @@ struct PassAddr {
 template <int CL, bool GU>
 CK_HD u32 row_u(const PassAddr &pa, u32 col) {
     const u32 v = pa.tile * (1u << CL) + col;
-    return GU ? ((v - pa.gc) * pa.gkinv) & ((1u << pa.n1) - 1) : v;
+    const u32 m1 = (1u << pa.n1) - 1;
+    return GU ? mul24((v - pa.gc) & m1, pa.gkinv) & m1 : v;
 }
 
 template <int L, int CL>
