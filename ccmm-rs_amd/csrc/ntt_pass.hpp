@@ -195,25 +195,31 @@ struct NttPass {
 #pragma unroll
                 for (int i = 0; i < 16; ++i) {
                     u32 x = (u32)i * T + tid;
-                    u32 e, off;
+                    u32 e, off, li;
                     if (KIND == PASS_FULL) {
                         e = bitrev(x, n);
                         off = x;
+                        li = lds_index<L, CL>(e);
                     } else {
                         e = stage_coord(x);
                         off = row_natural_offset<L, CL>(pa, e);
-                        if (GU) {
+                        if constexpr (GU) {
                             // off = R 2^n1 + v is source element gk n + gc of
                             // destination n = h 2^n1 + u, u = row_u(col):
-                            // gk h + ((gk u + gc) >> n1) = R (mod 2^n2)
+                            // gk h + ((gk u + gc) >> n1) = R (mod 2^n2); its
+                            // tile coordinate is bitrev_n2(h) | col << L
                             const u32 col = e >> L, m2 = (1u << pa.n2) - 1;
                             const u32 z = mul24(pa.gk, row_u<CL, true>(pa, col)) + pa.gc;
                             const u32 h = mul24(((off >> pa.n1) - (z >> pa.n1)) & m2, pa.gkinv) & m2;
-                            e = bitrev(h, pa.n2) | (col << L);
+                            CK_BOUNDS(h, 1u << L);
+                            li = kGuRow<L, CL>.v[h] ^ lds_swizzle(col);
+                        } else {
+                            li = lds_index<L, CL>(e);
                         }
                     }
                     CK_BOUNDS(off, 1u << n);
-                    lds[lds_index<L, CL>(e)] = src(job, off, FP && INITIAL);
+                    CK_BOUNDS(li, 1u << (L + CL));
+                    lds[li] = src(job, off, FP && INITIAL);
                 }
             });
             ex.barrier();

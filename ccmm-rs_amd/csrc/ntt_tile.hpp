@@ -178,6 +178,37 @@ CK_HD u32 lds_index(u32 e) {
     return lds_swizzle((t << CL) | col);
 }
 
+// Swizzled LDS row offsets for the Galois-read row pass, whose load places
+// each residue at a data-dependent row: entry h is
+// lds_swizzle(bitrev_L(h) << CL), so lds_index(bitrev_L(h) | col << L) =
+// table[h] ^ lds_swizzle(col) (the swizzle is linear over disjoint bits),
+// one cached read instead of a bit reversal and eight conditional XORs.
+constexpr u32 lds_swizzle_c(u32 a) {
+    u32 x = a;
+    for (int k = 0; k < 8; ++k)
+        if ((a >> (5 + k)) & 1u) x ^= kLdsSwizzle[k];
+    return x;
+}
+constexpr u32 bitrev_c(u32 x, u32 bits) {
+    u32 r = 0;
+    for (u32 i = 0; i < bits; ++i) r |= ((x >> i) & 1u) << (bits - 1 - i);
+    return r;
+}
+template <int L, int CL>
+struct GuRowTable {
+    unsigned short v[1 << L];
+    constexpr GuRowTable() : v() {
+        for (u32 h = 0; h < (1u << L); ++h) v[h] = (unsigned short)lds_swizzle_c(bitrev_c(h, L) << CL);
+    }
+};
+#if defined(__HIPCC__)
+template <int L, int CL>
+__device__ constexpr GuRowTable<L, CL> kGuRow{};
+#else
+template <int L, int CL>
+constexpr GuRowTable<L, CL> kGuRow{};
+#endif
+
 // ---- butterflies ------------------------------------------------------------
 
 CK_HD void ct_bfly(u64 &x, u64 &y, const Tw &w, u64 q) {
