@@ -31,7 +31,11 @@ constexpr int kMaxAlpha = 12;   // limbs per gadget block handled by ModUp
 constexpr int kMaxSpecial = 8;  // special primes handled by ModDown
 constexpr int kKsThreads = 256;
 constexpr int kFpRow = 4 + 2 * kMaxAlpha;      // doubles per FP64 ModUp row
-constexpr int kMdRow = 6 + 4 * kMaxSpecial;    // doubles per FP64 ModDown row
+// doubles per FP64 ModDown row: the fp_mulmod weights of each P digit, then
+// the split weights and constants of fp_split_target
+constexpr int kMdSplitW = 6 + 4 * kMaxSpecial;
+constexpr int kMdSplitK = kMdSplitW + 2 * kMaxSpecial;
+constexpr int kMdRow = kMdSplitK + 6;
 
 // Constants of the exact lift of one gadget block.
 struct ModUpBlock {
@@ -66,7 +70,9 @@ struct ModDownTable {
     const u64 *pmod, *pinv, *pinv_q;
     // FP64 target row tf + i * kMdRow (q_i < 2^41): q_i, 1/q_i, P mod q_i,
     // P^-1 mod q_i, fl(P^-1 / q_i), 0, then per P digit m (split at bit 30):
-    // w_im, fl(w_im / q_i), w_im 2^30 mod q_i, fl(that / q_i)
+    // w_im, fl(w_im / q_i), w_im 2^30 mod q_i, fl(that / q_i); from
+    // kMdSplitW per digit w_im mod 2^20, w_im >> 20, and from kMdSplitK
+    // 2^20, 2^30, 2^50 mod q_i each with fl(that / q_i)
     const double *tf;
 };
 
@@ -797,6 +803,7 @@ struct DownArgs {
     const DevPrime *primes;  // key basis
     u64 fp_targets;  // bit i: Q limb i takes the FP64 target path
     u32 limbs, q_limbs, logn, chunks;
+    u32 split;  // every special prime below 2^60: FP64 targets by fp_split_target
 };
 
 template <int K>
@@ -873,11 +880,17 @@ __global__ __launch_bounds__(kKsThreads) void k_moddown(DownArgs a) {
         if (i < 64 && ((fpm >> i) & 1)) {  // the mask is empty for bases over 64 limbs
             const CK_CONST_AS double *r = tf + (u64)i * kMdRow;
             const double q = r[0];
-            double acc = fp_mulmod(cl[0], r[6], r[7], q) + fp_mulmod(ch[0], r[8], r[9], q);
+            double acc;
+            if (a.split) {  // every P digit below 2^60
+                const CK_CONST_AS double *sk = r + kMdSplitK;
+                acc = fp_split_target<K>(cl, ch, r + kMdSplitW, q, r[1], sk[0], sk[1], sk[2], sk[3], sk[4], sk[5]);
+            } else {
+                acc = fp_mulmod(cl[0], r[6], r[7], q) + fp_mulmod(ch[0], r[8], r[9], q);
 #pragma unroll
-            for (int m = 1; m < K; ++m)
-                acc += fp_mulmod(cl[m], r[6 + 4 * m], r[7 + 4 * m], q) +
-                       fp_mulmod(ch[m], r[8 + 4 * m], r[9 + 4 * m], q);
+                for (int m = 1; m < K; ++m)
+                    acc += fp_mulmod(cl[m], r[6 + 4 * m], r[7 + 4 * m], q) +
+                           fp_mulmod(ch[m], r[8 + 4 * m], r[9 + 4 * m], q);
+            }
             double t = small_to_f64(ld_stream(x + (u64)i * N)) - acc;  // x_i - (x_P mod q_i)
             if (neg) t += r[2];                             // x_P - P
             double v = fp_mulmod(t, r[3], r[4], q);
@@ -1228,6 +1241,14 @@ static u64 fp_target_mask(const std::vector<u64> &mods, size_t count) {
 // P^-1 mod q_i and its Shoup quotient appended to consts (the table's
 // device pointers are set once consts is uploaded: md.w at the offset
 // consts.size() had on entry).
+// The ModDown digits are Garner digits below their special primes: with
+// every special prime below 2^60 the FP64 targets take fp_split_target.
+static u32 moddown_split(const std::vector<u64> &mods, size_t q_limbs) {
+    for (size_t m = q_limbs; m < mods.size(); ++m)
+        if (mods[m] >= (1ull << 60)) return 0;
+    return 1;
+}
+
 static void build_moddown(const std::vector<u64> &mods, size_t q_limbs, std::vector<u64> &consts,
                           ModDownTable &md) {
     const size_t K = mods.size() - q_limbs;
@@ -1296,6 +1317,14 @@ static void build_moddown(const std::vector<u64> &mods, size_t q_limbs, std::vec
                 push_fp_pair(consts, w, q);
                 push_fp_pair(consts, host::mulmod(w, (1ull << 30) % q, q), q);
             }
+            consts.resize(row0 + kMdSplitW, 0);
+            for (size_t m = 0; m < K; ++m) {
+                const u64 w = consts[md_w_off + i * kMaxSpecial + m];
+                consts.push_back(double_bits((double)(w & 0xfffffull)));
+                consts.push_back(double_bits((double)(w >> 20)));
+            }
+            consts.resize(row0 + kMdSplitK, 0);
+            for (int e : {20, 30, 50}) push_fp_pair(consts, (u64)(((unsigned __int128)1 << e) % q), q);
         }
         consts.resize(row0 + kMdRow, 0);
     }
@@ -1894,6 +1923,7 @@ static int ks_moddown(const KsJob &j, hipStream_t s) {
     da.md = k->d_moddown;
     da.primes = k->plan->d_primes;
     da.fp_targets = k->fp_q_targets;
+    da.split = moddown_split(k->plan->moduli, k->q_limbs);
     da.limbs = (u32)B;
     da.q_limbs = (u32)k->q_limbs;
     da.logn = k->plan->logn;
@@ -2247,6 +2277,7 @@ int ckks_mod_down(const ckks_rns_plan *qp_plan, size_t q_limbs, const uint64_t *
     da.md = (const ModDownTable *)it->second.second;
     da.primes = qp_plan->d_primes;
     da.fp_targets = fp_target_mask(qp_plan->moduli, q_limbs);
+    da.split = moddown_split(qp_plan->moduli, q_limbs);
     da.limbs = (u32)B;
     da.q_limbs = (u32)q_limbs;
     da.logn = qp_plan->logn;

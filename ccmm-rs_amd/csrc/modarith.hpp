@@ -159,6 +159,46 @@ CK_HD u64 f64_as_u64(double d) {
 CK_HD double small_to_f64(u64 v) { return u64_as_f64(v | 0x4330000000000000ull) - 4503599627370496.0; }
 CK_HD u64 f64_to_small(double d) { return f64_as_u64(d + 4503599627370496.0) - 0x4330000000000000ull; }
 
+// The base-conversion target sum_m c_m w_m mod q over K <= 8 digits
+// c_m < 2^60 given as halves cl_m, ch_m < 2^30 (c = ch 2^30 + cl), and
+// weights w_m < q < 2^41 given as w[2m] = wl_m < 2^20, w[2m+1] = wh_m < 2^21
+// (w = wh 2^20 + wl).  Each product of a half and a weight piece is below
+// 2^51, an exact double, and the partial sums stay exact integers below
+// 2^53: sum_m cl wl and sum_m ch wl have at most 8 terms below 2^50, the
+// wh sums are cut into groups of at most 4 terms below 2^51.  So
+//   sum = S_ll + 2^20 S_lh + 2^30 S_hl + 2^50 S_hh
+// takes FMAs, one fp_reduce per partial sum (below 2^53, q >= 2^20: the
+// quotient estimate is off by less than 2^-13, |r| <= 0.51 q) and three
+// fp_mulmod by k20 = 2^20, k30 = 2^30, k50 = 2^50 mod q (with fl(k / q)):
+// 28 FP64 operations for K = 7 where a fp_mulmod per half and weight takes
+// 84.  |result| <= 2.1 q.
+template <int K, class Row>
+CK_HD double fp_split_target(const double *cl, const double *ch, const Row &w, double q, double qinv, double k20,
+                             double k20p, double k30, double k30p, double k50, double k50p) {
+#pragma clang fp contract(off)
+    double sll = 0.0, shl = 0.0, slh0 = 0.0, slh1 = 0.0, shh0 = 0.0, shh1 = 0.0;
+#pragma unroll
+    for (int m = 0; m < K; ++m) {
+        const double wl = w[2 * m], wh = w[2 * m + 1];
+        sll = __builtin_fma(cl[m], wl, sll);
+        shl = __builtin_fma(ch[m], wl, shl);
+        if (m < 4) {
+            slh0 = __builtin_fma(cl[m], wh, slh0);
+            shh0 = __builtin_fma(ch[m], wh, shh0);
+        } else {
+            slh1 = __builtin_fma(cl[m], wh, slh1);
+            shh1 = __builtin_fma(ch[m], wh, shh1);
+        }
+    }
+    double rlh = fp_reduce(slh0, q, qinv), rhh = fp_reduce(shh0, q, qinv);
+    if (K > 4) {
+        rlh += fp_reduce(slh1, q, qinv);
+        rhh += fp_reduce(shh1, q, qinv);
+    }
+    return (fp_reduce(sll, q, qinv) + fp_mulmod(rlh, k20, k20p, q)) +
+           (fp_mulmod(fp_reduce(shl, q, qinv), k30, k30p, q) + fp_mulmod(rhh, k50, k50p, q));
+}
+
 // Canonical FP64 residue in [0, q) of an integer-valued double, |x| < 2^45.
 CK_HD double fp_canon(double x, double q, double qinv) {
     double r = fp_reduce(x, q, qinv);

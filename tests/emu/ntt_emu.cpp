@@ -192,6 +192,48 @@ extern "C" uint64_t emu_fp_moddown_target(uint64_t x, const uint64_t *c, const u
     return fp_residue(v, qd, 1.0 / qd);
 }
 
+// The same with the split-weight target (fp_split_target), which the ModDown
+// kernel uses when every special prime is below 2^60 (digits c_m < 2^60).
+template <int K>
+static double split_target(const uint64_t *c, const uint64_t *w, uint64_t q) {
+    double cl[K], ch[K], wr[2 * K];
+    for (int m = 0; m < K; ++m) {
+        cl[m] = small_to_f64(c[m] & 0x3fffffffull);
+        ch[m] = small_to_f64(c[m] >> 30);
+        wr[2 * m] = small_to_f64(w[m] & 0xfffffull);
+        wr[2 * m + 1] = small_to_f64(w[m] >> 20);
+    }
+    const uint64_t k20 = (1ull << 20) % q, k30 = (1ull << 30) % q,
+                   k50 = (uint64_t)(((unsigned __int128)1 << 50) % q);
+    const double qd = (double)q;
+    return fp_split_target<K>(cl, ch, wr, qd, 1.0 / qd, (double)k20, fp_quot(k20, q), (double)k30, fp_quot(k30, q),
+                              (double)k50, fp_quot(k50, q));
+}
+
+extern "C" uint64_t emu_fp_moddown_target_split(uint64_t x, const uint64_t *c, const uint64_t *w, int K, int neg,
+                                                uint64_t pmod, uint64_t pinv, uint64_t add, int add_neg,
+                                                uint64_t q) {
+    const double qd = (double)q;
+    double acc;
+    switch (K) {
+        case 1: acc = split_target<1>(c, w, q); break;
+        case 2: acc = split_target<2>(c, w, q); break;
+        case 3: acc = split_target<3>(c, w, q); break;
+        case 4: acc = split_target<4>(c, w, q); break;
+        case 5: acc = split_target<5>(c, w, q); break;
+        case 6: acc = split_target<6>(c, w, q); break;
+        case 7: acc = split_target<7>(c, w, q); break;
+        case 8: acc = split_target<8>(c, w, q); break;
+        default: return ~0ull;
+    }
+    double t = small_to_f64(x) - acc;
+    if (neg) t += (double)pmod;
+    double v = fp_mulmod(t, (double)pinv, fp_quot(pinv, q), qd);
+    const double ad = small_to_f64(add);
+    v = add_neg ? v - ad : v + ad;
+    return fp_residue(v, qd, 1.0 / qd);
+}
+
 // ---- wave-uniform twiddle stages ------------------------------------------
 // Stages that uniform_twiddle_mask flags read their twiddle through the
 // scalar unit (readfirstlane of the index).  For every pass geometry the

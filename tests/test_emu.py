@@ -198,6 +198,32 @@ def test_fp64_moddown_target_exact(emu, q):
                         assert got == want
 
 
+@pytest.mark.parametrize("q", _FP_TARGETS + [_prime_below(1 << 21)])
+def test_fp64_moddown_split_target_exact(emu, q):
+    """The split-weight FP64 ModDown target (digits below 2^60, used when
+    every special prime is) equals exact integer arithmetic: 1 to 8 digits
+    at 2^60 - 1 and 0, weights at q - 1 and at the split boundaries, x and
+    the added b component up to q - 1, both centering branches, both signs."""
+    f = emu.emu_fp_moddown_target_split
+    f.restype = C.c_uint64
+    f.argtypes = [C.c_uint64, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_uint64, C.c_uint64, C.c_uint64,
+                  C.c_int, C.c_uint64]
+    rng = np.random.default_rng(q % 991)
+    edge_w = [w for w in (q - 1, (1 << 20) - 1, 1 << 20, ((1 << 21) - 1) << 20) if w < q]
+    for K in range(1, 9):
+        for digits, x in _operand_sets(rng, q, K, 1 << 60, 1 << 60, 12):
+            for w in [[q - 1] * K, [int(v) for v in rng.integers(0, q, K)],
+                      [edge_w[i % len(edge_w)] for i in range(K)]]:
+                pmod, pinv = q - 1, int(rng.integers(1, q))
+                cd, wd = _u64(digits), _u64(w)  # kept alive across the calls
+                for neg in (0, 1):
+                    for add, add_neg in ((0, 0), (q - 1, 0), (q - 1, 1)):
+                        got = f(x, cd.ctypes.data, wd.ctypes.data, K, neg, pmod, pinv, add, add_neg, q)
+                        xp = sum(c * ww for c, ww in zip(digits, w))
+                        want = ((x - xp + neg * pmod) * pinv + (-add if add_neg else add)) % q
+                        assert got == want, (K, digits, w, x, neg, add, add_neg)
+
+
 @pytest.mark.parametrize("logn", [12, 16, 17])
 def test_emulated_fp64_inverse_extremes(emu, logn):
     """The inverse FP64 butterflies reduce the sum only once per round: pinned
