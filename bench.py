@@ -11,12 +11,13 @@ Workload (batched core primitives, N = 2^16):
   done per rotation does not depend on the values.
 
   The step (--rotation hoisted, the default) cuts and transforms each
-  batch's canonical digits once (ckks_hoist, then ckks_rehoist into the
-  same buffers on later steps) and then runs one
-  ckks_rotate_hoisted_batch per exponent; the hoisted MAC plus a
-  per-exponent correction of the negated coefficients' lift gives the
-  residues of the per-rotation key switch, which --rotation canonical times
-  (ckks_apply_galois_batch per exponent) and which is reported beside it.
+  batch's canonical digits and P NTT(b) once (ckks_hoist_with_b, then
+  ckks_rehoist into the same buffers on later steps) and then runs one
+  ckks_rotate_hoisted_batch per four exponents (their MACs read each digit
+  once); the hoisted MAC plus a per-exponent correction of the negated
+  coefficients' lift gives the residues of the per-rotation key switch,
+  which --rotation canonical times (ckks_apply_galois_batch per exponent)
+  and which is reported beside it.
 
 Reported (one JSON line):
   value           batched hybrid-key-switched rotations per second of the step
@@ -38,7 +39,11 @@ Reported (one JSON line):
   ntt_*           forward NTT over the batch's 96 digit limbs per ciphertext
   p50/p99_ms      latency of one batched rotation call of the step (64
                   ciphertexts at N = 2^16 and the 16 at N = 2^15 in one
-                  launch sequence; hoisted: after the step's hoist)
+                  launch sequence; hoisted: by --exponents-per-call
+                  exponents, after the step's hoist)
+  rotation_roofline
+                  the step against HBM peak: per exponent every ciphertext
+                  in and out plus each key once
   roofline        the forward NTT (dominant kernel pair) against HBM peak
   cpu_baseline    the CPU oracle's hybrid rotation on all the host cores the
                   job may use (one ciphertext per thread), bounded sample;
