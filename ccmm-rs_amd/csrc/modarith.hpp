@@ -167,11 +167,13 @@ CK_HD u64 f64_to_small(double d) { return f64_as_u64(d + 4503599627370496.0) - 0
 // 2^53: sum_m cl wl and sum_m ch wl have at most 8 terms below 2^50, the
 // wh sums are cut into groups of at most 4 terms below 2^51.  So
 //   sum = S_ll + 2^20 S_lh + 2^30 S_hl + 2^50 S_hh
-// takes FMAs, one fp_reduce per partial sum (below 2^53, q >= 2^20: the
-// quotient estimate is off by less than 2^-13, |r| <= 0.51 q) and three
-// fp_mulmod by k20 = 2^20, k30 = 2^30, k50 = 2^50 mod q (with fl(k / q)):
-// 28 FP64 operations for K = 7 where a fp_mulmod per half and weight takes
-// 84.  |result| <= 2.1 q.
+// takes FMAs, one fp_reduce per partial sum and three fp_mulmod by
+// k20 = 2^20, k30 = 2^30, k50 = 2^50 mod q (with fl(k / q)): 28 FP64
+// operations for K = 7 where a fp_mulmod per half and weight takes 84.
+// fp_reduce of a sum below 2^53 stays exact (the FMA's exact result is the
+// remainder); its quotient estimate is off by less than 2^-13 for
+// q >= 2^20 (|r| <= 0.51 q, |result| <= 2.1 q) and by less than 1 for any
+// q (|r| < 1.5 q), inside fp_mulmod's |y| < 2^45 either way.
 template <int K, class Row>
 CK_HD double fp_split_target(const double *cl, const double *ch, const Row &w, double q, double qinv, double k20,
                              double k20p, double k30, double k30p, double k50, double k50p) {
