@@ -186,9 +186,10 @@ def test_hoisted_batch_of_exponents_shares_the_digits(pkg, ctx, with_b):
     """Several exponents of one hoisted batch in one ckks_rotate_hoisted_batch
     call (their MACs read each digit once, up to 4 exponents per launch):
     5 exponents of an N = 2^15 batch, interleaved with 2 of an N = 2^13
-    batch and a lone RNS-gadget one, each equal to apply_galois."""
+    batch and 3 of an RNS-gadget batch (no special primes: P = 1), each
+    equal to apply_galois."""
     rng = np.random.default_rng(515)
-    shapes = [("hybrid", 15, 13, 4, [5, 4, 4], 2, 5), ("hybrid", 13, 9, 4, [5, 4], 3, 2), ("rns", 12, 4, 0, [2, 2], 2, 1)]
+    shapes = [("hybrid", 15, 13, 4, [5, 4, 4], 2, 5), ("hybrid", 13, 9, 4, [5, 4], 3, 2), ("rns", 12, 4, 0, [2, 2], 2, 3)]
     entries, hoisted = [], []
     per = []
     for kind, logn, L, K, blocks, n_cts, n_exp in shapes:
