@@ -437,6 +437,9 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=4.0)
     ap.add_argument("--split", choices=["auto", "off", "on"], default="auto",
                     help="key switches split over two streams: autotuned per key, off, on")
+    ap.add_argument("--ntt-variant", type=int, choices=[0, 1, 2, 3], default=None,
+                    help="two-pass NTT kernel variant for both degrees (bit set of NTT_VARIANT_BLOCKED / "
+                         "NTT_VARIANT_TILE8K), set after the autotuner")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default NTT variant and base-conversion engine")
     ap.add_argument("--base-conversion", choices=["auto", "valu", "mfma"], default="auto",
@@ -524,6 +527,11 @@ def main():
                     print(f"# autotune key switch {deg} (last key, ms per (engine, fusion)): "
                           f"{ {k: round(v, 2) for k, v in times.items()} }", file=sys.stderr, flush=True)
             tuned["key_switch"] = picks
+
+    if args.ntt_variant is not None:
+        ctx.set_ntt_variant(16, args.ntt_variant)
+        ctx.set_ntt_variant(15, args.ntt_variant)
+        tuned["ntt_variant"] = {"2^16": args.ntt_variant, "2^15": args.ntt_variant, "set": True}
 
     B = args.batch
     cts = rand_residues(torch, (B, 2, L, N), dev)
