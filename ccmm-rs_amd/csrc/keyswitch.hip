@@ -1863,7 +1863,9 @@ static int ks_mac(const KsJob &j, hipStream_t s) {
 
 // The MACs of hoisted jobs jobs[g] (g in grp) that share their digits:
 // one k_ks_mac_multi when 2..kMacMaxExp of them have dnum <= 8, else one
-// ks_mac each.
+// ks_mac each.  (More digits, chunked: a two-exponent chunk kernel took
+// 0.41 ms per chunk of 8 against 2 x 0.16 ms for the one-exponent chunks,
+// at 132 VGPRs and without their next-ciphertext prefetch.)
 static int ks_mac_group(const std::vector<KsJob> &jobs, const std::vector<size_t> &grp, hipStream_t s) {
     const KsJob &j0 = jobs[grp[0]];
     const ckks_ksk *k = j0.k;
@@ -2896,23 +2898,26 @@ int ckks_rotate_hoisted_batch(ckks_context *ctx, const ckks_hoisted_rotation *e,
         const ckks_ksk *key = r.key;
         const ckks_hoisted *h = r.hoisted;
         if (key->ctx != ctx) return fail(CKKS_ERR_INVALID, "hoisted rotation batch entries must share the context");
-        if (key->bounded) {
-            if ((rc = ckks_bounded_rotate_hoisted(h, key, r.exponent, r.in, r.out, r.in_stride, r.out_stride)))
-                return rc;
-            continue;
-        }
         size_t in_stride = r.in_stride, out_stride = r.out_stride;
         if ((rc = check_ct_args(key, r.in, r.out, h->n_cts, in_stride, out_stride, 2))) return rc;
         if (h->n_cts == 0) continue;
-        if (h->blocks.empty() || key->plan != h->plan || key->q_limbs != h->q_limbs || key->block_size != h->blocks)
-            return fail(CKKS_ERR_INVALID, "key does not match the hoisted digits (plan, Q limbs, gadget blocks)");
+        if (key->bounded ? (!h->blocks.empty() || key->plan != h->plan || key->q_limbs != h->q_limbs ||
+                            key->base_log != h->base_log || key->dnum != h->digit_count)
+                         : (h->blocks.empty() || key->plan != h->plan || key->q_limbs != h->q_limbs ||
+                            key->block_size != h->blocks))
+            return fail(CKKS_ERR_INVALID, "key does not match the hoisted digits (plan, Q limbs, gadget)");
         const u64 two_n = 2 * (u64)key->degree, k = r.exponent % two_n;
         if ((k & 1) == 0) return fail(CKKS_ERR_INVALID, "CKKS automorphism exponent must be odd modulo 2N");
         if (r.in == r.out) return fail(CKKS_ERR_INVALID, "rotation output must not alias its input");
         const u64 kinv = galois_inverse(k, two_n);
         const size_t poly = key->q_limbs * key->degree;
         const u64 *pk = nullptr;
-        rc = h->zero_block ? kNoRoomForKeyCopy : hoisted_key(key, k, &pk);
+        // the canonical path: a zero block value (canonical digits) or a
+        // negated tie (bounded digits, see ckks_bounded_rotate_hoisted)
+        bool canonical = h->zero_block;
+        for (const auto &t : h->ties)
+            canonical = canonical || ((u64)t.second * k & (two_n - 1)) >= key->degree;
+        rc = canonical ? kNoRoomForKeyCopy : hoisted_key(key, k, &pk);
         if (rc == kNoRoomForKeyCopy) {  // the canonical key switch, same residues
             if ((rc = key_switch_core(key, r.in + poly, in_stride, r.in, nullptr, in_stride, r.out, out_stride,
                                       h->n_cts, kinv)))
@@ -2943,8 +2948,9 @@ int ckks_rotate_hoisted_batch(ckks_context *ctx, const ckks_hoisted_rotation *e,
         const Fast &f = fast[i];
         const ckks_ksk *key = f.key;
         const size_t B = key->limbs, N = key->degree, DBN = key->dnum * B * N, poly = key->q_limbs * N;
-        hms[i] = HoistedMac{f.h->d_digits, f.pk, f.pk + DBN, f.k, f.pk + 2 * DBN, f.pk + 2 * DBN + B * N,
-                            f.h->d_bhat};
+        // bounded keys keep no correction tables after the permuted copy
+        const u64 *tb = key->bounded ? nullptr : f.pk + 2 * DBN;
+        hms[i] = HoistedMac{f.h->d_digits, f.pk, f.pk + DBN, f.k, tb, tb ? tb + B * N : nullptr, f.h->d_bhat};
         const u64 kinv = galois_inverse(f.k, 2 * (u64)N);
         KsJob &j = jobs[i];
         j = KsJob{key, f.in + poly, f.in_stride, f.in, nullptr, f.in_stride, f.out, f.out_stride, f.h->n_cts, kinv,

@@ -221,6 +221,39 @@ def test_hoisted_batch_of_exponents_shares_the_digits(pkg, ctx, with_b):
             h.close()
 
 
+def test_bounded_hoisted_batch_shares_the_digits(pkg, ctx):
+    """Bounded (base 2^w) digits of one hoisted batch rotated by five
+    exponents in one ckks_rotate_hoisted_batch call (more than 8 digits: one
+    chunked MAC per exponent, one inverse NTT launch for all); a batch
+    holding a -2^(w-1) tie takes the canonical path for the exponents that
+    negate it.  Every entry equals apply_galois."""
+    rng = np.random.default_rng(2323)
+    n, L, K, w = 1 << 13, 6, 2, 32
+    q = [Q0[0]] + Q40[:L - 1]
+    basis = q + P60[:K]
+    plan = pkg.PreparedRnsNttPlan(ctx, basis, n)
+    cnt = O.bounded_digit_count(q, w)
+    keys = [pkg.BoundedKeySwitchKey(plan, L, w, rand_residues(rng, basis, n, cnt), rand_residues(rng, basis, n, cnt))
+            for _ in range(5)]
+    ct = rand_residues(rng, q, n, polys=6).reshape(3, 2, L, n)
+    batches = [ct]
+    tie = ct.copy()
+    # coefficient 7 of ciphertext 1: a = -2^(w-1), whose balanced digit vector holds -2^(w-1)
+    tie[1, 1, :, 7] = [(-(1 << (w - 1))) % qi for qi in q]
+    batches.append(tie)
+    for b in batches:
+        d = pkg.to_device(b)
+        h = keys[0].hoist(d)
+        try:
+            exps = [O.rotation_exponent_left(n, 1 << e) for e in range(4)] + [2 * n - 1]
+            entries = [(h, key, k, d, torch.empty_like(d)) for key, k in zip(keys, exps)]
+            pkg.rotate_hoisted_batch(ctx, entries)
+            for _, key, k, _, out in entries:
+                assert torch.equal(out, key.apply_galois(d, k)), k
+        finally:
+            h.close()
+
+
 def test_hoisted_mac_inside_the_inverse_pass(pkg, ctx):
     """With KS_FUSION_INVERSE, hoisted rotations of two-pass degrees compute
     the key MAC (and the canonical-lift correction) inside the Galois-read
